@@ -1,0 +1,21 @@
+"""BoostFractor on MI355X: drop-in mirror of BoostFractor.jl's public API.
+
+The module layout follows the reference (src/BoostFractor.jl:5-20): the
+coordinate/boundary data model, the propagator plug-ins, the recursive
+Fourier solvers and the mode-matching transformer.  Every numerical call runs
+in libboostfractor_hip.so (hand-written gfx950 HIP kernels) through its C ABI
+(include/boostfractor_hip.h); there is no CPU fallback.
+"""
+from .setup import (CoordinateSystem, DimensionMismatch, SeedCoordinateSystem, SeedSetupBoundaries,
+                    SetupBoundaries, get_kspace_coords, wavelength)
+from .jlrange import julia_range
+from ._lib import ArgumentError, HipError, UnsupportedError, Context, context, device_count
+from .propagators import propagator, propagator1D, propagatorMomentumSpace, propagatorNoTilts
+from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
+
+__all__ = [
+    "CoordinateSystem", "SeedCoordinateSystem", "get_kspace_coords", "SetupBoundaries", "SeedSetupBoundaries",
+    "wavelength", "julia_range", "DimensionMismatch", "ArgumentError", "HipError", "UnsupportedError", "Context",
+    "context", "device_count", "propagator", "propagatorNoTilts", "propagatorMomentumSpace", "propagator1D",
+    "dancer", "dance_intro", "cheerleader", "DancerPlan",
+]

@@ -1,0 +1,217 @@
+"""ctypes binding of libboostfractor_hip.so (include/boostfractor_hip.h).
+
+There is no CPU fallback: if the HIP library is missing or no device is
+visible, every engine call raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from .setup import DimensionMismatch
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("BF_LIBRARY", os.path.join(PKG_DIR, "lib", "libboostfractor_hip.so"))
+
+BF_OK, BF_ERR_DIM, BF_ERR_ARG, BF_ERR_UNSUPPORTED, BF_ERR_HIP, BF_ERR_NOMEM = 0, 1, 2, 3, 4, 5
+PROP_PROPAGATOR, PROP_NOTILTS, PROP_1D, PROP_MOMENTUM = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError / error(...)"""
+
+
+class UnsupportedError(NotImplementedError):
+    pass
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+_P = ctypes.c_void_p
+
+
+class Coords(ctypes.Structure):
+    _fields_ = [("nx", _I32), ("ny", _I32), ("X", _P), ("Y", _P), ("kX", _P), ("kY", _P)]
+
+
+class Setup(ctypes.Structure):
+    _fields_ = [("nreg", _I32), ("distance", _P), ("r", _P), ("eps", _P), ("tilt_x", _P), ("tilt_y", _P),
+                ("surfaces", _P)]
+
+
+class ModesS(ctypes.Structure):
+    _fields_ = [("n", _I32), ("ncomp", _I32), ("patterns", _P), ("mode_kt", _P)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "bf_version": (ctypes.c_int, []),
+    "bf_last_error": (ctypes.c_char_p, [_P]),
+    "bf_device_count": (ctypes.c_int, [ctypes.POINTER(_I32)]),
+    "bf_ctx_create": (ctypes.c_int, [_I32, ctypes.POINTER(_P)]),
+    "bf_ctx_destroy": (ctypes.c_int, [_P]),
+    "bf_ctx_stream": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "bf_ctx_synchronize": (ctypes.c_int, [_P]),
+    "bf_propagate": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Coords), _P, _D, _D, _D, _D, _D, _D, _P, _D]),
+    "bf_dance_intro": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _P, _D, _D, _D, _P]),
+    "bf_dancer": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32, _P, _P,
+                                 _D, _I32, _I32, _I32, _P, _P, _P]),
+    "bf_cheerleader": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32,
+                                      _P, _P, _D, _I32, _P, _P]),
+    "bf_dancer_plan_create": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32,
+                                             _I32, _D, _I32, _I32, ctypes.POINTER(_P)]),
+    "bf_dancer_plan_run": (ctypes.c_int, [_P]),
+    "bf_dancer_plan_fetch": (ctypes.c_int, [_P, _P, _P]),
+    "bf_dancer_plan_timing": (ctypes.c_int, [_P, _I32]),
+    "bf_dancer_plan_kernel_times": (ctypes.c_int, [_P, _P, _P]),
+    "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
+    "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),
+    "bf_field2modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), ctypes.POINTER(ModesS), _P, _D, _D, _D, _P]),
+    "bf_modes2field": (ctypes.c_int, [_P, ctypes.POINTER(Coords), ctypes.POINTER(ModesS), _P, _P]),
+    "bf_propagation_matrices": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords),
+                                               ctypes.POINTER(ModesS), _P, _I32, _I32, _D, _I32, _P]),
+    "bf_transformer": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), ctypes.POINTER(ModesS), _P,
+                                      _I32, _I32, _P, _D, _P, _P, _I32, _P, _P]),
+    "bf_transformer_trace_back": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Setup), ctypes.POINTER(Coords),
+                                                 ctypes.POINTER(ModesS), _D, _I32, _P, _D, _I32, _P, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_tls = threading.local()
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"libboostfractor_hip.so not found at {LIB_PATH}: build it with "
+                        "`make -C boostfractor.jl_amd` (or __graft_entry__.build()); there is no CPU fallback")
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in _SIGS.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def check(rc, ctx=None):
+    if rc == BF_OK:
+        return
+    msg = lib().bf_last_error(ctx)
+    msg = msg.decode() if msg else ""
+    if rc == BF_ERR_DIM:
+        raise DimensionMismatch(msg)
+    if rc == BF_ERR_ARG:
+        raise ArgumentError(msg)
+    if rc == BF_ERR_UNSUPPORTED:
+        raise UnsupportedError(msg)
+    if rc == BF_ERR_NOMEM:
+        raise MemoryError(msg)
+    raise HipError(msg)
+
+
+class Context:
+    """One engine context (device + HIP stream).  One per thread by default."""
+
+    def __init__(self, device=None):
+        if device is None:
+            device = int(os.environ.get("BF_DEVICE", "0"))
+        h = _P()
+        check(lib().bf_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().bf_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        s = _P()
+        check(lib().bf_ctx_stream(self.handle, ctypes.byref(s)), self.handle)
+        return s.value
+
+    def synchronize(self):
+        check(lib().bf_ctx_synchronize(self.handle), self.handle)
+
+
+def context():
+    """The calling thread's context (created on first use)."""
+    c = getattr(_tls, "ctx", None)
+    if c is None:
+        c = Context()
+        _tls.ctx = c
+    return c
+
+
+def device_count():
+    n = _I32()
+    rc = lib().bf_device_count(ctypes.byref(n))
+    return n.value if rc == BF_OK else 0
+
+
+# ---- marshalling helpers ---------------------------------------------------
+def cptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def as_c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def as_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Marshalled:
+    """Holds the numpy buffers a ctypes struct points into."""
+
+    def __init__(self):
+        self.keep = []
+
+    def hold(self, a):
+        self.keep.append(a)
+        return a
+
+    def coords(self, c):
+        X, Y, kX, kY = (self.hold(as_f64(v)) for v in (c.X, c.Y, c.kX, c.kY))
+        return Coords(len(X), len(Y), cptr(X), cptr(Y), cptr(kX), cptr(kY))
+
+    def setup(self, b, ny, nx):
+        dist = self.hold(as_f64(b.distance))
+        eps = self.hold(as_c128(b.eps))
+        r = self.hold(as_c128(b.r))
+        tx = self.hold(as_f64(np.broadcast_to(b.relative_tilt_x, dist.shape)))
+        ty = self.hold(as_f64(np.broadcast_to(b.relative_tilt_y, dist.shape)))
+        surf = None
+        s = np.asarray(b.relative_surfaces)
+        if s.size and np.any(s != 0):
+            if s.shape != (len(dist), ny, nx):
+                raise DimensionMismatch(
+                    f"relative_surfaces has shape {s.shape}, expected {(len(dist), ny, nx)}")
+            surf = self.hold(as_c128(s))
+        return Setup(len(dist), cptr(dist), cptr(r), cptr(eps), cptr(tx), cptr(ty), cptr(surf))
+
+    def modes(self, m):
+        pat = self.hold(as_c128(m.patterns_flat()))
+        kt = self.hold(as_c128(np.asarray(m.mode_kt).reshape(-1)))
+        n = m.n
+        return ModesS(n, m.ncomp, cptr(pat), cptr(kt))

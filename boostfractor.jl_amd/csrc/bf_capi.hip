@@ -1,0 +1,371 @@
+// bf_capi.hip -- extern "C" entry points of libboostfractor_hip.so.
+//
+// Every entry point validates its arguments on the host first (so the error
+// paths are testable without a device), then runs the engine, converting any
+// internal bf::Error into a status code and a message for bf_last_error().
+#include "bf_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+
+struct bf_dancer_plan {
+    bf::DancerPlanBase* impl = nullptr;
+};
+
+static thread_local std::string g_last_error;
+
+static int fail(const bf_ctx* ctx, int code, const std::string& msg) {
+    g_last_error = msg;
+    if (ctx) const_cast<bf_ctx*>(ctx)->err = msg;
+    return code;
+}
+
+#define BF_REQUIRE(ctx, cond, code, msg) \
+    do {                                 \
+        if (!(cond)) return fail(ctx, code, msg); \
+    } while (0)
+
+template <typename F>
+static int guarded(bf_ctx* ctx, F&& body) {
+    try {
+        if (ctx) BF_HIP(hipSetDevice(ctx->device));
+        body();
+        return BF_OK;
+    } catch (const bf::Error& e) {
+        return fail(ctx, e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, BF_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ctx, BF_ERR_HIP, e.what());
+    }
+}
+
+// ---- argument checks ------------------------------------------------------
+static int check_coords(const bf_ctx* ctx, const bf_coords* c) {
+    BF_REQUIRE(ctx, c != nullptr, BF_ERR_ARG, "coords is NULL");
+    BF_REQUIRE(ctx, c->nx >= 1 && c->ny >= 1, BF_ERR_DIM, "grid dimensions must be >= 1");
+    BF_REQUIRE(ctx, c->nx <= BF_MAX_GRID && c->ny <= BF_MAX_GRID, BF_ERR_UNSUPPORTED,
+               "grid dimensions above 1024 are not supported");
+    BF_REQUIRE(ctx, c->X && c->Y && c->kX && c->kY, BF_ERR_ARG, "coordinate arrays must not be NULL");
+    return BF_OK;
+}
+
+static int check_setup(const bf_ctx* ctx, const bf_setup* s, int min_reg, bool need_r) {
+    BF_REQUIRE(ctx, s != nullptr, BF_ERR_ARG, "setup is NULL");
+    BF_REQUIRE(ctx, s->nreg >= min_reg, BF_ERR_DIM,
+               "the arrays in your SetupBoundaries objects don't fit together! (too few regions)");
+    BF_REQUIRE(ctx, s->distance && s->eps && s->tilt_x && s->tilt_y, BF_ERR_ARG, "setup arrays must not be NULL");
+    BF_REQUIRE(ctx, !need_r || s->r, BF_ERR_ARG, "setup reflectivities must not be NULL");
+    return BF_OK;
+}
+
+static int check_kind(const bf_ctx* ctx, int kind) {
+    BF_REQUIRE(ctx, kind >= BF_PROP_PROPAGATOR && kind <= BF_PROP_MOMENTUM, BF_ERR_ARG,
+               "unknown propagator kind (user-defined Julia propagators have no device equivalent)");
+    return BF_OK;
+}
+
+static int check_prec(const bf_ctx* ctx, int p) {
+    BF_REQUIRE(ctx, p == BF_PREC_FP64 || p == BF_PREC_MIXED, BF_ERR_ARG, "unknown precision");
+    return BF_OK;
+}
+
+static int check_freqs(const bf_ctx* ctx, const double* f, int n) {
+    BF_REQUIRE(ctx, f != nullptr && n >= 1, BF_ERR_ARG, "at least one frequency is required");
+    for (int i = 0; i < n; i++) BF_REQUIRE(ctx, std::isfinite(f[i]) && f[i] > 0.0, BF_ERR_ARG, "frequencies must be > 0");
+    return BF_OK;
+}
+
+static int check_modes(const bf_ctx* ctx, const bf_modes* m) {
+    BF_REQUIRE(ctx, m != nullptr && m->patterns && m->mode_kt, BF_ERR_ARG, "modes or its arrays are NULL");
+    BF_REQUIRE(ctx, m->n >= 1 && m->ncomp >= 1, BF_ERR_DIM, "modes must have n >= 1 and ncomp >= 1");
+    BF_REQUIRE(ctx, m->n <= BF_MAX_MODES, BF_ERR_UNSUPPORTED, "more than 128 modes are not supported");
+    return BF_OK;
+}
+
+#define TRY(x)              \
+    do {                    \
+        int rc_ = (x);      \
+        if (rc_) return rc_; \
+    } while (0)
+
+extern "C" {
+
+int bf_version(void) { return 10000; }
+
+const char* bf_last_error(const bf_ctx* ctx) {
+    if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+int bf_device_count(int32_t* count) {
+    if (!count) return fail(nullptr, BF_ERR_ARG, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(nullptr, BF_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    *count = n;
+    return BF_OK;
+}
+
+int bf_ctx_create(int32_t device, bf_ctx** out) {
+    if (!out) return fail(nullptr, BF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    bf_ctx* c = new bf_ctx();
+    c->device = device;
+    int rc = guarded(nullptr, [&] {
+        int n = 0;
+        BF_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw bf::Error(BF_ERR_ARG, "device index out of range");
+        BF_HIP(hipSetDevice(device));
+        BF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    });
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return BF_OK;
+}
+
+int bf_ctx_destroy(bf_ctx* ctx) {
+    if (!ctx) return BF_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->twiddles.clear();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BF_OK;
+}
+
+int bf_ctx_stream(bf_ctx* ctx, void** stream) {
+    BF_REQUIRE(ctx, ctx && stream, BF_ERR_ARG, "ctx or stream is NULL");
+    *stream = (void*)ctx->stream;
+    return BF_OK;
+}
+
+int bf_ctx_synchronize(bf_ctx* ctx) {
+    BF_REQUIRE(ctx, ctx, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { BF_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int bf_propagate(bf_ctx* ctx, int32_t prop_kind, const bf_coords* coords, double* E, double dz, double diskR,
+                 double eps_re, double eps_im, double tilt_x, double tilt_y, const double* surface, double lambda) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_coords(ctx, coords));
+    BF_REQUIRE(ctx, E != nullptr, BF_ERR_ARG, "E is NULL");
+    BF_REQUIRE(ctx, lambda > 0.0, BF_ERR_ARG, "lambda must be > 0");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::propagate_one(ctx, prop_kind, coords, E, dz, diskR, cmk<double>(eps_re, eps_im), tilt_x, tilt_y, surface,
+                          lambda);
+    });
+}
+
+int bf_dance_intro(bf_ctx* ctx, const bf_coords* coords, int32_t nd, const double* eps, double r1_re, double r1_im,
+                   double diskR, double* out) {
+    TRY(check_coords(ctx, coords));
+    BF_REQUIRE(ctx, nd >= 1 && eps && out, BF_ERR_ARG, "dance_intro needs nd >= 1, eps and out");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::dance_intro_dev(ctx, coords, nd, eps, cmk<double>(r1_re, r1_im), diskR, out); });
+}
+
+int bf_dancer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, double amin, int32_t nmax,
+              const double* freqs, int32_t nfreq, int32_t prop_kind, const double* emit, const double* reflect,
+              double diskR, int32_t returnsum, int32_t immediatesum, int32_t precision, double* Eout, double* power,
+              int32_t* bounces) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, true));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nmax >= -1, BF_ERR_ARG, "nmax must be >= -1");
+    BF_REQUIRE(ctx, Eout || power, BF_ERR_ARG, "Eout and power are both NULL");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::DancerOptions o;
+        o.reflect = reflect;  // reflect takes precedence over emit (src/boost_fractor_recurprop.jl:53-63)
+        o.emit = reflect ? nullptr : emit;
+        o.returnsum = returnsum ? 1 : 0;
+        o.immediatesum = immediatesum ? 1 : 0;
+        std::unique_ptr<bf::DancerPlanBase> p(bf::make_dancer_plan(ctx, setup, coords, amin, nmax, freqs, nfreq,
+                                                                   prop_kind, diskR, precision, o));
+        p->run();
+        p->fetch(Eout, power, bounces);
+    });
+}
+
+int bf_cheerleader(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, double amin, int32_t nmax,
+                   const double* freqs, int32_t nfreq, int32_t prop_kind, const double* emit, const double* reflect,
+                   double diskR, int32_t precision, double* out_right, double* out_left) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 3, true));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, out_right != nullptr, BF_ERR_ARG, "out_right is NULL");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        const double* em = reflect ? nullptr : emit;
+        if (precision == BF_PREC_MIXED)
+            bf::cheerleader_run<float>(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, em, reflect, diskR,
+                                       out_right, out_left);
+        else
+            bf::cheerleader_run<double>(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, em, reflect, diskR,
+                                        out_right, out_left);
+    });
+}
+
+int bf_dancer_plan_create(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, double amin, int32_t nmax,
+                          const double* freqs, int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                          int32_t freq_batch, bf_dancer_plan** out) {
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, true));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nmax >= -1, BF_ERR_ARG, "nmax must be >= -1");
+    BF_REQUIRE(ctx, freq_batch >= 0, BF_ERR_ARG, "freq_batch must be >= 0");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::DancerOptions o;
+        o.freq_batch = freq_batch;
+        const char* tx = getenv("BF_COL_TX");
+        const char* rows = getenv("BF_ROWS");
+        if (tx) o.col_tx = atoi(tx);
+        if (rows) o.rows = atoi(rows);
+        auto* p = new bf_dancer_plan();
+        try {
+            p->impl = bf::make_dancer_plan(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, diskR, precision, o);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+int bf_dancer_plan_run(bf_dancer_plan* plan) {
+    BF_REQUIRE(nullptr, plan && plan->impl, BF_ERR_ARG, "plan is NULL");
+    bf_ctx* ctx = plan->impl->ctx;
+    return guarded(ctx, [&] { plan->impl->run(); });
+}
+
+int bf_dancer_plan_fetch(bf_dancer_plan* plan, double* Eout, double* power) {
+    BF_REQUIRE(nullptr, plan && plan->impl, BF_ERR_ARG, "plan is NULL");
+    bf_ctx* ctx = plan->impl->ctx;
+    return guarded(ctx, [&] { plan->impl->fetch(Eout, power, nullptr); });
+}
+
+int bf_dancer_plan_timing(bf_dancer_plan* plan, int32_t enable) {
+    BF_REQUIRE(nullptr, plan && plan->impl, BF_ERR_ARG, "plan is NULL");
+    bf_ctx* ctx = plan->impl->ctx;
+    return guarded(ctx, [&] { plan->impl->set_timing(enable != 0); });
+}
+
+int bf_dancer_plan_kernel_times(bf_dancer_plan* plan, double* ms, int64_t* launches) {
+    BF_REQUIRE(nullptr, plan && plan->impl, BF_ERR_ARG, "plan is NULL");
+    bf_ctx* ctx = plan->impl->ctx;
+    return guarded(ctx, [&] { plan->impl->kernel_times(ms, launches); });
+}
+
+int bf_dancer_plan_destroy(bf_dancer_plan* plan) {
+    if (!plan) return BF_OK;
+    if (plan->impl) {
+        (void)hipSetDevice(plan->impl->ctx->device);
+        (void)hipStreamSynchronize(plan->impl->ctx->stream);
+        delete plan->impl;
+    }
+    delete plan;
+    return BF_OK;
+}
+
+int bf_seed_modes(bf_ctx* ctx, const bf_coords* coords, int32_t M, int32_t L, double diskR, double* patterns,
+                  double* mode_kt) {
+    TRY(check_coords(ctx, coords));
+    BF_REQUIRE(ctx, M >= 1 && L >= 0, BF_ERR_ARG, "SeedModes needs Mmax >= 1 and Lmax >= 0");
+    BF_REQUIRE(ctx, (int64_t)M * (2 * L + 1) <= BF_MAX_MODES, BF_ERR_UNSUPPORTED, "more than 128 modes");
+    BF_REQUIRE(ctx, patterns && mode_kt, BF_ERR_ARG, "output arrays are NULL");
+    BF_REQUIRE(ctx, diskR > 0.0, BF_ERR_ARG, "diskR must be > 0");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::seed_modes_dev(ctx, coords, M, L, diskR, patterns, mode_kt); });
+}
+
+int bf_field2modes(bf_ctx* ctx, const bf_coords* coords, const bf_modes* modes, const double* B, double velocity_x,
+                   double diskR, double f, double* coeffs) {
+    TRY(check_coords(ctx, coords));
+    TRY(check_modes(ctx, modes));
+    BF_REQUIRE(ctx, coeffs != nullptr, BF_ERR_ARG, "coeffs is NULL");
+    BF_REQUIRE(ctx, velocity_x == 0.0 || f > 0.0, BF_ERR_ARG, "frequency must be > 0");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::field2modes_dev(ctx, coords, modes, B, velocity_x, diskR, f, coeffs); });
+}
+
+int bf_modes2field(bf_ctx* ctx, const bf_coords* coords, const bf_modes* modes, const double* coeffs,
+                   double* field) {
+    TRY(check_coords(ctx, coords));
+    TRY(check_modes(ctx, modes));
+    BF_REQUIRE(ctx, coeffs && field, BF_ERR_ARG, "coeffs or field is NULL");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::modes2field_dev(ctx, coords, modes, coeffs, field); });
+}
+
+int bf_propagation_matrices(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, const bf_modes* modes,
+                            const double* freqs, int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                            double* out) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 1, false));
+    TRY(check_modes(ctx, modes));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::propagation_matrices_dev(ctx, setup, coords, modes, freqs, nfreq, prop_kind, diskR, precision, out);
+    });
+}
+
+int bf_transformer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, const bf_modes* modes,
+                   const double* freqs, int32_t nfreq, int32_t prop_kind, const double* prop_mats, double diskR,
+                   const double* emit, const double* reflect, int32_t precision, double* boost, double* refl) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, false));
+    TRY(check_modes(ctx, modes));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, emit && boost, BF_ERR_ARG, "emit and boost must not be NULL");
+    BF_REQUIRE(ctx, !reflect || refl, BF_ERR_ARG, "refl output is NULL while reflect is given");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::transformer_dev(ctx, setup, coords, modes, freqs, nfreq, prop_kind, prop_mats, diskR, emit, reflect,
+                            precision, boost, refl);
+    });
+}
+
+int bf_transformer_trace_back(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* setup,
+                              const bf_coords* coords, const bf_modes* modes, double f, int32_t prop_kind,
+                              const double* prop_mats, double diskR, int32_t includes_axion, const double* emit,
+                              double* fields) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, false));
+    TRY(check_modes(ctx, modes));
+    TRY(check_freqs(ctx, &f, 1));
+    BF_REQUIRE(ctx, reflected && input && fields, BF_ERR_ARG, "reflected, input and fields must not be NULL");
+    BF_REQUIRE(ctx, !includes_axion || emit, BF_ERR_ARG, "emit is required when includes_axion is set");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::trace_back_dev(ctx, reflected, input, setup, coords, modes, f, prop_kind, prop_mats, diskR, includes_axion,
+                           emit, fields);
+    });
+}
+
+}  // extern "C"

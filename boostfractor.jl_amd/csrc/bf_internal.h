@@ -1,0 +1,140 @@
+// bf_internal.h -- host-side internals shared by the engine translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/boostfractor_hip.h"
+#include "bf_device.h"
+
+namespace bf {
+
+// Internal failure carrying a C-ABI status code; converted to a return code at
+// the extern "C" boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define BF_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::bf::Error(BF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+#define BF_CHECK_LAUNCH() BF_HIP(hipGetLastError())
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(e == hipErrorOutOfMemory ? BF_ERR_NOMEM : BF_ERR_HIP,
+                        std::string("hipMalloc(") + std::to_string(n) + " bytes): " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename U>
+    U* as() const { return static_cast<U*>(p); }
+};
+
+}  // namespace bf
+
+// Per-caller context: one device, one stream, cached twiddle tables.
+struct bf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // twiddle tables keyed by (n, bytes per real)
+    std::map<std::pair<int, int>, bf::DevBuf> twiddles;
+};
+
+namespace bf {
+
+// Device-resident coordinate grids.
+struct Grid {
+    int nx = 0, ny = 0;
+    DevBuf X, Y, kX, kY;  // double
+    void upload(const bf_coords* c, hipStream_t s);
+    size_t K() const { return (size_t)nx * ny; }
+};
+
+template <typename T>
+const cplx<T>* twiddle_table(bf_ctx* ctx, int n);
+
+struct DancerOptions {
+    const double* emit = nullptr;     // host [R][2][K]
+    const double* reflect = nullptr;  // host [K]
+    int returnsum = 1, immediatesum = 1;
+    int freq_batch = 0;
+    int col_tx = 0, rows = 0;  // launch shapes (0: defaults)
+};
+
+struct DancerPlanBase {
+    virtual ~DancerPlanBase() = default;
+    virtual void run() = 0;
+    virtual void fetch(double* Eout, double* power, int32_t* bounces) = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual void kernel_times(double* ms, int64_t* launches) = 0;
+    bf_ctx* ctx = nullptr;
+};
+
+// recursive Fourier propagation (bf_engine.hip)
+void propagate_one(bf_ctx* ctx, int kind, const bf_coords* coords, double* E, double dz, double diskR, zd eps,
+                   double tx, double ty, const double* surface, double lambda);
+void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double* eps, zd r1, double diskR,
+                     double* out);
+DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax,
+                                 const double* fr, int nfr, int kind, double diskR, int precision,
+                                 const DancerOptions& o);
+template <typename T>
+void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax, const double* fr,
+                     int nfr, int kind, const double* emit_h, const double* reflect_h, double diskR, double* out_right,
+                     double* out_left);
+
+// mode matching (bf_transformer.hip)
+void seed_modes_dev(bf_ctx* ctx, const bf_coords* coords, int M, int L, double diskR, double* patterns, double* kt);
+void field2modes_dev(bf_ctx* ctx, const bf_coords* coords, const bf_modes* modes, const double* B, double velocity_x,
+                     double diskR, double f, double* coeffs);
+void modes2field_dev(bf_ctx* ctx, const bf_coords* coords, const bf_modes* modes, const double* coeffs,
+                     double* field);
+void propagation_matrices_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* md,
+                              const double* freqs, int nfreq, int kind, double diskR, int precision, double* out);
+void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* md, const double* freqs,
+                     int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
+                     const double* reflect, int precision, double* boost, double* refl);
+void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* st,
+                    const bf_coords* co, const bf_modes* md, double f, int kind, const double* prop_mats,
+                    double diskR, int includes_axion, const double* emit, double* fields);
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+void download(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+}  // namespace bf

@@ -1,0 +1,196 @@
+/*
+ * boostfractor_hip.h -- C ABI of the MI355X (gfx950) engine for BoostFractor.jl's
+ * two propagation solvers.
+ *
+ * The reference is the Julia package BoostFractor.jl; every entry point below
+ * replaces one of its Julia functions and is what its `ccall` binding would
+ * target.  Conventions shared by all entry points:
+ *
+ *   - complex numbers are interleaved (re, im) doubles, binary compatible with
+ *     Julia's Complex{Float64};
+ *   - a field is [ny][nx] with x fastest, i.e. exactly Julia's column-major
+ *     E[x, y] (Array{ComplexF64,2}(undef, Nx, Ny));
+ *   - stacked objects put their stack index slowest: surfaces [nreg][ny][nx],
+ *     dancer fields [nreg-1][2][ny][nx], mode patterns [n][ncomp][ny][nx] with
+ *     mode index (m-1)(2L+1) + l + L, n x n matrices column-major;
+ *   - all buffers are caller-owned host memory; nothing is retained after a
+ *     call returns (plans below keep their own device copies);
+ *   - functions return a status (BF_OK or BF_ERR_*); the message of the last
+ *     failure is bf_last_error(ctx) (bf_last_error(NULL) for failures raised
+ *     before a context was available).  Argument and dimension checks run on
+ *     the host before any device work.
+ *
+ * A context owns one HIP stream on one device.  Contexts are independent: use
+ * one per concurrent caller thread.  Calls on one context are serialised by
+ * the caller.
+ */
+#ifndef BOOSTFRACTOR_HIP_H
+#define BOOSTFRACTOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BF_OK 0
+#define BF_ERR_DIM 1         /* Julia DimensionMismatch (src/boost_fractor.jl:105) */
+#define BF_ERR_ARG 2         /* Julia ArgumentError / error(...) */
+#define BF_ERR_UNSUPPORTED 3 /* request outside the engine's limits (grid > 1024, n > 128, ...) */
+#define BF_ERR_HIP 4         /* HIP runtime failure */
+#define BF_ERR_NOMEM 5       /* device allocation failed */
+
+/* the reference's propagator plug-ins (src/boost_fractor.jl:137,157,187,218) */
+#define BF_PROP_PROPAGATOR 0 /* propagator: mask, FFT, kz phase, IFFT, tilt, surface */
+#define BF_PROP_NOTILTS 1    /* propagatorNoTilts */
+#define BF_PROP_1D 2         /* propagator1D: scalar phase only */
+#define BF_PROP_MOMENTUM 3   /* propagatorMomentumSpace: field held in (fftshifted) k-space */
+
+/* arithmetic of the hot path */
+#define BF_PREC_FP64 0  /* everything in FP64 */
+#define BF_PREC_MIXED 1 /* FP32 fields/FFTs/overlaps, FP64 phase arguments, outputs and chains */
+
+#define BF_MAX_GRID 1024
+#define BF_MAX_MODES 128
+
+typedef struct bf_ctx bf_ctx;
+
+/* CoordinateSystem (src/boost_fractor.jl:21-28).  kX/kY are passed in, not
+ * recomputed, so that the engine uses the exact doubles the caller's grid holds. */
+typedef struct bf_coords {
+    int32_t nx, ny;
+    const double *X, *Y;   /* [nx], [ny] */
+    const double *kX, *kY; /* [nx], [ny] */
+} bf_coords;
+
+/* SetupBoundaries (src/boost_fractor.jl:63-71), mirror region first. */
+typedef struct bf_setup {
+    int32_t nreg;                  /* number of regions = length(distance) */
+    const double *distance;        /* [nreg] */
+    const double *r;               /* complex [nreg-1] boundary reflectivities */
+    const double *eps;             /* complex [nreg]; real values carry imag -0.0 */
+    const double *tilt_x, *tilt_y; /* [nreg] */
+    const double *surfaces;        /* complex [nreg][ny][nx], or NULL for flat */
+} bf_setup;
+
+/* Modes (src/boost_fractor_transformer.jl:31-38) */
+typedef struct bf_modes {
+    int32_t n;              /* number of modes M*(2L+1) */
+    int32_t ncomp;          /* field components per pattern (1: scalar field) */
+    const double *patterns; /* complex [n][ncomp][ny][nx] */
+    const double *mode_kt;  /* complex [n] transverse wavenumbers */
+} bf_modes;
+
+/* ---- context ----------------------------------------------------------- */
+int bf_version(void);
+const char *bf_last_error(const bf_ctx *ctx);
+int bf_device_count(int32_t *count);
+int bf_ctx_create(int32_t device, bf_ctx **out);
+int bf_ctx_destroy(bf_ctx *ctx);
+/* HIP stream of the context (hipStream_t), for callers that time or order work on it */
+int bf_ctx_stream(bf_ctx *ctx, void **stream);
+int bf_ctx_synchronize(bf_ctx *ctx);
+
+/* ---- propagators --------------------------------------------------------
+ * replaces prop(E0, dz, diskR, eps, tilt_x, tilt_y, surface, lambda, coords)
+ *   propagator              src/boost_fractor.jl:137-149
+ *   propagatorNoTilts       src/boost_fractor.jl:157-179
+ *   propagatorMomentumSpace src/boost_fractor.jl:187-210
+ *   propagator1D            src/boost_fractor.jl:218-227
+ * E: complex [ny][nx], replaced by the propagated field.  surface: complex
+ * [ny][nx] or NULL (flat).  eps is complex; propagator (kind 0) casts a real eps
+ * to re-0.0im as the reference does, the other kinds use it as given. */
+int bf_propagate(bf_ctx *ctx, int32_t prop_kind, const bf_coords *coords, double *E, double dz, double diskR,
+                 double eps_re, double eps_im, double tilt_x, double tilt_y, const double *surface, double lambda);
+
+/* ---- recursive Fourier propagation ----------------------------------- */
+/* dance_intro (src/boost_fractor_recurprop.jl:153-201), default B field, zero
+ * axion velocity.  nd = length(bdry.distance), eps complex [nd], r1 = bdry.r[1].
+ * out: complex [nd][2][ny][nx]. */
+int bf_dance_intro(bf_ctx *ctx, const bf_coords *coords, int32_t nd, const double *eps, double r1_re,
+                   double r1_im, double diskR, double *out);
+
+/* dancer (src/boost_fractor_recurprop.jl:25-146) for nfreq frequencies of one
+ * setup.  setup is the untrimmed SetupBoundaries (mirror region first).
+ * emit: complex [nreg-1][2][ny][nx] or NULL (axion source from dance_intro);
+ * reflect: complex [ny][nx] or NULL.  Output Eout:
+ *   returnsum != 0           -> [nfreq][ny][nx]
+ *   returnsum == 0           -> [nfreq][S][ny][nx], S = immediatesum ? 1 : nmax+1
+ * power (optional, [nfreq]): sum |Eout|^2 over the grid of the summed output.
+ * bounces (optional, int32 [nfreq]): iterations executed (nmax+1 unless amin stops early). */
+int bf_dancer(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, double amin, int32_t nmax,
+              const double *freqs, int32_t nfreq, int32_t prop_kind, const double *emit, const double *reflect,
+              double diskR, int32_t returnsum, int32_t immediatesum, int32_t precision, double *Eout,
+              double *power, int32_t *bounces);
+
+/* cheerleader (src/boost_fractor_recurprop.jl:226-363) for nfreq frequencies.
+ * emit: complex [nreg][2][ny][nx] or NULL; reflect: complex [ny][nx] or NULL.
+ * out_right: complex [nfreq][ny][nx] (fields leaving on the right);
+ * out_left: complex [nfreq][ny][nx] or NULL (returnboth=true). */
+int bf_cheerleader(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, double amin, int32_t nmax,
+                   const double *freqs, int32_t nfreq, int32_t prop_kind, const double *emit,
+                   const double *reflect, double diskR, int32_t precision, double *out_right, double *out_left);
+
+/* Resident dancer sweep: device buffers and phase tables are built once; run
+ * executes all frequencies with inputs already in HBM; fetch copies results.
+ * Same semantics as bf_dancer with emit = reflect = NULL, returnsum = immediatesum = 1. */
+typedef struct bf_dancer_plan bf_dancer_plan;
+int bf_dancer_plan_create(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, double amin, int32_t nmax,
+                          const double *freqs, int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                          int32_t freq_batch, bf_dancer_plan **out);
+int bf_dancer_plan_run(bf_dancer_plan *plan);
+int bf_dancer_plan_fetch(bf_dancer_plan *plan, double *Eout, double *power);
+/* Per-kernel timing over the next runs (HIP events on the plan's stream):
+ * enable != 0 starts collecting; query returns accumulated milliseconds and
+ * launch counts for the row pass (index 0) and the column pass (index 1). */
+int bf_dancer_plan_timing(bf_dancer_plan *plan, int32_t enable);
+int bf_dancer_plan_kernel_times(bf_dancer_plan *plan, double *ms, int64_t *launches);
+int bf_dancer_plan_destroy(bf_dancer_plan *plan);
+
+/* ---- mode matching ----------------------------------------------------- */
+/* SeedModes Bessel patterns (src/boost_fractor_transformer.jl:46-107, mode :88):
+ * J_l(j_{|l|,m} r / R) exp(-i l phi), zero for r > R, normalised to sum |.|^2 = 1.
+ * patterns: complex [M(2L+1)][ny][nx]; mode_kt: complex [M(2L+1)]. */
+int bf_seed_modes(bf_ctx *ctx, const bf_coords *coords, int32_t M, int32_t L, double diskR, double *patterns,
+                  double *mode_kt);
+
+/* axion_induced_modes / field2modes (src/boost_fractor_transformer.jl:115-156):
+ * B: complex [ncomp][ny][nx] or NULL (uniform); masked to r <= diskR,
+ * normalised, projected on the modes.  coeffs: complex [n]. */
+int bf_field2modes(bf_ctx *ctx, const bf_coords *coords, const bf_modes *modes, const double *B,
+                   double velocity_x, double diskR, double f, double *coeffs);
+
+/* modes2field (src/boost_fractor_transformer.jl:161-167); field complex [ncomp][ny][nx] */
+int bf_modes2field(bf_ctx *ctx, const bf_coords *coords, const bf_modes *modes, const double *coeffs,
+                   double *field);
+
+/* calc_propagation_matrices (src/boost_fractor_transformer.jl:274-280) -> propagation_matrix
+ * (:174-227) for every region of the setup at each frequency.
+ * out: complex [nfreq][nreg][n][n] (column-major n x n). */
+int bf_propagation_matrices(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, const bf_modes *modes,
+                            const double *freqs, int32_t nfreq, int32_t prop_kind, double diskR,
+                            int32_t precision, double *out);
+
+/* transformer (src/boost_fractor_transformer.jl:285-359) at nfreq frequencies.
+ * NaN permittivities are treated as 1e30 (the reference mutates bdry.eps so).
+ * prop_mats: complex [nfreq][nreg][n][n] or NULL (computed on the device).
+ * emit: complex [n]; reflect: complex [n] or NULL.
+ * boost: complex [nfreq][n]; refl: complex [nfreq][n] (only if reflect != NULL). */
+int bf_transformer(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, const bf_modes *modes,
+                   const double *freqs, int32_t nfreq, int32_t prop_kind, const double *prop_mats, double diskR,
+                   const double *emit, const double *reflect, int32_t precision, double *boost, double *refl);
+
+/* transformer_trace_back (src/boost_fractor_transformer.jl:372-421) at one frequency.
+ * reflected, input: complex [n]; emit: complex [n] (used when includes_axion);
+ * prop_mats: complex [nreg][n][n] or NULL.  fields: complex [nreg][2][n]. */
+int bf_transformer_trace_back(bf_ctx *ctx, const double *reflected, const double *input, const bf_setup *setup,
+                              const bf_coords *coords, const bf_modes *modes, double f, int32_t prop_kind,
+                              const double *prop_mats, double diskR, int32_t includes_axion, const double *emit,
+                              double *fields);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BOOSTFRACTOR_HIP_H */

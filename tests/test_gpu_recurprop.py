@@ -1,0 +1,227 @@
+"""GPU parity of the propagators and the recursive Fourier solvers.
+
+FP64 path: results must match the CPU oracle (and the reference goldens) to
+1e-10 relative (max |diff| / max |oracle| for fields, per-point relative for
+boost factors).  Mixed precision: relative beta^2 error <= 1e-4 against FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import BoostFractor as B
+from BoostFractor import workloads as W
+from helpers import (DISKR, GOLDEN, O, deps_distance, ocoords, osetup, pmap, ref_coords, ref_dx,
+                     ref_setup_default, ref_setup_eps9, rel, with_distance)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+KINDS = [(B.propagator, O.PROP_PROPAGATOR), (B.propagatorNoTilts, O.PROP_NOTILTS),
+         (B.propagator1D, O.PROP_1D), (B.propagatorMomentumSpace, O.PROP_MOMENTUM)]
+
+
+def _field(coords, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((coords.ny, coords.nx)) + 1j * rng.standard_normal((coords.ny, coords.nx))
+
+
+def _grid(nx, ny):
+    X = np.array([((j - nx // 2) / nx) for j in range(nx)]) if nx > 1 else np.array([1e-7])
+    Y = np.array([((j - ny // 2) / ny) for j in range(ny)]) if ny > 1 else np.array([1e-7])
+    return B.CoordinateSystem(X, Y, B.get_kspace_coords(X), B.get_kspace_coords(Y))
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (101, 101), (128, 32), (100, 64), (1, 1)])
+@pytest.mark.parametrize("kind", range(4))
+def test_propagators_match_oracle(nx, ny, kind):
+    coords = _grid(nx, ny) if (nx, ny) != (101, 101) else ref_coords()
+    oc = ocoords(coords)
+    prop, okind = KINDS[kind]
+    lam = B.wavelength(20e9)
+    rng = np.random.default_rng(7)
+    surf = 1e-5 * (rng.standard_normal((coords.ny, coords.nx)) + 0.1j * rng.standard_normal((coords.ny, coords.nx)))
+    cases = [(complex(1.0, -0.0), 8e-3, 0.0, 0.0, None), (complex(24.0, -0.0), 1e-3, 2e-4, -1e-4, surf),
+             (complex(1.5, -0.02), 5e-3, 1e-4, 3e-4, surf), (complex(1.0, -0.0), 0.0, 0.0, 0.0, None)]
+    for i, (eps, dz, tx, ty, s) in enumerate(cases):
+        E0 = _field(coords, 11 + i)
+        got = prop(E0, dz, 0.3, eps, tx, ty, s, lam, coords)
+        want = O.propagate(okind, E0, oc, dz, 0.3, eps, tx, ty, s, lam)
+        assert rel(got, want) < TOL, (nx, ny, kind, i, rel(got, want))
+
+
+def test_propagator_dz0_is_mask():
+    """dz = 0, no tilt: the FFT round trip leaves the aperture-masked field."""
+    coords = W.grid_coords(128)
+    E0 = _field(coords, 3)
+    got = B.propagator(E0, 0.0, 0.2, complex(1.0, -0.0), 0.0, 0.0, None, 0.015, coords)
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    assert np.max(np.abs(got - np.where(XX ** 2 + YY ** 2 < 0.04, E0, 0))) < 1e-13
+
+
+def test_dancer_1d_golden():
+    """test/runtests.jl:214 (dancer with propagator1D, nmax=200)."""
+    coords = ref_coords()
+    sb = ref_setup_eps9(coords)
+    dx = ref_dx(coords)
+    got = []
+    for d in np.arange(0, 3 * math.pi, 1.0):
+        E = B.dancer(0, 200, with_distance(sb, deps_distance(d)), coords, prop=B.propagator1D, f=10e9, diskR=DISKR)
+        got.append(np.sum(np.abs(E) ** 2) * dx * dx / (math.pi * DISKR ** 2))
+    g = np.array(GOLDEN["dancer_1d_deps_0_1_3pi"]["beta2"])
+    assert np.max(np.abs(np.array(got) / g - 1)) < TOL
+
+
+def test_dancer_3d_golden():
+    """test/runtests.jl:215 (3D dancer, nmax=200, Deps = 0:0.1:3pi), a spread of points."""
+    coords = ref_coords()
+    sb = ref_setup_eps9(coords)
+    dx = ref_dx(coords)
+    Deps = np.arange(0, 3 * math.pi, 0.1)
+    g = np.array(GOLDEN["dancer_3d_deps_0_0.1_3pi"]["beta2"])
+    for i in [0, 17, 47, 80, 94]:
+        E = B.dancer(0, 200, with_distance(sb, deps_distance(Deps[i])), coords, f=10e9, diskR=DISKR)
+        b = np.sum(np.abs(E) ** 2) * dx * dx / (math.pi * DISKR ** 2)
+        assert abs(b / g[i] - 1) < TOL, (i, b, g[i])
+
+
+def test_cheerleader_golden():
+    """test/runtests.jl:211 (3D cheerleader, nmax=100), all 10 points."""
+    coords = ref_coords()
+    sb = ref_setup_default(coords)
+    dx = ref_dx(coords)
+    got = []
+    for d in np.arange(0, 3 * math.pi, 1.0):
+        E = B.cheerleader(0, 100, with_distance(sb, deps_distance(d)), coords, prop=B.propagator, f=10e9,
+                          diskR=DISKR)
+        got.append(np.sum(np.abs(E) ** 2) * dx * dx / (math.pi * DISKR ** 2))
+    g = np.array(GOLDEN["cheerleader_3d_deps_0_1_3pi"]["beta2"])
+    assert np.max(np.abs(np.array(got) / g - 1)) < TOL
+
+
+def _small_booster(N=64, ndisk=3, seed=1001, tilt=True):
+    coords = W.grid_coords(N)
+    sb = W.booster(coords, ndisk=ndisk, seed=seed)
+    if tilt:
+        rng = W.SplitMix64(seed + 7)
+        n = len(sb.distance)
+        sb.relative_tilt_x = np.array([2e-4 * rng.normal() for _ in range(n)])
+        sb.relative_tilt_y = np.array([2e-4 * rng.normal() for _ in range(n)])
+        surf = np.zeros((n, N, N), dtype=complex)
+        XX, YY = np.meshgrid(coords.X, coords.Y)
+        for i in range(2, n - 1, 2):
+            surf[i] = 5e-6 * np.cos(20 * XX + i) * np.sin(13 * YY)
+        sb.relative_surfaces = surf
+    return coords, sb
+
+
+@pytest.mark.parametrize("prop,okind", [(B.propagator, O.PROP_PROPAGATOR), (B.propagatorNoTilts, O.PROP_NOTILTS),
+                                        (B.propagator1D, O.PROP_1D),
+                                        (B.propagatorMomentumSpace, O.PROP_MOMENTUM)])
+def test_dancer_batched_matches_oracle(prop, okind):
+    coords, sb = _small_booster()
+    freqs = [19.7e9, 20.0e9, 20.3e9]
+    got = B.dancer(0, 25, sb, coords, f=freqs, prop=prop, diskR=DISKR)
+    os_, oc = osetup(sb), ocoords(coords)
+    want = pmap(lambda f: O.dancer(os_, oc, 0, 25, f, okind, diskR=DISKR), freqs)
+    for q in range(len(freqs)):
+        assert rel(got[q], want[q]) < TOL, (q, rel(got[q], want[q]))
+
+
+def test_dancer_history_and_reverse_sum():
+    coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
+    os_, oc = osetup(sb), ocoords(coords)
+    hist = B.dancer(0, 12, sb, coords, f=20e9, diskR=DISKR, returnsum=False, immediatesum=False)
+    want = O.dancer(os_, oc, 0, 12, 20e9, diskR=DISKR, returnsum=False, immediatesum=False)
+    assert hist.shape == (13, 32, 32)
+    assert rel(hist, want) < TOL
+    summed = B.dancer(0, 12, sb, coords, f=20e9, diskR=DISKR, returnsum=True, immediatesum=False)
+    assert rel(summed, O.dancer(os_, oc, 0, 12, 20e9, diskR=DISKR, returnsum=True, immediatesum=False)) < TOL
+    one = B.dancer(0, 12, sb, coords, f=20e9, diskR=DISKR, returnsum=False, immediatesum=True)
+    assert one.shape == (1, 32, 32)
+    assert rel(one[0], summed) < 1e-12
+
+
+def test_dancer_amin_early_stop():
+    coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
+    os_, oc = osetup(sb), ocoords(coords)
+    for amin in [0.3, 0.05, 2.0]:
+        got = B.dancer(amin, 40, sb, coords, f=20e9, diskR=DISKR)
+        want = O.dancer(os_, oc, amin, 40, 20e9, diskR=DISKR)
+        assert rel(got, want) < TOL if np.any(want) else np.all(got == 0), amin
+
+
+def test_dancer_emit_and_reflect():
+    coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
+    os_, oc = osetup(sb), ocoords(coords)
+    R = len(sb.distance) - 1
+    rng = np.random.default_rng(5)
+    emit = rng.standard_normal((R, 2, 32, 32)) + 1j * rng.standard_normal((R, 2, 32, 32))
+    got = B.dancer(0, 10, sb, coords, f=20e9, emit=emit, diskR=DISKR)
+    assert rel(got, O.dancer(os_, oc, 0, 10, 20e9, emit=emit, diskR=DISKR)) < TOL
+    refl = rng.standard_normal((32, 32)) + 0j
+    got = B.dancer(0, 10, sb, coords, f=20e9, reflect=refl, emit=emit, diskR=DISKR)
+    assert rel(got, O.dancer(os_, oc, 0, 10, 20e9, reflect=refl, diskR=DISKR)) < TOL
+
+
+@pytest.mark.parametrize("prop,okind", [(B.propagator, O.PROP_PROPAGATOR), (B.propagator1D, O.PROP_1D),
+                                        (B.propagatorMomentumSpace, O.PROP_MOMENTUM)])
+def test_cheerleader_matches_oracle(prop, okind):
+    coords, sb = _small_booster(ndisk=2)
+    freqs = [19.9e9, 20.2e9]
+    left, right = B.cheerleader(0, 15, sb, coords, f=freqs, prop=prop, diskR=DISKR, returnboth=True)
+    os_, oc = osetup(sb), ocoords(coords)
+    for q, f in enumerate(freqs):
+        wl, wr = O.cheerleader(os_, oc, 0, 15, f, okind, diskR=DISKR, returnboth=True)
+        assert rel(right[q], wr) < TOL and rel(left[q], wl) < TOL, q
+
+
+def test_cheerleader_amin_reflect():
+    coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
+    os_, oc = osetup(sb), ocoords(coords)
+    got = B.cheerleader(0.05, 40, sb, coords, f=20e9, diskR=DISKR)
+    assert rel(got, O.cheerleader(os_, oc, 0.05, 40, 20e9, diskR=DISKR)) < TOL
+    refl = np.ones((32, 32), dtype=complex)
+    got = B.cheerleader(0, 10, sb, coords, f=20e9, reflect=refl, diskR=DISKR)
+    assert rel(got, O.cheerleader(os_, oc, 0, 10, 20e9, reflect=refl, diskR=DISKR)) < TOL
+
+
+def test_dancer_mixed_precision_beta2():
+    """20-disk booster, 128^2, FP32 fields/FFT: beta^2 within 1e-4 of the FP64 engine."""
+    coords = W.grid_coords(128)
+    sb = W.booster(coords, ndisk=20, seed=1000)
+    freqs = np.linspace(19.5e9, 20.5e9, 6)
+    e64 = B.dancer(0, 300, sb, coords, f=freqs, diskR=DISKR)
+    e32 = B.dancer(0, 300, sb, coords, f=freqs, diskR=DISKR, precision="mixed")
+    b64 = W.beta2_dancer(e64, coords, DISKR)
+    b32 = W.beta2_dancer(e32, coords, DISKR)
+    assert np.max(np.abs(b32 / b64 - 1)) < 1e-4, np.abs(b32 / b64 - 1)
+
+
+def test_dance_intro_matches_oracle():
+    coords = ref_coords()
+    sb = ref_setup_eps9(coords)
+    # untrimmed setup: the NaN mirror permittivity propagates exactly as in the reference
+    got = B.dance_intro(sb, coords, diskR=DISKR)
+    want = O.dance_intro(sb.eps, sb.r[0], ocoords(coords), DISKR)
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=0, equal_nan=True)
+    # the trimmed view dancer() uses (src/boost_fractor_recurprop.jl:28-34)
+    t = sb.copy()
+    t.distance, t.eps = sb.distance[1:], sb.eps[1:]
+    got = B.dance_intro(t, coords, diskR=DISKR)
+    want = O.dance_intro(sb.eps[1:], sb.r[0], ocoords(coords), DISKR)
+    assert np.all(np.isfinite(got)) and rel(got, want) < 1e-14
+
+
+def test_dancer_plan_matches_one_shot():
+    coords = W.grid_coords(64)
+    sb = W.booster(coords, ndisk=4, seed=1003)
+    freqs = np.linspace(19.5e9, 20.5e9, 5)
+    E1 = B.dancer(0, 30, sb, coords, f=freqs, diskR=DISKR)
+    plan = B.DancerPlan(0, 30, sb, coords, freqs, diskR=DISKR, freq_batch=2)
+    plan.run()
+    plan.run()  # idempotent: every run restarts from the axion source
+    E2, power = plan.fetch()
+    assert rel(E2, E1) < 1e-13
+    assert np.allclose(power, np.sum(np.abs(E1) ** 2, axis=(1, 2)), rtol=1e-12)
+    plan.close()
