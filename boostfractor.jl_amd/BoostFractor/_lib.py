@@ -71,6 +71,7 @@ _SIGS = {
     "bf_dancer_plan_timing": (ctypes.c_int, [_P, _I32]),
     "bf_dancer_plan_kernel_times": (ctypes.c_int, [_P, _P, _P]),
     "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
+    "bf_stream_copy_probe": (ctypes.c_int, [_P, ctypes.c_int64, _I32, ctypes.POINTER(_D)]),
     "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),
     "bf_field2modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), ctypes.POINTER(ModesS), _P, _D, _D, _D, _P]),
     "bf_modes2field": (ctypes.c_int, [_P, ctypes.POINTER(Coords), ctypes.POINTER(ModesS), _P, _P]),
@@ -151,6 +152,12 @@ class Context:
 
     def synchronize(self):
         check(lib().bf_ctx_synchronize(self.handle), self.handle)
+
+    def stream_copy_probe(self, nbytes, reps=20):
+        """mean ms per device-to-device copy of nbytes with the engine's copy kernel"""
+        ms = _D()
+        check(lib().bf_stream_copy_probe(self.handle, int(nbytes), int(reps), ctypes.byref(ms)), self.handle)
+        return ms.value
 
 
 def context():

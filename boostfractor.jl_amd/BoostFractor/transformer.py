@@ -1,0 +1,269 @@
+"""Mode-matching transformer (src/boost_fractor_transformer.jl), on the GPU.
+
+Array conventions (Julia column-major -> numpy C order, spatial axes last):
+  mode_patterns (M, 2L+1, Nx, Ny, C)  ->  (M, 2L+1, C, ny, nx)
+  mode_kt       (M, 2L+1)             ->  (M, 2L+1)
+  modes2field   (Nx, Ny, C)           ->  (C, ny, nx)
+  fields_regions (Nreg, 2, n)         ->  (Nreg, 2, n)   (same element indexing)
+Mode vectors use the flat index (m-1)(2L+1) + l + L (m = 1..M, l = -L..L).
+
+`f` may be a scalar (reference signature) or a sequence: the frequencies then
+run batched on the device and results gain a leading frequency axis.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .propagators import precision_code, prop_kind, propagator
+from .setup import SPEED_OF_LIGHT, SetupBoundaries, wavelength
+
+
+class ErrorException(RuntimeError):
+    """Julia's error(...)"""
+
+
+class Modes:
+    """src/boost_fractor_transformer.jl:31-38"""
+
+    def __init__(self, M, L_, mode_patterns, mode_kt, id_=None, zeromatrix=None):
+        self.M = int(M)
+        self.L = int(L_)
+        self.mode_patterns = np.asarray(mode_patterns, dtype=np.complex128)
+        self.mode_kt = np.asarray(mode_kt, dtype=np.complex128)
+        n = self.M * (2 * self.L + 1)
+        self.id = np.eye(n, dtype=np.complex128) if id_ is None else id_
+        self.zeromatrix = np.zeros((n, n), dtype=np.complex128) if zeromatrix is None else zeromatrix
+
+    @property
+    def n(self):
+        return self.M * (2 * self.L + 1)
+
+    @property
+    def ncomp(self):
+        return self.mode_patterns.shape[2]
+
+    def patterns_flat(self):
+        """[n][C][ny][nx] with n = (m-1)(2L+1) + l + L"""
+        p = self.mode_patterns
+        return p.reshape((self.n,) + p.shape[2:])
+
+
+def e_field_dimensions(modes):
+    """src/boost_fractor_transformer.jl:23: 1 (scalar field) or 3."""
+    return modes.mode_patterns.shape[2]
+
+
+def index(modes, k):
+    """src/boost_fractor_transformer.jl:76-78 (0-based slice of sub-block k = 1, 2)."""
+    n = modes.n
+    return slice((k - 1) * n, k * n)
+
+
+def _seed_bessel(coords, M, L_, diskR):
+    n = M * (2 * L_ + 1)
+    pat = np.zeros((n, coords.ny, coords.nx), dtype=np.complex128)
+    kt = np.zeros(n, dtype=np.complex128)
+    m = L.Marshalled()
+    c = m.coords(coords)
+    ctx = L.context()
+    L.check(L.lib().bf_seed_modes(ctx.handle, ctypes.byref(c), int(M), int(L_), float(diskR), L.cptr(pat),
+                                  L.cptr(kt)), ctx.handle)
+    return pat, kt
+
+
+def SeedModes(coords, ThreeDim=False, Mmax=1, Lmax=0, diskR=0.15, pattern_input=None, kT_arr=None):
+    """src/boost_fractor_transformer.jl:46-69"""
+    if ThreeDim:
+        M, Lm = int(Mmax), int(Lmax)
+        if pattern_input is None:
+            pat, kt = _seed_bessel(coords, M, Lm, diskR)
+            patterns = pat.reshape(M, 2 * Lm + 1, 1, coords.ny, coords.nx)
+            mode_kt = kt.reshape(M, 2 * Lm + 1)
+        else:
+            if kT_arr is None:
+                raise ErrorException("you have to define kTs as well when putting in an input pattern")
+            pi = np.asarray(pattern_input, dtype=np.complex128)
+            patterns = np.ascontiguousarray(pi[:M, :2 * Lm + 1])
+            mode_kt = np.asarray(kT_arr, dtype=np.complex128)[:M, :2 * Lm + 1].copy()
+    else:
+        M, Lm = 1, 0
+        patterns = np.ones((1, 1, 1, coords.ny, coords.nx), dtype=np.complex128)
+        mode_kt = np.zeros((1, 1), dtype=np.complex128)
+    return Modes(M, Lm, patterns, mode_kt)
+
+
+def mode(m, l, L_, coords, diskR=0.15, k0=2 * np.pi / 0.03, pattern_input=None, kT_arr=None):
+    """src/boost_fractor_transformer.jl:88-107 -> (kt, pattern (C, ny, nx))."""
+    if pattern_input is None:
+        pat, kt = _seed_bessel(coords, m, abs(l), diskR)
+        idx = (m - 1) * (2 * abs(l) + 1) + l + abs(l)
+        return kt[idx], pat[idx][None]
+    if kT_arr is None:
+        raise ErrorException("you have to define kTs as well when putting in an input pattern")
+    return complex(np.asarray(kT_arr)[m - 1, l + L_]), np.asarray(pattern_input)[m - 1, l + L_]
+
+
+def _modes_struct(m, modes):
+    return m.modes(modes)
+
+
+def axion_induced_modes(coords, modes, B=None, velocity_x=0, diskR=0.15, f=20e9):
+    """src/boost_fractor_transformer.jl:115-151 -> mode vector (n,)."""
+    C = modes.ncomp
+    if B is None and C == 3:
+        B = np.zeros((3, coords.ny, coords.nx), dtype=np.complex128)
+        B[1] = 1.0
+    m = L.Marshalled()
+    c = m.coords(coords)
+    md = _modes_struct(m, modes)
+    b = None
+    if B is not None:
+        b = np.asarray(B, dtype=np.complex128)
+        if b.ndim == 2:
+            b = np.broadcast_to(b, (C,) + b.shape)
+        b = m.hold(L.as_c128(b))
+        if b.shape != (C, coords.ny, coords.nx):
+            raise L.DimensionMismatch(f"B has shape {b.shape}, expected {(C, coords.ny, coords.nx)}")
+    out = np.zeros(modes.n, dtype=np.complex128)
+    ctx = L.context()
+    L.check(L.lib().bf_field2modes(ctx.handle, ctypes.byref(c), ctypes.byref(md), L.cptr(b), float(velocity_x),
+                                   float(diskR), float(f), L.cptr(out)), ctx.handle)
+    return out
+
+
+def field2modes(pattern, coords, modes, diskR=0.15):
+    """src/boost_fractor_transformer.jl:156 (the caller's array is not modified)."""
+    return axion_induced_modes(coords, modes, B=pattern, diskR=diskR)
+
+
+def modes2field(mode_coeffs, coords, modes):
+    """src/boost_fractor_transformer.jl:161-167 -> field (C, ny, nx)."""
+    m = L.Marshalled()
+    c = m.coords(coords)
+    md = _modes_struct(m, modes)
+    coeffs = m.hold(L.as_c128(np.asarray(mode_coeffs).reshape(-1)))
+    if len(coeffs) != modes.n:
+        raise L.DimensionMismatch(f"{len(coeffs)} coefficients for {modes.n} modes")
+    out = np.zeros((modes.ncomp, coords.ny, coords.nx), dtype=np.complex128)
+    ctx = L.context()
+    L.check(L.lib().bf_modes2field(ctx.handle, ctypes.byref(c), ctypes.byref(md), L.cptr(coeffs), L.cptr(out)),
+            ctx.handle)
+    return out
+
+
+def _fortran_stack(mats, n):
+    return np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.complex128).reshape(n, n).T for x in mats]))
+
+
+def _run_matrices(bdry, coords, modes, freqs, prop, diskR, precision):
+    m = L.Marshalled()
+    c = m.coords(coords)
+    s = m.setup(bdry, coords.ny, coords.nx)
+    md = _modes_struct(m, modes)
+    fr = m.hold(L.as_f64(freqs))
+    n, nreg = modes.n, len(bdry.distance)
+    out = np.zeros((len(fr), nreg, n, n), dtype=np.complex128)  # column-major matrices
+    ctx = L.context()
+    L.check(L.lib().bf_propagation_matrices(ctx.handle, ctypes.byref(s), ctypes.byref(c), ctypes.byref(md),
+                                            L.cptr(fr), len(fr), prop_kind(prop), float(diskR),
+                                            precision_code(precision), L.cptr(out)), ctx.handle)
+    return np.swapaxes(out, -1, -2)  # -> row-major indexing P[f][reg][i, j]
+
+
+def propagation_matrix(dz, diskR, eps, tilt_x, tilt_y, surface, lambda_, coords, modes, is_air=None,
+                       onlydiagonal=False, prop=propagator):
+    """src/boost_fractor_transformer.jl:174-227 (one region)."""
+    eps = complex(eps)
+    if is_air is not None and bool(is_air) != (eps.real == 1.0):
+        raise L.UnsupportedError("is_air must follow real(eps) == 1 on the device engine")
+    surf = np.zeros((1, coords.ny, coords.nx), dtype=np.complex128)
+    if surface is not None:
+        surf[0] = np.broadcast_to(np.asarray(surface), (coords.ny, coords.nx))
+    b = SetupBoundaries([dz], np.zeros(0), [eps], [tilt_x], [tilt_y], surf)
+    P = _run_matrices(b, coords, modes, [SPEED_OF_LIGHT / float(lambda_)], prop, diskR, "fp64")[0, 0]
+    if onlydiagonal:
+        P = np.diag(np.diag(P))
+    return P
+
+
+def calc_propagation_matrices(bdry, coords, modes, f=10.0e9, prop=propagator, diskR=0.15, precision="fp64"):
+    """src/boost_fractor_transformer.jl:274-280: list of Nreg n x n matrices (list of lists for several f)."""
+    freqs = np.atleast_1d(np.asarray(f, dtype=np.float64))
+    P = _run_matrices(bdry, coords, modes, freqs, prop, diskR, precision)
+    res = [[P[q, r] for r in range(P.shape[1])] for q in range(P.shape[0])]
+    return res[0] if np.ndim(f) == 0 else res
+
+
+def _nan_to_1e30(bdry):
+    bad = np.isnan(bdry.eps.real) | np.isnan(bdry.eps.imag)
+    bdry.eps[bad] = 1e30  # the reference mutates its argument (src/boost_fractor_transformer.jl:289)
+
+
+def transformer(bdry, coords, modes, f=10.0e9, velocity_x=0, prop=propagator, propagation_matrices=None, diskR=0.15,
+                emit=None, reflect=None, precision="fp64"):
+    """src/boost_fractor_transformer.jl:285-359 -> boost (n,) or (boost, refl); batched over f."""
+    _nan_to_1e30(bdry)
+    freqs = np.atleast_1d(np.asarray(f, dtype=np.float64))
+    scalar = np.ndim(f) == 0
+    n, nreg = modes.n, len(bdry.distance)
+    if emit is None and velocity_x != 0 and len(freqs) > 1:
+        # the default emission depends on f through the velocity phase: one call per frequency
+        res = [transformer(bdry, coords, modes, f=float(q), velocity_x=velocity_x, prop=prop,
+                           propagation_matrices=(propagation_matrices[i] if propagation_matrices else None),
+                           diskR=diskR, reflect=reflect, precision=precision) for i, q in enumerate(freqs)]
+        if reflect is None:
+            return np.stack(res)
+        return np.stack([r[0] for r in res]), np.stack([r[1] for r in res])
+    if emit is None:
+        emit = axion_induced_modes(coords, modes, B=None, velocity_x=velocity_x, diskR=diskR, f=float(freqs[0]))
+    m = L.Marshalled()
+    c = m.coords(coords)
+    s = m.setup(bdry, coords.ny, coords.nx)
+    md = _modes_struct(m, modes)
+    fr = m.hold(L.as_f64(freqs))
+    em = m.hold(L.as_c128(np.asarray(emit).reshape(-1)))
+    rf = m.hold(L.as_c128(np.broadcast_to(reflect, (n,)))) if reflect is not None else None
+    pm = None
+    if propagation_matrices is not None and len(propagation_matrices) > 0:
+        mats = propagation_matrices
+        if scalar or not isinstance(mats[0], (list, tuple)):
+            mats = [mats] * len(fr)
+        pm = m.hold(np.ascontiguousarray(np.stack([_fortran_stack(x, n) for x in mats])))
+        if pm.shape != (len(fr), nreg, n, n):
+            raise L.DimensionMismatch(f"propagation_matrices has shape {pm.shape}, expected {(len(fr), nreg, n, n)}")
+    boost = np.zeros((len(fr), n), dtype=np.complex128)
+    refl = np.zeros((len(fr), n), dtype=np.complex128) if reflect is not None else None
+    ctx = L.context()
+    L.check(L.lib().bf_transformer(ctx.handle, ctypes.byref(s), ctypes.byref(c), ctypes.byref(md), L.cptr(fr),
+                                   len(fr), prop_kind(prop), L.cptr(pm), float(diskR), L.cptr(em), L.cptr(rf),
+                                   precision_code(precision), L.cptr(boost), L.cptr(refl)), ctx.handle)
+    if scalar:
+        boost = boost[0]
+        refl = refl[0] if refl is not None else None
+    return boost if reflect is None else (boost, refl)
+
+
+def transformer_trace_back(reflected_beam, input_beam, bdry, coords, modes, inlcudes_axion=False, f=10.0e9,
+                           velocity_x=0, prop=propagator, propagation_matrices=None, diskR=0.15, emit=None):
+    """src/boost_fractor_transformer.jl:372-421 -> fields_regions (Nreg, 2, n)."""
+    n, nreg = modes.n, len(bdry.distance)
+    if emit is None:
+        emit = axion_induced_modes(coords, modes, B=np.ones((coords.ny, coords.nx)), diskR=diskR)
+    m = L.Marshalled()
+    c = m.coords(coords)
+    s = m.setup(bdry, coords.ny, coords.nx)
+    md = _modes_struct(m, modes)
+    refl = m.hold(L.as_c128(np.broadcast_to(reflected_beam, (n,))))
+    inp = m.hold(L.as_c128(np.broadcast_to(input_beam, (n,))))
+    em = m.hold(L.as_c128(np.asarray(emit).reshape(-1)))
+    pm = None
+    if propagation_matrices is not None:
+        pm = m.hold(_fortran_stack(propagation_matrices, n))
+    out = np.zeros((nreg, 2, n), dtype=np.complex128)
+    ctx = L.context()
+    L.check(L.lib().bf_transformer_trace_back(ctx.handle, L.cptr(refl), L.cptr(inp), ctypes.byref(s),
+                                              ctypes.byref(c), ctypes.byref(md), float(f), prop_kind(prop),
+                                              L.cptr(pm), float(diskR), int(bool(inlcudes_axion)), L.cptr(em),
+                                              L.cptr(out)), ctx.handle)
+    return out

@@ -286,6 +286,12 @@ int bf_dancer_plan_destroy(bf_dancer_plan* plan) {
     return BF_OK;
 }
 
+int bf_stream_copy_probe(bf_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy) {
+    BF_REQUIRE(ctx, bytes >= 16 && reps >= 1 && ms_per_copy, BF_ERR_ARG, "bytes >= 16, reps >= 1 and an output");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { *ms_per_copy = bf::stream_copy_probe(ctx, bytes, reps); });
+}
+
 int bf_seed_modes(bf_ctx* ctx, const bf_coords* coords, int32_t M, int32_t L, double diskR, double* patterns,
                   double* mode_kt) {
     TRY(check_coords(ctx, coords));
@@ -361,6 +367,9 @@ int bf_transformer_trace_back(bf_ctx* ctx, const double* reflected, const double
     TRY(check_freqs(ctx, &f, 1));
     BF_REQUIRE(ctx, reflected && input && fields, BF_ERR_ARG, "reflected, input and fields must not be NULL");
     BF_REQUIRE(ctx, !includes_axion || emit, BF_ERR_ARG, "emit is required when includes_axion is set");
+    // the reference adds a 2 x n matrix to the 2n solution vector: only n == 1 conforms
+    BF_REQUIRE(ctx, !includes_axion || modes->n == 1, BF_ERR_DIM,
+               "includes_axion needs a single mode (2n vector + 2 x n matrix, src/boost_fractor_transformer.jl:411-413)");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
         bf::trace_back_dev(ctx, reflected, input, setup, coords, modes, f, prop_kind, prop_mats, diskR, includes_axion,

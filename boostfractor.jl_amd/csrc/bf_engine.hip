@@ -27,27 +27,6 @@ namespace bf {
 // ---------------------------------------------------------------------------
 // descriptors
 // ---------------------------------------------------------------------------
-struct FieldDesc {
-    int64_t off;  // element offset of the field in its buffer
-    int ktab;     // k-space table index (-1: none)
-    int rtab;     // real-space (tilt*surface) table index (-1: none)
-    int group;    // activity group (frequency slot), index into `active`
-    int pad;
-};
-
-struct KTabDesc {
-    zd k0sq;      // k0*k0 (Julia complex product)
-    double dz;
-    int shifted;  // 1: table laid out in fftshifted order (momentum-space propagator)
-    int pad;
-};
-
-struct RTabDesc {
-    zd a;            // (-1im)*k0
-    double tx, ty;
-    int64_t surf;    // element offset into the surfaces buffer, -1: flat
-};
-
 enum : uint32_t { RF_INV = 1, RF_RTAB = 2, RF_MASK = 4, RF_FWD = 8 };
 enum : uint32_t { CF_FWD = 1, CF_KMID = 2, CF_INV = 4, CF_KLOAD = 8, CF_INSHIFT = 16, CF_OUTSHIFT = 32 };
 enum : uint32_t { DF_FWD = 1, DF_AMAX = 2, DF_EOUT_ASSIGN = 4 };
@@ -470,6 +449,11 @@ __global__ void k_power(const zd* __restrict__ E, size_t K, double* out) {
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+void allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    BF_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes) BF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
 }
@@ -538,6 +522,7 @@ struct RowLaunch {
         const int lpb = std::max(1, 256 / TPL);
         const long nlines = (long)a.nf * a.ny;
         const size_t smem = sizeof(cplx<T>) * (size_t)n * lpb * (N >= 16 ? 1 : 2);
+        allow_lds((const void*)k_rows<T, N>, smem);
         hipLaunchKernelGGL((k_rows<T, N>), dim3((unsigned)((nlines + lpb - 1) / lpb)), dim3(lpb * TPL), smem, s, a);
         BF_CHECK_LAUNCH();
     }
@@ -553,6 +538,7 @@ struct ColLaunch {
         while (tx > 1 && sizeof(cplx<T>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2) > 96 * 1024) tx >>= 1;
         a.tx = tx;
         const size_t smem = sizeof(cplx<T>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2);
+        allow_lds((const void*)k_cols<T, N>, smem);
         hipLaunchKernelGGL((k_cols<T, N>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * TPL), smem, s, a);
         BF_CHECK_LAUNCH();
     }
@@ -566,6 +552,7 @@ struct DancerRowLaunch {
         int rows = rows_req > 0 ? rows_req : std::max(1, 128 / TPL);
         rows = std::max(1, std::min(rows, 512 / TPL));
         const size_t smem = sizeof(cplx<T>) * (size_t)n * (2 * rows) * 2;
+        allow_lds((const void*)k_dancer_rows<T, N>, smem);
         hipLaunchKernelGGL((k_dancer_rows<T, N>), dim3((a.ny + rows - 1) / rows, a.R + 1, fb), dim3(2 * rows * TPL),
                            smem, s, a);
         BF_CHECK_LAUNCH();
@@ -1314,6 +1301,88 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         }
         BF_HIP(hipStreamSynchronize(s));
     }
+}
+
+// ---------------------------------------------------------------------------
+// helpers for the mode-matching unit
+// ---------------------------------------------------------------------------
+// unnormalised 2D FFT of nf contiguous fields (optionally aperture-masked first)
+void fft_fields(bf_ctx* ctx, const Grid& g, zd* base, int nf, bool mask, double diskR) {
+    PropEngine<double> pe(ctx, &g, BF_PROP_NOTILTS, diskR, nullptr);
+    std::vector<FieldDesc> fd(nf);
+    for (int j = 0; j < nf; j++) fd[j] = FieldDesc{(int64_t)j * (int64_t)g.K(), -1, -1, 0, 0};
+    DevBuf dfd(sizeof(FieldDesc) * nf);
+    upload(dfd.p, fd.data(), sizeof(FieldDesc) * nf, ctx->stream);
+    pe.rows(base, dfd.as<FieldDesc>(), nf, (mask ? RF_MASK : 0u) | RF_FWD, nullptr);
+    pe.cols(base, base, dfd.as<FieldDesc>(), nf, CF_FWD, nullptr);
+    BF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// prop(...) applied to nf contiguous fields with one region's parameters
+void propagate_fields(bf_ctx* ctx, const Grid& g, int kind, double diskR, const zd* surfaces, zd* base, int nf,
+                      zd eps, double dz, double tx, double ty, int64_t surf_off, double lambda, zd* scratch) {
+    PropEngine<double> pe(ctx, &g, kind, diskR, surfaces);
+    pe.build_tables({RegionParams{eps, dz, tx, ty, lambda, surf_off}});
+    const int rt = pe.rtab_of.empty() ? -1 : pe.rtab_of[0];
+    std::vector<FieldDesc> fd(nf);
+    for (int j = 0; j < nf; j++) fd[j] = FieldDesc{(int64_t)j * (int64_t)g.K(), 0, rt, 0, 0};
+    std::vector<zd> ph(nf, phase_1d(eps, lambda, dz));
+    DevBuf dfd(sizeof(FieldDesc) * nf), dph(sizeof(zd) * nf);
+    upload(dfd.p, fd.data(), sizeof(FieldDesc) * nf, ctx->stream);
+    upload(dph.p, ph.data(), sizeof(zd) * nf, ctx->stream);
+    pe.propagate(base, dfd.as<FieldDesc>(), nf, dph.as<zd>(), scratch, nullptr);
+    BF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void build_ktabs(bf_ctx* ctx, const Grid& g, const std::vector<KTabDesc>& d, double scale, zd* out) {
+    if (d.empty()) return;
+    DevBuf dd(sizeof(KTabDesc) * d.size());
+    upload(dd.p, d.data(), sizeof(KTabDesc) * d.size(), ctx->stream);
+    hipLaunchKernelGGL(k_build_ktab<double>, dim3(grid1d(g.K(), 256, 1024), (unsigned)d.size()), dim3(256), 0,
+                       ctx->stream, out, dd.as<KTabDesc>(), g.nx, g.ny, g.kX.as<double>(), g.kY.as<double>(), scale);
+    BF_CHECK_LAUNCH();
+    BF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void build_rtabs(bf_ctx* ctx, const Grid& g, const std::vector<RTabDesc>& d, const zd* surfaces, zd* out) {
+    if (d.empty()) return;
+    DevBuf dd(sizeof(RTabDesc) * d.size());
+    upload(dd.p, d.data(), sizeof(RTabDesc) * d.size(), ctx->stream);
+    hipLaunchKernelGGL(k_build_rtab<double>, dim3(grid1d(g.K(), 256, 1024), (unsigned)d.size()), dim3(256), 0,
+                       ctx->stream, out, dd.as<RTabDesc>(), g.nx, g.ny, g.X.as<double>(), g.Y.as<double>(), surfaces);
+    BF_CHECK_LAUNCH();
+    BF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ---------------------------------------------------------------------------
+// stream bandwidth probe
+// ---------------------------------------------------------------------------
+__global__ void k_stream_copy(double2* __restrict__ dst, const double2* __restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+double stream_copy_probe(bf_ctx* ctx, int64_t bytes, int reps) {
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)bytes / sizeof(double2);
+    DevBuf a(n * sizeof(double2)), b(n * sizeof(double2));
+    BF_HIP(hipMemsetAsync(a.p, 0, n * sizeof(double2), s));
+    hipEvent_t e0, e1;
+    BF_HIP(hipEventCreate(&e0));
+    BF_HIP(hipEventCreate(&e1));
+    const unsigned blocks = 256 * 16;
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, b.as<double2>(), a.as<double2>(), n);
+    BF_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s, b.as<double2>(), a.as<double2>(), n);
+    BF_CHECK_LAUNCH();
+    BF_HIP(hipEventRecord(e1, s));
+    BF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    BF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / std::max(reps, 1);
 }
 
 template void cheerleader_run<double>(bf_ctx*, const bf_setup*, const bf_coords*, double, int, const double*, int, int,

@@ -89,6 +89,38 @@ struct Grid {
 template <typename T>
 const cplx<T>* twiddle_table(bf_ctx* ctx, int n);
 
+// per-field descriptor of the propagation passes
+struct FieldDesc {
+    int64_t off;  // element offset of the field in its buffer
+    int ktab;     // k-space table index (-1: none)
+    int rtab;     // real-space (tilt*surface) table index (-1: none)
+    int group;    // activity group (frequency slot), index into `active`
+    int pad;
+};
+
+struct KTabDesc {
+    zd k0sq;      // k0*k0 (Julia complex product)
+    double dz;
+    int shifted;  // 1: table laid out in fftshifted order (momentum-space propagator)
+    int pad;
+};
+
+struct RTabDesc {
+    zd a;          // (-1im)*k0
+    double tx, ty;
+    int64_t surf;  // element offset into the surfaces buffer, -1: flat
+};
+
+// allow a kernel more than the default dynamic LDS (gfx950 has 160 KB per CU)
+void allow_lds(const void* kernel, size_t bytes);
+
+// helpers used by the mode-matching unit
+void fft_fields(bf_ctx* ctx, const Grid& g, zd* base, int nf, bool mask, double diskR);
+void propagate_fields(bf_ctx* ctx, const Grid& g, int kind, double diskR, const zd* surfaces, zd* base, int nf,
+                      zd eps, double dz, double tx, double ty, int64_t surf_off, double lambda, zd* scratch);
+void build_ktabs(bf_ctx* ctx, const Grid& g, const std::vector<KTabDesc>& d, double scale, zd* out);
+void build_rtabs(bf_ctx* ctx, const Grid& g, const std::vector<RTabDesc>& d, const zd* surfaces, zd* out);
+
 struct DancerOptions {
     const double* emit = nullptr;     // host [R][2][K]
     const double* reflect = nullptr;  // host [K]
@@ -118,6 +150,8 @@ template <typename T>
 void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax, const double* fr,
                      int nfr, int kind, const double* emit_h, const double* reflect_h, double diskR, double* out_right,
                      double* out_left);
+
+double stream_copy_probe(bf_ctx* ctx, int64_t bytes, int reps);
 
 // mode matching (bf_transformer.hip)
 void seed_modes_dev(bf_ctx* ctx, const bf_coords* coords, int M, int L, double diskR, double* patterns, double* kt);

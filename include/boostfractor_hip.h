@@ -148,6 +148,12 @@ int bf_dancer_plan_timing(bf_dancer_plan *plan, int32_t enable);
 int bf_dancer_plan_kernel_times(bf_dancer_plan *plan, double *ms, int64_t *launches);
 int bf_dancer_plan_destroy(bf_dancer_plan *plan);
 
+/* Stream-bandwidth probe: copies `bytes` between two device buffers `reps`
+ * times with the engine's copy kernel (k_stream_copy) and returns the mean
+ * milliseconds per copy.  Used to state measured HBM bandwidth next to the
+ * roofline fractions and to calibrate profiler byte counters. */
+int bf_stream_copy_probe(bf_ctx *ctx, int64_t bytes, int32_t reps, double *ms_per_copy);
+
 /* ---- mode matching ----------------------------------------------------- */
 /* SeedModes Bessel patterns (src/boost_fractor_transformer.jl:46-107, mode :88):
  * J_l(j_{|l|,m} r / R) exp(-i l phi), zero for r > R, normalised to sum |.|^2 = 1.

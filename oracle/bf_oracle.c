@@ -1045,8 +1045,12 @@ int bfo_transformer_trace_back(const double *reflected, const double *input, int
     cx *sol = (cx *)malloc(sizeof(cx) * (size_t)N2), *nsol = (cx *)malloc(sizeof(cx) * (size_t)N2);
     cx *P = (cx *)malloc(sizeof(cx) * (size_t)n * n), *Bm = (cx *)malloc(sizeof(cx) * (size_t)N2 * N2);
     cx *out = (cx *)fields_out;
+    /* fields_regions[r,:,:] = solution: the 2n vector fills the 2 x n slice in
+     * column-major order, element (d, m) = solution[d + 2m]; out is [nreg][2][n] */
+#define PUT(r, v) do { for (int d_ = 0; d_ < 2; d_++) for (int m_ = 0; m_ < n; m_++) \
+        out[((size_t)(r) * 2 + d_) * n + m_] = (v)[d_ + 2 * m_]; } while (0)
     for (int i = 0; i < n; i++) { sol[i] = cload(input, i); sol[n + i] = cload(reflected, i); }
-    memcpy(out + (size_t)(nreg - 1) * N2, sol, sizeof(cx) * (size_t)N2);
+    PUT(nreg - 1, sol);
     fft2 pl;
     fft2_init(&pl, nx, ny);
     for (int s = 1; s <= nreg - 1; s++) {
@@ -1069,8 +1073,9 @@ int bfo_transformer_trace_back(const double *reflected, const double *input, int
             cx S = rmul(0.5, csub(cdiv(C(1.0, 0.0), cmul(nr, nr)), cdiv(C(1.0, 0.0), cmul(nl, nl))));
             for (int i = 0; i < N2; i++) sol[i] = cadd(sol[i], cmul(S, cload(emit, i % n)));
         }
-        memcpy(out + (size_t)ir1 * N2, sol, sizeof(cx) * (size_t)N2);
+        PUT(ir1, sol);
     }
+#undef PUT
     fft2_free(&pl);
     free(sol); free(nsol); free(P); free(Bm);
     return 0;

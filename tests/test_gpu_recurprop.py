@@ -56,7 +56,7 @@ def test_propagator_dz0_is_mask():
     E0 = _field(coords, 3)
     got = B.propagator(E0, 0.0, 0.2, complex(1.0, -0.0), 0.0, 0.0, None, 0.015, coords)
     XX, YY = np.meshgrid(coords.X, coords.Y)
-    assert np.max(np.abs(got - np.where(XX ** 2 + YY ** 2 < 0.04, E0, 0))) < 1e-13
+    assert np.max(np.abs(got - np.where(XX ** 2 + YY ** 2 < 0.2 ** 2, E0, 0))) < 1e-13
 
 
 def test_dancer_1d_golden():

@@ -1,0 +1,311 @@
+"""GPU parity of the mode-matching transformer (src/boost_fractor_transformer.jl).
+
+FP64 path at 1e-10 relative against the reference goldens (test/runtests.jl,
+examples/Beam Shapes.ipynb) and the CPU oracle on richer cases (several
+modes and azimuthal orders, tilts, rough surfaces, all propagator kinds,
+reflectivity, precomputed matrices, trace-back).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import BoostFractor as B
+from BoostFractor import workloads as W
+from helpers import (DISKR, GOLDEN, O, deps_distance, ocoords, pmap, ref_coords, ref_setup_eps9, rel,
+                     with_distance)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def gauss_profile(coords, z=1e-9, omega0=0.1, f=10e9):
+    """input recipe of src/beam_gaussian.jl:24-37 (numpy), used to build golden inputs"""
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    radius = np.sqrt(XX ** 2 + YY ** 2)
+    lam = B.wavelength(f)
+    k = 2 * math.pi / lam
+    zR = math.pi * omega0 ** 2 / lam
+    om = omega0 * math.sqrt(1 + (z / zR) ** 2)
+    Rz = z * (1 + (zR / z) ** 2)
+    psi = math.atan(z / zR)
+    return omega0 / om * np.exp(-radius ** 2 / om ** 2) * np.exp(-1j * (k * z + k * radius ** 2 / (2 * Rz) - psi))
+
+
+def test_transformer_golden():
+    """test/runtests.jl:217: 3D transformer, M=10, L=0, Deps = 0:1:3pi."""
+    coords = ref_coords()
+    sb = ref_setup_eps9(coords)
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=10, Lmax=0, diskR=DISKR)
+    got = []
+    for d in np.arange(0, 3 * math.pi, 1.0):
+        s = with_distance(sb, deps_distance(d))
+        boost = B.transformer(s, coords, modes, prop=B.propagator, f=10e9, diskR=DISKR)
+        got.append(np.sum(np.abs(boost) ** 2))
+    g = np.array(GOLDEN["transformer_3d_deps_0_1_3pi"]["beta2"])
+    assert np.max(np.abs(np.array(got) / g - 1)) < TOL
+
+
+def test_transformer_golden_batched_geometry_and_frequency():
+    """the same golden through the batched frequency axis (one geometry, f = [10 GHz] * 2)."""
+    coords = ref_coords()
+    sb = with_distance(ref_setup_eps9(coords), deps_distance(2.0))
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=10, Lmax=0, diskR=DISKR)
+    boost = B.transformer(sb, coords, modes, f=[10e9, 10e9], diskR=DISKR)
+    g = GOLDEN["transformer_3d_deps_0_1_3pi"]["beta2"][2]
+    assert np.all(np.abs(np.sum(np.abs(boost) ** 2, axis=1) / g - 1) < TOL)
+
+
+def test_modes_golden():
+    """test/runtests.jl:166-171"""
+    coords = ref_coords()
+    sb = ref_setup_eps9(coords)
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=10, Lmax=0, diskR=DISKR)
+    coeffs = B.field2modes(None, coords, modes, diskR=DISKR)
+    field = B.modes2field(coeffs, coords, modes)
+    g = GOLDEN["modes2field_field5000"]
+    assert abs(field.ravel()[4999] - (g["re"] + 1j * g["im"])) < TOL * abs(g["re"])
+    sb.eps[0] = 1e30  # state after the transformer sweep of the reference test (it mutates bdry.eps)
+    propmat = B.calc_propagation_matrices(sb, coords, modes)
+    g = GOLDEN["propmat1_4_6"]
+    # a Gram entry of nearly orthogonal patterns: absolute agreement relative to the unit diagonal
+    assert abs(propmat[0][3, 5] - (g["re"] + 1j * g["im"])) < 1e-14
+    assert abs(propmat[0][3, 3] - 1.0) < 1e-13
+
+
+def test_trace_back_golden():
+    """test/runtests.jl:174-207,221 (1D, with the Deps=9 geometry left in the global by the loop)."""
+    c1 = B.SeedCoordinateSystem(X=[1e-7], Y=[1e-7])
+    sb = ref_setup_eps9(c1)
+    modes = B.SeedModes(c1, ThreeDim=False, Mmax=1, Lmax=0, diskR=DISKR)
+    res = []
+    for d in np.arange(0, 3 * math.pi, 1.0):
+        sb.distance = deps_distance(d)
+        res.append(B.transformer(sb, c1, modes, prop=B.propagator1D, f=10e9, diskR=DISKR))
+    full = B.transformer_trace_back(res[4], 0, sb, c1, modes, prop=B.propagator1D, f=10e9)
+    g = np.array(GOLDEN["trace_back_full_fields"]["re"]) + 1j * np.array(GOLDEN["trace_back_full_fields"]["im"])
+    assert np.max(np.abs(full[:, :, 0] - g)) < 1e-12
+
+
+def test_beam_shapes_goldens():
+    """examples/Beam Shapes.ipynb: 128^2 grid (dx=0.007), uniform beam (M=7) and Gaussian beam (M=4)."""
+    coords = B.SeedCoordinateSystem(X=(-0.4445, 0.007, 0.4445), Y=(-0.4445, 0.007, 0.4445))
+    modes7 = B.SeedModes(coords, ThreeDim=True, Mmax=7, Lmax=0, diskR=0.15)
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    uniform = np.where(XX ** 2 + YY ** 2 > 0.15 ** 2, 0.0, 1.0)
+    m = B.field2modes(uniform, coords, modes7, diskR=0.15)
+    g = np.array(GOLDEN["beam_shapes_uniform_field2modes_M7"]["re"])
+    assert np.max(np.abs(m - g)) < TOL
+    modes4 = B.SeedModes(coords, ThreeDim=True, Mmax=4, Lmax=0, diskR=0.15)
+    gs = gauss_profile(coords, z=1e-9, omega0=0.097, f=22e9)
+    m = B.field2modes(gs, coords, modes4)
+    g = np.array(GOLDEN["beam_shapes_gauss_field2modes_M4"]["re"]) + \
+        1j * np.array(GOLDEN["beam_shapes_gauss_field2modes_M4"]["im"])
+    assert np.max(np.abs(m - g)) < 1e-9 * np.max(np.abs(g))
+
+
+def test_seed_modes_match_oracle():
+    coords = W.grid_coords(64)
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=3, Lmax=2, diskR=0.15)
+    pat, kt = O.seed_modes(ocoords(coords), 3, 2, 0.15)
+    assert rel(modes.mode_kt.reshape(-1), kt) < 1e-14
+    assert rel(modes.patterns_flat()[:, 0], pat) < 1e-12
+
+
+def _lg_like_modes(coords, M, Lm, seed=3, kspace=False):
+    """user patterns through pattern_input/kT_arr (smooth, non-orthogonal, complex).
+    kspace=True gives their fftshifted spectra: the momentum-space propagator
+    expects its input in that form (a real-space pattern fed to it lands at the
+    grid edge after the inverse FFT and is masked away to roundoff)."""
+    rng = np.random.default_rng(seed)
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    r2 = XX ** 2 + YY ** 2
+    phi = np.arctan2(YY, XX)
+    pat = np.zeros((M, 2 * Lm + 1, 1, coords.ny, coords.nx), dtype=complex)
+    kt = np.zeros((M, 2 * Lm + 1), dtype=complex)
+    for m in range(M):
+        for li, l in enumerate(range(-Lm, Lm + 1)):
+            w = 0.08 + 0.02 * m
+            p = (np.sqrt(r2) / w) ** abs(l) * np.exp(-r2 / w ** 2) * np.exp(-1j * l * phi) * (1 + 0.1 * m * r2 / w ** 2)
+            p *= np.exp(1j * rng.uniform(0, 0.1))
+            if kspace:
+                p = np.fft.fftshift(np.fft.fft2(p))
+            pat[m, li, 0] = p / np.sqrt(np.sum(np.abs(p) ** 2))
+            kt[m, li] = 2.0 * math.sqrt(2 * m + abs(l) + 1) / w
+    return B.SeedModes(coords, ThreeDim=True, Mmax=M, Lmax=Lm, pattern_input=pat, kT_arr=kt)
+
+
+def _rough_booster(N=48, ndisk=2, seed=1100):
+    coords = W.grid_coords(N)
+    sb = W.booster(coords, ndisk=ndisk, seed=seed)
+    n = len(sb.distance)
+    rng = W.SplitMix64(seed + 1)
+    sb.relative_tilt_x = np.array([3e-4 * rng.normal() for _ in range(n)])
+    sb.relative_tilt_y = np.array([3e-4 * rng.normal() for _ in range(n)])
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    surf = np.zeros((n, N, N), dtype=complex)
+    for i in range(1, n - 1):
+        surf[i] = 4e-6 * np.cos(17 * XX + i) * np.sin(11 * YY - i)
+    sb.relative_surfaces = surf
+    return coords, sb
+
+
+def _flat_booster(N=48, ndisk=2, seed=1100):
+    coords = W.grid_coords(N)
+    return coords, W.booster(coords, ndisk=ndisk, seed=seed)
+
+
+def _otransformer(sb, coords, modes, f, okind, emit, reflect=None):
+    surf = sb.relative_surfaces if np.any(sb.relative_surfaces != 0) else None
+    eps = sb.eps.copy()
+    eps[np.isnan(eps.real)] = 1e30
+    os_ = O.Setup(sb.distance, eps, None, sb.relative_tilt_x, sb.relative_tilt_y, surf)
+    return O.transformer(os_, ocoords(coords), modes.patterns_flat()[:, 0], modes.mode_kt.reshape(-1), f, okind,
+                         diskR=DISKR, emit=emit, reflect=reflect)
+
+
+def _oracle_mats(sb, coords, modes, f, okind, eps=None):
+    eps = sb.eps if eps is None else eps
+    lam = B.wavelength(f)
+    pat, kt = modes.patterns_flat()[:, 0], modes.mode_kt.reshape(-1)
+    return [O.propagation_matrix(sb.distance[r], DISKR, eps[r], sb.relative_tilt_x[r], sb.relative_tilt_y[r],
+                                 sb.relative_surfaces[r], lam, ocoords(coords), pat, kt, okind)
+            for r in range(len(sb.distance))]
+
+
+def chain_sensitivity(Ps, eps, emit, trials=3, seed=0):
+    """relative change of the boost vector when every propagation matrix is
+    perturbed at the 1e-15 level: the intrinsic FP64 sensitivity of the chain"""
+    n = len(emit)
+    N = len(eps)
+
+    def run(rng):
+        T_ = np.eye(2 * n, dtype=complex)
+        ab = np.zeros(n, complex)
+        for s in range(N - 1, 0, -1):
+            ir, ir1 = N - s, N - s - 1
+            n1, n0 = np.sqrt(eps[ir1]), np.sqrt(eps[ir])
+            ab += (1 / n1 ** 2 - 1 / n0 ** 2) / 2 * (T_[n:, :n] @ emit + T_[n:, n:] @ emit)
+            P = Ps[ir]
+            if rng is not None:
+                P = P * (1 + 1e-15 * (rng.standard_normal(P.shape) + 1j * rng.standard_normal(P.shape)))
+            a, b = (n1 + n0) / (2 * n1), (n1 - n0) / (2 * n1)
+            Pi = np.linalg.inv(P)
+            T_ = T_ @ np.block([[a * P, b * Pi], [b * P, a * Pi]])
+        return -np.linalg.solve(T_[n:, n:], ab)
+
+    b0 = run(None)
+    rng = np.random.default_rng(seed)
+    return max(np.max(np.abs(run(rng) - b0)) / np.max(np.abs(b0)) for _ in range(trials))
+
+
+KINDS = [(B.propagator, O.PROP_PROPAGATOR), (B.propagatorNoTilts, O.PROP_NOTILTS), (B.propagator1D, O.PROP_1D),
+         (B.propagatorMomentumSpace, O.PROP_MOMENTUM)]
+
+
+@pytest.mark.parametrize("builder", [_flat_booster, _rough_booster])
+@pytest.mark.parametrize("prop,okind", KINDS)
+def test_propagation_matrices_per_kind(builder, prop, okind):
+    """every region's matrix (no inversion involved) at 1e-10"""
+    coords, sb = builder()
+    modes = _lg_like_modes(coords, 2, 1, kspace=(okind == O.PROP_MOMENTUM))
+    got = B.calc_propagation_matrices(sb, coords, modes, f=[19.8e9, 20.25e9], prop=prop, diskR=DISKR)
+    for q, f in enumerate([19.8e9, 20.25e9]):
+        want = _oracle_mats(sb, coords, modes, f, okind)
+        for r in range(1, len(sb.distance)):
+            assert rel(got[q][r], want[r]) < TOL, (q, r, rel(got[q][r], want[r]))
+
+
+@pytest.mark.parametrize("builder", [_flat_booster, _rough_booster])
+@pytest.mark.parametrize("prop,okind", KINDS)
+def test_transformer_matches_oracle(builder, prop, okind):
+    """end to end, with the tolerance scaled by the problem's own FP64 sensitivity
+    (these synthetic multimode boosters have cond(T22) up to ~1e7)"""
+    coords, sb = builder()
+    modes = _lg_like_modes(coords, 2, 1, kspace=(okind == O.PROP_MOMENTUM))
+    emit = B.axion_induced_modes(coords, modes, diskR=DISKR)
+    refl_in = np.linspace(1, 2, modes.n) + 0.3j
+    freqs = [19.8e9, 20.25e9]
+    boost, refl = B.transformer(sb.copy(), coords, modes, f=freqs, prop=prop, diskR=DISKR, emit=emit,
+                                reflect=refl_in)
+    want = pmap(lambda f: _otransformer(sb, coords, modes, f, okind, emit, refl_in), freqs)
+    eps = sb.eps.copy()
+    eps[np.isnan(eps.real)] = 1e30
+    for q, f in enumerate(freqs):
+        sens = chain_sensitivity(_oracle_mats(sb, coords, modes, f, okind, eps), eps, emit)
+        tol = max(TOL, 10 * sens)
+        assert tol < 1e-7
+        assert rel(boost[q], want[q][0]) < tol, (q, rel(boost[q], want[q][0]), sens)
+        assert rel(refl[q], want[q][1]) < tol, (q, rel(refl[q], want[q][1]), sens)
+
+
+def test_transformer_many_modes_matches_oracle():
+    """45 orthonormal Bessel modes (M=5, L=4): several MFMA tiles, off-diagonal tiles, padding rows"""
+    coords, sb = _flat_booster(N=128, ndisk=1)
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=4, diskR=DISKR)
+    emit = B.axion_induced_modes(coords, modes, diskR=DISKR)
+    got = B.transformer(sb.copy(), coords, modes, f=20e9, diskR=DISKR, emit=emit)
+    mats = B.calc_propagation_matrices(sb, coords, modes, f=20e9, diskR=DISKR)
+    want_m = _oracle_mats(sb, coords, modes, 20e9, O.PROP_PROPAGATOR)
+    for r in range(1, len(sb.distance)):
+        assert rel(mats[r], want_m[r]) < TOL, r
+    want = _otransformer(sb, coords, modes, 20e9, O.PROP_PROPAGATOR, emit)
+    eps = sb.eps.copy()
+    eps[np.isnan(eps.real)] = 1e30
+    sens = chain_sensitivity(_oracle_mats(sb, coords, modes, 20e9, O.PROP_PROPAGATOR, eps), eps, emit)
+    assert rel(got, want) < max(TOL, 10 * sens), (rel(got, want), sens)
+
+
+def test_propagation_matrices_match_oracle():
+    coords, sb = _rough_booster(N=32)
+    modes = _lg_like_modes(coords, 2, 1)
+    mats = B.calc_propagation_matrices(sb, coords, modes, f=20e9, diskR=DISKR)
+    lam = B.wavelength(20e9)
+    pat, kt = modes.patterns_flat()[:, 0], modes.mode_kt.reshape(-1)
+    for r in range(1, len(sb.distance)):
+        want = O.propagation_matrix(sb.distance[r], DISKR, sb.eps[r], sb.relative_tilt_x[r], sb.relative_tilt_y[r],
+                                    sb.relative_surfaces[r], lam, ocoords(coords), pat, kt)
+        assert rel(mats[r], want) < TOL, r
+    # the NaN mirror region propagates NaN as in the reference
+    assert np.all(np.isnan(mats[0]))
+    one = B.propagation_matrix(sb.distance[2], DISKR, sb.eps[2], sb.relative_tilt_x[2], sb.relative_tilt_y[2],
+                               sb.relative_surfaces[2], lam, coords, modes, onlydiagonal=True)
+    assert np.allclose(np.diag(one), np.diag(mats[2]), rtol=1e-12) and np.count_nonzero(one - np.diag(np.diag(one))) == 0
+
+
+def test_transformer_precomputed_matrices_and_eps_mutation():
+    coords, sb = _flat_booster(N=32)
+    modes = _lg_like_modes(coords, 2, 0)
+    s1 = sb.copy()
+    direct = B.transformer(s1, coords, modes, f=20e9, diskR=DISKR)
+    assert s1.eps[0] == 1e30  # bdry.eps[isnan] .= 1e30
+    mats = B.calc_propagation_matrices(s1, coords, modes, f=20e9, diskR=DISKR)
+    via = B.transformer(s1, coords, modes, f=20e9, diskR=DISKR, propagation_matrices=mats)
+    assert rel(via, direct) < 1e-13
+
+
+def test_trace_back_multimode_matches_oracle():
+    coords, sb = _flat_booster(N=32)
+    modes = _lg_like_modes(coords, 2, 1)
+    sb.eps[0] = 1e30
+    rng = np.random.default_rng(2)
+    reflected = rng.standard_normal(modes.n) + 1j * rng.standard_normal(modes.n)
+    inp = rng.standard_normal(modes.n) + 0j
+    got = B.transformer_trace_back(reflected, inp, sb, coords, modes, f=20e9)
+    os_ = O.Setup(sb.distance, sb.eps, None)
+    want = O.transformer_trace_back(reflected, inp, os_, ocoords(coords), modes.patterns_flat()[:, 0],
+                                    modes.mode_kt.reshape(-1), 20e9)
+    assert rel(got, want) < TOL
+
+
+def test_transformer_reference_vs_cheerleader_near_resonance():
+    """1 disk in front of a mirror: the transformer with many modes approaches the 3D
+    cheerleader's boost factor (examples/Single Disk.ipynb, cell 33) -- a coarse cross-check
+    between the two independent device solvers."""
+    coords = ref_coords()
+    sb = with_distance(ref_setup_eps9(coords), deps_distance(2.0))
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=10, Lmax=0, diskR=DISKR)
+    bt = np.sum(np.abs(B.transformer(sb.copy(), coords, modes, f=10e9, diskR=DISKR)) ** 2)
+    E = B.cheerleader(0, 100, sb, coords, f=10e9, diskR=DISKR)
+    bc = np.sum(np.abs(E) ** 2) * 1e-4 / (math.pi * DISKR ** 2)
+    assert abs(bt / bc - 1) < 0.05

@@ -1,0 +1,149 @@
+"""Host-side tests (no device): grids, setup, the C-ABI library's exports and
+its argument validation, the Python mirror's marshalling and error mapping."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import BoostFractor as B
+from BoostFractor import _lib as L
+from BoostFractor import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "boostfractor_hip.h")
+
+
+def test_julia_ranges():
+    X = B.julia_range(-0.5, 0.01, 0.5)
+    assert len(X) == 101 and X[0] == -0.5 and X[50] == 0.0 and X[-1] == 0.5
+    assert X[1] == -0.49 and X[37] == -0.13  # exact decimal elements
+    X = B.julia_range(-0.4445, 0.007, 0.4445)
+    assert len(X) == 128 and X[0] == -0.4445 and X[-1] == 0.4445
+    k = B.get_kspace_coords(X)
+    assert len(k) == 128 and k[0] == -448.79895051282756 and k[-1] == 448.79895051282756
+    assert k[1] == -441.73125050475153  # as printed by examples/Beam Shapes.ipynb cell 13
+    c = B.SeedCoordinateSystem()
+    assert c.nx == 101 and c.kX[50] == 0.0 and abs(c.kX[1] - c.kX[0] - 2 * math.pi) < 1e-12
+
+
+def test_kspace_even_grid_dc_offset():
+    """dk = 2pi/(2 max X); an even grid's fftshifted DC bin carries dk/2 (not 0)"""
+    c = W.grid_coords(256)
+    dk = 2 * math.pi / (2 * c.X.max())
+    assert abs(c.kX[128] - dk / 2) < 1e-9 and abs(c.kX[1] - c.kX[0] - dk) < 1e-9
+
+
+def test_seed_setup_boundaries():
+    c = B.SeedCoordinateSystem()
+    sb = B.SeedSetupBoundaries(c, diskno=1)
+    assert np.isnan(sb.eps[0].real)
+    assert list(sb.distance) == [0.0, 8e-3, 1e-3, 0.0]
+    # real eps stored as re - 0.0im
+    assert all(math.copysign(1.0, e.imag) == -1.0 for e in sb.eps)
+    assert np.allclose(sb.r, [1.0, -0.5, 0.5])
+    with pytest.raises(B.DimensionMismatch):
+        B.SeedSetupBoundaries(c, diskno=1, distance=[0.0, 1.0, 0.0])
+
+
+def test_booster_generator_is_seeded():
+    c = W.grid_coords(32)
+    a = W.booster(c, ndisk=20, seed=1000)
+    b = W.booster(c, ndisk=20, seed=1000)
+    assert np.array_equal(a.distance, b.distance) and len(a.distance) == 42
+    s = a.distance[1:-1:2]
+    assert np.all(np.abs(s - 8e-3) <= 0.3e-3) and np.all(a.distance[2:-1:2] == 1e-3)
+    assert a.eps[2] == 24 and a.eps[1] == 1
+    assert W.SplitMix64(0).next_u64() == 0xE220A8397B1DCDAF  # published SplitMix64 first output for seed 0
+
+
+def _header_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(bf_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = L.lib()
+    syms = _header_symbols()
+    assert len(syms) >= 24
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.bf_version() >= 10000
+
+
+def _coords_struct(nx=8, ny=8):
+    m = L.Marshalled()
+    c = W.grid_coords(nx)
+    return m, m.coords(c)
+
+
+def test_cabi_validation_without_device():
+    """argument and dimension checks run on the host before any device work"""
+    lib = L.lib()
+    m, c = _coords_struct()
+    E = np.zeros((8, 8), dtype=complex)
+    # unknown propagator kind -> ArgumentError
+    rc = lib.bf_propagate(None, 7, ctypes.byref(c), L.cptr(E), 0.0, 0.1, 1.0, 0.0, 0.0, 0.0, None, 0.03)
+    assert rc == L.BF_ERR_ARG and b"propagator" in lib.bf_last_error(None)
+    # grid too large
+    big = L.Coords(2048, 8, c.X, c.Y, c.kX, c.kY)
+    rc = lib.bf_propagate(None, 0, ctypes.byref(big), L.cptr(E), 0.0, 0.1, 1.0, 0.0, 0.0, 0.0, None, 0.03)
+    assert rc == L.BF_ERR_UNSUPPORTED
+    # valid arguments but no context -> ArgumentError, raised only after the checks
+    rc = lib.bf_propagate(None, 0, ctypes.byref(c), L.cptr(E), 0.0, 0.1, 1.0, 0.0, 0.0, 0.0, None, 0.03)
+    assert rc == L.BF_ERR_ARG and b"ctx" in lib.bf_last_error(None)
+    # too few regions -> DimensionMismatch
+    sb = W.booster(W.grid_coords(8), ndisk=1)
+    s = m.setup(sb, 8, 8)
+    s.nreg = 1
+    f = np.array([20e9])
+    out = np.zeros((1, 8, 8), dtype=complex)
+    rc = lib.bf_dancer(None, ctypes.byref(s), ctypes.byref(c), 0.0, 10, L.cptr(f), 1, 0, None, None, 0.15, 1, 1, 0,
+                       L.cptr(out), None, None)
+    assert rc == L.BF_ERR_DIM
+    with pytest.raises(B.DimensionMismatch):
+        L.check(rc)
+    # bad precision / negative frequency
+    s.nreg = len(sb.distance)
+    rc = lib.bf_dancer(None, ctypes.byref(s), ctypes.byref(c), 0.0, 10, L.cptr(f), 1, 0, None, None, 0.15, 1, 1, 9,
+                       L.cptr(out), None, None)
+    assert rc == L.BF_ERR_ARG
+    fneg = np.array([-1.0])
+    rc = lib.bf_dancer(None, ctypes.byref(s), ctypes.byref(c), 0.0, 10, L.cptr(fneg), 1, 0, None, None, 0.15, 1, 1, 0,
+                       L.cptr(out), None, None)
+    assert rc == L.BF_ERR_ARG
+    # includes_axion with several modes is a DimensionMismatch in the reference
+    pat = np.ones((3, 8, 8), dtype=complex)
+    kt = np.zeros(3, dtype=complex)
+    md = L.ModesS(3, 1, L.cptr(pat), L.cptr(kt))
+    v = np.zeros(3, dtype=complex)
+    fields = np.zeros((len(sb.distance), 2, 3), dtype=complex)
+    rc = lib.bf_transformer_trace_back(None, L.cptr(v), L.cptr(v), ctypes.byref(s), ctypes.byref(c),
+                                       ctypes.byref(md), 20e9, 0, None, 0.15, 1, L.cptr(v), L.cptr(fields))
+    assert rc == L.BF_ERR_DIM
+
+
+def test_mirror_rejects_user_propagators():
+    with pytest.raises(B.ArgumentError):
+        from BoostFractor.propagators import prop_kind
+        prop_kind(lambda *a: a[0])
+
+
+def test_no_cpu_fallback_when_library_missing(monkeypatch, tmp_path):
+    """the mirror raises instead of computing anything on the host"""
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(ImportError):
+        L.lib()
+
+
+def test_oracle_not_imported_by_product():
+    """the product never imports, links or loads the CPU oracle"""
+    pkg = os.path.join(ROOT, "boostfractor.jl_amd")
+    pat = re.compile(r"(import\s+oracle|from\s+oracle|libbf_oracle|bf_oracle\.c|bfo_)")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp", "Makefile")) or f == "Makefile":
+                assert not pat.search(open(os.path.join(dirpath, f)).read()), f
