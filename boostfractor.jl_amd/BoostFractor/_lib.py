@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("BF_LIBRARY", os.path.join(PKG_DIR, "lib", "libboostfr
 
 BF_OK, BF_ERR_DIM, BF_ERR_ARG, BF_ERR_UNSUPPORTED, BF_ERR_HIP, BF_ERR_NOMEM = 0, 1, 2, 3, 4, 5
 PROP_PROPAGATOR, PROP_NOTILTS, PROP_1D, PROP_MOMENTUM = 0, 1, 2, 3
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
 
 
 class ArgumentError(ValueError):

@@ -77,6 +77,8 @@ def prop_kind(prop):
 def precision_code(precision):
     if precision in ("fp64", "float64", L.PREC_FP64, None):
         return L.PREC_FP64
-    if precision in ("mixed", "fp32", L.PREC_MIXED):
+    if precision in ("mixed", L.PREC_MIXED):
         return L.PREC_MIXED
-    raise L.ArgumentError(f"unknown precision {precision!r} (use 'fp64' or 'mixed')")
+    if precision in ("fp32", L.PREC_FP32):
+        return L.PREC_FP32
+    raise L.ArgumentError(f"unknown precision {precision!r} (use 'fp64', 'mixed' or 'fp32')")

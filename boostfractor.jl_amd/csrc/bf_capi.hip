@@ -68,7 +68,7 @@ static int check_kind(const bf_ctx* ctx, int kind) {
 }
 
 static int check_prec(const bf_ctx* ctx, int p) {
-    BF_REQUIRE(ctx, p == BF_PREC_FP64 || p == BF_PREC_MIXED, BF_ERR_ARG, "unknown precision");
+    BF_REQUIRE(ctx, p == BF_PREC_FP64 || p == BF_PREC_MIXED || p == BF_PREC_FP32, BF_ERR_ARG, "unknown precision");
     return BF_OK;
 }
 
@@ -174,6 +174,17 @@ int bf_dance_intro(bf_ctx* ctx, const bf_coords* coords, int32_t nd, const doubl
     return guarded(ctx, [&] { bf::dance_intro_dev(ctx, coords, nd, eps, cmk<double>(r1_re, r1_im), diskR, out); });
 }
 
+// Launch-shape and band overrides for tuning and A/B checks: BF_COL_TX
+// (columns per column-pass block), BF_ROWS (rows per dancer row block),
+// BF_BAND=0 (process every row instead of the aperture's row band).
+static bf::DancerOptions env_options() {
+    bf::DancerOptions o;
+    if (const char* v = getenv("BF_COL_TX")) o.col_tx = atoi(v);
+    if (const char* v = getenv("BF_ROWS")) o.rows = atoi(v);
+    if (const char* v = getenv("BF_BAND")) o.band = atoi(v);
+    return o;
+}
+
 int bf_dancer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, double amin, int32_t nmax,
               const double* freqs, int32_t nfreq, int32_t prop_kind, const double* emit, const double* reflect,
               double diskR, int32_t returnsum, int32_t immediatesum, int32_t precision, double* Eout, double* power,
@@ -187,7 +198,7 @@ int bf_dancer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, doubl
     BF_REQUIRE(ctx, Eout || power, BF_ERR_ARG, "Eout and power are both NULL");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
-        bf::DancerOptions o;
+        bf::DancerOptions o = env_options();
         o.reflect = reflect;  // reflect takes precedence over emit (src/boost_fractor_recurprop.jl:53-63)
         o.emit = reflect ? nullptr : emit;
         o.returnsum = returnsum ? 1 : 0;
@@ -214,6 +225,9 @@ int bf_cheerleader(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, 
         if (precision == BF_PREC_MIXED)
             bf::cheerleader_run<float>(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, em, reflect, diskR,
                                        out_right, out_left);
+        else if (precision == BF_PREC_FP32)
+            bf::cheerleader_run<bf::f32pure>(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, em, reflect,
+                                             diskR, out_right, out_left);
         else
             bf::cheerleader_run<double>(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, em, reflect, diskR,
                                         out_right, out_left);
@@ -234,12 +248,8 @@ int bf_dancer_plan_create(bf_ctx* ctx, const bf_setup* setup, const bf_coords* c
     BF_REQUIRE(ctx, freq_batch >= 0, BF_ERR_ARG, "freq_batch must be >= 0");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
-        bf::DancerOptions o;
+        bf::DancerOptions o = env_options();
         o.freq_batch = freq_batch;
-        const char* tx = getenv("BF_COL_TX");
-        const char* rows = getenv("BF_ROWS");
-        if (tx) o.col_tx = atoi(tx);
-        if (rows) o.rows = atoi(rows);
         auto* p = new bf_dancer_plan();
         try {
             p->impl = bf::make_dancer_plan(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, diskR, precision, o);

@@ -58,6 +58,11 @@ __host__ __device__ __forceinline__ cplx<To> ccast(cplx<Ti> a) {
 
 typedef cplx<double> zd;
 
+// precision tag: FP32 storage with FP32 arithmetic
+namespace bf {
+struct f32pure {};
+}  // namespace bf
+
 // (-1im) * z evaluated as Julia does: Complex(0,-1) * z
 __host__ __device__ __forceinline__ zd neg_i(zd z) {
 #pragma clang fp contract(off)
@@ -107,6 +112,61 @@ __host__ __device__ inline zd kz_phase(zd k0sq, double Kx, double Ky, double dz)
     zd kz = csqrt_j(arg);
     zd e = neg_i(kz);
     return cexp_j(cmk<double>(dz * e.re, dz * e.im));
+}
+
+// Aperture predicate of the propagators, abs(x^2 + y^2) < diskR^2
+// (src/boost_fractor.jl:159, :198), with separately rounded products.
+__host__ __device__ inline bool in_aperture(double x, double y, double R2) {
+#pragma clang fp contract(off)
+    return fabs(x * x + y * y) < R2;
+}
+
+// ---------------------------------------------------------------------------
+// Raw buffer access.  A buffer load whose byte offset lies beyond the
+// resource's size returns zero without touching memory, and such a store is
+// dropped; predicated loads and stores therefore need no branches, and every
+// load of a pass can be in flight at once.  Offsets are 32-bit byte offsets
+// (a resource covers at most one field or table, < 2 GiB).
+// ---------------------------------------------------------------------------
+constexpr uint32_t BUF_OOB = 0x80000000u;  // out of range for every resource used here
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    // word 3 = 0x00020000: 32-bit data format, the raw-buffer setting of the gfx9 family
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename S>
+__device__ __forceinline__ cplx<S> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ cplx<float> buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    return cmk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+template <>
+__device__ __forceinline__ cplx<double> buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return cmk<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+}
+
+template <typename S>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, cplx<S> a);
+template <>
+__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, uint32_t off, cplx<float> a) {
+    unsigned int v __attribute__((__vector_size__(8)));
+    v[0] = __float_as_uint(a.re);
+    v[1] = __float_as_uint(a.im);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, uint32_t off, cplx<double> a) {
+    unsigned int v __attribute__((__vector_size__(16)));
+    const unsigned long long re = (unsigned long long)__double_as_longlong(a.re);
+    const unsigned long long im = (unsigned long long)__double_as_longlong(a.im);
+    v[0] = (unsigned int)re;
+    v[1] = (unsigned int)(re >> 32);
+    v[2] = (unsigned int)im;
+    v[3] = (unsigned int)(im >> 32);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
 }
 
 // ---------------------------------------------------------------------------

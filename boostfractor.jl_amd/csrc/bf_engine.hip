@@ -17,8 +17,10 @@
 // reading and writing every field once.  The per-region k-space phase tables
 // are built once per frequency batch from FP64 arguments.
 #include "bf_internal.h"
+#include "bf_fftreg.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 
@@ -30,6 +32,18 @@ namespace bf {
 enum : uint32_t { RF_INV = 1, RF_RTAB = 2, RF_MASK = 4, RF_FWD = 8 };
 enum : uint32_t { CF_FWD = 1, CF_KMID = 2, CF_INV = 4, CF_KLOAD = 8, CF_INSHIFT = 16, CF_OUTSHIFT = 32 };
 enum : uint32_t { DF_FWD = 1, DF_AMAX = 2, DF_EOUT_ASSIGN = 4 };
+enum : int { FD_WHOLE = 1 };  // FieldDesc.pad: column pass stores every row of this field
+
+// Precision tags: `double` = FP64 storage and arithmetic; `float` = fields stored
+// in FP32 in HBM, all in-register arithmetic (FFTs, phases, boundary updates) in
+// FP64; `f32pure` = FP32 storage and FP32 arithmetic.
+template <typename T> struct Storage { using type = T; };
+template <> struct Storage<f32pure> { using type = float; };
+template <typename T> struct Compute { using type = double; };
+template <> struct Compute<f32pure> { using type = float; };
+#define BF_TYPES(T)                           \
+    using S = typename Storage<T>::type;      \
+    using C = typename Compute<T>::type
 
 // ---------------------------------------------------------------------------
 // small kernels
@@ -80,15 +94,17 @@ __global__ void k_build_rtab(cplx<T>* out, const RTabDesc* __restrict__ d, int n
 
 // fields[j] *= ph[j] (propagator1D, src/boost_fractor.jl:218-227)
 template <typename T>
-__global__ void k_scale_fields(cplx<T>* base, const FieldDesc* __restrict__ fd, const zd* __restrict__ ph, size_t K,
-                               const int* __restrict__ active) {
+__global__ void k_scale_fields(typename Storage<T>::type* base_, const FieldDesc* __restrict__ fd,
+                               const zd* __restrict__ ph, size_t K, const int* __restrict__ active) {
+    BF_TYPES(T);
+    cplx<S>* base = reinterpret_cast<cplx<S>*>(base_);
     const int j = blockIdx.y;
     const FieldDesc d = fd[j];
     if (active && !active[d.group]) return;
-    const cplx<T> p = cmk<T>((T)ph[j].re, (T)ph[j].im);
-    cplx<T>* f = base + d.off;
+    const cplx<C> p = cmk<C>((C)ph[j].re, (C)ph[j].im);
+    cplx<S>* f = base + d.off;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x)
-        f[i] = cmul(f[i], p);
+        f[i] = ccast<S>(cmul(ccast<C>(f[i]), p));
 }
 
 // ---------------------------------------------------------------------------
@@ -96,12 +112,12 @@ __global__ void k_scale_fields(cplx<T>* base, const FieldDesc* __restrict__ fd, 
 // ---------------------------------------------------------------------------
 template <typename T>
 struct RowArgs {
-    cplx<T>* base;
+    cplx<typename Storage<T>::type>* base;
     const FieldDesc* fd;
     int nf, nx, ny;
-    const cplx<T>* tw;
+    const cplx<typename Compute<T>::type>* tw;
     uint32_t flags;
-    const cplx<T>* rtab;
+    const cplx<typename Compute<T>::type>* rtab;
     const double* X;
     const double* Y;
     double R2;
@@ -110,14 +126,15 @@ struct RowArgs {
 
 template <typename T, int N>
 __global__ void __launch_bounds__(1024) k_rows(RowArgs<T> a) {
+    BF_TYPES(T);
     extern __shared__ unsigned char smem_raw[];
-    cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<C>* smem = reinterpret_cast<cplx<C>*>(smem_raw);
     constexpr int TPL = LineCfg<N>::TPL;
     const int n = (N >= 16) ? N : a.nx;
     const int lpb = blockDim.x / TPL;
     const int li = threadIdx.x / TPL, t = threadIdx.x % TPL;
-    cplx<T>* s = smem + (size_t)li * n;
-    cplx<T>* s2 = smem + (size_t)(lpb + li) * n;
+    cplx<C>* s = smem + (size_t)li * n;
+    cplx<C>* s2 = smem + (size_t)(lpb + li) * n;
     const long nlines = (long)a.nf * a.ny;
     const long line = (long)blockIdx.x * lpb + li;
     bool valid = line < nlines;
@@ -125,29 +142,29 @@ __global__ void __launch_bounds__(1024) k_rows(RowArgs<T> a) {
     const int y = valid ? (int)(line % a.ny) : 0;
     const FieldDesc d = a.fd[j];
     if (valid && a.active && !a.active[d.group]) valid = false;
-    cplx<T>* row = a.base + d.off + (size_t)y * a.nx;
-    for (int x = t; x < n; x += TPL) s[x] = valid ? row[x] : cmk<T>(0, 0);
+    cplx<S>* row = a.base + d.off + (size_t)y * a.nx;
+    for (int x = t; x < n; x += TPL) s[x] = valid ? ccast<C>(row[x]) : cmk<C>(0, 0);
     __syncthreads();
-    if (a.flags & RF_INV) line_fft<T, N, true>(s, s2, n, t, a.tw);
+    if (a.flags & RF_INV) line_fft<C, N, true>(s, s2, n, t, a.tw);
     if (a.flags & (RF_RTAB | RF_MASK)) {
-        const cplx<T>* tab = ((a.flags & RF_RTAB) && d.rtab >= 0)
+        const cplx<C>* tab = ((a.flags & RF_RTAB) && d.rtab >= 0)
                                  ? a.rtab + ((size_t)d.rtab * a.ny + y) * a.nx
                                  : nullptr;
         const double yy = a.Y[y];
         for (int x = t; x < n; x += TPL) {
-            cplx<T> v = s[x];
+            cplx<C> v = s[x];
             if (tab) v = cmul(v, tab[x]);
             if (a.flags & RF_MASK) {
                 const double xx = a.X[x];
-                if (!(fabs(xx * xx + yy * yy) < a.R2)) v = cmk<T>(0, 0);
+                if (!(fabs(xx * xx + yy * yy) < a.R2)) v = cmk<C>(0, 0);
             }
             s[x] = v;
         }
         __syncthreads();
     }
-    if (a.flags & RF_FWD) line_fft<T, N, false>(s, s2, n, t, a.tw);
+    if (a.flags & RF_FWD) line_fft<C, N, false>(s, s2, n, t, a.tw);
     if (valid)
-        for (int x = t; x < n; x += TPL) row[x] = s[x];
+        for (int x = t; x < n; x += TPL) row[x] = ccast<S>(s[x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,56 +172,70 @@ __global__ void __launch_bounds__(1024) k_rows(RowArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ColArgs {
-    const cplx<T>* src;
-    cplx<T>* dst;
+    const cplx<typename Storage<T>::type>* src;
+    cplx<typename Storage<T>::type>* dst;
     const FieldDesc* fd;
     int nf, nx, ny;
-    const cplx<T>* tw;
+    const cplx<typename Compute<T>::type>* tw;
     uint32_t flags;
-    const cplx<T>* ktab;
+    const cplx<typename Compute<T>::type>* ktab;
     int tx;  // columns per block
     const int* active;
+    // Row band: rows outside [ylo, yhi) are known to be zero on input (they are
+    // not loaded) and are not needed on output (not stored), except for the
+    // fields whose descriptor carries FD_WHOLE, stored whole.
+    int ylo, yhi;
 };
+
+template <typename T>
+__device__ __forceinline__ bool in_band(const ColArgs<T>& a, int y) {
+    return (unsigned)(y - a.ylo) < (unsigned)(a.yhi - a.ylo);
+}
+__device__ __forceinline__ bool store_whole(const FieldDesc& d) { return (d.pad & FD_WHOLE) != 0; }
 
 template <typename T, int N>
 __global__ void __launch_bounds__(1024) k_cols(ColArgs<T> a) {
+    BF_TYPES(T);
     extern __shared__ unsigned char smem_raw[];
-    cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<C>* smem = reinterpret_cast<cplx<C>*>(smem_raw);
     constexpr int TPL = LineCfg<N>::TPL;
     const int n = (N >= 16) ? N : a.ny;
     const int ld = n + 1;
     const int tx = a.tx;
-    cplx<T>* s = smem;
-    cplx<T>* s2 = smem + (size_t)tx * ld;
+    cplx<C>* s = smem;
+    cplx<C>* s2 = smem + (size_t)tx * ld;
     const FieldDesc d = a.fd[blockIdx.y];
     if (a.active && !a.active[d.group]) return;  // uniform over the block
     const int x0 = blockIdx.x * tx;
     const int nx = a.nx, ny = a.ny;
     const int hx = nx / 2, hy = ny / 2;
-    const cplx<T>* src = a.src + d.off;
-    cplx<T>* dst = a.dst + d.off;
-    const cplx<T>* kt = (d.ktab >= 0) ? a.ktab + (size_t)d.ktab * ny * nx : nullptr;
+    const cplx<S>* src = a.src + d.off;
+    cplx<S>* dst = a.dst + d.off;
+    const cplx<C>* kt = (d.ktab >= 0) ? a.ktab + (size_t)d.ktab * ny * nx : nullptr;
     const uint32_t fl = a.flags;
+    const bool whole = store_whole(d);
     for (int i = threadIdx.x; i < tx * n; i += blockDim.x) {
         const int c = i % tx, my = i / tx, mx = x0 + c;
-        cplx<T> v = cmk<T>(0, 0);
+        cplx<C> v = cmk<C>(0, 0);
         if (mx < nx) {
             int sx = mx, sy = my;
             if (fl & CF_INSHIFT) {  // ifftshift on load
                 sx = (mx + hx) % nx;
                 sy = (my + hy) % ny;
             }
-            const size_t o = (size_t)sy * nx + sx;
-            v = src[o];
-            if ((fl & CF_KLOAD) && kt) v = cmul(v, kt[o]);
+            if (in_band(a, sy)) {
+                const size_t o = (size_t)sy * nx + sx;
+                v = ccast<C>(src[o]);
+                if ((fl & CF_KLOAD) && kt) v = cmul(v, kt[o]);
+            }
         }
         s[c * ld + my] = v;
     }
     __syncthreads();
     const int c = threadIdx.x / TPL, t = threadIdx.x % TPL;
-    cplx<T>* line = s + c * ld;
-    cplx<T>* line2 = s2 + c * ld;
-    if (fl & CF_FWD) line_fft<T, N, false>(line, line2, n, t, a.tw);
+    cplx<C>* line = s + c * ld;
+    cplx<C>* line2 = s2 + c * ld;
+    if (fl & CF_FWD) line_fft<C, N, false>(line, line2, n, t, a.tw);
     if ((fl & CF_KMID) && kt) {
         for (int i = threadIdx.x; i < tx * n; i += blockDim.x) {
             const int cc = i % tx, my = i / tx, mx = x0 + cc;
@@ -212,7 +243,7 @@ __global__ void __launch_bounds__(1024) k_cols(ColArgs<T> a) {
         }
         __syncthreads();
     }
-    if (fl & CF_INV) line_fft<T, N, true>(line, line2, n, t, a.tw);
+    if (fl & CF_INV) line_fft<C, N, true>(line, line2, n, t, a.tw);
     for (int i = threadIdx.x; i < tx * n; i += blockDim.x) {
         const int cc = i % tx, my = i / tx, mx = x0 + cc;
         if (mx < nx) {
@@ -221,7 +252,7 @@ __global__ void __launch_bounds__(1024) k_cols(ColArgs<T> a) {
                 dx = (mx + hx) % nx;
                 dy = (my + hy) % ny;
             }
-            dst[(size_t)dy * nx + dx] = s[cc * ld + my];
+            if (whole || in_band(a, dy)) dst[(size_t)dy * nx + dx] = ccast<S>(s[cc * ld + my]);
         }
     }
 }
@@ -231,11 +262,11 @@ __global__ void __launch_bounds__(1024) k_cols(ColArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct DancerRowArgs {
-    cplx<T>* fields;  // [fb][R][2][ny][nx]
+    cplx<typename Storage<T>::type>* fields;  // [fb][R][2][ny][nx]
     int R, nx, ny;
     int rm;  // slot holding the right-moving fields this bounce
-    const cplx<T>* tw;
-    const cplx<T>* rtab;
+    const cplx<typename Compute<T>::type>* tw;
+    const cplx<typename Compute<T>::type>* rtab;
     const int* rtab_idx;  // [fb][R], -1: none
     const zd* coef;       // boundary reflectivities r_b, b = 0..R (r_R = 0 appended)
     const double* X;
@@ -248,7 +279,31 @@ struct DancerRowArgs {
     double* amax;  // [fb]
     const int* active;
     int* count;  // [fb] bounce counters
+    // Row band [ylo, yhi): outside it the next bounce's aperture zeroes every
+    // row, so those rows are only needed for Eout at the exit boundary.  The
+    // launcher fills the tile map from the band and its rows per block.
+    int ylo, yhi;
+    int tile0, nband, ntiles;
 };
+
+// Block -> (boundary b, row tile).  Blocks [0, nband (R+1)) cover the row tiles
+// that meet the band at every boundary; the remaining ntiles - nband blocks
+// cover the other row tiles at the exit boundary only (exit_only: Eout is
+// accumulated, nothing is stored because the aperture zeroes those rows).
+template <typename T>
+__device__ __forceinline__ void dancer_block(const DancerRowArgs<T>& a, int i, int& b, int& tile, bool& exit_only) {
+    const int nb = a.nband * (a.R + 1);
+    if (i < nb) {
+        b = i / a.nband;
+        tile = a.tile0 + i % a.nband;
+        exit_only = false;
+    } else {
+        const int j = i - nb;
+        b = a.R;
+        tile = j < a.tile0 ? j : j + a.nband;
+        exit_only = true;
+    }
+}
 
 __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
     // non-negative doubles order like their bit patterns
@@ -257,48 +312,52 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
 
 template <typename T, int N>
 __global__ void __launch_bounds__(1024) k_dancer_rows(DancerRowArgs<T> a) {
+    BF_TYPES(T);
     extern __shared__ unsigned char smem_raw[];
-    cplx<T>* smem = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<C>* smem = reinterpret_cast<cplx<C>*>(smem_raw);
     __shared__ double red[32];
     constexpr int TPL = LineCfg<N>::TPL;
     const int n = (N >= 16) ? N : a.nx;
-    const int f = blockIdx.z, b = blockIdx.y;
+    const int f = blockIdx.y;
+    int b, tile;
+    bool exit_only;
+    dancer_block(a, blockIdx.x, b, tile, exit_only);
     if (a.active && !a.active[f]) return;  // uniform over the block
-    if (a.count && blockIdx.x == 0 && b == 0 && threadIdx.x == 0) a.count[f] += 1;
+    if (a.count && blockIdx.x == 0 && threadIdx.x == 0) a.count[f] += 1;
     const int lpb = blockDim.x / TPL;  // 2 * rows per block
     const int li = threadIdx.x / TPL, t = threadIdx.x % TPL;
     const int side = li & 1;  // 0: A = region b-1 right-moving, 1: B = region b left-moving
-    const int y = blockIdx.x * (lpb >> 1) + (li >> 1);
+    const int y = tile * (lpb >> 1) + (li >> 1);
     const int reg = side == 0 ? b - 1 : b;
     const int slot = side == 0 ? a.rm : 1 - a.rm;
     const bool exists = reg >= 0 && reg < a.R && y < a.ny;
     const size_t K = (size_t)a.nx * a.ny;
-    cplx<T>* s = smem + (size_t)li * n;
-    cplx<T>* s2 = smem + (size_t)(lpb + li) * n;
-    cplx<T>* row = a.fields + (((size_t)f * a.R + (exists ? reg : 0)) * 2 + slot) * K + (size_t)(exists ? y : 0) * a.nx;
-    for (int x = t; x < n; x += TPL) s[x] = exists ? row[x] : cmk<T>(0, 0);
+    cplx<C>* s = smem + (size_t)li * n;
+    cplx<C>* s2 = smem + (size_t)(lpb + li) * n;
+    cplx<S>* row = a.fields + (((size_t)f * a.R + (exists ? reg : 0)) * 2 + slot) * K + (size_t)(exists ? y : 0) * a.nx;
+    for (int x = t; x < n; x += TPL) s[x] = exists ? ccast<C>(row[x]) : cmk<C>(0, 0);
     __syncthreads();
-    line_fft<T, N, true>(s, s2, n, t, a.tw);
+    line_fft<C, N, true>(s, s2, n, t, a.tw);
     if (a.rtab) {
         const int ti = exists ? a.rtab_idx[f * a.R + reg] : -1;
         if (ti >= 0) {
-            const cplx<T>* tab = a.rtab + (size_t)ti * K + (size_t)y * a.nx;
+            const cplx<C>* tab = a.rtab + (size_t)ti * K + (size_t)y * a.nx;
             for (int x = t; x < n; x += TPL) s[x] = cmul(s[x], tab[x]);
         }
         __syncthreads();
     }
     // boundary update (src/boost_fractor_recurprop.jl:90-119), result staged in s2
     const zd r = a.coef[b];
-    const cplx<T> rT = cmk<T>((T)r.re, (T)r.im);
-    const cplx<T> mrT = cmk<T>((T)(-r.re), (T)(-r.im));
-    const cplx<T> omrT = cmk<T>((T)(1.0 - r.re), (T)(-r.im));
-    const cplx<T> oprT = cmk<T>((T)(1.0 + r.re), (T)r.im);
-    const cplx<T>* p = smem + (size_t)(li ^ 1) * n;  // partner line, same y
+    const cplx<C> rT = cmk<C>((C)r.re, (C)r.im);
+    const cplx<C> mrT = cmk<C>((C)(-r.re), (C)(-r.im));
+    const cplx<C> omrT = cmk<C>((C)(1.0 - r.re), (C)(-r.im));
+    const cplx<C> oprT = cmk<C>((C)(1.0 + r.re), (C)r.im);
+    const cplx<C>* p = smem + (size_t)(li ^ 1) * n;  // partner line, same y
     double vmax = 0.0;
     const double yy = a.Y[exists ? y : 0];
     for (int x = t; x < n; x += TPL) {
-        const cplx<T> v = s[x], w = p[x];
-        cplx<T> nv = cmk<T>(0, 0);
+        const cplx<C> v = s[x], w = p[x];
+        cplx<C> nv = cmk<C>(0, 0);
         if (exists) {
             if (side == 0) {
                 if (b == a.R) {  // exit boundary: Eout += (1+r)*A, A *= r
@@ -319,7 +378,7 @@ __global__ void __launch_bounds__(1024) k_dancer_rows(DancerRowArgs<T> a) {
             if (a.flags & DF_AMAX) vmax = fmax(vmax, hypot((double)nv.re, (double)nv.im));
             if (a.flags & DF_FWD) {
                 const double xx = a.X[x];
-                if (!(fabs(xx * xx + yy * yy) < a.R2)) nv = cmk<T>(0, 0);  // next bounce's aperture
+                if (!(fabs(xx * xx + yy * yy) < a.R2)) nv = cmk<C>(0, 0);  // next bounce's aperture
             }
         }
         s2[x] = nv;
@@ -334,44 +393,317 @@ __global__ void __launch_bounds__(1024) k_dancer_rows(DancerRowArgs<T> a) {
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
         if (m > 0.0) atomic_max_pos(a.amax + f, m);
     }
-    if (!(a.flags & DF_FWD)) return;  // last bounce: only Eout is needed
+    if (!(a.flags & DF_FWD) || exit_only) return;  // only Eout is needed
     for (int x = t; x < n; x += TPL) s[x] = s2[x];
     __syncthreads();
-    line_fft<T, N, false>(s, s2, n, t, a.tw);
+    line_fft<C, N, false>(s, s2, n, t, a.tw);
     if (exists)
-        for (int x = t; x < n; x += TPL) row[x] = s[x];
+        for (int x = t; x < n; x += TPL) row[x] = ccast<S>(s[x]);
+}
+
+// ---------------------------------------------------------------------------
+// register-FFT versions of the passes (power-of-two lines)
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+struct RegE {
+    // 16 points per thread: a line of <= 256 points needs a single LDS exchange
+    static constexpr int E = 16;
+    static constexpr int TT = N / E;
+    static constexpr int LD = N + N / E;
+    // Column-pass line stride.  The threads of a wave span 16 columns, so the
+    // stride decides how the columns' lines fall on the LDS banks; LD itself
+    // (a multiple of the bank row) puts all 16 on the same banks.  Measured on
+    // gfx950 (SQ_LDS_BANK_CONFLICT per active LDS cycle, 256-point columns):
+    // LD 9.5, LD+1 0.56, LD+2 1.1, LD+3 0.56, LD+4 3.8, LD+8 7.3 -> odd stride.
+    static constexpr int LDC = LD + 1;
+};
+
+// Aperture bits of the E points x = t + TT m of row y: bit m set when the
+// point lies inside (all coordinate loads are issued together).
+template <int E, int TT>
+__device__ __forceinline__ uint32_t aperture_bits(const double* __restrict__ X, double yy, double R2, int t) {
+    double xs[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) xs[m] = X[t + TT * m];
+    uint32_t bits = 0;
+#pragma unroll
+    for (int m = 0; m < E; m++) bits |= (in_aperture(xs[m], yy, R2) ? 1u : 0u) << m;
+    return bits;
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(512) k_rows_reg(RowArgs<T> a) {
+    BF_TYPES(T);
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int E = RegE<T, N>::E, TT = RegE<T, N>::TT, LD = RegE<T, N>::LD;
+    cplx<S>* smem = reinterpret_cast<cplx<S>*>(smem_raw);
+    const int lpb = blockDim.x / TT;
+    const int li = threadIdx.x / TT, t = threadIdx.x % TT;
+    cplx<S>* s = smem + (size_t)li * LD;
+    const long nlines = (long)a.nf * a.ny;
+    const long line = (long)blockIdx.x * lpb + li;
+    bool valid = line < nlines;
+    const int j = valid ? (int)(line / a.ny) : 0;
+    const int y = valid ? (int)(line % a.ny) : 0;
+    const FieldDesc d = a.fd[j];
+    if (valid && a.active && !a.active[d.group]) valid = false;
+    const uint32_t K = (uint32_t)N * (uint32_t)a.ny;
+    const auto rf = buf_rsrc(a.base + d.off, K * (uint32_t)sizeof(cplx<S>));
+    const uint32_t o0 = (uint32_t)y * N + t;
+    cplx<C> v[E];
+    {
+        cplx<S> raw[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) raw[m] = buf_load<S>(rf, valid ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB);
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
+    }
+    if (a.flags & RF_INV) fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    if ((a.flags & RF_RTAB) && d.rtab >= 0) {
+        const auto rt = buf_rsrc(a.rtab + (size_t)d.rtab * K, K * (uint32_t)sizeof(cplx<C>));
+        cplx<C> tab[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) tab[m] = buf_load<C>(rt, (o0 + TT * m) * (uint32_t)sizeof(cplx<C>));
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
+    }
+    if (a.flags & RF_MASK) {
+        const uint32_t keep = aperture_bits<E, TT>(a.X, a.Y[y], a.R2, t);
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            if (!((keep >> m) & 1u)) v[m] = cmk<C>(0, 0);
+    }
+    if (a.flags & RF_FWD) fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        buf_store<S>(rf, valid ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB, ccast<S>(v[m]));
+}
+
+// FL != 0: the flags are fixed at compile time (the dancer's FWD|KMID|INV).
+template <typename T, int N, uint32_t FL>
+__global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
+    BF_TYPES(T);
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int E = RegE<T, N>::E, TT = RegE<T, N>::TT, LDC = RegE<T, N>::LDC;
+    cplx<S>* smem = reinterpret_cast<cplx<S>*>(smem_raw);
+    const int jf = blockIdx.y;
+    const FieldDesc d = a.fd[jf];
+    if (a.active && !a.active[d.group]) return;  // uniform over the block
+    const uint32_t fl = FL ? FL : a.flags;
+    const int tx = a.tx;
+    const int c = threadIdx.x % tx, t = threadIdx.x / tx;
+    const int nx = a.nx;
+    const int mx = blockIdx.x * tx + c;
+    const bool ok = mx < nx;
+    const uint32_t K = (uint32_t)N * (uint32_t)nx;
+    const auto rs = buf_rsrc(a.src + d.off, K * (uint32_t)sizeof(cplx<S>));
+    const auto rd = buf_rsrc(a.dst + d.off, K * (uint32_t)sizeof(cplx<S>));
+    const bool has_kt = d.ktab >= 0;
+    const auto rk = buf_rsrc(a.ktab + (size_t)(has_kt ? d.ktab : 0) * K, K * (uint32_t)sizeof(cplx<C>));
+    const bool whole = store_whole(d);
+    const int sx = (fl & CF_INSHIFT) ? (mx + nx / 2) % nx : mx;
+    cplx<S>* s = smem + (size_t)c * LDC;
+    cplx<C> v[E];
+    {
+        uint32_t off[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int my = t + TT * m;
+            const int sy = (fl & CF_INSHIFT) ? ((my + N / 2) & (N - 1)) : my;
+            off[m] = (ok && in_band(a, sy)) ? (uint32_t)(sy * nx + sx) : BUF_OOB;
+        }
+        cplx<S> raw[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = buf_load<S>(rs, off[m] == BUF_OOB ? BUF_OOB : off[m] * (uint32_t)sizeof(cplx<S>));
+        if ((fl & CF_KLOAD) && has_kt) {
+            cplx<C> k[E];
+#pragma unroll
+            for (int m = 0; m < E; m++)
+                k[m] = buf_load<C>(rk, off[m] == BUF_OOB ? BUF_OOB : off[m] * (uint32_t)sizeof(cplx<C>));
+#pragma unroll
+            for (int m = 0; m < E; m++) v[m] = cmul(ccast<C>(raw[m]), k[m]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
+        }
+    }
+    if (fl & CF_FWD) fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+    if ((fl & CF_KMID) && has_kt) {
+        // in chunks of KCH loads: bounds the registers held by loads in flight
+        constexpr int KCH = sizeof(C) == 8 ? 8 : E;
+#pragma unroll
+        for (int m0 = 0; m0 < E; m0 += KCH) {
+            cplx<C> k[KCH];
+#pragma unroll
+            for (int q = 0; q < KCH; q++)
+                k[q] = buf_load<C>(rk, ok ? (uint32_t)((t + TT * (m0 + q)) * nx + mx) * (uint32_t)sizeof(cplx<C>)
+                                          : BUF_OOB);
+#pragma unroll
+            for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
+        }
+    }
+    if (fl & CF_INV) fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    const int dx = (fl & CF_OUTSHIFT) ? (mx + nx / 2) % nx : mx;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        int dy = t + TT * m;
+        if (fl & CF_OUTSHIFT) dy = (dy + N / 2) & (N - 1);
+        const bool st = ok && (whole || in_band(a, dy));
+        buf_store<S>(rd, st ? (uint32_t)(dy * nx + dx) * (uint32_t)sizeof(cplx<S>) : BUF_OOB, ccast<S>(v[m]));
+    }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
+    BF_TYPES(T);
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int E = RegE<T, N>::E, TT = RegE<T, N>::TT, LD = RegE<T, N>::LD;
+    cplx<S>* smem = reinterpret_cast<cplx<S>*>(smem_raw);
+    __shared__ double red[32];
+    const int f = blockIdx.y;
+    int b, tile;
+    bool exit_only;
+    dancer_block(a, blockIdx.x, b, tile, exit_only);
+    if (a.active && !a.active[f]) return;  // uniform over the block
+    if (a.count && blockIdx.x == 0 && threadIdx.x == 0) a.count[f] += 1;
+    const int lpb = blockDim.x / TT;  // 2 * rows per block
+    const int li = threadIdx.x / TT, t = threadIdx.x % TT;
+    const int side = li & 1;  // 0: A = region b-1 right-moving, 1: B = region b left-moving
+    const int y = tile * (lpb >> 1) + (li >> 1);
+    const int reg = side == 0 ? b - 1 : b;
+    const int slot = side == 0 ? a.rm : 1 - a.rm;
+    const bool exists = reg >= 0 && reg < a.R && y < a.ny;
+    const int yc = y < a.ny ? y : a.ny - 1;
+    const uint32_t K = (uint32_t)N * (uint32_t)a.ny;
+    cplx<S>* s = smem + (size_t)li * LD;
+    cplx<S>* sp = smem + (size_t)(li ^ 1) * LD;  // partner line, same y
+    const auto rf = buf_rsrc(a.fields + (((size_t)f * a.R + (exists ? reg : 0)) * 2 + slot) * K,
+                             K * (uint32_t)sizeof(cplx<S>));
+    const uint32_t o0 = (uint32_t)yc * N + t;
+    cplx<C> v[E];
+    {
+        cplx<S> raw[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            raw[m] = buf_load<S>(rf, exists ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB);
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
+    }
+    // next bounce's aperture (bit m: keep point t + TT m)
+    const uint32_t keep = (a.flags & DF_FWD) ? aperture_bits<E, TT>(a.X, a.Y[yc], a.R2, t) : ~0u;
+    fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    if (a.rtab) {
+        const int ti = exists ? a.rtab_idx[f * a.R + reg] : -1;
+        const auto rt = buf_rsrc(a.rtab + (size_t)(ti >= 0 ? ti : 0) * K, K * (uint32_t)sizeof(cplx<C>));
+        cplx<C> tab[E];
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            tab[m] = buf_load<C>(rt, ti >= 0 ? (o0 + TT * m) * (uint32_t)sizeof(cplx<C>) : BUF_OOB);
+        if (ti >= 0)
+#pragma unroll
+            for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
+    }
+    // exchange with the partner line through LDS (the FFT's buffers are free now)
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) s[t + TT * m] = ccast<S>(v[m]);
+    __syncthreads();
+    // boundary update (src/boost_fractor_recurprop.jl:90-119) as own*c_own + partner*c_w:
+    //   side 0 (A):  b < R: A' = r A + (1-r) B;    b = R: A' = r A, Eout += (1+r) A
+    //   side 1 (B):  b > 0: B' = -r B + (1+r) A;   b = 0: B' = -r B
+    // c_w = 0 exactly where the partner line does not exist (its values are zero).
+    const zd r = a.coef[b];
+    cplx<C> cown, cw;
+    if (side == 0) {
+        cown = cmk<C>((C)r.re, (C)r.im);
+        cw = (b == a.R) ? cmk<C>(0, 0) : cmk<C>((C)(1.0 - r.re), (C)(-r.im));
+    } else {
+        cown = cmk<C>((C)(-r.re), (C)(-r.im));
+        cw = (b == 0) ? cmk<C>(0, 0) : cmk<C>((C)(1.0 + r.re), (C)r.im);
+    }
+    if (b == a.R) {  // exit boundary (uniform over the block): Eout += (1+r) * A on the A lines
+        const bool eo_line = exists && side == 0;
+        const auto re = buf_rsrc(a.Eout + (size_t)f * a.eout_fstride + (size_t)a.slice * K, K * (uint32_t)sizeof(zd));
+        const bool assign = (a.flags & DF_EOUT_ASSIGN) != 0;
+        const zd opr = cmk<double>(1.0 + r.re, r.im);
+        constexpr int ECH = 4;  // exit blocks are few: keep their extra registers low
+#pragma unroll
+        for (int m0 = 0; m0 < E; m0 += ECH) {
+            zd prev[ECH];
+#pragma unroll
+            for (int q = 0; q < ECH; q++)
+                prev[q] = buf_load<double>(re, (eo_line && !assign) ? (o0 + TT * (m0 + q)) * (uint32_t)sizeof(zd)
+                                                                    : BUF_OOB);
+#pragma unroll
+            for (int q = 0; q < ECH; q++)
+                buf_store<double>(re, eo_line ? (o0 + TT * (m0 + q)) * (uint32_t)sizeof(zd) : BUF_OOB,
+                                  cadd(prev[q], cmul(opr, ccast<double>(v[m0 + q]))));
+        }
+    }
+    cplx<S> wp[E];  // partner values, all LDS reads issued together
+#pragma unroll
+    for (int m = 0; m < E; m++) wp[m] = sp[t + TT * m];
+    double vmax = 0.0;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const cplx<C> w = ccast<C>(wp[m]);
+        cplx<C> nv = exists ? cadd(cmul(v[m], cown), cmul(cw, w)) : cmk<C>(0, 0);
+        if (a.flags & DF_AMAX) vmax = fmax(vmax, hypot((double)nv.re, (double)nv.im));
+        if (!((keep >> m) & 1u)) nv = cmk<C>(0, 0);
+        v[m] = nv;
+    }
+    if (a.flags & DF_AMAX) {
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) mm = fmax(mm, red[w]);
+            if (mm > 0.0) atomic_max_pos(a.amax + f, mm);
+        }
+    }
+    if (!(a.flags & DF_FWD) || exit_only) return;  // only Eout is needed
+    fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        buf_store<S>(rf, exists ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB, ccast<S>(v[m]));
 }
 
 // pointwise bounce update for the unfused propagators (propagator1D, momentum space)
 template <typename T>
-__global__ void k_dancer_update(cplx<T>* fields, int R, size_t K, int rm, const zd* __restrict__ coef, zd* Eout,
-                                size_t eout_fstride, int slice, int assign, double* amax, const int* active,
-                                int* count) {
+__global__ void k_dancer_update(cplx<typename Storage<T>::type>* fields, int R, size_t K, int rm,
+                                const zd* __restrict__ coef, zd* Eout, size_t eout_fstride, int slice, int assign,
+                                double* amax, const int* active, int* count) {
+    BF_TYPES(T);
     const int f = blockIdx.z, b = blockIdx.y;
     if (active && !active[f]) return;
     if (count && blockIdx.x == 0 && b == 0 && threadIdx.x == 0) count[f] += 1;
     const zd r = coef[b];
-    const cplx<T> rT = cmk<T>((T)r.re, (T)r.im), mrT = cmk<T>((T)(-r.re), (T)(-r.im));
-    const cplx<T> omrT = cmk<T>((T)(1.0 - r.re), (T)(-r.im)), oprT = cmk<T>((T)(1.0 + r.re), (T)r.im);
-    cplx<T>* A = (b >= 1) ? fields + (((size_t)f * R + (b - 1)) * 2 + rm) * K : nullptr;
-    cplx<T>* B = (b < R) ? fields + (((size_t)f * R + b) * 2 + (1 - rm)) * K : nullptr;
+    const cplx<C> rT = cmk<C>((C)r.re, (C)r.im), mrT = cmk<C>((C)(-r.re), (C)(-r.im));
+    const cplx<C> omrT = cmk<C>((C)(1.0 - r.re), (C)(-r.im)), oprT = cmk<C>((C)(1.0 + r.re), (C)r.im);
+    cplx<S>* A = (b >= 1) ? fields + (((size_t)f * R + (b - 1)) * 2 + rm) * K : nullptr;
+    cplx<S>* B = (b < R) ? fields + (((size_t)f * R + b) * 2 + (1 - rm)) * K : nullptr;
     double vmax = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
         if (b == 0) {
-            B[i] = cmul(B[i], mrT);
-            vmax = fmax(vmax, hypot((double)B[i].re, (double)B[i].im));
+            const cplx<C> nb = cmul(ccast<C>(B[i]), mrT);
+            B[i] = ccast<S>(nb);
+            vmax = fmax(vmax, hypot((double)nb.re, (double)nb.im));
         } else if (b == R) {
-            const zd e = cmul(cmk<double>(1.0 + r.re, r.im), ccast<double>(A[i]));
+            const cplx<C> va = ccast<C>(A[i]);
+            const zd e = cmul(cmk<double>(1.0 + r.re, r.im), ccast<double>(va));
             zd* eo = Eout + (size_t)f * eout_fstride + (size_t)slice * K + i;
             *eo = assign ? e : cadd(*eo, e);
-            A[i] = cmul(A[i], rT);
-            vmax = fmax(vmax, hypot((double)A[i].re, (double)A[i].im));
+            const cplx<C> na = cmul(va, rT);
+            A[i] = ccast<S>(na);
+            vmax = fmax(vmax, hypot((double)na.re, (double)na.im));
         } else {
-            const cplx<T> va = A[i], vb = B[i];
-            const cplx<T> na = cadd(cmul(va, rT), cmul(omrT, vb));
-            const cplx<T> nb = cadd(cmul(vb, mrT), cmul(oprT, va));
-            A[i] = na;
-            B[i] = nb;
+            const cplx<C> va = ccast<C>(A[i]), vb = ccast<C>(B[i]);
+            const cplx<C> na = cadd(cmul(va, rT), cmul(omrT, vb));
+            const cplx<C> nb = cadd(cmul(vb, mrT), cmul(oprT, va));
+            A[i] = ccast<S>(na);
+            B[i] = ccast<S>(nb);
             vmax = fmax(vmax, fmax(hypot((double)na.re, (double)na.im), hypot((double)nb.re, (double)nb.im)));
         }
     }
@@ -391,28 +723,33 @@ __global__ void k_amin_check(int* active, double* amax, double amin, int fb) {
 
 // dance_intro fill (src/boost_fractor_recurprop.jl:153-201): fields[f][n][s] = ax[n][s] * [x^2+y^2 <= R^2]
 template <typename T>
-__global__ void k_dance_intro(cplx<T>* fields, int nd, size_t K, int nx, const zd* __restrict__ ax,
-                              const double* __restrict__ X, const double* __restrict__ Y, double R2,
-                              size_t fstride) {
+__global__ void k_dance_intro(cplx<typename Storage<T>::type>* fields, int nd, size_t K, int nx,
+                              const zd* __restrict__ ax, const double* __restrict__ X, const double* __restrict__ Y,
+                              double R2, size_t fstride) {
+    BF_TYPES(T);
+    (void)sizeof(C);
     const int f = blockIdx.z, q = blockIdx.y;  // q = region*2 + slot
     const zd v = ax[q];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
         const double xx = X[i % nx], yy = Y[i / nx];
         const double bb = (xx * xx + yy * yy > R2) ? 0.0 : 1.0;
-        fields[(size_t)f * fstride + (size_t)q * K + i] = cmk<T>((T)(bb * v.re), (T)(bb * v.im));
+        fields[(size_t)f * fstride + (size_t)q * K + i] = cmk<S>((S)(bb * v.re), (S)(bb * v.im));
     }
 }
 
 // broadcast a double field block [nq][K] into every frequency slot of a T buffer
 template <typename T>
-__global__ void k_broadcast(cplx<T>* dst, const zd* __restrict__ src, size_t n, size_t fstride) {
+__global__ void k_broadcast(cplx<typename Storage<T>::type>* dst, const zd* __restrict__ src, size_t n,
+                            size_t fstride) {
+    BF_TYPES(T);
+    (void)sizeof(C);
     const int f = blockIdx.y;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[(size_t)f * fstride + i] = cmk<T>((T)src[i].re, (T)src[i].im);
+        dst[(size_t)f * fstride + i] = cmk<S>((S)src[i].re, (S)src[i].im);
 }
 
 template <typename T>
-__global__ void k_to_double(zd* dst, const cplx<T>* __restrict__ src, size_t n) {
+__global__ void k_to_double(zd* dst, const cplx<typename Storage<T>::type>* __restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = ccast<double>(src[i]);
 }
@@ -511,17 +848,36 @@ static void dispatch_n(int n, A&&... args) {
     }
 }
 
-template <int N>
-static int tpl_of() { return LineCfg<N>::TPL; }
+static bool use_reg_fft() {
+    static int v = -1;
+    if (v < 0) {
+        const char* s = getenv("BF_FFT");
+        v = (s && std::string(s) == "lds") ? 0 : 1;
+    }
+    return v == 1;
+}
 
 template <typename T, int N>
 struct RowLaunch {
     static void run(const RowArgs<T>& a, hipStream_t s) {
+        if constexpr (N >= 16) {
+            if (use_reg_fft()) {
+                using C = RegE<T, N>;
+                const int lpb = std::max(1, 256 / C::TT);
+                const long nlines = (long)a.nf * a.ny;
+                const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * lpb;
+                allow_lds((const void*)k_rows_reg<T, N>, smem);
+                hipLaunchKernelGGL((k_rows_reg<T, N>), dim3((unsigned)((nlines + lpb - 1) / lpb)), dim3(lpb * C::TT),
+                                   smem, s, a);
+                BF_CHECK_LAUNCH();
+                return;
+            }
+        }
         const int TPL = LineCfg<N>::TPL;
         const int n = N >= 16 ? N : a.nx;
         const int lpb = std::max(1, 256 / TPL);
         const long nlines = (long)a.nf * a.ny;
-        const size_t smem = sizeof(cplx<T>) * (size_t)n * lpb * (N >= 16 ? 1 : 2);
+        const size_t smem = sizeof(cplx<typename Compute<T>::type>) * (size_t)n * lpb * (N >= 16 ? 1 : 2);
         allow_lds((const void*)k_rows<T, N>, smem);
         hipLaunchKernelGGL((k_rows<T, N>), dim3((unsigned)((nlines + lpb - 1) / lpb)), dim3(lpb * TPL), smem, s, a);
         BF_CHECK_LAUNCH();
@@ -531,30 +887,78 @@ struct RowLaunch {
 template <typename T, int N>
 struct ColLaunch {
     static void run(ColArgs<T> a, int tx_req, hipStream_t s) {
+        if constexpr (N >= 16) {
+            if (use_reg_fft()) {
+                using C = RegE<T, N>;
+                int tx = tx_req > 0 ? tx_req : (int)(128 / sizeof(cplx<typename Storage<T>::type>));
+                tx = std::max(1, std::min(tx, 512 / C::TT));
+                while (tx > 1 && sizeof(cplx<typename Storage<T>::type>) * (size_t)tx * C::LDC > 144 * 1024) tx >>= 1;
+                a.tx = tx;
+                const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)tx * C::LDC;
+                constexpr uint32_t PROP = CF_FWD | CF_KMID | CF_INV;
+                if (a.flags == PROP) {
+                    allow_lds((const void*)k_cols_reg<T, N, PROP>, smem);
+                    hipLaunchKernelGGL((k_cols_reg<T, N, PROP>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * C::TT),
+                                       smem, s, a);
+                } else {
+                    allow_lds((const void*)k_cols_reg<T, N, 0>, smem);
+                    hipLaunchKernelGGL((k_cols_reg<T, N, 0>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * C::TT),
+                                       smem, s, a);
+                }
+                BF_CHECK_LAUNCH();
+                return;
+            }
+        }
         const int TPL = LineCfg<N>::TPL;
         const int n = N >= 16 ? N : a.ny;
-        int tx = tx_req > 0 ? tx_req : (int)(128 / sizeof(cplx<T>));
+        int tx = tx_req > 0 ? tx_req : (int)(128 / sizeof(cplx<typename Storage<T>::type>));
         tx = std::max(1, std::min(tx, 1024 / TPL));
-        while (tx > 1 && sizeof(cplx<T>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2) > 96 * 1024) tx >>= 1;
+        while (tx > 1 && sizeof(cplx<typename Compute<T>::type>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2) > 96 * 1024) tx >>= 1;
         a.tx = tx;
-        const size_t smem = sizeof(cplx<T>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2);
+        const size_t smem = sizeof(cplx<typename Compute<T>::type>) * (size_t)tx * (n + 1) * (N >= 16 ? 1 : 2);
         allow_lds((const void*)k_cols<T, N>, smem);
         hipLaunchKernelGGL((k_cols<T, N>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * TPL), smem, s, a);
         BF_CHECK_LAUNCH();
     }
 };
 
+template <typename T>
+static unsigned dancer_tiles(DancerRowArgs<T>& a, int rows) {
+    a.ntiles = (a.ny + rows - 1) / rows;
+    if (a.yhi > a.ylo) {
+        a.tile0 = a.ylo / rows;
+        a.nband = (a.yhi + rows - 1) / rows - a.tile0;
+    } else {
+        a.tile0 = 0;
+        a.nband = 0;
+    }
+    return (unsigned)(a.nband * (a.R + 1) + (a.ntiles - a.nband));
+}
+
 template <typename T, int N>
 struct DancerRowLaunch {
-    static void run(const DancerRowArgs<T>& a, int fb, int rows_req, hipStream_t s) {
+    static void run(DancerRowArgs<T> a, int fb, int rows_req, hipStream_t s) {
+        if constexpr (N >= 16) {
+            if (use_reg_fft()) {
+                using C = RegE<T, N>;
+                int rows = rows_req > 0 ? rows_req : std::max(1, 128 / C::TT);
+                rows = std::max(1, std::min(rows, 256 / C::TT));
+                const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * (2 * rows);
+                const unsigned nblk = dancer_tiles(a, rows);
+                allow_lds((const void*)k_dancer_rows_reg<T, N>, smem);
+                hipLaunchKernelGGL((k_dancer_rows_reg<T, N>), dim3(nblk, fb), dim3(2 * rows * C::TT), smem, s, a);
+                BF_CHECK_LAUNCH();
+                return;
+            }
+        }
         const int TPL = LineCfg<N>::TPL;
         const int n = N >= 16 ? N : a.nx;
         int rows = rows_req > 0 ? rows_req : std::max(1, 128 / TPL);
         rows = std::max(1, std::min(rows, 512 / TPL));
-        const size_t smem = sizeof(cplx<T>) * (size_t)n * (2 * rows) * 2;
+        const size_t smem = sizeof(cplx<typename Compute<T>::type>) * (size_t)n * (2 * rows) * 2;
+        const unsigned nblk = dancer_tiles(a, rows);
         allow_lds((const void*)k_dancer_rows<T, N>, smem);
-        hipLaunchKernelGGL((k_dancer_rows<T, N>), dim3((a.ny + rows - 1) / rows, a.R + 1, fb), dim3(2 * rows * TPL),
-                           smem, s, a);
+        hipLaunchKernelGGL((k_dancer_rows<T, N>), dim3(nblk, fb), dim3(2 * rows * TPL), smem, s, a);
         BF_CHECK_LAUNCH();
     }
 };
@@ -574,39 +978,61 @@ struct RegionParams {
 
 template <typename T>
 struct PropEngine {
+    BF_TYPES(T);
     bf_ctx* ctx;
     const Grid* g;
     int kind;
     double R2;
-    const cplx<T>* twx;
-    const cplx<T>* twy;
+    const cplx<C>* twx;
+    const cplx<C>* twy;
     const zd* surfaces;  // device, may be null
     DevBuf ktab, rtab, kdesc, rdesc, scratch;
-    int ntab = 0;
-    std::vector<int> rtab_of;  // per table: rtab index or -1
+    int ntab = 0;              // k-space tables built
+    std::vector<int> ktab_of;  // per region entry: k-space table index
+    std::vector<int> rtab_of;  // per region entry: rtab index or -1
     int col_tx = 0;            // autotuned column tile (0: default)
 
     PropEngine(bf_ctx* c, const Grid* gr, int k, double diskR, const zd* surf)
         : ctx(c), g(gr), kind(k), R2(diskR * diskR), surfaces(surf) {
-        twx = twiddle_table<T>(ctx, g->nx);
-        twy = twiddle_table<T>(ctx, g->ny);
+        twx = twiddle_table<C>(ctx, g->nx);
+        twy = twiddle_table<C>(ctx, g->ny);
     }
 
-    // Build the k-space and real-space tables for a list of regions.
-    void build_tables(const std::vector<RegionParams>& regs) {
+    // Build the k-space and real-space tables for a list of regions.  With
+    // dedup, regions whose k-space tables would be bit-identical (same k0^2,
+    // dz and layout, e.g. every disk of a booster) share one table; ktab_of
+    // maps region entries to tables either way.
+    void build_tables(const std::vector<RegionParams>& regs, bool dedup = false) {
         const size_t K = g->K();
-        ntab = (int)regs.size();
-        std::vector<KTabDesc> kd(ntab);
+        const int nreg = (int)regs.size();
+        std::vector<KTabDesc> kd;
         std::vector<RTabDesc> rd;
-        rtab_of.assign(ntab, -1);
-        for (int i = 0; i < ntab; i++) {
+        rtab_of.assign(nreg, -1);
+        ktab_of.assign(nreg, -1);
+        std::map<std::array<uint64_t, 4>, int> seen;
+        for (int i = 0; i < nreg; i++) {
             const RegionParams& p = regs[i];
             zd eps = p.eps;
             if (kind == BF_PROP_PROPAGATOR && eps.im == 0.0) eps.im = -0.0;  // src/boost_fractor.jl:140
             const zd k0 = k0_of(eps, p.lambda);
-            kd[i].k0sq = cmul(k0, k0);
-            kd[i].dz = p.dz;
-            kd[i].shifted = (kind == BF_PROP_MOMENTUM) ? 1 : 0;
+            KTabDesc d;
+            d.k0sq = cmul(k0, k0);
+            d.dz = p.dz;
+            d.shifted = (kind == BF_PROP_MOMENTUM) ? 1 : 0;
+            d.pad = 0;
+            std::array<uint64_t, 4> key;
+            std::memcpy(&key[0], &d.k0sq.re, 8);
+            std::memcpy(&key[1], &d.k0sq.im, 8);
+            std::memcpy(&key[2], &d.dz, 8);
+            key[3] = (uint64_t)d.shifted;
+            auto it = dedup ? seen.find(key) : seen.end();
+            if (it != seen.end()) {
+                ktab_of[i] = it->second;
+            } else {
+                ktab_of[i] = (int)kd.size();
+                if (dedup) seen.emplace(key, ktab_of[i]);
+                kd.push_back(d);
+            }
             bool need_r = false;
             if (kind == BF_PROP_PROPAGATOR) need_r = (p.tx != 0.0 || p.ty != 0.0 || p.surf >= 0);
             if (kind == BF_PROP_MOMENTUM) need_r = (p.tx != 0.0 || p.ty != 0.0);  // :201 tilt only
@@ -620,21 +1046,22 @@ struct PropEngine {
                 rd.push_back(r);
             }
         }
+        ntab = (int)kd.size();
         if (kind == BF_PROP_1D) return;
         const double scale = 1.0 / (double)((int64_t)g->nx * g->ny);  // ifft normalisation folded in
-        if (ktab.bytes < sizeof(cplx<T>) * K * ntab) ktab.alloc(sizeof(cplx<T>) * K * ntab);
+        if (ktab.bytes < sizeof(cplx<C>) * K * ntab) ktab.alloc(sizeof(cplx<C>) * K * ntab);
         if (kdesc.bytes < sizeof(KTabDesc) * ntab) kdesc.alloc(sizeof(KTabDesc) * ntab);
         upload(kdesc.p, kd.data(), sizeof(KTabDesc) * ntab, ctx->stream);
-        hipLaunchKernelGGL(k_build_ktab<T>, dim3(grid1d(K, 256, 1024), ntab), dim3(256), 0, ctx->stream,
-                           ktab.as<cplx<T>>(), kdesc.as<KTabDesc>(), g->nx, g->ny, g->kX.as<double>(),
+        hipLaunchKernelGGL(k_build_ktab<C>, dim3(grid1d(K, 256, 1024), ntab), dim3(256), 0, ctx->stream,
+                           ktab.as<cplx<C>>(), kdesc.as<KTabDesc>(), g->nx, g->ny, g->kX.as<double>(),
                            g->kY.as<double>(), scale);
         BF_CHECK_LAUNCH();
         if (!rd.empty()) {
-            if (rtab.bytes < sizeof(cplx<T>) * K * rd.size()) rtab.alloc(sizeof(cplx<T>) * K * rd.size());
+            if (rtab.bytes < sizeof(cplx<C>) * K * rd.size()) rtab.alloc(sizeof(cplx<C>) * K * rd.size());
             if (rdesc.bytes < sizeof(RTabDesc) * rd.size()) rdesc.alloc(sizeof(RTabDesc) * rd.size());
             upload(rdesc.p, rd.data(), sizeof(RTabDesc) * rd.size(), ctx->stream);
-            hipLaunchKernelGGL(k_build_rtab<T>, dim3(grid1d(K, 256, 1024), (unsigned)rd.size()), dim3(256), 0,
-                               ctx->stream, rtab.as<cplx<T>>(), rdesc.as<RTabDesc>(), g->nx, g->ny, g->X.as<double>(),
+            hipLaunchKernelGGL(k_build_rtab<C>, dim3(grid1d(K, 256, 1024), (unsigned)rd.size()), dim3(256), 0,
+                               ctx->stream, rtab.as<cplx<C>>(), rdesc.as<RTabDesc>(), g->nx, g->ny, g->X.as<double>(),
                                g->Y.as<double>(), surfaces);
             BF_CHECK_LAUNCH();
         }
@@ -643,22 +1070,25 @@ struct PropEngine {
     bool has_rtab() const { return rtab.p != nullptr && !rtab_of.empty() &&
                                    std::any_of(rtab_of.begin(), rtab_of.end(), [](int v) { return v >= 0; }); }
 
-    void rows(cplx<T>* base, const FieldDesc* fd, int nf, uint32_t flags, const int* active) {
+    void rows(cplx<S>* base, const FieldDesc* fd, int nf, uint32_t flags, const int* active) {
         RowArgs<T> a;
         a.base = base; a.fd = fd; a.nf = nf; a.nx = g->nx; a.ny = g->ny; a.tw = twx; a.flags = flags;
-        a.rtab = rtab.as<cplx<T>>(); a.X = g->X.as<double>(); a.Y = g->Y.as<double>(); a.R2 = R2; a.active = active;
+        a.rtab = rtab.as<cplx<C>>(); a.X = g->X.as<double>(); a.Y = g->Y.as<double>(); a.R2 = R2; a.active = active;
         dispatch_n<RowLaunch, T>(g->nx, a, ctx->stream);
     }
-    void cols(const cplx<T>* src, cplx<T>* dst, const FieldDesc* fd, int nf, uint32_t flags, const int* active) {
+    // ylo/yhi: row band (yhi < 0: all rows)
+    void cols(const cplx<S>* src, cplx<S>* dst, const FieldDesc* fd, int nf, uint32_t flags, const int* active,
+              int ylo = 0, int yhi = -1) {
         ColArgs<T> a;
         a.src = src; a.dst = dst; a.fd = fd; a.nf = nf; a.nx = g->nx; a.ny = g->ny; a.tw = twy; a.flags = flags;
-        a.ktab = ktab.as<cplx<T>>(); a.tx = 0; a.active = active;
+        a.ktab = ktab.as<cplx<C>>(); a.tx = 0; a.active = active;
+        a.ylo = ylo; a.yhi = yhi < 0 ? g->ny : yhi;
         dispatch_n<ColLaunch, T>(g->ny, a, col_tx, ctx->stream);
     }
 
     // Full propagation of nf fields (field j uses tables fd[j].ktab / fd[j].rtab).
     // `scr` must hold as many elements as `base` spans (momentum-space kind only).
-    void propagate(cplx<T>* base, const FieldDesc* fd, int nf, const zd* ph1d, cplx<T>* scr, const int* active) {
+    void propagate(cplx<S>* base, const FieldDesc* fd, int nf, const zd* ph1d, cplx<S>* scr, const int* active) {
         const size_t K = g->K();
         switch (kind) {
             case BF_PROP_PROPAGATOR:
@@ -673,7 +1103,7 @@ struct PropEngine {
                 cols(scr, base, fd, nf, CF_FWD | CF_OUTSHIFT, active);
                 break;
             default:
-                hipLaunchKernelGGL(k_scale_fields<T>, dim3(grid1d(K, 256, 256), nf), dim3(256), 0, ctx->stream, base,
+                hipLaunchKernelGGL(k_scale_fields<T>, dim3(grid1d(K, 256, 256), nf), dim3(256), 0, ctx->stream, (S*)base,
                                    fd, ph1d, K, active);
                 BF_CHECK_LAUNCH();
                 break;
@@ -773,6 +1203,7 @@ void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double*
 // ---------------------------------------------------------------------------
 template <typename T>
 struct DancerPlan : DancerPlanBase {
+    BF_TYPES(T);
     Grid g;
     int kind, R, nmax, nfreq, fb, K_;
     double amin, diskR;
@@ -780,11 +1211,13 @@ struct DancerPlan : DancerPlanBase {
     std::vector<zd> eps;  // trimmed: regions 1..nreg-1 of the setup
     std::vector<double> dist, tx, ty;
     bool has_surf = false;
-    DevBuf surf, coef, fields, scratch, Eout, hist, power, active, amax, count, fdesc, ridx, ax, emit, reflect, ph1d;
+    DevBuf surf, coef, fields, scratch, Eout, hist, power, active, amax, count, fdesc, fdesc_col, ridx, ax, emit,
+        reflect, ph1d;
     std::unique_ptr<PropEngine<T>> pe;
     DancerOptions opt;
     int nslices;
     std::vector<FieldDesc> fd_host;
+    int band_lo = 0, band_hi = 0;
     // timing: HIP events recorded around every launch of the row and column
     // passes on the plan's stream, reduced once per frequency batch
     bool timing = false;
@@ -858,7 +1291,7 @@ struct DancerPlan : DancerPlanBase {
         coef.alloc(sizeof(zd) * (R + 1));
         upload(coef.p, r.data(), sizeof(zd) * (R + 1), s);
         // frequency batch: keep the fields of a batch around 192 MB (Infinity Cache sized)
-        const size_t per = sizeof(cplx<T>) * K * 2 * R;
+        const size_t per = sizeof(cplx<S>) * K * 2 * R;
         fb = opt.freq_batch > 0 ? opt.freq_batch : (int)std::max<size_t>(1, (192u << 20) / std::max<size_t>(per, 1));
         fb = std::max(1, std::min(fb, std::min(nfreq, 64)));
         nslices = opt.immediatesum ? 1 : nmax + 1;
@@ -883,6 +1316,7 @@ struct DancerPlan : DancerPlanBase {
                 d.pad = 0;
             }
         fdesc.alloc(sizeof(FieldDesc) * fd.size());
+        fdesc_col.alloc(2 * sizeof(FieldDesc) * fd.size());
         upload(fdesc.p, fd.data(), sizeof(FieldDesc) * fd.size(), s);
         fd_host = fd;
         ridx.alloc(sizeof(int) * (size_t)fb * R);
@@ -900,6 +1334,26 @@ struct DancerPlan : DancerPlanBase {
         }
         pe.reset(new PropEngine<T>(ctx, &g, kind, diskR, surf.as<zd>()));
         pe->col_tx = opt.col_tx;
+        // Row band of the aperture: rows y with Y[y]^2 + min_x X[x]^2 < diskR^2
+        // (a relative margin of 1e-9 keeps every row the device mask could keep).
+        // The mask zeroes every other row before each propagation, so the fused
+        // bounce skips them (except for the exit field's Eout).  The amin
+        // criterion reads every row, so it keeps the full range.
+        band_lo = 0;
+        band_hi = g.ny;
+        if (opt.band && amin == 0.0) {
+            double minx2 = INFINITY;
+            for (int x = 0; x < g.nx; x++) minx2 = std::min(minx2, co->X[x] * co->X[x]);
+            const double lim = diskR * diskR * (1.0 + 1e-9);
+            int lo = g.ny, hi = 0;
+            for (int y = 0; y < g.ny; y++)
+                if (co->Y[y] * co->Y[y] + minx2 < lim) {
+                    lo = std::min(lo, y);
+                    hi = y + 1;
+                }
+            band_lo = lo < hi ? lo : 0;
+            band_hi = lo < hi ? hi : 0;
+        }
         BF_HIP(hipStreamSynchronize(s));
     }
 
@@ -917,13 +1371,37 @@ struct DancerPlan : DancerPlanBase {
                 for (int i = 0; i < R; i++)
                     regs.push_back(RegionParams{eps[i], dist[i], tx[i], ty[i], lam, has_surf ? (int64_t)(K * i) : -1});
             }
-            pe->build_tables(regs);
+            pe->build_tables(regs, true);
             std::vector<int> ri((size_t)fb * R);
             for (size_t q = 0; q < ri.size(); q++) ri[q] = pe->rtab_of.empty() ? -1 : pe->rtab_of[q];
             upload(ridx.p, ri.data(), sizeof(int) * ri.size(), s);
-            if (!fused) {  // the unfused propagators read the tilt/surface table index from the descriptors
-                for (auto& d : fd_host) d.rtab = ri[d.ktab];
-                upload(fdesc.p, fd_host.data(), sizeof(FieldDesc) * fd_host.size(), s);
+            // field (f, region, slot) -> shared k-space table; the unfused propagators
+            // also read the tilt/surface table index from the descriptors
+            for (int f = 0; f < fb; f++)
+                for (int q = 0; q < 2 * R; q++) {
+                    FieldDesc& d = fd_host[(size_t)f * 2 * R + q];
+                    d.ktab = pe->ktab_of[(size_t)f * R + q / 2];
+                    d.rtab = fused ? -1 : ri[(size_t)f * R + q / 2];
+                }
+            upload(fdesc.p, fd_host.data(), sizeof(FieldDesc) * fd_host.size(), s);
+            if (fused) {
+                // Column-pass order: fields grouped by k-space table, so the blocks that
+                // read one table run back to back and find it in L2 (every disk of a
+                // booster shares a table).  Two copies: the exit field (region R-1,
+                // right-moving slot rm), needed whole for Eout, is flagged per slot.
+                const size_t nfd = fd_host.size();
+                std::vector<size_t> ord(nfd);
+                for (size_t q = 0; q < nfd; q++) ord[q] = q;
+                std::stable_sort(ord.begin(), ord.end(),
+                                 [&](size_t x, size_t y) { return fd_host[x].ktab < fd_host[y].ktab; });
+                std::vector<FieldDesc> col(2 * nfd);
+                for (int r = 0; r < 2; r++)
+                    for (size_t q = 0; q < nfd; q++) {
+                        FieldDesc d = fd_host[ord[q]];
+                        d.pad = ((int)(ord[q] % (2 * R)) == 2 * (R - 1) + r) ? FD_WHOLE : 0;
+                        col[(size_t)r * nfd + q] = d;
+                    }
+                upload(fdesc_col.p, col.data(), sizeof(FieldDesc) * col.size(), s);
             }
             if (kind == BF_PROP_1D) {
                 std::vector<zd> ph((size_t)fb * 2 * R);
@@ -936,15 +1414,15 @@ struct DancerPlan : DancerPlanBase {
             const size_t fstride = K * 2 * R;
             if (opt.emit) {
                 hipLaunchKernelGGL(k_broadcast<T>, dim3(grid1d(fstride, 256, 2048), nb), dim3(256), 0, s,
-                                   fields.as<cplx<T>>(), emit.as<zd>(), fstride, fstride);
+                                   fields.as<cplx<S>>(), emit.as<zd>(), fstride, fstride);
             } else if (opt.reflect) {
-                BF_HIP(hipMemsetAsync(fields.p, 0, sizeof(cplx<T>) * fstride * nb, s));
+                BF_HIP(hipMemsetAsync(fields.p, 0, sizeof(cplx<S>) * fstride * nb, s));
                 // fields[R-1][leftmoving = slot 1] = reflect
                 hipLaunchKernelGGL(k_broadcast<T>, dim3(grid1d(K, 256, 1024), nb), dim3(256), 0, s,
-                                   fields.as<cplx<T>>() + ((size_t)(R - 1) * 2 + 1) * K, reflect.as<zd>(), K, fstride);
+                                   fields.as<cplx<S>>() + ((size_t)(R - 1) * 2 + 1) * K, reflect.as<zd>(), K, fstride);
             } else {
                 hipLaunchKernelGGL(k_dance_intro<T>, dim3(grid1d(K, 256, 512), 2 * R, nb), dim3(256), 0, s,
-                                   fields.as<cplx<T>>(), R, K, g.nx, ax.as<zd>(), g.X.as<double>(), g.Y.as<double>(),
+                                   fields.as<cplx<S>>(), R, K, g.nx, ax.as<zd>(), g.X.as<double>(), g.Y.as<double>(),
                                    diskR * diskR, fstride);
             }
             BF_CHECK_LAUNCH();
@@ -960,16 +1438,18 @@ struct DancerPlan : DancerPlanBase {
             const FieldDesc* fd = fdesc.as<FieldDesc>();
             int rm = 0;
             if (fused) {
-                pe->rows(fields.as<cplx<T>>(), fd, nf, RF_MASK | RF_FWD, active.as<int>());
+                pe->rows(fields.as<cplx<S>>(), fd, nf, RF_MASK | RF_FWD, active.as<int>());
                 for (int n = 0; n <= nmax; n++) {
                     tmark(n, 1, 0);
-                    pe->cols(fields.as<cplx<T>>(), fields.as<cplx<T>>(), fd, nf, CF_FWD | CF_KMID | CF_INV,
-                             active.as<int>());
+                    // every slot of the table-ordered list (inactive slots return at once)
+                    const int ncol = (int)fd_host.size();
+                    pe->cols(fields.as<cplx<S>>(), fields.as<cplx<S>>(), fdesc_col.as<FieldDesc>() + (size_t)rm * ncol,
+                             ncol, CF_FWD | CF_KMID | CF_INV, active.as<int>(), band_lo, band_hi);
                     tmark(n, 1, 1);
                     DancerRowArgs<T> a;
-                    a.fields = fields.as<cplx<T>>();
+                    a.fields = fields.as<cplx<S>>();
                     a.R = R; a.nx = g.nx; a.ny = g.ny; a.rm = rm; a.tw = pe->twx;
-                    a.rtab = pe->has_rtab() ? pe->rtab.template as<cplx<T>>() : nullptr;
+                    a.rtab = pe->has_rtab() ? pe->rtab.template as<cplx<C>>() : nullptr;
                     a.rtab_idx = ridx.as<int>();
                     a.coef = coef.as<zd>();
                     a.X = g.X.as<double>(); a.Y = g.Y.as<double>(); a.R2 = diskR * diskR;
@@ -981,6 +1461,8 @@ struct DancerPlan : DancerPlanBase {
                     a.amax = amax.as<double>();
                     a.active = active.as<int>();
                     a.count = count.as<int>() + f0;
+                    a.ylo = band_lo;
+                    a.yhi = band_hi;
                     tmark(n, 0, 0);
                     dispatch_n<DancerRowLaunch, T>(g.nx, a, nb, opt.rows, s);
                     tmark(n, 0, 1);
@@ -994,10 +1476,10 @@ struct DancerPlan : DancerPlanBase {
                 tcollect(nmax + 1);
             } else {
                 for (int n = 0; n <= nmax; n++) {
-                    pe->propagate(fields.as<cplx<T>>(), fd, nf, ph1d.as<zd>(), scratch.as<cplx<T>>(),
+                    pe->propagate(fields.as<cplx<S>>(), fd, nf, ph1d.as<zd>(), scratch.as<cplx<S>>(),
                                   active.as<int>());
                     hipLaunchKernelGGL(k_dancer_update<T>, dim3(grid1d(K, 256, 256), R + 1, nb), dim3(256), 0, s,
-                                       fields.as<cplx<T>>(), R, K, rm, coef.as<zd>(), eo_target, K * nslices,
+                                       fields.as<cplx<S>>(), R, K, rm, coef.as<zd>(), eo_target, K * nslices,
                                        nslices > 1 ? n : 0, nslices > 1 ? 1 : 0,
                                        amin != 0.0 ? amax.as<double>() : nullptr, active.as<int>(),
                                        count.as<int>() + f0);
@@ -1057,6 +1539,11 @@ DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coord
         try { p->init(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, o); } catch (...) { delete p; throw; }
         return p;
     }
+    if (precision == BF_PREC_FP32) {
+        auto* p = new DancerPlan<f32pure>();
+        try { p->init(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, o); } catch (...) { delete p; throw; }
+        return p;
+    }
     auto* p = new DancerPlan<double>();
     try { p->init(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, o); } catch (...) { delete p; throw; }
     return p;
@@ -1068,63 +1555,72 @@ DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coord
 // ---------------------------------------------------------------------------
 // F[i+1] += t*F[i]; F[i] *= r      (prologue and last region, :285-286, :343-344)
 template <typename T>
-__global__ void k_ch_push(cplx<T>* F, int Nd, size_t K, int i, zd t_, zd r_, const int* active) {
+__global__ void k_ch_push(cplx<typename Storage<T>::type>* F, int Nd, size_t K, int i, zd t_, zd r_,
+                          const int* active) {
+    BF_TYPES(T);
     const int f = blockIdx.y;
     if (active && !active[f]) return;
-    const cplx<T> t = cmk<T>((T)t_.re, (T)t_.im), r = cmk<T>((T)r_.re, (T)r_.im);
-    cplx<T>* Fi = F + ((size_t)f * Nd + i) * K;
-    cplx<T>* Fn = Fi + K;
+    const cplx<C> t = cmk<C>((C)t_.re, (C)t_.im), r = cmk<C>((C)r_.re, (C)r_.im);
+    cplx<S>* Fi = F + ((size_t)f * Nd + i) * K;
+    cplx<S>* Fn = Fi + K;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < K; p += (size_t)gridDim.x * blockDim.x) {
-        const cplx<T> v = Fi[p];
-        Fn[p] = cadd(Fn[p], cmul(t, v));
-        Fi[p] = cmul(v, r);
+        const cplx<C> v = ccast<C>(Fi[p]);
+        Fn[p] = ccast<S>(cadd(ccast<C>(Fn[p]), cmul(t, v)));
+        Fi[p] = ccast<S>(cmul(v, r));
     }
 }
 // F[0] += tl*F[1]; F[1] *= rl    (first region, :318-320)
 template <typename T>
-__global__ void k_ch_first(cplx<T>* F, int Nd, size_t K, zd tl_, zd rl_, const int* active) {
+__global__ void k_ch_first(cplx<typename Storage<T>::type>* F, int Nd, size_t K, zd tl_, zd rl_, const int* active) {
+    BF_TYPES(T);
     const int f = blockIdx.y;
     if (active && !active[f]) return;
-    const cplx<T> tl = cmk<T>((T)tl_.re, (T)tl_.im), rl = cmk<T>((T)rl_.re, (T)rl_.im);
-    cplx<T>* F0 = F + (size_t)f * Nd * K;
-    cplx<T>* F1 = F0 + K;
+    const cplx<C> tl = cmk<C>((C)tl_.re, (C)tl_.im), rl = cmk<C>((C)rl_.re, (C)rl_.im);
+    cplx<S>* F0 = F + (size_t)f * Nd * K;
+    cplx<S>* F1 = F0 + K;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < K; p += (size_t)gridDim.x * blockDim.x) {
-        const cplx<T> v = F1[p];
-        F0[p] = cadd(F0[p], cmul(tl, v));
-        F1[p] = cmul(v, rl);
+        const cplx<C> v = ccast<C>(F1[p]);
+        F0[p] = ccast<S>(cadd(ccast<C>(F0[p]), cmul(tl, v)));
+        F1[p] = ccast<S>(cmul(v, rl));
     }
 }
 // pair update at boundary i (:330-334)
 template <typename T>
-__global__ void k_ch_pair(cplx<T>* F, int Nd, size_t K, int i, zd rl_, zd tr_, zd rr_, zd tl_, const int* active) {
+__global__ void k_ch_pair(cplx<typename Storage<T>::type>* F, int Nd, size_t K, int i, zd rl_, zd tr_, zd rr_, zd tl_,
+                          const int* active) {
+    BF_TYPES(T);
     const int f = blockIdx.y;
     if (active && !active[f]) return;
-    const cplx<T> rl = cmk<T>((T)rl_.re, (T)rl_.im), tr = cmk<T>((T)tr_.re, (T)tr_.im);
-    const cplx<T> rr = cmk<T>((T)rr_.re, (T)rr_.im), tl = cmk<T>((T)tl_.re, (T)tl_.im);
-    cplx<T>* Fi = F + ((size_t)f * Nd + i) * K;
-    cplx<T>* Fn = Fi + K;
+    const cplx<C> rl = cmk<C>((C)rl_.re, (C)rl_.im), tr = cmk<C>((C)tr_.re, (C)tr_.im);
+    const cplx<C> rr = cmk<C>((C)rr_.re, (C)rr_.im), tl = cmk<C>((C)tl_.re, (C)tl_.im);
+    cplx<S>* Fi = F + ((size_t)f * Nd + i) * K;
+    cplx<S>* Fn = Fi + K;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < K; p += (size_t)gridDim.x * blockDim.x) {
-        const cplx<T> a = Fi[p], c = Fn[p];
-        Fn[p] = cadd(cmul(c, rl), cmul(tr, a));
-        Fi[p] = cadd(cmul(a, rr), cmul(tl, c));
+        const cplx<C> a = ccast<C>(Fi[p]), c = ccast<C>(Fn[p]);
+        Fn[p] = ccast<S>(cadd(cmul(c, rl), cmul(tr, a)));
+        Fi[p] = ccast<S>(cadd(cmul(a, rr), cmul(tl, c)));
     }
 }
 // emission of the left-moving fields (:290-300)
 template <typename T>
-__global__ void k_ch_emit_left(cplx<T>* F, const cplx<T>* __restrict__ E, int Nd, size_t K, zd tl_, zd rl_) {
+__global__ void k_ch_emit_left(cplx<typename Storage<T>::type>* F, const cplx<typename Storage<T>::type>* __restrict__ E,
+                               int Nd, size_t K, zd tl_, zd rl_) {
+    BF_TYPES(T);
     const int f = blockIdx.y;
-    const cplx<T> tl = cmk<T>((T)tl_.re, (T)tl_.im), rl = cmk<T>((T)rl_.re, (T)rl_.im);
-    cplx<T>* Fb = F + (size_t)f * Nd * K;
-    const cplx<T>* Eb = E + (size_t)f * Nd * 2 * K;
+    const cplx<C> tl = cmk<C>((C)tl_.re, (C)tl_.im), rl = cmk<C>((C)rl_.re, (C)rl_.im);
+    cplx<S>* Fb = F + (size_t)f * Nd * K;
+    const cplx<S>* Eb = E + (size_t)f * Nd * 2 * K;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < K; p += (size_t)gridDim.x * blockDim.x) {
-        for (int i = 0; i <= Nd - 2; i++) Fb[(size_t)i * K + p] = cadd(Fb[(size_t)i * K + p], Eb[((size_t)i * 2 + 1) * K + p]);
-        const cplx<T> e = Eb[((size_t)(Nd - 1) * 2 + 1) * K + p];
-        Fb[(size_t)(Nd - 2) * K + p] = cadd(Fb[(size_t)(Nd - 2) * K + p], cmul(tl, e));
-        Fb[(size_t)(Nd - 1) * K + p] = cadd(Fb[(size_t)(Nd - 1) * K + p], cmul(rl, e));
+        for (int i = 0; i <= Nd - 2; i++)
+            Fb[(size_t)i * K + p] = ccast<S>(cadd(ccast<C>(Fb[(size_t)i * K + p]), ccast<C>(Eb[((size_t)i * 2 + 1) * K + p])));
+        const cplx<C> e = ccast<C>(Eb[((size_t)(Nd - 1) * 2 + 1) * K + p]);
+        Fb[(size_t)(Nd - 2) * K + p] = ccast<S>(cadd(ccast<C>(Fb[(size_t)(Nd - 2) * K + p]), cmul(tl, e)));
+        Fb[(size_t)(Nd - 1) * K + p] = ccast<S>(cadd(ccast<C>(Fb[(size_t)(Nd - 1) * K + p]), cmul(rl, e)));
     }
 }
 template <typename T>
-__global__ void k_ch_copy_right(cplx<T>* F, const cplx<T>* __restrict__ E, int Nd, size_t K) {
+__global__ void k_ch_copy_right(cplx<typename Storage<T>::type>* F, const cplx<typename Storage<T>::type>* __restrict__ E,
+                                int Nd, size_t K) {
     const int f = blockIdx.y;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < (size_t)Nd * K; p += (size_t)gridDim.x * blockDim.x) {
         const size_t i = p / K, q = p % K;
@@ -1132,13 +1628,13 @@ __global__ void k_ch_copy_right(cplx<T>* F, const cplx<T>* __restrict__ E, int N
     }
 }
 template <typename T>
-__global__ void k_amax_fields(const cplx<T>* F, size_t n, double* amax, const int* active) {
+__global__ void k_amax_fields(const cplx<typename Storage<T>::type>* F, size_t n, double* amax, const int* active) {
     const int f = blockIdx.y;
     if (active && !active[f]) return;
     double m = 0.0;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const cplx<T> v = F[(size_t)f * n + p];
-        m = fmax(m, hypot((double)v.re, (double)v.im));
+        const zd v = ccast<double>(F[(size_t)f * n + p]);
+        m = fmax(m, hypot(v.re, v.im));
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & 63) == 0 && m > 0.0) atomic_max_pos(amax + f, m);
@@ -1148,6 +1644,7 @@ template <typename T>
 void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax, const double* fr,
                      int nfr, int kind, const double* emit_h, const double* reflect_h, double diskR, double* out_right,
                      double* out_left) {
+    BF_TYPES(T);
     hipStream_t s = ctx->stream;
     Grid g;
     g.upload(co, s);
@@ -1167,7 +1664,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         tl[i] = cmk<double>(1.0 - r[i].re, -r[i].im);
         tr[i] = cmk<double>(1.0 + r[i].re, r[i].im);
     }
-    const size_t per = sizeof(cplx<T>) * K * Nd;
+    const size_t per = sizeof(cplx<S>) * K * Nd;
     int fb = (int)std::max<size_t>(1, (192u << 20) / std::max<size_t>(per * 3, 1));
     fb = std::max(1, std::min(fb, std::min(nfr, 64)));
     DevBuf F(per * fb), EM(per * 2 * fb), scr, dax, demit, dref, out(sizeof(zd) * K * fb), act(sizeof(int) * fb),
@@ -1224,27 +1721,27 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         const size_t estride = K * 2 * Nd;
         if (emit_h) {
             hipLaunchKernelGGL(k_broadcast<T>, dim3(grid1d(estride, 256, 2048), nb), dim3(256), 0, s,
-                               EM.as<cplx<T>>(), demit.as<zd>(), estride, estride);
+                               EM.as<cplx<S>>(), demit.as<zd>(), estride, estride);
         } else {
             BF_HIP(hipMemsetAsync(EM.p, 0, per * 2 * nb, s));
             if (reflect_h)
                 hipLaunchKernelGGL(k_broadcast<T>, dim3(gb, nb), dim3(256), 0, s,
-                                   EM.as<cplx<T>>() + ((size_t)(Nd - 1) * 2 + 1) * K, dref.as<zd>(), K, estride);
+                                   EM.as<cplx<S>>() + ((size_t)(Nd - 1) * 2 + 1) * K, dref.as<zd>(), K, estride);
             else
                 hipLaunchKernelGGL(k_dance_intro<T>, dim3(grid1d(K, 256, 512), 2 * (Nd - 1), nb), dim3(256), 0, s,
-                                   EM.as<cplx<T>>() + 2 * K, Nd - 1, K, g.nx, dax.as<zd>(), g.X.as<double>(),
+                                   EM.as<cplx<S>>() + 2 * K, Nd - 1, K, g.nx, dax.as<zd>(), g.X.as<double>(),
                                    g.Y.as<double>(), diskR * diskR, estride);
         }
         BF_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_ch_copy_right<T>, dim3(grid1d(K * Nd, 256, 4096), nb), dim3(256), 0, s, F.as<cplx<T>>(),
-                           EM.as<cplx<T>>(), Nd, K);
+        hipLaunchKernelGGL(k_ch_copy_right<T>, dim3(grid1d(K * Nd, 256, 4096), nb), dim3(256), 0, s, F.as<cplx<S>>(),
+                           EM.as<cplx<S>>(), Nd, K);
         BF_CHECK_LAUNCH();
         std::vector<int> a0(fb, 0);
         for (int f = 0; f < nb; f++) a0[f] = (1.0 >= amin) ? 1 : 0;
         const FieldDesc* D = dfd.as<FieldDesc>();
         const zd* P1 = dph.as<zd>();
-        cplx<T>* Fp = F.as<cplx<T>>();
-        cplx<T>* Sp = scr.as<cplx<T>>();
+        cplx<S>* Fp = F.as<cplx<S>>();
+        cplx<S>* Sp = scr.as<cplx<S>>();
         auto prop_single = [&](int i, const int* active) {
             pe.propagate(Fp, D + single_off[i], nb, P1 + single_off[i], Sp, active);
         };
@@ -1257,7 +1754,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
             hipLaunchKernelGGL(k_ch_push<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, i, tr[i], rr[i], nullptr);
             BF_CHECK_LAUNCH();
         }
-        hipLaunchKernelGGL(k_ch_emit_left<T>, dim3(gb, nb), dim3(256), 0, s, Fp, EM.as<cplx<T>>(), Nd, K, tl[Nd - 2],
+        hipLaunchKernelGGL(k_ch_emit_left<T>, dim3(gb, nb), dim3(256), 0, s, Fp, EM.as<cplx<S>>(), Nd, K, tl[Nd - 2],
                            rl[Nd - 2]);
         BF_CHECK_LAUNCH();
         upload(act.p, a0.data(), sizeof(int) * fb, s);
@@ -1387,6 +1884,8 @@ double stream_copy_probe(bf_ctx* ctx, int64_t bytes, int reps) {
 
 template void cheerleader_run<double>(bf_ctx*, const bf_setup*, const bf_coords*, double, int, const double*, int, int,
                                       const double*, const double*, double, double*, double*);
+template void cheerleader_run<f32pure>(bf_ctx*, const bf_setup*, const bf_coords*, double, int, const double*, int,
+                                       int, const double*, const double*, double, double*, double*);
 template void cheerleader_run<float>(bf_ctx*, const bf_setup*, const bf_coords*, double, int, const double*, int, int,
                                      const double*, const double*, double, double*, double*);
 

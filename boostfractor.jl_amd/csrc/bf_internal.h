@@ -127,6 +127,7 @@ struct DancerOptions {
     int returnsum = 1, immediatesum = 1;
     int freq_batch = 0;
     int col_tx = 0, rows = 0;  // launch shapes (0: defaults)
+    int band = 1;              // skip the rows the aperture zeroes (0: process every row)
 };
 
 struct DancerPlanBase {

@@ -49,7 +49,9 @@ extern "C" {
 
 /* arithmetic of the hot path */
 #define BF_PREC_FP64 0  /* everything in FP64 */
-#define BF_PREC_MIXED 1 /* FP32 fields/FFTs/overlaps, FP64 phase arguments, outputs and chains */
+#define BF_PREC_MIXED 1 /* fields stored in FP32 in HBM, in-register FFT/phase/update arithmetic in FP64,
+                           FP64 outputs and transfer-matrix chains */
+#define BF_PREC_FP32 2  /* FP32 storage and FP32 arithmetic (accuracy study; not the mixed path) */
 
 #define BF_MAX_GRID 1024
 #define BF_MAX_MODES 128
