@@ -12,11 +12,27 @@ relative beta^2 error of every point against FP64 is reported.
 
 Kernel timing: HIP events recorded on the engine's stream around every launch
 of the two bounce kernels during the timed steps.
+
+Roofline: `achieved` counts the algorithmic bytes of the dancer bounce, every
+field read once and written once per bounce and frequency (2 F K B_c with
+F = 82 fields, K = 256^2, B_c = 8 B for complex64 storage), times the
+frequency-bounces one launch of the dominant kernel processes, over that
+kernel's mean launch time.  Skipping the rows the aperture zeroes and sharing
+the phase tables reduce the bytes actually moved, which `traffic` reports (HBM
+bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the
+same counter's reading for a copy kernel of known size); they are never folded
+into `achieved`.
+
+  python bench.py --summarize-pmc FETCH_DB WRITE_DB
+  python bench.py --summarize-trace TRACE_DB
+print the per-kernel counter summary and the kernel-trace statistics (the
+profiles/ file format) from rocprofv3's rocpd databases.
 """
 import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -29,6 +45,57 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 NDISK, N, NFREQ, NMAX, DISKR = 20, 256, 50, 2000, 0.15
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cols_reg_dancer_mixed_256x256_20disk.md")
+
+
+def pmc_per_kernel(db_path, counter):
+    """mean counter value per kernel name from a rocprofv3 (rocpd) database"""
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    acc = {}
+    for name, val in db.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                                (counter,)):
+        a = acc.setdefault(name, [0.0, 0])
+        a[0] += val
+        a[1] += 1
+    return {k: (v[0] / v[1], v[1]) for k, v in acc.items()}
+
+
+def summarize_pmc(fetch_db, write_db):
+    f = pmc_per_kernel(fetch_db, "FETCH_SIZE")
+    w = pmc_per_kernel(write_db, "WRITE_SIZE")
+    lines = ["| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch |", "|---|---|---|---|"]
+    for k in sorted(f, key=lambda k: -f[k][0]):
+        lines.append(f"| `{k}` | {f[k][1]} | {f[k][0]:.1f} | {w.get(k, (float('nan'), 0))[0]:.1f} |")
+    return "\n".join(lines)
+
+
+def summarize_trace(db_path, top=12):
+    import sqlite3
+    db = sqlite3.connect(db_path)
+    lines = ["| kernel | calls | total us | mean us | % |", "|---|---|---|---|---|"]
+    for name, calls, total, mean, pct in db.execute(
+            "select name, total_calls, total_duration, average, percentage from top_kernels limit ?", (top,)):
+        lines.append(f"| `{name}` | {calls} | {total:.1f} | {mean:.2f} | {pct:.1f} |")
+    return "\n".join(lines)
+
+
+def traffic_from_profile(kernel_key, fb):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary, calibrated by the copy kernel's readings and scaled to the
+    frequency batch used here (a launch moves bytes in proportion to it)."""
+    try:
+        txt = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"calibration: copy_bytes=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+)", txt)
+    k = re.search(r"traffic: kernel=%s freq_batch=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+)" % kernel_key, txt)
+    if not (m and k):
+        return None
+    copy_kib = int(m.group(1)) / 1024.0
+    fetch = float(k.group(2)) * copy_kib / float(m.group(2))
+    write = float(k.group(3)) * copy_kib / float(m.group(3))
+    return (fetch + write) * 1024.0 * fb / int(k.group(1))
 
 
 def workload():
@@ -89,7 +156,17 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--freq-batch", type=int, default=0)
+    ap.add_argument("--nmax", type=int, default=NMAX,
+                    help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
+    ap.add_argument("--summarize-trace", metavar="TRACE_DB")
     args = ap.parse_args()
+    if args.summarize_pmc or args.summarize_trace:
+        if args.summarize_trace:
+            print(summarize_trace(args.summarize_trace))
+        if args.summarize_pmc:
+            print(summarize_pmc(*args.summarize_pmc))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the workload is a single-GPU sweep and this build has no multi-GPU "
               "path (frequency points would shard across GPUs with no exchange); run with --gpus 1",
@@ -104,25 +181,35 @@ def main():
     K = N * N
     F = 2 * (2 * NDISK + 1)  # fields per bounce (dancer regions x 2 directions)
 
-    print(f"[bench] workload: {NDISK}-disk booster, {N}x{N}, {NFREQ} freqs, nmax={NMAX}", file=sys.stderr, flush=True)
-    plan32 = B.DancerPlan(0, NMAX, sb, coords, freqs, diskR=DISKR, precision="mixed", freq_batch=args.freq_batch)
+    nmax = args.nmax
+    print(f"[bench] workload: {NDISK}-disk booster, {N}x{N}, {NFREQ} freqs, nmax={nmax}", file=sys.stderr, flush=True)
+    plan32 = B.DancerPlan(0, nmax, sb, coords, freqs, diskR=DISKR, precision="mixed", freq_batch=args.freq_batch)
+    info32 = plan32.info()
+    fb32 = info32["freq_batch"]
     dt32, kt32 = timed_steps(plan32, ctx, args.steps, args.warmup, timing=True)
     _, p32 = plan32.fetch(fields=False)
     plan32.close()
     print(f"[bench] mixed: {dt32 / args.steps:.3f} s/step", file=sys.stderr, flush=True)
 
-    plan64 = B.DancerPlan(0, NMAX, sb, coords, freqs, diskR=DISKR, precision="fp64", freq_batch=args.freq_batch)
-    dt64, kt64 = timed_steps(plan64, ctx, args.fp64_steps, 1, timing=True)
-    _, p64 = plan64.fetch(fields=False)
-    plan64.close()
-    print(f"[bench] fp64: {dt64 / args.fp64_steps:.3f} s/step", file=sys.stderr, flush=True)
-
-    b32 = p32 * dx * dx / (math.pi * DISKR ** 2)
-    b64 = p64 * dx * dx / (math.pi * DISKR ** 2)
-    err = np.abs(b32 / b64 - 1.0)
-
     pts32 = NFREQ * args.steps / dt32
-    pts64 = NFREQ * args.fp64_steps / dt64
+    b32 = p32 * dx * dx / (math.pi * DISKR ** 2)
+    fp64 = None
+    if args.fp64_steps > 0:  # 0 only for profiler runs of the mixed sweep
+        plan64 = B.DancerPlan(0, nmax, sb, coords, freqs, diskR=DISKR, precision="fp64", freq_batch=args.freq_batch)
+        dt64, kt64 = timed_steps(plan64, ctx, args.fp64_steps, 1, timing=True)
+        _, p64 = plan64.fetch(fields=False)
+        fb64 = plan64.freq_batch
+        plan64.close()
+        print(f"[bench] fp64: {dt64 / args.fp64_steps:.3f} s/step", file=sys.stderr, flush=True)
+        b64 = p64 * dx * dx / (math.pi * DISKR ** 2)
+        err = np.abs(b32 / b64 - 1.0)
+        pts64 = NFREQ * args.fp64_steps / dt64
+        fp64 = {"value": round(pts64, 4), "unit": "points/s", "ms_per_step": round(dt64 / args.fp64_steps * 1e3, 3),
+                "steps": args.fp64_steps, "freq_batch": fb64,
+                "mixed_over_fp64": round(pts32 / pts64, 3),
+                "beta2_rel_err_vs_fp64": {"max": float(np.max(err)), "median": float(np.median(err)),
+                                          "p90": float(np.percentile(err, 90))},
+                "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt64.items()}}
 
     # roofline of the dominant kernel of the mixed path
     names = {"row_pass": "k_dancer_rows (row IFFT, tilt/surface, boundary update, mask, row FFT)",
@@ -130,9 +217,10 @@ def main():
     dom = max(kt32, key=lambda k: kt32[k][0])
     ms_tot, launches = kt32[dom]
     avg_ms = ms_tot / launches
-    nb_per_launch = NFREQ * args.steps * (NMAX + 1) / launches  # frequency-bounces per launch
+    nb_per_launch = NFREQ * args.steps * (nmax + 1) / launches  # frequency-bounces per launch
     alg_bytes = nb_per_launch * 2 * F * K * 8  # every field read once and written once, complex64
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile("k_cols_reg", fb32) if dom == "col_pass" else None
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=10)
     stream_gbs = 2 * (1 << 30) / (bw_ms * 1e-3) / 1e9
 
@@ -147,23 +235,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields/FFT, f64 phase arguments and Eout",
+        "dtype": "f64",
+        "storage": "complex64 fields in HBM (mixed precision: every FFT, phase and boundary update in f64)",
         "data": "synthetic (seeded SplitMix64 booster geometry, axion source from dance_intro)",
-        "config": {"workload": "dancer, 20-disk booster eps=24, 256x256 grid over 1 m, 50 freqs 19.5-20.5 GHz, "
-                               "nmax=2000, FFT propagator, one batched device-resident sweep",
-                   "disks": NDISK, "grid": N, "frequencies": NFREQ, "nmax": NMAX},
-        "fp64": {"value": round(pts64, 4), "unit": "points/s", "ms_per_step": round(dt64 / args.fp64_steps * 1e3, 3),
-                 "steps": args.fp64_steps},
-        "mixed_over_fp64": round(pts32 / pts64, 3),
-        "beta2_rel_err_vs_fp64": {"max": float(np.max(err)), "median": float(np.median(err)),
-                                  "p90": float(np.percentile(err, 90))},
+        "config": {"workload": f"dancer, 20-disk booster eps=24, 256x256 grid over 1 m, 50 freqs 19.5-20.5 GHz, "
+                               f"nmax={nmax}, FFT propagator, one batched device-resident sweep",
+                   "disks": NDISK, "grid": N, "frequencies": NFREQ, "nmax": nmax, "freq_batch": fb32},
+        "fp64": fp64,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic if traffic is None else int(traffic),
                      "kernel": names[dom], "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "measured_stream_copy_GBs": round(stream_gbs, 1)},
-        "kernel_ms": {"mixed": {k: [round(v[0], 3), v[1]] for k, v in kt32.items()},
-                      "fp64": {k: [round(v[0], 3), v[1]] for k, v in kt64.items()}},
+        "work_reductions": {"row_band": list(info32["band"]), "rows": N,
+                            "note": "rows outside the aperture band skipped by both passes (exit field kept whole); "
+                                    "identical k-space tables shared; not folded into roofline.achieved"},
+        "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt32.items()},
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)

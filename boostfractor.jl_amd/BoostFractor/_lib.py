@@ -70,6 +70,7 @@ _SIGS = {
     "bf_dancer_plan_fetch": (ctypes.c_int, [_P, _P, _P]),
     "bf_dancer_plan_timing": (ctypes.c_int, [_P, _I32]),
     "bf_dancer_plan_kernel_times": (ctypes.c_int, [_P, _P, _P]),
+    "bf_dancer_plan_info": (ctypes.c_int, [_P, _P, _P, _P]),
     "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
     "bf_stream_copy_probe": (ctypes.c_int, [_P, ctypes.c_int64, _I32, ctypes.POINTER(_D)]),
     "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),

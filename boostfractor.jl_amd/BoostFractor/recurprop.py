@@ -147,6 +147,17 @@ class DancerPlan:
         L.check(L.lib().bf_dancer_plan_kernel_times(self.handle, L.cptr(ms), L.cptr(n)), self.ctx.handle)
         return {"row_pass": (ms[0], int(n[0])), "col_pass": (ms[1], int(n[1]))}
 
+    def info(self):
+        """layout chosen by the library: frequencies per batch and the row band"""
+        fb, lo, hi = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        L.check(L.lib().bf_dancer_plan_info(self.handle, ctypes.byref(fb), ctypes.byref(lo), ctypes.byref(hi)),
+                self.ctx.handle)
+        return {"freq_batch": fb.value, "band": (lo.value, hi.value)}
+
+    @property
+    def freq_batch(self):
+        return self.info()["freq_batch"]
+
     def close(self):
         if self.handle:
             L.lib().bf_dancer_plan_destroy(self.handle)

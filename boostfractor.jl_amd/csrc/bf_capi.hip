@@ -285,6 +285,15 @@ int bf_dancer_plan_kernel_times(bf_dancer_plan* plan, double* ms, int64_t* launc
     return guarded(ctx, [&] { plan->impl->kernel_times(ms, launches); });
 }
 
+int bf_dancer_plan_info(bf_dancer_plan* plan, int32_t* freq_batch, int32_t* band_lo, int32_t* band_hi) {
+    BF_REQUIRE(nullptr, plan && plan->impl, BF_ERR_ARG, "plan is NULL");
+    const bf::DancerPlanBase* p = plan->impl;
+    if (freq_batch) *freq_batch = p->freq_batch;
+    if (band_lo) *band_lo = p->band_lo;
+    if (band_hi) *band_hi = p->band_hi;
+    return BF_OK;
+}
+
 int bf_dancer_plan_destroy(bf_dancer_plan* plan) {
     if (!plan) return BF_OK;
     if (plan->impl) {

@@ -1217,7 +1217,6 @@ struct DancerPlan : DancerPlanBase {
     DancerOptions opt;
     int nslices;
     std::vector<FieldDesc> fd_host;
-    int band_lo = 0, band_hi = 0;
     // timing: HIP events recorded around every launch of the row and column
     // passes on the plan's stream, reduced once per frequency batch
     bool timing = false;
@@ -1294,6 +1293,7 @@ struct DancerPlan : DancerPlanBase {
         const size_t per = sizeof(cplx<S>) * K * 2 * R;
         fb = opt.freq_batch > 0 ? opt.freq_batch : (int)std::max<size_t>(1, (192u << 20) / std::max<size_t>(per, 1));
         fb = std::max(1, std::min(fb, std::min(nfreq, 64)));
+        freq_batch = fb;
         nslices = opt.immediatesum ? 1 : nmax + 1;
         fields.alloc(per * fb);
         if (kind == BF_PROP_MOMENTUM) scratch.alloc(per * fb);
@@ -1341,7 +1341,7 @@ struct DancerPlan : DancerPlanBase {
         // criterion reads every row, so it keeps the full range.
         band_lo = 0;
         band_hi = g.ny;
-        if (opt.band && amin == 0.0) {
+        if (opt.band && amin == 0.0 && (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS)) {
             double minx2 = INFINITY;
             for (int x = 0; x < g.nx; x++) minx2 = std::min(minx2, co->X[x] * co->X[x]);
             const double lim = diskR * diskR * (1.0 + 1e-9);

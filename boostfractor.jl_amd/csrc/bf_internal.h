@@ -137,6 +137,8 @@ struct DancerPlanBase {
     virtual void set_timing(bool on) = 0;
     virtual void kernel_times(double* ms, int64_t* launches) = 0;
     bf_ctx* ctx = nullptr;
+    int freq_batch = 0;             // frequencies per batch
+    int band_lo = 0, band_hi = 0;   // row band of the fused bounce
 };
 
 // recursive Fourier propagation (bf_engine.hip)

@@ -148,6 +148,11 @@ int bf_dancer_plan_fetch(bf_dancer_plan *plan, double *Eout, double *power);
  * launch counts for the row pass (index 0) and the column pass (index 1). */
 int bf_dancer_plan_timing(bf_dancer_plan *plan, int32_t enable);
 int bf_dancer_plan_kernel_times(bf_dancer_plan *plan, double *ms, int64_t *launches);
+/* Plan layout chosen at create time (no reference counterpart; for tuning and
+ * reports): frequencies per batch, and the row band [band_lo, band_hi) the
+ * fused bounce processes at every boundary (rows outside it are zeroed by the
+ * aperture and only feed Eout at the exit boundary). */
+int bf_dancer_plan_info(bf_dancer_plan *plan, int32_t *freq_batch, int32_t *band_lo, int32_t *band_hi);
 int bf_dancer_plan_destroy(bf_dancer_plan *plan);
 
 /* Stream-bandwidth probe: copies `bytes` between two device buffers `reps`
