@@ -44,7 +44,11 @@ for p in (ROOT, os.path.join(ROOT, "boostfractor.jl_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
 NDISK, N, NFREQ, NMAX, DISKR = 20, 256, 50, 2000, 0.15
+# mode-matching workload: the same booster with per-disk tilts and roughness,
+# 45 Laguerre-Gauss modes on 512^2, 200 frequencies
+N3, NF3 = 512, 200
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cols_reg_dancer_mixed_256x256_20disk.md")
 
 
@@ -148,6 +152,89 @@ def cpu_baseline(coords, sb, freqs, budget_s=20.0):
                       f"(FP64 C restatement of the reference dancer), extrapolated linearly in bounces"}
 
 
+def transformer_cpu_baseline(coords, sb, pat, kt):
+    """FP64 CPU restatement on a bounded sample: the propagation matrix of one
+    air region and one disk region (single thread), scaled to the 21 air and
+    20 disk regions of one frequency point (the chain is negligible)."""
+    from oracle import oracle as O
+    oc = O.Coords(coords.X, coords.Y, coords.kX, coords.kY)
+    P = pat.reshape(-1, coords.ny, coords.nx)
+    K = kt.reshape(-1)
+    lam = 299792458.0 / 20e9
+    times = {}
+    for name, r in (("air", 1), ("disk", 2)):
+        t = time.perf_counter()
+        O.propagation_matrix(sb.distance[r], DISKR, sb.eps[r], sb.relative_tilt_x[r], sb.relative_tilt_y[r],
+                             sb.relative_surfaces[r], lam, oc, P, K)
+        times[name] = time.perf_counter() - t
+    n_air = sum(1 for r in range(1, len(sb.eps)) if sb.eps[r].real == 1.0)
+    n_disk = len(sb.eps) - 1 - n_air
+    per_point = n_air * times["air"] + n_disk * times["disk"]
+    return {"value": 1.0 / per_point, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"propagation matrices of one air ({times['air']:.1f} s) and one disk region "
+                      f"({times['disk']:.1f} s) at 512^2 with 45 modes, one thread, scaled to {n_air} air + "
+                      f"{n_disk} disk regions per point (FP64 C restatement of the reference transformer)"}
+
+
+def transformer_main(args):
+    import copy
+    import BoostFractor as B
+    from BoostFractor import workloads as W
+    coords = W.grid_coords(N3)
+    sb0 = W.rough_booster(coords, ndisk=NDISK, seed=1002)
+    pat, kt = W.lg_modes(coords)
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=4, pattern_input=pat, kT_arr=kt)
+    freqs = np.linspace(19.5e9, 20.5e9, NF3)
+    n = modes.n
+    ctx = B.context()
+    print(f"[bench] transformer: {NDISK}-disk rough booster, {N3}x{N3}, {n} LG modes, {NF3} freqs",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        B.transformer(copy.deepcopy(sb0), coords, modes, f=freqs[:8], diskR=DISKR)
+    ctx.synchronize()
+    setups = [copy.deepcopy(sb0) for _ in range(args.steps)]  # transformer mutates eps (NaN -> 1e30)
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for sb in setups:
+        boost = B.transformer(sb, coords, modes, f=freqs, diskR=DISKR)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "inverse", "chain")}
+    ctx.timing(False)
+    ms, launches, flops = kt_["wgram"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    beta2 = np.sum(np.abs(boost) ** 2, axis=-1)
+    out = {
+        "metric": "transformer boost-factor freq points/s on 1 MI355X (FP64)",
+        "value": round(NF3 * args.steps / dt, 3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 booster, tilts N(0,50 urad), 10 um roughness; LG modes w0=0.1 m)",
+        "note": "host buffers in and out: each step includes copying modes and surfaces to the device",
+        "config": {"workload": f"transformer, 20-disk booster with per-disk tilt and roughness, {N3}x{N3} grid over "
+                               f"1 m, {n} Laguerre-Gauss modes (p<=4, |l|<=4), {NF3} freqs 19.5-20.5 GHz",
+                   "grid": N3, "modes": n, "frequencies": NF3},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4), "traffic": None,
+                     "kernel": "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)",
+                     "avg_launch_ms": round(ms / max(launches, 1), 3),
+                     "alg_flops_per_launch": flops / max(launches, 1)},
+        "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
+        "beta2": {"min": float(beta2.min()), "max": float(beta2.max())},
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = transformer_cpu_baseline(coords, sb0, pat, kt)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +247,8 @@ def main():
                     help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
+    ap.add_argument("--workload", choices=("dancer", "transformer"), default="dancer",
+                    help="dancer: the headline sweep (default); transformer: the mode-matching sweep")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
         if args.summarize_trace:
@@ -172,6 +261,9 @@ def main():
               "path (frequency points would shard across GPUs with no exchange); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "transformer":
+        transformer_main(args)
+        return
 
     import BoostFractor as B
     from BoostFractor import workloads as W

@@ -71,6 +71,8 @@ _SIGS = {
     "bf_dancer_plan_timing": (ctypes.c_int, [_P, _I32]),
     "bf_dancer_plan_kernel_times": (ctypes.c_int, [_P, _P, _P]),
     "bf_dancer_plan_info": (ctypes.c_int, [_P, _P, _P, _P]),
+    "bf_ctx_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "bf_ctx_kernel_time": (ctypes.c_int, [_P, ctypes.c_char_p, _P, _P, _P]),
     "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
     "bf_stream_copy_probe": (ctypes.c_int, [_P, ctypes.c_int64, _I32, ctypes.POINTER(_D)]),
     "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),
@@ -153,6 +155,18 @@ class Context:
 
     def synchronize(self):
         check(lib().bf_ctx_synchronize(self.handle), self.handle)
+
+    def timing(self, enable=True):
+        """start (clearing the counters) or stop per-kernel-class timing"""
+        check(lib().bf_ctx_timing(self.handle, int(bool(enable))), self.handle)
+
+    def kernel_time(self, kernel):
+        """(milliseconds, launches, algorithmic flops) accumulated for a kernel class"""
+        ms, fl = _D(), _D()
+        n = ctypes.c_int64()
+        check(lib().bf_ctx_kernel_time(self.handle, kernel.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                       ctypes.byref(fl)), self.handle)
+        return ms.value, n.value, fl.value
 
     def stream_copy_probe(self, nbytes, reps=20):
         """mean ms per device-to-device copy of nbytes with the engine's copy kernel"""

@@ -36,6 +36,22 @@ class SplitMix64:
         u2 = self.uniform()
         return math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(2.0 * math.pi * u2)
 
+    def uniforms(self, count):
+        """`count` successive uniform() values at once (same sequence)"""
+        i = np.arange(1, count + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + i * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        self.state = (self.state + count * 0x9E3779B97F4A7C15) & _MASK
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+    def normals(self, count):
+        """`count` successive normal() values at once (same sequence)"""
+        u = self.uniforms(2 * count)
+        return np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+
 
 def grid_coords(N, span=1.0):
     """N x N grid over `span` metres: X = ((0:N-1) .- N/2) ./ N * span (dx = span/N)."""
@@ -58,6 +74,77 @@ def booster(coords, ndisk=20, eps_disk=24.0, thickness=1e-3, spacing=8e-3, jitte
     distance.append(0.0)
     eps.append(1.0)
     return SeedSetupBoundaries(coords, diskno=ndisk, distance=distance, epsilon=eps)
+
+
+def _genlaguerre(p, a, x):
+    """generalised Laguerre polynomial L_p^a(x) by its three-term recurrence"""
+    l0 = np.ones_like(x)
+    if p == 0:
+        return l0
+    l1 = 1.0 + a - x
+    for k in range(1, p):
+        l0, l1 = l1, ((2 * k + 1 + a - x) * l1 - (k + a) * l0) / (k + 1)
+    return l1
+
+
+def lg_modes(coords, pmax=4, lmax=4, w0=0.1):
+    """Laguerre-Gauss modes LG_{p,l}, p = 0..pmax, l = -lmax..lmax, in the
+    layout of SeedModes' pattern_input (src/boost_fractor_transformer.jl:46-57,
+    mode m = p + 1): patterns (pmax+1, 2 lmax+1, 1, ny, nx), each
+    (sqrt(2) r/w0)^|l| L_p^|l|(2 r^2/w0^2) exp(-r^2/w0^2) exp(-i l phi)
+    normalised to sum |.|^2 = 1 on the grid, and kT_arr (pmax+1, 2 lmax+1)
+    with the paraxial kT^2 = 4 (2p + |l| + 1) / w0^2.  The reference has no LG
+    modes; this is the workload's own convention."""
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    r2 = XX ** 2 + YY ** 2
+    phi = np.arctan2(YY, XX)
+    M, nl = pmax + 1, 2 * lmax + 1
+    pat = np.zeros((M, nl, 1, coords.ny, coords.nx), dtype=np.complex128)
+    kt = np.zeros((M, nl), dtype=np.complex128)
+    for p in range(M):
+        for l in range(-lmax, lmax + 1):
+            a = abs(l)
+            u = (np.sqrt(2.0 * r2) / w0) ** a * _genlaguerre(p, a, 2.0 * r2 / w0 ** 2) * np.exp(-r2 / w0 ** 2)
+            f = u * np.exp(-1j * l * phi)
+            pat[p, l + lmax, 0] = f / np.sqrt(np.sum(np.abs(f) ** 2))
+            kt[p, l + lmax] = math.sqrt(4.0 * (2 * p + a + 1)) / w0
+    return pat, kt
+
+
+def rough_surface(coords, rng, sigma=10e-6, corr=1e-2, trunc=10.0, diskR=0.15):
+    """Surface height map (ny, nx) after the reference's surface_roughness
+    (examples/transformer_optim_utilities.jl:166-195): white N(0,1) noise
+    convolved with exp(-r^2/(2 corr^2)) (circular FFT convolution in place of
+    conv2), rescaled to rms `sigma` over the grid, truncated at +-trunc*sigma,
+    zero outside r <= diskR."""
+    ny, nx = coords.ny, coords.nx
+    w = rng.normals(ny * nx).reshape(ny, nx)
+    XX, YY = np.meshgrid(coords.X, coords.Y)
+    g = np.exp(-(XX ** 2 + YY ** 2) / (2.0 * corr ** 2))
+    s = np.real(np.fft.ifft2(np.fft.fft2(w) * np.fft.fft2(np.fft.ifftshift(g))))
+    s *= sigma / np.sqrt(np.mean(s ** 2))
+    s = np.clip(s, -trunc * sigma, trunc * sigma)
+    s[XX ** 2 + YY ** 2 > diskR ** 2] = 0.0
+    return s
+
+
+def rough_booster(coords, ndisk=20, seed=1002, tilt_sigma=50e-6, sigma=10e-6, corr=1e-2, diskR=0.15):
+    """The 20-disk booster with per-disk random tilts N(0, tilt_sigma) in x and
+    y and Gaussian surface roughness on every disk region (all from one
+    SplitMix64 stream)."""
+    sb = booster(coords, ndisk=ndisk, seed=seed)
+    rng = SplitMix64(seed + 1)
+    n = len(sb.distance)
+    tx, ty = np.zeros(n), np.zeros(n)
+    surf = np.zeros((n, coords.ny, coords.nx), dtype=np.complex128)
+    for i in range(2, n - 1, 2):  # disk regions: eps = eps_disk at even indices 2 .. 2 ndisk
+        tx[i] = tilt_sigma * rng.normal()
+        ty[i] = tilt_sigma * rng.normal()
+        surf[i] = rough_surface(coords, rng, sigma=sigma, corr=corr, diskR=diskR)
+    sb.relative_tilt_x = tx
+    sb.relative_tilt_y = ty
+    sb.relative_surfaces = surf
+    return sb
 
 
 def beta2_dancer(E, coords, diskR):

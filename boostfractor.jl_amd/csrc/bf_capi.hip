@@ -136,6 +136,7 @@ int bf_ctx_destroy(bf_ctx* ctx) {
     if (!ctx) return BF_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    bf::clear_timers(ctx);
     ctx->twiddles.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -146,6 +147,26 @@ int bf_ctx_stream(bf_ctx* ctx, void** stream) {
     BF_REQUIRE(ctx, ctx && stream, BF_ERR_ARG, "ctx or stream is NULL");
     *stream = (void*)ctx->stream;
     return BF_OK;
+}
+
+int bf_ctx_timing(bf_ctx* ctx, int32_t enable) {
+    BF_REQUIRE(ctx, ctx, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::clear_timers(ctx);
+        ctx->timing = enable != 0;
+    });
+}
+
+int bf_ctx_kernel_time(bf_ctx* ctx, const char* kernel, double* ms, int64_t* launches, double* flops) {
+    BF_REQUIRE(ctx, ctx && kernel, BF_ERR_ARG, "ctx or kernel is NULL");
+    return guarded(ctx, [&] {
+        bf::resolve_timers(ctx);
+        auto it = ctx->timers.find(kernel);
+        const bool have = it != ctx->timers.end();
+        if (ms) *ms = have ? it->second.ms : 0.0;
+        if (launches) *launches = have ? it->second.launches : 0;
+        if (flops) *flops = have ? it->second.flops : 0.0;
+    });
 }
 
 int bf_ctx_synchronize(bf_ctx* ctx) {

@@ -1859,6 +1859,31 @@ __global__ void k_stream_copy(double2* __restrict__ dst, const double2* __restri
         dst[i] = src[i];
 }
 
+void resolve_timers(bf_ctx* ctx) {
+    for (auto& kv : ctx->timers) {
+        bf_kernel_timer& t = kv.second;
+        for (auto& pr : t.pending) {
+            BF_HIP(hipEventSynchronize(pr.second));
+            float m = 0.f;
+            BF_HIP(hipEventElapsedTime(&m, pr.first, pr.second));
+            t.ms += m;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        t.pending.clear();
+    }
+}
+
+void clear_timers(bf_ctx* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->timers)
+        for (auto& pr : kv.second.pending) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    ctx->timers.clear();
+}
+
 double stream_copy_probe(bf_ctx* ctx, int64_t bytes, int reps) {
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)bytes / sizeof(double2);

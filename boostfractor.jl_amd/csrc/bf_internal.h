@@ -67,6 +67,14 @@ struct DevBuf {
 
 }  // namespace bf
 
+// Launch timing of one kernel class: HIP events recorded on the context's
+// stream around each launch, resolved into milliseconds when queried.
+struct bf_kernel_timer {
+    double ms = 0.0, flops = 0.0;
+    int64_t launches = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
 // Per-caller context: one device, one stream, cached twiddle tables.
 struct bf_ctx {
     int device = 0;
@@ -74,9 +82,37 @@ struct bf_ctx {
     std::string err;
     // twiddle tables keyed by (n, bytes per real)
     std::map<std::pair<int, int>, bf::DevBuf> twiddles;
+    bool timing = false;
+    std::map<std::string, bf_kernel_timer> timers;
 };
 
 namespace bf {
+
+// Brackets one launch of kernel class `name` with events while the context's
+// timing is on; `flops` is the launch's algorithmic work.  Construct it right
+// before the launch, destroy it right after.
+struct TimedLaunch {
+    bf_ctx* ctx;
+    hipEvent_t e1 = nullptr;
+    TimedLaunch(bf_ctx* c, const char* name, double flops) : ctx(c) {
+        if (!c->timing) return;
+        hipEvent_t e0;
+        BF_HIP(hipEventCreate(&e0));
+        BF_HIP(hipEventCreate(&e1));
+        BF_HIP(hipEventRecord(e0, c->stream));
+        bf_kernel_timer& t = c->timers[name];
+        t.pending.push_back({e0, e1});
+        t.flops += flops;
+        t.launches += 1;
+    }
+    ~TimedLaunch() {
+        if (e1) (void)hipEventRecord(e1, ctx->stream);
+    }
+    TimedLaunch(const TimedLaunch&) = delete;
+    TimedLaunch& operator=(const TimedLaunch&) = delete;
+};
+void resolve_timers(bf_ctx* ctx);
+void clear_timers(bf_ctx* ctx);
 
 // Device-resident coordinate grids.
 struct Grid {

@@ -219,9 +219,11 @@ typedef double __attribute__((ext_vector_type(4))) d4;
 
 constexpr int KC = 32;          // k values staged per chunk
 constexpr int LDA = KC + 1;     // padded LDS row (complex elements)
-constexpr int TILES_PER_WAVE = 4;
+constexpr int TILES_PER_WAVE = 2;  // 2 tiles x 4 accumulators x 8 registers: no spills at 1024 threads
 
-// grid: (splits, problems); block: 64 * waves.  Output partial [prob][split][n][n] col-major.
+// grid: (splits, problems, tile batches); block: 64 * waves; batch z covers the
+// output tiles [z * waves * TILES_PER_WAVE, ...).  Output partial
+// [prob][split][n][n] col-major.
 __global__ void __launch_bounds__(1024) k_wgram(const GemmProb* __restrict__ probs, int n, size_t Ktot, size_t Kw,
                                                  size_t kper, zd* partial) {
     extern __shared__ unsigned char smem_raw[];
@@ -242,6 +244,7 @@ __global__ void __launch_bounds__(1024) k_wgram(const GemmProb* __restrict__ pro
         for (int q = 0; q < 4; q++) acc[t][q] = d4{0.0, 0.0, 0.0, 0.0};
 
     const int ntiles = T * T;
+    const int tile0 = blockIdx.z * nwaves * TILES_PER_WAVE;
     for (size_t k0 = k_begin; k0 < k_end; k0 += KC) {
         // stage A and (weighted) B: row i = mode, column kk
         for (int e = threadIdx.x; e < npad * KC; e += blockDim.x) {
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(1024) k_wgram(const GemmProb* __restrict__ pro
         __syncthreads();
         const zd* Bsrc = same ? As : Bs;
         for (int t = 0; t < TILES_PER_WAVE; t++) {
-            const int tile = wave + t * nwaves;
+            const int tile = tile0 + wave + t * nwaves;
             if (tile >= ntiles) break;
             const int ti = tile / T, tj = tile % T;
 #pragma unroll
@@ -281,9 +284,103 @@ __global__ void __launch_bounds__(1024) k_wgram(const GemmProb* __restrict__ pro
     // D layout (v_mfma_f64_16x16x4f64): register v of lane holds D[4*v + lane/16][lane%16]
     zd* out = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (size_t)n * n;
     for (int t = 0; t < TILES_PER_WAVE; t++) {
-        const int tile = wave + t * nwaves;
+        const int tile = tile0 + wave + t * nwaves;
         if (tile >= ntiles) break;
         const int ti = tile / T, tj = tile % T;
+        const int j = tj * 16 + (lane & 15);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = ti * 16 + 4 * v + (lane >> 4);
+            if (i < n && j < n)
+                out[i + (size_t)n * j] = cmk<double>(acc[t][0][v] + acc[t][1][v], acc[t][2][v] - acc[t][3][v]);
+        }
+    }
+}
+
+// Grouped variant: the problems of a group share A and B (every flat air region
+// of every frequency shares FFT(Phi) and FFT(mask Phi); every disk region shares
+// Phi), so a block stages one chunk of A and B for up to WG_G problems and
+// applies each problem's weight to B on the fly.  A wave owns WG_TPW output
+// tiles (problem, ti, tj).  Same output layout as k_wgram.
+constexpr int WG_G = 4;
+constexpr int WG_TPW = 3;
+
+struct GemmGroup {
+    const zd* A;
+    const zd* B;
+    int first;  // index of the group's first problem in the problem list
+    int count;  // problems in the group (<= WG_G)
+};
+
+__global__ void __launch_bounds__(1024) k_wgram_grouped(const GemmGroup* __restrict__ groups,
+                                                         const GemmProb* __restrict__ probs, int n, size_t Ktot,
+                                                         size_t Kw, size_t kper, zd* partial) {
+    extern __shared__ unsigned char smem_raw[];
+    const GemmGroup gr = groups[blockIdx.y];
+    const int T = (n + 15) / 16;
+    const int npad = T * 16, tpp = T * T;
+    zd* As = reinterpret_cast<zd*>(smem_raw);  // [npad][LDA]
+    zd* Bs = As + (size_t)npad * LDA;          // [npad][LDA]
+    zd* Ws = Bs + (size_t)npad * LDA;          // [WG_G][KC]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const size_t k_begin = (size_t)blockIdx.x * kper;
+    const size_t k_end = std::min(Ktot, k_begin + kper);
+    const int ntiles = gr.count * tpp;
+
+    d4 acc[WG_TPW][4];
+#pragma unroll
+    for (int t = 0; t < WG_TPW; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[t][q] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (size_t k0 = k_begin; k0 < k_end; k0 += KC) {
+        for (int e = threadIdx.x; e < npad * KC; e += blockDim.x) {
+            const int i = e / KC, kk = e % KC;
+            const size_t k = k0 + kk;
+            zd a = cmk<double>(0.0, 0.0), b = cmk<double>(0.0, 0.0);
+            if (i < n && k < k_end) {
+                a = gr.A[(size_t)i * Ktot + k];
+                b = gr.B[(size_t)i * Ktot + k];
+            }
+            As[i * LDA + kk] = a;
+            Bs[i * LDA + kk] = b;
+        }
+        for (int e = threadIdx.x; e < WG_G * KC; e += blockDim.x) {
+            const int g = e / KC, kk = e % KC;
+            const size_t k = k0 + kk;
+            zd w = cmk<double>(0.0, 0.0);
+            if (g < gr.count && k < k_end) {
+                const zd* wp = probs[gr.first + g].w;
+                w = wp ? wp[k % Kw] : cmk<double>(1.0, 0.0);
+            }
+            Ws[g * KC + kk] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < WG_TPW; t++) {
+            const int tile = wave + t * nwaves;
+            if (tile >= ntiles) break;
+            const int g = tile / tpp, tt = tile % tpp;
+            const int ti = tt / T, tj = tt % T;
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 4) {
+                const int kq = kk + (lane >> 4);
+                const zd a = As[(ti * 16 + (lane & 15)) * LDA + kq];
+                const zd b = cmul(Ws[g * KC + kq], Bs[(tj * 16 + (lane & 15)) * LDA + kq]);
+                acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.im, acc[t][1], 0, 0, 0);
+                acc[t][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, acc[t][2], 0, 0, 0);
+                acc[t][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, acc[t][3], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    for (int t = 0; t < WG_TPW; t++) {
+        const int tile = wave + t * nwaves;
+        if (tile >= ntiles) break;
+        const int g = tile / tpp, tt = tile % tpp;
+        const int ti = tt / T, tj = tt % T;
+        zd* out = partial + ((size_t)(gr.first + g) * gridDim.x + blockIdx.x) * (size_t)n * n;
         const int j = tj * 16 + (lane & 15);
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -535,7 +632,7 @@ struct Gram {
     bf_ctx* ctx;
     int n;
     size_t Ktot, Kw;
-    DevBuf dprobs, partial;
+    DevBuf dprobs, dgroups, partial;
     void run(const std::vector<GemmProb>& probs, zd* P) {
         if (probs.empty()) return;
         hipStream_t s = ctx->stream;
@@ -548,15 +645,65 @@ struct Gram {
         size_t kper = (Ktot + splits - 1) / splits;
         kper = (kper + KC - 1) / KC * KC;
         splits = (Ktot + kper - 1) / kper;
+        const size_t nn = (size_t)n * n;
+        const int gmax = std::min(WG_G, 16 * WG_TPW / (T * T));
+        if (gmax >= 2) {
+            // problems sorted so that groups share A and B; a group holds <= gmax of them
+            std::vector<int> order(np);
+            for (int q = 0; q < np; q++) order[q] = q;
+            std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+                return std::make_pair(probs[x].A, probs[x].B) < std::make_pair(probs[y].A, probs[y].B);
+            });
+            std::vector<GemmProb> sorted(np);
+            for (int q = 0; q < np; q++) sorted[q] = probs[order[q]];
+            std::vector<GemmGroup> groups;
+            for (int q = 0; q < np;) {
+                int c = 1;
+                while (q + c < np && c < gmax && sorted[q + c].A == sorted[q].A && sorted[q + c].B == sorted[q].B) c++;
+                groups.push_back(GemmGroup{sorted[q].A, sorted[q].B, q, c});
+                q += c;
+            }
+            const int ng = (int)groups.size();
+            // enough blocks for several rounds of the GPU, so the last round's idle share is small
+            size_t gsplits = std::max<size_t>(1, (4096 + ng - 1) / ng);
+            gsplits = std::min(gsplits, (Ktot + KC - 1) / KC);
+            size_t gkper = (Ktot + gsplits - 1) / gsplits;
+            gkper = (gkper + KC - 1) / KC * KC;
+            gsplits = (Ktot + gkper - 1) / gkper;
+            dprobs.alloc(sizeof(GemmProb) * np);
+            upload(dprobs.p, sorted.data(), sizeof(GemmProb) * np, s);
+            dgroups.alloc(sizeof(GemmGroup) * ng);
+            upload(dgroups.p, groups.data(), sizeof(GemmGroup) * ng, s);
+            if (partial.bytes < sizeof(zd) * nn * gsplits * np) partial.alloc(sizeof(zd) * nn * gsplits * np);
+            const int gwaves = std::max(1, std::min(16, (gmax * T * T + WG_TPW - 1) / WG_TPW));
+            const size_t smem = sizeof(zd) * ((size_t)T * 16 * LDA * 2 + (size_t)WG_G * KC);
+            allow_lds((const void*)k_wgram_grouped, smem);
+            {
+                TimedLaunch tl(ctx, "wgram", 8.0 * (double)nn * (double)Ktot * np);
+                hipLaunchKernelGGL(k_wgram_grouped, dim3((unsigned)gsplits, ng), dim3(64 * gwaves), smem, s,
+                                   dgroups.as<GemmGroup>(), dprobs.as<GemmProb>(), n, Ktot, Kw, gkper,
+                                   partial.as<zd>());
+                BF_CHECK_LAUNCH();
+            }
+            hipLaunchKernelGGL(k_wgram_reduce, dim3(g1(nn, 64), np), dim3(256), 0, s, dprobs.as<GemmProb>(),
+                               partial.as<zd>(), (int)gsplits, (int)nn, P);
+            BF_CHECK_LAUNCH();
+            BF_HIP(hipStreamSynchronize(s));  // dprobs/dgroups/partial are reused by the next call
+            return;
+        }
         dprobs.alloc(sizeof(GemmProb) * np);
         upload(dprobs.p, probs.data(), sizeof(GemmProb) * np, s);
-        const size_t nn = (size_t)n * n;
         if (partial.bytes < sizeof(zd) * nn * splits * np) partial.alloc(sizeof(zd) * nn * splits * np);
         const size_t smem = sizeof(zd) * (size_t)T * 16 * LDA * 2;
         allow_lds((const void*)k_wgram, smem);
-        hipLaunchKernelGGL(k_wgram, dim3((unsigned)splits, np), dim3(64 * waves), smem, s, dprobs.as<GemmProb>(), n,
-                           Ktot, Kw, kper, partial.as<zd>());
-        BF_CHECK_LAUNCH();
+        {
+            // algorithmic work: one complex multiply-add (8 flop) per (i, j, k) and problem
+            TimedLaunch tl(ctx, "wgram", 8.0 * (double)nn * (double)Ktot * np);
+            const int zb = (T * T + waves * TILES_PER_WAVE - 1) / (waves * TILES_PER_WAVE);
+            hipLaunchKernelGGL(k_wgram, dim3((unsigned)splits, np, zb), dim3(64 * waves), smem, s,
+                               dprobs.as<GemmProb>(), n, Ktot, Kw, kper, partial.as<zd>());
+            BF_CHECK_LAUNCH();
+        }
         hipLaunchKernelGGL(k_wgram_reduce, dim3(g1(nn, 64), np), dim3(256), 0, s, dprobs.as<GemmProb>(),
                            partial.as<zd>(), (int)splits, (int)nn, P);
         BF_CHECK_LAUNCH();
@@ -858,9 +1005,12 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
         for (int r = 1; r < nreg; r++) sel.push_back(f * nreg + r);
     DevBuf dsel(sizeof(int) * sel.size()), work(sizeof(zd) * nn * sel.size());
     upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
-    hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P.as<zd>(), Pinv.as<zd>(), n,
-                       work.as<zd>(), dsel.as<int>());
-    BF_CHECK_LAUNCH();
+    {
+        TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
+        hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P.as<zd>(),
+                           Pinv.as<zd>(), n, work.as<zd>(), dsel.as<int>());
+        BF_CHECK_LAUNCH();
+    }
     // chain scalars per frequency: s = Nreg-1 .. 1
     const int nsteps = nreg - 1;
     std::vector<ChainStep> steps((size_t)nfreq * nsteps);
@@ -891,10 +1041,14 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
         drefl.alloc(sizeof(zd) * n * nfreq);
         upload(dref.p, reflect, sizeof(zd) * n, s);
     }
-    hipLaunchKernelGGL(k_chain, dim3(nfreq), dim3(256), 0, s, dsteps.as<ChainStep>(), nsteps, P.as<zd>(),
-                       Pinv.as<zd>(), nreg, n, de0.as<zd>(), dref.as<zd>(), Tb.as<zd>(), cw.as<zd>(), dboost.as<zd>(),
-                       drefl.as<zd>());
-    BF_CHECK_LAUNCH();
+    {
+        // per region: T (2n x 2n) times the structured boundary-propagation matrix, 64 n^3 flop
+        TimedLaunch tl(ctx, "chain", 64.0 * (double)n * n * n * (double)nsteps * nfreq);
+        hipLaunchKernelGGL(k_chain, dim3(nfreq), dim3(256), 0, s, dsteps.as<ChainStep>(), nsteps, P.as<zd>(),
+                           Pinv.as<zd>(), nreg, n, de0.as<zd>(), dref.as<zd>(), Tb.as<zd>(), cw.as<zd>(),
+                           dboost.as<zd>(), drefl.as<zd>());
+        BF_CHECK_LAUNCH();
+    }
     download(boost, dboost.p, sizeof(zd) * n * nfreq, s);
     if (reflect) download(refl, drefl.p, sizeof(zd) * n * nfreq, s);
     BF_HIP(hipStreamSynchronize(s));

@@ -93,6 +93,15 @@ int bf_ctx_destroy(bf_ctx *ctx);
 /* HIP stream of the context (hipStream_t), for callers that time or order work on it */
 int bf_ctx_stream(bf_ctx *ctx, void **stream);
 int bf_ctx_synchronize(bf_ctx *ctx);
+/* Per-kernel-class timing on the context's stream (no reference counterpart;
+ * for measurement): enable != 0 clears the counters and starts recording HIP
+ * events around the launches of each class, 0 stops.  bf_ctx_kernel_time
+ * returns the accumulated milliseconds, launch count and algorithmic flops of
+ * one class: "wgram" (propagation-matrix Gram products, 8 n^2 K flop per
+ * region and frequency), "inverse" (batched LU inverses), "chain"
+ * (transfer-matrix chain and solves, 64 n^3 flop per region and frequency). */
+int bf_ctx_timing(bf_ctx *ctx, int32_t enable);
+int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *launches, double *flops);
 
 /* ---- propagators --------------------------------------------------------
  * replaces prop(E0, dz, diskR, eps, tilt_x, tilt_y, surface, lambda, coords)

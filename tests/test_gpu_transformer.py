@@ -309,3 +309,30 @@ def test_transformer_reference_vs_cheerleader_near_resonance():
     E = B.cheerleader(0, 100, sb, coords, f=10e9, diskR=DISKR)
     bc = np.sum(np.abs(E) ** 2) * 1e-4 / (math.pi * DISKR ** 2)
     assert abs(bt / bc - 1) < 0.05
+
+
+@pytest.mark.parametrize("M,Lm", [(13, 2), (9, 4)])
+def test_propagation_matrices_many_modes(M, Lm):
+    """n = 65 and 81 modes: more output tiles than one block holds (the Gram
+    product runs in several tile batches); flat air (Parseval), tilted/rough
+    disks and explicitly propagated air regions, all against the oracle."""
+    coords, sb = _rough_booster(N=32)
+    sb.relative_tilt_x[1] = 2e-4  # one tilted air region: propagated explicitly
+    modes = _lg_like_modes(coords, M, Lm)
+    got = B.calc_propagation_matrices(sb, coords, modes, f=20.1e9, diskR=DISKR)
+    want = _oracle_mats(sb, coords, modes, 20.1e9, O.PROP_PROPAGATOR)
+    for r in range(1, len(sb.distance)):
+        assert rel(got[r], want[r]) < TOL, (r, rel(got[r], want[r]))
+
+
+def test_propagation_matrices_grouped_batches():
+    """45 modes, several frequencies: problems sharing A and B are grouped (up to
+    four per block) with a partial last group; every matrix against the oracle."""
+    coords, sb = _rough_booster(N=32, ndisk=3)
+    modes = _lg_like_modes(coords, 5, 4)
+    freqs = [19.7e9, 20.0e9, 20.3e9]
+    got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    for q, f in enumerate(freqs):
+        want = _oracle_mats(sb, coords, modes, f, O.PROP_PROPAGATOR)
+        for r in range(1, len(sb.distance)):
+            assert rel(got[q][r], want[r]) < TOL, (q, r, rel(got[q][r], want[r]))
