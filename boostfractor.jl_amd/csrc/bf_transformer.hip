@@ -312,20 +312,74 @@ struct GemmGroup {
     int count;  // problems in the group (<= WG_G)
 };
 
-__global__ void __launch_bounds__(1024) k_wgram_grouped(const GemmGroup* __restrict__ groups,
-                                                         const GemmProb* __restrict__ probs, int n, size_t Ktot,
-                                                         size_t Kw, size_t kper, zd* partial) {
+constexpr int WG_WAVES = 12;  // waves per block at most (tiles per block <= WG_WAVES * WG_TPW)
+
+// launch shape for T x T output tiles per problem
+template <int T>
+struct WgShape {
+    static constexpr int G = (WG_G < WG_WAVES * WG_TPW / (T * T)) ? WG_G : WG_WAVES * WG_TPW / (T * T);
+    static constexpr int WAVES = (G * T * T + WG_TPW - 1) / WG_TPW;
+    static constexpr int Q = (16 * T * KC + 64 * WAVES - 1) / (64 * WAVES);  // staged elements per thread
+    static_assert(G >= 1 && WAVES <= WG_WAVES && G * KC <= 64 * WAVES, "Gram block shape");
+};
+
+// Double-buffered: while the waves run the MFMAs of chunk c from one LDS
+// buffer, chunk c+1 travels from HBM into registers; it is written to the
+// other buffer afterwards, one barrier per chunk.
+template <int T>
+__global__ void __launch_bounds__(64 * WgShape<T>::WAVES) k_wgram_grouped(const GemmGroup* __restrict__ groups,
+                                                                           const GemmProb* __restrict__ probs, int n,
+                                                                           size_t Ktot, size_t Kw, size_t kper,
+                                                                           zd* partial) {
     extern __shared__ unsigned char smem_raw[];
+    constexpr int WG_Q = WgShape<T>::Q;
     const GemmGroup gr = groups[blockIdx.y];
-    const int T = (n + 15) / 16;
-    const int npad = T * 16, tpp = T * T;
-    zd* As = reinterpret_cast<zd*>(smem_raw);  // [npad][LDA]
-    zd* Bs = As + (size_t)npad * LDA;          // [npad][LDA]
-    zd* Ws = Bs + (size_t)npad * LDA;          // [WG_G][KC]
+    constexpr int npad = T * 16, tpp = T * T;
+    const size_t buf = (size_t)npad * LDA * 2 + (size_t)WG_G * KC;  // one stage: A, B, weights
+    zd* base = reinterpret_cast<zd*>(smem_raw);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const size_t k_begin = (size_t)blockIdx.x * kper;
     const size_t k_end = std::min(Ktot, k_begin + kper);
     const int ntiles = gr.count * tpp;
+    // the weight vector this thread stages (threads < WG_G * KC)
+    const int wg = threadIdx.x / KC;
+    const zd* wsrc = (wg < gr.count) ? probs[gr.first + wg].w : nullptr;
+
+    zd ra[WG_Q], rb[WG_Q], rw;
+    auto fetch = [&](size_t k0) {
+#pragma unroll
+        for (int q = 0; q < WG_Q; q++) {
+            const int e = threadIdx.x + q * blockDim.x;
+            const int i = e / KC, kk = e % KC;
+            const size_t k = k0 + kk;
+            ra[q] = cmk<double>(0.0, 0.0);
+            rb[q] = cmk<double>(0.0, 0.0);
+            if (e < npad * KC && i < n && k < k_end) {
+                ra[q] = gr.A[(size_t)i * Ktot + k];
+                rb[q] = gr.B[(size_t)i * Ktot + k];
+            }
+        }
+        rw = cmk<double>(0.0, 0.0);
+        if ((int)threadIdx.x < WG_G * KC) {
+            const size_t k = k0 + threadIdx.x % KC;
+            if (wg < gr.count && k < k_end) rw = wsrc ? wsrc[k % Kw] : cmk<double>(1.0, 0.0);
+        }
+    };
+    auto store = [&](zd* st) {
+        zd* As = st;
+        zd* Bs = st + (size_t)npad * LDA;
+        zd* Ws = Bs + (size_t)npad * LDA;
+#pragma unroll
+        for (int q = 0; q < WG_Q; q++) {
+            const int e = threadIdx.x + q * blockDim.x;
+            if (e < npad * KC) {
+                const int i = e / KC, kk = e % KC;
+                As[i * LDA + kk] = ra[q];
+                Bs[i * LDA + kk] = rb[q];
+            }
+        }
+        if ((int)threadIdx.x < WG_G * KC) Ws[threadIdx.x] = rw;
+    };
 
     d4 acc[WG_TPW][4];
 #pragma unroll
@@ -333,29 +387,18 @@ __global__ void __launch_bounds__(1024) k_wgram_grouped(const GemmGroup* __restr
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[t][q] = d4{0.0, 0.0, 0.0, 0.0};
 
+    int stage = 0;
+    if (k_begin < k_end) {
+        fetch(k_begin);
+        store(base);
+    }
+    __syncthreads();
     for (size_t k0 = k_begin; k0 < k_end; k0 += KC) {
-        for (int e = threadIdx.x; e < npad * KC; e += blockDim.x) {
-            const int i = e / KC, kk = e % KC;
-            const size_t k = k0 + kk;
-            zd a = cmk<double>(0.0, 0.0), b = cmk<double>(0.0, 0.0);
-            if (i < n && k < k_end) {
-                a = gr.A[(size_t)i * Ktot + k];
-                b = gr.B[(size_t)i * Ktot + k];
-            }
-            As[i * LDA + kk] = a;
-            Bs[i * LDA + kk] = b;
-        }
-        for (int e = threadIdx.x; e < WG_G * KC; e += blockDim.x) {
-            const int g = e / KC, kk = e % KC;
-            const size_t k = k0 + kk;
-            zd w = cmk<double>(0.0, 0.0);
-            if (g < gr.count && k < k_end) {
-                const zd* wp = probs[gr.first + g].w;
-                w = wp ? wp[k % Kw] : cmk<double>(1.0, 0.0);
-            }
-            Ws[g * KC + kk] = w;
-        }
-        __syncthreads();
+        const bool more = k0 + KC < k_end;
+        if (more) fetch(k0 + KC);
+        const zd* As = base + (size_t)stage * buf;
+        const zd* Bs = As + (size_t)npad * LDA;
+        const zd* Ws = Bs + (size_t)npad * LDA;
 #pragma unroll
         for (int t = 0; t < WG_TPW; t++) {
             const int tile = wave + t * nwaves;
@@ -373,7 +416,9 @@ __global__ void __launch_bounds__(1024) k_wgram_grouped(const GemmGroup* __restr
                 acc[t][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, acc[t][3], 0, 0, 0);
             }
         }
-        __syncthreads();
+        if (more) store(base + (size_t)(stage ^ 1) * buf);
+        __syncthreads();  // the next chunk is staged; every wave is done with this one
+        stage ^= 1;
     }
     for (int t = 0; t < WG_TPW; t++) {
         const int tile = wave + t * nwaves;
@@ -633,6 +678,13 @@ struct Gram {
     int n;
     size_t Ktot, Kw;
     DevBuf dprobs, dgroups, partial;
+    template <int T>
+    void launch_grouped(size_t smem, dim3 grid, int n_, size_t kper) {
+        allow_lds((const void*)k_wgram_grouped<T>, smem);
+        hipLaunchKernelGGL(k_wgram_grouped<T>, grid, dim3(64 * WgShape<T>::WAVES), smem, ctx->stream,
+                           dgroups.as<GemmGroup>(), dprobs.as<GemmProb>(), n_, Ktot, Kw, kper, partial.as<zd>());
+        BF_CHECK_LAUNCH();
+    }
     void run(const std::vector<GemmProb>& probs, zd* P) {
         if (probs.empty()) return;
         hipStream_t s = ctx->stream;
@@ -646,8 +698,9 @@ struct Gram {
         kper = (kper + KC - 1) / KC * KC;
         splits = (Ktot + kper - 1) / kper;
         const size_t nn = (size_t)n * n;
-        const int gmax = std::min(WG_G, 16 * WG_TPW / (T * T));
-        if (gmax >= 2) {
+        // grouped kernel for n <= 48 (at T = 4 its double buffer would not fit the registers)
+        const int gmax = T == 1 ? WgShape<1>::G : T == 2 ? WgShape<2>::G : T == 3 ? WgShape<3>::G : 1;
+        if (T <= 3 && gmax >= 2) {
             // problems sorted so that groups share A and B; a group holds <= gmax of them
             std::vector<int> order(np);
             for (int q = 0; q < np; q++) order[q] = q;
@@ -675,15 +728,14 @@ struct Gram {
             dgroups.alloc(sizeof(GemmGroup) * ng);
             upload(dgroups.p, groups.data(), sizeof(GemmGroup) * ng, s);
             if (partial.bytes < sizeof(zd) * nn * gsplits * np) partial.alloc(sizeof(zd) * nn * gsplits * np);
-            const int gwaves = std::max(1, std::min(16, (gmax * T * T + WG_TPW - 1) / WG_TPW));
-            const size_t smem = sizeof(zd) * ((size_t)T * 16 * LDA * 2 + (size_t)WG_G * KC);
-            allow_lds((const void*)k_wgram_grouped, smem);
+            const size_t smem = 2 * sizeof(zd) * ((size_t)T * 16 * LDA * 2 + (size_t)WG_G * KC);
             {
                 TimedLaunch tl(ctx, "wgram", 8.0 * (double)nn * (double)Ktot * np);
-                hipLaunchKernelGGL(k_wgram_grouped, dim3((unsigned)gsplits, ng), dim3(64 * gwaves), smem, s,
-                                   dgroups.as<GemmGroup>(), dprobs.as<GemmProb>(), n, Ktot, Kw, gkper,
-                                   partial.as<zd>());
-                BF_CHECK_LAUNCH();
+                switch (T) {
+                    case 1: launch_grouped<1>(smem, dim3((unsigned)gsplits, ng), n, gkper); break;
+                    case 2: launch_grouped<2>(smem, dim3((unsigned)gsplits, ng), n, gkper); break;
+                    default: launch_grouped<3>(smem, dim3((unsigned)gsplits, ng), n, gkper); break;
+                }
             }
             hipLaunchKernelGGL(k_wgram_reduce, dim3(g1(nn, 64), np), dim3(256), 0, s, dprobs.as<GemmProb>(),
                                partial.as<zd>(), (int)gsplits, (int)nn, P);

@@ -311,11 +311,12 @@ def test_transformer_reference_vs_cheerleader_near_resonance():
     assert abs(bt / bc - 1) < 0.05
 
 
-@pytest.mark.parametrize("M,Lm", [(13, 2), (9, 4)])
+@pytest.mark.parametrize("M,Lm", [(7, 3), (13, 2), (9, 4)])
 def test_propagation_matrices_many_modes(M, Lm):
-    """n = 65 and 81 modes: more output tiles than one block holds (the Gram
-    product runs in several tile batches); flat air (Parseval), tilted/rough
-    disks and explicitly propagated air regions, all against the oracle."""
+    """n = 49, 65 and 81 modes (4x4 to 6x6 output tiles of 16 per problem: the
+    ungrouped Gram kernel, in several tile batches for the larger two); flat
+    air (Parseval), tilted/rough disks and an explicitly propagated air region,
+    all against the oracle."""
     coords, sb = _rough_booster(N=32)
     sb.relative_tilt_x[1] = 2e-4  # one tilted air region: propagated explicitly
     modes = _lg_like_modes(coords, M, Lm)
@@ -325,11 +326,14 @@ def test_propagation_matrices_many_modes(M, Lm):
         assert rel(got[r], want[r]) < TOL, (r, rel(got[r], want[r]))
 
 
-def test_propagation_matrices_grouped_batches():
-    """45 modes, several frequencies: problems sharing A and B are grouped (up to
-    four per block) with a partial last group; every matrix against the oracle."""
+@pytest.mark.parametrize("M,Lm", [(3, 1), (5, 2), (5, 4)])
+def test_propagation_matrices_grouped_batches(M, Lm):
+    """n = 9, 25, 45 modes (1x1, 2x2, 3x3 output tiles: the grouped, double-
+    buffered Gram kernel), several frequencies: problems sharing A and B are
+    grouped (up to four per block) with a partial last group; every matrix
+    against the oracle."""
     coords, sb = _rough_booster(N=32, ndisk=3)
-    modes = _lg_like_modes(coords, 5, 4)
+    modes = _lg_like_modes(coords, M, Lm)
     freqs = [19.7e9, 20.0e9, 20.3e9]
     got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
     for q, f in enumerate(freqs):
