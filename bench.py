@@ -50,6 +50,7 @@ NDISK, N, NFREQ, NMAX, DISKR = 20, 256, 50, 2000, 0.15
 # 45 Laguerre-Gauss modes on 512^2, 200 frequencies
 N3, NF3 = 512, 200
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cols_reg_dancer_mixed_256x256_20disk.md")
+TRAFFIC_PROFILE_TR = os.path.join(ROOT, "profiles", "k_wgram_grouped_transformer_512x512_45modes.md")
 
 
 def pmc_per_kernel(db_path, counter):
@@ -84,12 +85,12 @@ def summarize_trace(db_path, top=12):
     return "\n".join(lines)
 
 
-def traffic_from_profile(kernel_key, fb):
+def traffic_from_profile(kernel_key, fb, path=TRAFFIC_PROFILE):
     """HBM bytes per launch of the dominant kernel from the committed counter
     summary, calibrated by the copy kernel's readings and scaled to the
     frequency batch used here (a launch moves bytes in proportion to it)."""
     try:
-        txt = open(TRAFFIC_PROFILE).read()
+        txt = open(path).read()
     except OSError:
         return None
     m = re.search(r"calibration: copy_bytes=(\d+) fetch_kib=([\d.]+) write_kib=([\d.]+)", txt)
@@ -203,6 +204,7 @@ def transformer_main(args):
     ctx.timing(False)
     ms, launches, flops = kt_["wgram"]
     achieved = flops / (ms * 1e-3) / 1e12
+    bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)  # also the byte-counter calibration of profiler runs
     beta2 = np.sum(np.abs(boost) ** 2, axis=-1)
     out = {
         "metric": "transformer boost-factor freq points/s on 1 MI355X (FP64)",
@@ -222,10 +224,13 @@ def transformer_main(args):
                                f"1 m, {n} Laguerre-Gauss modes (p<=4, |l|<=4), {NF3} freqs 19.5-20.5 GHz",
                    "grid": N3, "modes": n, "frequencies": NF3},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4),
+                     "traffic": (lambda t: None if t is None else int(t))(
+                         traffic_from_profile("k_wgram_grouped", 1, TRAFFIC_PROFILE_TR)),
                      "kernel": "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)",
                      "avg_launch_ms": round(ms / max(launches, 1), 3),
-                     "alg_flops_per_launch": flops / max(launches, 1)},
+                     "alg_flops_per_launch": flops / max(launches, 1),
+                     "measured_stream_copy_GBs": round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)},
         "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
         "beta2": {"min": float(beta2.min()), "max": float(beta2.max())},
     }
