@@ -10,8 +10,9 @@ all frequencies as one batched, device-resident sweep.  `value` is the mixed
 precision sweep's points/s; the same sweep in FP64 is timed beside it and the
 relative beta^2 error of every point against FP64 is reported.
 
-Kernel timing: HIP events recorded on the engine's stream around every launch
-of the two bounce kernels during the timed steps.
+Kernel timing: the timed steps run without instrumentation; one more step
+then records HIP events on the engine's stream around every launch of the two
+bounce kernels, which gives the kernels' mean launch times.
 
 Roofline: `achieved` counts the algorithmic bytes of the dancer bounce, every
 field read once and written once per bounce and frequency (2 F K B_c with
@@ -113,19 +114,22 @@ def workload():
 
 
 def timed_steps(plan, ctx, steps, warmup, timing):
+    """wall time of `steps` uninstrumented runs; with `timing`, per-kernel HIP
+    event times of one extra instrumented run"""
     for _ in range(warmup):
         plan.run()
-    ctx.synchronize()
-    if timing:
-        plan.timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    kt = plan.kernel_times() if timing else None
+    kt = None
     if timing:
+        plan.timing(True)
+        plan.run()
+        ctx.synchronize()
+        kt = plan.kernel_times()
         plan.timing(False)
     return dt, kt
 
@@ -314,7 +318,7 @@ def main():
     dom = max(kt32, key=lambda k: kt32[k][0])
     ms_tot, launches = kt32[dom]
     avg_ms = ms_tot / launches
-    nb_per_launch = NFREQ * args.steps * (nmax + 1) / launches  # frequency-bounces per launch
+    nb_per_launch = NFREQ * (nmax + 1) / launches  # frequency-bounces per launch (one instrumented run)
     alg_bytes = nb_per_launch * 2 * F * K * 8  # every field read once and written once, complex64
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic = traffic_from_profile("k_cols_reg", fb32) if dom == "col_pass" else None

@@ -1665,8 +1665,11 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         tr[i] = cmk<double>(1.0 + r[i].re, r[i].im);
     }
     const size_t per = sizeof(cplx<S>) * K * Nd;
-    int fb = (int)std::max<size_t>(1, (192u << 20) / std::max<size_t>(per * 3, 1));
-    fb = std::max(1, std::min(fb, std::min(nfr, 64)));
+    // Gauss-Seidel is sequential over regions: a launch touches only the one or
+    // two fields of a region per frequency, so batch as many frequencies as an
+    // 8 GB state allows to give each launch enough work-groups.
+    int fb = (int)std::max<size_t>(1, ((size_t)8 << 30) / std::max<size_t>(per * 3, 1));
+    fb = std::max(1, std::min(fb, std::min(nfr, 256)));
     DevBuf F(per * fb), EM(per * 2 * fb), scr, dax, demit, dref, out(sizeof(zd) * K * fb), act(sizeof(int) * fb),
         amx(sizeof(double) * fb), dfd, dph;
     if (kind == BF_PROP_MOMENTUM) scr.alloc(per * fb);
