@@ -348,6 +348,9 @@ def main():
                      "traffic": traffic if traffic is None else int(traffic),
                      "kernel": names[dom], "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
+                     # the same algorithmic bytes over the whole step (both bounce kernels, the gaps between
+                     # launches and the per-batch setup): the rate a user of the sweep sees
+                     "whole_step_GBs": round(NFREQ * (nmax + 1) * 2 * F * K * 8 / (dt32 / args.steps) / 1e9, 1),
                      "measured_stream_copy_GBs": round(stream_gbs, 1)},
         "work_reductions": {"row_band": list(info32["band"]), "rows": N,
                             "note": "rows outside the aperture band skipped by both passes (exit field kept whole); "
