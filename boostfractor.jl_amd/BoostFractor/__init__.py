@@ -9,7 +9,8 @@ in libboostfractor_hip.so (hand-written gfx950 HIP kernels) through its C ABI
 from .setup import (CoordinateSystem, DimensionMismatch, SeedCoordinateSystem, SeedSetupBoundaries,
                     SetupBoundaries, get_kspace_coords, wavelength)
 from .jlrange import julia_range
-from ._lib import ArgumentError, HipError, UnsupportedError, Context, context, device_count
+from ._lib import ArgumentError, HipError, InexactError, UnsupportedError, Context, context, device_count
+from .beam import gauss_profile, unnormalized_match
 from .propagators import propagator, propagator1D, propagatorMomentumSpace, propagatorNoTilts
 from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
 from .transformer import (ErrorException, Modes, SeedModes, axion_induced_modes, calc_propagation_matrices,
@@ -22,5 +23,6 @@ __all__ = [
     "context", "device_count", "propagator", "propagatorNoTilts", "propagatorMomentumSpace", "propagator1D",
     "dancer", "dance_intro", "cheerleader", "DancerPlan", "Modes", "SeedModes", "mode", "index",
     "e_field_dimensions", "axion_induced_modes", "field2modes", "modes2field", "propagation_matrix",
-    "calc_propagation_matrices", "transformer", "transformer_trace_back", "ErrorException",
+    "calc_propagation_matrices", "transformer", "transformer_trace_back", "ErrorException", "InexactError",
+    "gauss_profile", "unnormalized_match",
 ]

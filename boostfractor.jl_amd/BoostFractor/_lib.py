@@ -27,6 +27,10 @@ class UnsupportedError(NotImplementedError):
     pass
 
 
+class InexactError(ValueError):
+    """Julia's InexactError (a complex value stored into a Float64 array)"""
+
+
 class HipError(RuntimeError):
     pass
 
@@ -59,7 +63,9 @@ _SIGS = {
     "bf_ctx_stream": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "bf_ctx_synchronize": (ctypes.c_int, [_P]),
     "bf_propagate": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Coords), _P, _D, _D, _D, _D, _D, _D, _P, _D]),
-    "bf_dance_intro": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _P, _D, _D, _D, _P]),
+    "bf_dance_intro": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _P, _D, _D, _D, _P, _D, _D, _P]),
+    "bf_gauss_profile": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _D, _D, _D, _P]),
+    "bf_unnormalized_match": (ctypes.c_int, [_P, _I32, ctypes.c_int64, _P, _P, _D, _D, _P]),
     "bf_dancer": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32, _P, _P,
                                  _D, _I32, _I32, _I32, _P, _P, _P]),
     "bf_cheerleader": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32,

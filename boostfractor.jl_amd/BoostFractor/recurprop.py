@@ -20,19 +20,33 @@ def _freqs(f):
 
 
 def dance_intro(bdry, coords, bfield=None, velocity_x=0, f=10e9, diskR=0.1):
-    """src/boost_fractor_recurprop.jl:153-201 -> fields (length(distance), 2, ny, nx)."""
-    if bfield is not None or velocity_x != 0:
-        # With the default Float64 B field the reference's `bfield .*= exp(...)` throws InexactError.
-        raise L.UnsupportedError("dance_intro with a custom bfield or velocity_x is not supported by the engine")
+    """src/boost_fractor_recurprop.jl:153-201 -> fields (length(distance), 2, ny, nx).
+
+    bfield: (length(distance)+1, ny, nx) or None (the real aperture mask).  A
+    nonzero velocity_x multiplies bfield by exp(-1im*Ma*(-velocity_x)*x) in place,
+    as the reference does; with a real bfield (or the default mask) the
+    reference raises InexactError, and so does this."""
     m = L.Marshalled()
     c = m.coords(coords)
     nd = len(bdry.distance)
     eps = m.hold(L.as_c128(bdry.eps[:nd]))
     r1 = complex(bdry.r[0])
     out = np.zeros((nd, 2, coords.ny, coords.nx), dtype=np.complex128)
+    b = None
+    if bfield is not None:
+        arr = np.asarray(bfield)
+        if arr.shape != (nd + 1, coords.ny, coords.nx):
+            raise L.DimensionMismatch(f"bfield has shape {arr.shape}, expected {(nd + 1, coords.ny, coords.nx)}")
+        if velocity_x != 0 and not np.iscomplexobj(arr):
+            raise L.InexactError("velocity_x needs a complex bfield: the phase cannot be stored in a real array")
+        b = m.hold(L.as_c128(arr).copy())
+    elif velocity_x != 0:
+        raise L.InexactError("velocity_x with the default (real) B field")
     ctx = L.context()
     L.check(L.lib().bf_dance_intro(ctx.handle, ctypes.byref(c), nd, L.cptr(eps), r1.real, r1.imag, float(diskR),
-                                   L.cptr(out)), ctx.handle)
+                                   L.cptr(b), float(velocity_x), float(f), L.cptr(out)), ctx.handle)
+    if b is not None and velocity_x != 0 and isinstance(bfield, np.ndarray) and bfield.flags.writeable:
+        bfield[...] = b  # the reference's `bfield .*= ...` mutates the caller's array
     return out
 
 

@@ -188,11 +188,32 @@ int bf_propagate(bf_ctx* ctx, int32_t prop_kind, const bf_coords* coords, double
 }
 
 int bf_dance_intro(bf_ctx* ctx, const bf_coords* coords, int32_t nd, const double* eps, double r1_re, double r1_im,
-                   double diskR, double* out) {
+                   double diskR, double* bfield, double velocity_x, double f, double* out) {
     TRY(check_coords(ctx, coords));
     BF_REQUIRE(ctx, nd >= 1 && eps && out, BF_ERR_ARG, "dance_intro needs nd >= 1, eps and out");
+    BF_REQUIRE(ctx, velocity_x == 0.0 || bfield != nullptr, BF_ERR_ARG,
+               "InexactError: velocity_x needs a complex bfield (the default B field is real)");
+    BF_REQUIRE(ctx, velocity_x == 0.0 || f > 0.0, BF_ERR_ARG, "frequency must be positive");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
-    return guarded(ctx, [&] { bf::dance_intro_dev(ctx, coords, nd, eps, cmk<double>(r1_re, r1_im), diskR, out); });
+    return guarded(ctx, [&] {
+        bf::dance_intro_dev(ctx, coords, nd, eps, cmk<double>(r1_re, r1_im), diskR, bfield, velocity_x, f, out);
+    });
+}
+
+int bf_gauss_profile(bf_ctx* ctx, const bf_coords* coords, double z, double omega0, double f, double* out) {
+    TRY(check_coords(ctx, coords));
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    BF_REQUIRE(ctx, omega0 > 0.0 && f > 0.0, BF_ERR_ARG, "gauss_profile needs omega0 > 0 and f > 0");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::gauss_profile_dev(ctx, coords, z, omega0, f, out); });
+}
+
+int bf_unnormalized_match(bf_ctx* ctx, int32_t nfield, int64_t n, const double* Eout, const double* Eantenna,
+                          double dx, double dy, double* out) {
+    BF_REQUIRE(ctx, nfield >= 1 && n >= 1 && Eout && Eantenna && out, BF_ERR_ARG,
+               "unnormalized_match needs nfield >= 1, n >= 1 and non-NULL buffers");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] { bf::unnormalized_match_dev(ctx, nfield, n, Eout, Eantenna, dx, dy, out); });
 }
 
 // Launch-shape and band overrides for tuning and A/B checks: BF_COL_TX

@@ -722,18 +722,39 @@ __global__ void k_amin_check(int* active, double* amax, double amin, int fb) {
 }
 
 // dance_intro fill (src/boost_fractor_recurprop.jl:153-201): fields[f][n][s] = ax[n][s] * [x^2+y^2 <= R^2]
+// bfield: null -> the default real mask x^2 + y^2 > R^2 ? 0 : 1 (ax .* mask, complex x real);
+// else complex [nd+1][K]: region n's right-moving amplitude multiplies layer n, its
+// left-moving amplitude layer n+1 (complex x complex, src/boost_fractor_recurprop.jl:196-197).
 template <typename T>
 __global__ void k_dance_intro(cplx<typename Storage<T>::type>* fields, int nd, size_t K, int nx,
                               const zd* __restrict__ ax, const double* __restrict__ X, const double* __restrict__ Y,
-                              double R2, size_t fstride) {
+                              double R2, size_t fstride, const zd* __restrict__ bfield) {
+#pragma clang fp contract(off)
     BF_TYPES(T);
     (void)sizeof(C);
     const int f = blockIdx.z, q = blockIdx.y;  // q = region*2 + slot
     const zd v = ax[q];
+    const zd* layer = bfield ? bfield + (size_t)(q / 2 + (q & 1)) * K : nullptr;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
-        const double xx = X[i % nx], yy = Y[i / nx];
-        const double bb = (xx * xx + yy * yy > R2) ? 0.0 : 1.0;
-        fields[(size_t)f * fstride + (size_t)q * K + i] = cmk<S>((S)(bb * v.re), (S)(bb * v.im));
+        zd e;
+        if (layer) {
+            e = cmul(v, layer[i]);
+        } else {
+            const double xx = X[i % nx], yy = Y[i / nx];
+            const double bb = (xx * xx + yy * yy > R2) ? 0.0 : 1.0;
+            e = cmk<double>(bb * v.re, bb * v.im);
+        }
+        fields[(size_t)f * fstride + (size_t)q * K + i] = cmk<S>((S)e.re, (S)e.im);
+    }
+}
+
+// bfield[z][y][x] *= exp(a.re*x + 1im*a.im*x) for every layer: the axion velocity
+// phase exp(-1im*Ma*(-velocity_x)*x) with a = (-1im*Ma)*(-velocity_x) (:162-166)
+__global__ void k_bfield_velocity(zd* b, size_t n, size_t K, int nx, const double* __restrict__ X, zd a) {
+#pragma clang fp contract(off)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double x = X[(i % K) % nx];
+        b[i] = cmul(b[i], cexp_j(cmk<double>(a.re * x, a.im * x)));
     }
 }
 
@@ -1182,17 +1203,31 @@ std::vector<zd> dance_intro_amplitudes(int nd, const zd* eps, zd r1) {
 }
 
 void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double* eps, zd r1, double diskR,
-                     double* out) {
+                     double* bfield, double velocity_x, double f, double* out) {
     hipStream_t s = ctx->stream;
     Grid g;
     g.upload(coords, s);
     const size_t K = g.K();
     std::vector<zd> ax = dance_intro_amplitudes(nd, reinterpret_cast<const zd*>(eps), r1);
-    DevBuf dax(sizeof(zd) * ax.size()), dout(sizeof(zd) * K * 2 * nd);
+    DevBuf dax(sizeof(zd) * ax.size()), dout(sizeof(zd) * K * 2 * nd), db;
     upload(dax.p, ax.data(), sizeof(zd) * ax.size(), s);
+    if (bfield) {
+        const size_t nb = K * (size_t)(nd + 1);
+        db.alloc(sizeof(zd) * nb);
+        upload(db.p, bfield, sizeof(zd) * nb, s);
+        if (velocity_x != 0.0) {
+            const double ma = 2.0 * M_PI / (299792458.0 / f);
+            zd a = cmul(cmk<double>(0.0, -1.0), cmk<double>(ma, 0.0));
+            a = cmk<double>(a.re * -velocity_x, a.im * -velocity_x);
+            hipLaunchKernelGGL(k_bfield_velocity, dim3(grid1d(nb, 256, 2048)), dim3(256), 0, s, db.as<zd>(), nb, K,
+                               g.nx, g.X.as<double>(), a);
+            BF_CHECK_LAUNCH();
+            download(bfield, db.p, sizeof(zd) * nb, s);  // the reference multiplies the caller's array in place
+        }
+    }
     hipLaunchKernelGGL(k_dance_intro<double>, dim3(grid1d(K, 256, 512), 2 * nd, 1), dim3(256), 0, s,
                        dout.as<zd>(), nd, K, g.nx, dax.as<zd>(), g.X.as<double>(), g.Y.as<double>(), diskR * diskR,
-                       (size_t)0);
+                       (size_t)0, bfield ? db.as<zd>() : nullptr);
     BF_CHECK_LAUNCH();
     download(out, dout.p, sizeof(zd) * K * 2 * nd, s);
     BF_HIP(hipStreamSynchronize(s));
@@ -1423,7 +1458,7 @@ struct DancerPlan : DancerPlanBase {
             } else {
                 hipLaunchKernelGGL(k_dance_intro<T>, dim3(grid1d(K, 256, 512), 2 * R, nb), dim3(256), 0, s,
                                    fields.as<cplx<S>>(), R, K, g.nx, ax.as<zd>(), g.X.as<double>(), g.Y.as<double>(),
-                                   diskR * diskR, fstride);
+                                   diskR * diskR, fstride, (const zd*)nullptr);
             }
             BF_CHECK_LAUNCH();
             std::vector<int> act(fb, 0);
@@ -1733,7 +1768,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
             else
                 hipLaunchKernelGGL(k_dance_intro<T>, dim3(grid1d(K, 256, 512), 2 * (Nd - 1), nb), dim3(256), 0, s,
                                    EM.as<cplx<S>>() + 2 * K, Nd - 1, K, g.nx, dax.as<zd>(), g.X.as<double>(),
-                                   g.Y.as<double>(), diskR * diskR, estride);
+                                   g.Y.as<double>(), diskR * diskR, estride, (const zd*)nullptr);
         }
         BF_CHECK_LAUNCH();
         hipLaunchKernelGGL(k_ch_copy_right<T>, dim3(grid1d(K * Nd, 256, 4096), nb), dim3(256), 0, s, F.as<cplx<S>>(),

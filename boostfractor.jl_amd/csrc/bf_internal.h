@@ -181,7 +181,10 @@ struct DancerPlanBase {
 void propagate_one(bf_ctx* ctx, int kind, const bf_coords* coords, double* E, double dz, double diskR, zd eps,
                    double tx, double ty, const double* surface, double lambda);
 void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double* eps, zd r1, double diskR,
-                     double* out);
+                     double* bfield, double velocity_x, double f, double* out);
+void gauss_profile_dev(bf_ctx* ctx, const bf_coords* co, double z, double omega0, double f, double* out);
+void unnormalized_match_dev(bf_ctx* ctx, int nb, int64_t n, const double* Eout, const double* Eant, double dx,
+                            double dy, double* out);
 DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax,
                                  const double* fr, int nfr, int kind, double diskR, int precision,
                                  const DancerOptions& o);

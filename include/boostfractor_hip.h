@@ -116,11 +116,23 @@ int bf_propagate(bf_ctx *ctx, int32_t prop_kind, const bf_coords *coords, double
                  double eps_re, double eps_im, double tilt_x, double tilt_y, const double *surface, double lambda);
 
 /* ---- recursive Fourier propagation ----------------------------------- */
-/* dance_intro (src/boost_fractor_recurprop.jl:153-201), default B field, zero
- * axion velocity.  nd = length(bdry.distance), eps complex [nd], r1 = bdry.r[1].
- * out: complex [nd][2][ny][nx]. */
+/* dance_intro (src/boost_fractor_recurprop.jl:153-201).  nd = length(bdry.distance),
+ * eps complex [nd], r1 = bdry.r[1].  bfield: complex [nd+1][ny][nx], or NULL for the
+ * default real mask (1 on x^2 + y^2 <= diskR^2).  velocity_x != 0 multiplies bfield in
+ * place by exp(-1im*Ma*(-velocity_x)*x), Ma = 2 pi f/c (:162-166), as the reference does
+ * to the caller's array; it needs a bfield (with the default real mask the reference
+ * throws InexactError: BF_ERR_ARG here).  out: complex [nd][2][ny][nx]. */
 int bf_dance_intro(bf_ctx *ctx, const bf_coords *coords, int32_t nd, const double *eps, double r1_re,
-                   double r1_im, double diskR, double *out);
+                   double r1_im, double diskR, double *bfield, double velocity_x, double f, double *out);
+
+/* gauss_profile (src/beam_gaussian.jl:24-37): normalised Gaussian beam at distance z
+ * from its waist omega0, frequency f.  out: complex [ny][nx]. */
+int bf_gauss_profile(bf_ctx *ctx, const bf_coords *coords, double z, double omega0, double f, double *out);
+
+/* unnormalized_match (src/beam_coupling.jl:13-15): out[b] = sum(Eout[b] .* conj(Eantenna)) * dx * dy
+ * for nfield fields Eout complex [nfield][n] against one Eantenna complex [n]; out complex [nfield]. */
+int bf_unnormalized_match(bf_ctx *ctx, int32_t nfield, int64_t n, const double *Eout, const double *Eantenna,
+                          double dx, double dy, double *out);
 
 /* dancer (src/boost_fractor_recurprop.jl:25-146) for nfreq frequencies of one
  * setup.  setup is the untrimmed SetupBoundaries (mirror region first).

@@ -406,6 +406,104 @@ int bfo_dance_intro(int nd, const double *eps, double r1_re, double r1_im, int n
     return 0;
 }
 
+/* dance_intro with a user B field and/or an axion velocity
+ * (src/boost_fractor_recurprop.jl:153-201, options :157-166).
+ * bfield: complex [nd+1][ny][nx], or NULL for the default real mask x^2+y^2 > R^2 ? 0 : 1.
+ * velocity_x != 0 multiplies the B field by exp(-1im*Ma*(-velocity_x)*x), Ma = 2pi/lambda
+ * (:162-166); with the default Float64 mask the reference's in-place `bfield .*=` throws
+ * InexactError: returns -1 then.  bfield_out (optional): the multiplied B field, which the
+ * reference leaves in the caller's array. */
+int bfo_dance_intro_ex(int nd, const double *eps, double r1_re, double r1_im, int nx, int ny, const double *X,
+                       const double *Y, double diskR, const double *bfield, double velocity_x, double f,
+                       double *bfield_out, double *out) {
+    size_t K = (size_t)nx * ny;
+    if (bfield == NULL) {
+        if (velocity_x != 0.0) return -1;
+        return bfo_dance_intro(nd, eps, r1_re, r1_im, nx, ny, X, Y, diskR, out);
+    }
+    cx *b = (cx *)malloc(sizeof(cx) * (size_t)(nd + 1) * K);
+    memcpy(b, bfield, sizeof(cx) * (size_t)(nd + 1) * K);
+    if (velocity_x != 0.0) {
+        double lambda = bfo_wavelength(f);
+        double ma = 2.0 * M_PI / lambda;
+        /* -1im*Ma*(-velocity_x)*x, evaluated left to right in complex arithmetic */
+        cx a = cmul(C(0.0, -1.0), C(ma, 0.0));
+        a = C(a.re * (-velocity_x), a.im * (-velocity_x));
+        for (int z = 0; z <= nd; z++)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) {
+                    cx ph = cexp_j(C(a.re * X[x], a.im * X[x]));
+                    size_t i = (size_t)z * K + (size_t)y * nx + x;
+                    b[i] = cmul(b[i], ph);
+                }
+    }
+    if (bfield_out) memcpy(bfield_out, b, sizeof(cx) * (size_t)(nd + 1) * K);
+    /* amplitudes as dance_intro_c, then ax (+0.0im) .* bfield[n] / bfield[n+1] */
+    cx *ones_ = (cx *)malloc(sizeof(cx) * (size_t)nd * 2 * K);
+    coords_t c = {nx, ny, X, Y, NULL, NULL};
+    dance_intro_c(nd, (const cx *)eps, C(r1_re, r1_im), &c, INFINITY, ones_);  /* mask of all ones */
+    cx *o = (cx *)out;
+    for (int n = 0; n < nd; n++)
+        for (size_t i = 0; i < K; i++) {
+            o[(size_t)(2 * n) * K + i] = cmul(ones_[(size_t)(2 * n) * K + i], b[(size_t)n * K + i]);
+            o[(size_t)(2 * n + 1) * K + i] = cmul(ones_[(size_t)(2 * n + 1) * K + i], b[(size_t)(n + 1) * K + i]);
+        }
+    free(ones_);
+    free(b);
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* antenna helpers (src/beam_gaussian.jl:24-37, src/beam_coupling.jl:13-15) */
+/* ----------------------------------------------------------------------- */
+/* gauss_profile: omega0/omega(z) .* exp.(-radius.^2/omega(z)^2)
+ *                .* exp.(-1im .* (k*z .+ k*radius.^2. /(2*R.(z)) .- Psi.(z)))
+ * out: complex [ny][nx]. */
+int bfo_gauss_profile(int nx, int ny, const double *X, const double *Y, double z, double omega0, double f,
+                      double *out) {
+    double lambda = bfo_wavelength(f);
+    double k = 2.0 * M_PI / lambda;
+    double zR = M_PI * omega0 * omega0 / lambda;
+    double om = omega0 * sqrt(1.0 + (z / zR) * (z / zR));
+    double Rz = z * (1.0 + (zR / z) * (zR / z));
+    double psi = atan(z / zR);
+    double amp0 = omega0 / om;
+    double om2 = om * om;
+    cx *o = (cx *)out;
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            double r = sqrt(X[x] * X[x] + Y[y] * Y[y]);
+            double r2 = r * r;
+            double ph = k * z + k * r2 / (2.0 * Rz) - psi;
+            /* -1im .* ph: Complex{Int}(0,-1) * ph = (0*ph, -ph) */
+            cx e = cexp_j(C(0.0 * ph, -ph));
+            double a = amp0 * exp(-r2 / om2);
+            o[(size_t)y * nx + x] = rmul(a, e);
+        }
+    return 0;
+}
+
+/* unnormalized_match: sum(Eout .* conj(Eantenna)) .* dx .* dy, Julia's pairwise sum
+ * (blocks of 1024 summed sequentially, halves combined recursively). */
+static cx pairwise_dot(const cx *a, const cx *b, size_t lo, size_t hi) {
+    if (hi - lo <= 1024) {
+        cx s = C(0.0, 0.0);
+        for (size_t i = lo; i < hi; i++) s = cadd(s, cmul(a[i], conj_(b[i])));
+        return s;
+    }
+    size_t mid = lo + (hi - lo + 1) / 2; /* Julia: mid = (first + last) >>> 1, left half inclusive */
+    return cadd(pairwise_dot(a, b, lo, mid), pairwise_dot(a, b, mid, hi));
+}
+
+int bfo_unnormalized_match(int64_t n, const double *Eout, const double *Eant, double dx, double dy, double *out) {
+    cx s = n > 0 ? pairwise_dot((const cx *)Eout, (const cx *)Eant, 0, (size_t)n) : C(0.0, 0.0);
+    s = C(s.re * dx, s.im * dx);
+    s = C(s.re * dy, s.im * dy);
+    out[0] = s.re;
+    out[1] = s.im;
+    return 0;
+}
+
 /* ----------------------------------------------------------------------- */
 /* dancer (src/boost_fractor_recurprop.jl:25-146)                           */
 /* ----------------------------------------------------------------------- */

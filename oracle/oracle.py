@@ -112,6 +112,36 @@ def dance_intro(eps, r1, coords, diskR):
     return out
 
 
+def dance_intro_ex(eps, r1, coords, diskR, bfield=None, velocity_x=0.0, f=10e9):
+    """dance_intro with bfield (nd+1, ny, nx) and velocity; returns (fields, bfield after the
+    reference's in-place multiplication), or raises ValueError where the reference throws
+    InexactError (velocity with the default Float64 mask)."""
+    eps = _c(eps)
+    nd = len(eps)
+    out = np.zeros((nd, 2, coords.ny, coords.nx), dtype=np.complex128)
+    bout = np.zeros((nd + 1, coords.ny, coords.nx), dtype=np.complex128) if bfield is not None else None
+    rc = lib().bfo_dance_intro_ex(_I(nd), _p(eps), _D(complex(r1).real), _D(complex(r1).imag), _I(coords.nx),
+                                  _I(coords.ny), _p(coords.X), _p(coords.Y), _D(diskR), _p(_c(bfield)),
+                                  _D(velocity_x), _D(f), _p(bout), _p(out))
+    if rc != 0:
+        raise ValueError("InexactError: velocity_x with the default Float64 B field")
+    return out, bout
+
+
+def gauss_profile(coords, z=1e-9, omega0=0.1, f=10e9):
+    out = np.zeros((coords.ny, coords.nx), dtype=np.complex128)
+    lib().bfo_gauss_profile(_I(coords.nx), _I(coords.ny), _p(coords.X), _p(coords.Y), _D(z), _D(omega0), _D(f),
+                            _p(out))
+    return out
+
+
+def unnormalized_match(Eout, Eant, dx=0.01, dy=0.01):
+    a, b = _c(Eout), _c(Eant)
+    out = np.zeros(2)
+    lib().bfo_unnormalized_match(ctypes.c_int64(a.size), _p(a), _p(b), _D(dx), _D(dy), _p(out))
+    return complex(out[0], out[1])
+
+
 def dancer(setup, coords, amin, nmax, f, kind=PROP_PROPAGATOR, emit=None, reflect=None, diskR=0.1,
            returnsum=True, immediatesum=True):
     K = (coords.ny, coords.nx)

@@ -123,6 +123,17 @@ def test_cabi_validation_without_device():
     rc = lib.bf_transformer_trace_back(None, L.cptr(v), L.cptr(v), ctypes.byref(s), ctypes.byref(c),
                                        ctypes.byref(md), 20e9, 0, None, 0.15, 1, L.cptr(v), L.cptr(fields))
     assert rc == L.BF_ERR_DIM
+    # dance_intro: an axion velocity needs a complex B field (the reference throws InexactError)
+    eps = np.ones(4, dtype=complex)
+    out = np.zeros((4, 2, 8, 8), dtype=complex)
+    rc = lib.bf_dance_intro(None, ctypes.byref(c), 4, L.cptr(eps), 1.0, 0.0, 0.15, None, 1e-3, 20e9, L.cptr(out))
+    assert rc == L.BF_ERR_ARG and b"InexactError" in lib.bf_last_error(None)
+    with pytest.raises(B.InexactError):
+        B.dance_intro(sb, W.grid_coords(8), velocity_x=1e-3)
+    # gauss_profile / unnormalized_match argument checks
+    g = np.zeros((8, 8), dtype=complex)
+    assert lib.bf_gauss_profile(None, ctypes.byref(c), 1e-9, -0.1, 10e9, L.cptr(g)) == L.BF_ERR_ARG
+    assert lib.bf_unnormalized_match(None, 0, 64, L.cptr(g), L.cptr(g), 0.01, 0.01, L.cptr(g)) == L.BF_ERR_ARG
 
 
 def test_mirror_rejects_user_propagators():

@@ -169,3 +169,56 @@ def test_dft_against_brute_force():
     want = np.fft.ifft2(np.fft.ifftshift(np.fft.fftshift(np.fft.fft2(E)) * ph))
     got = O.propagate(O.PROP_NOTILTS, E, oc, dz, 10.0, complex(1.0, -0.0), 0, 0, None, lam)
     assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_gauss_profile_golden_and_closed_forms():
+    """test/runtests.jl:46-47: gauss_profile(coords; z=1e-9, omega0=0.097, f=22e9)[4000] on the
+    default 101^2 grid; near the waist |profile| = exp(-r^2/omega0^2) and the beam's power
+    integral is pi omega0^2 / 2."""
+    c = B.SeedCoordinateSystem()
+    oc = ocoords(c)
+    g = O.gauss_profile(oc, z=1e-9, omega0=0.097, f=22e9)
+    gd = GOLDEN["gauss_profile_4000"]
+    assert g.ravel()[3999] == complex(gd["re"], gd["im"])  # Julia linear index 4000, x fastest
+    XX, YY = np.meshgrid(c.X, c.Y)
+    assert np.max(np.abs(np.abs(g) - np.exp(-(XX ** 2 + YY ** 2) / 0.097 ** 2))) < 1e-15
+    assert abs(np.sum(np.abs(g) ** 2) * 1e-4 / (math.pi * 0.097 ** 2 / 2) - 1) < 1e-3
+    # far from the waist the amplitude drops as omega0/omega(z): omega(z)^2 = omega0^2 (1 + (z/zR)^2)
+    z, lam = 0.5, 299792458.0 / 22e9
+    zR = math.pi * 0.097 ** 2 / lam
+    gz = O.gauss_profile(oc, z=z, omega0=0.097, f=22e9)
+    assert abs(np.abs(gz[50, 50]) - 1 / math.sqrt(1 + (z / zR) ** 2)) < 1e-14
+
+
+def test_unnormalized_match_closed_forms():
+    """src/beam_coupling.jl:13-15: self match = sum |E|^2 dx dy (real); match(a, b) = conj(match(b, a))"""
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((37, 41)) + 1j * rng.standard_normal((37, 41))
+    b = rng.standard_normal((37, 41)) + 1j * rng.standard_normal((37, 41))
+    s = O.unnormalized_match(a, a, 0.02, 0.03)
+    assert s.imag == 0.0 and abs(s.real / (np.sum(np.abs(a) ** 2) * 0.02 * 0.03) - 1) < 1e-14
+    assert O.unnormalized_match(a, b) == np.conj(O.unnormalized_match(b, a))
+    assert abs(O.unnormalized_match(a, b) - np.sum(a * np.conj(b)) * 1e-4) < 1e-13
+
+
+def test_dance_intro_bfield_and_velocity():
+    """src/boost_fractor_recurprop.jl:157-166: the default B field is the real aperture mask;
+    a velocity multiplies the B field by exp(1im*Ma*v*x) in place; with the real default the
+    reference's in-place multiplication throws InexactError."""
+    c = B.SeedCoordinateSystem()
+    oc = ocoords(c)
+    sb = B.SeedSetupBoundaries(c, diskno=2)
+    nd = len(sb.eps)
+    XX, YY = np.meshgrid(c.X, c.Y)
+    mask = np.where(XX ** 2 + YY ** 2 > 0.15 ** 2, 0.0, 1.0).astype(complex)
+    f0 = O.dance_intro(sb.eps, sb.r[0], oc, 0.15)
+    f1, _ = O.dance_intro_ex(sb.eps, sb.r[0], oc, 0.15, bfield=np.broadcast_to(mask, (nd + 1,) + mask.shape).copy())
+    assert np.array_equal(f0, f1, equal_nan=True)
+    bf = (np.broadcast_to(mask, (nd + 1,) + mask.shape) * (1 + 0.5j)).copy()
+    f2, b2 = O.dance_intro_ex(sb.eps, sb.r[0], oc, 0.15, bfield=bf, velocity_x=1e-3, f=20e9)
+    ph = np.exp(1j * (2 * math.pi / (299792458.0 / 20e9)) * 1e-3 * XX)
+    assert np.max(np.abs(b2 - bf * ph[None])) < 1e-15
+    ok = np.isfinite(f0[:, 0])
+    assert np.max(np.abs((f2[:, 0] - f0[:, 0] * (1 + 0.5j) * ph[None])[ok])) < 1e-15
+    with pytest.raises(ValueError):
+        O.dance_intro_ex(sb.eps, sb.r[0], oc, 0.15, velocity_x=1e-3)
