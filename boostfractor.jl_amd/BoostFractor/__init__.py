@@ -15,7 +15,7 @@ from .propagators import propagator, propagator1D, propagatorMomentumSpace, prop
 from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
 from .transformer import (ErrorException, Modes, SeedModes, axion_induced_modes, calc_propagation_matrices,
                           e_field_dimensions, field2modes, index, mode, modes2field, propagation_matrix, transformer,
-                          transformer_trace_back)
+                          transformer_batch, transformer_trace_back)
 
 __all__ = [
     "CoordinateSystem", "SeedCoordinateSystem", "get_kspace_coords", "SetupBoundaries", "SeedSetupBoundaries",
@@ -23,6 +23,6 @@ __all__ = [
     "context", "device_count", "propagator", "propagatorNoTilts", "propagatorMomentumSpace", "propagator1D",
     "dancer", "dance_intro", "cheerleader", "DancerPlan", "Modes", "SeedModes", "mode", "index",
     "e_field_dimensions", "axion_induced_modes", "field2modes", "modes2field", "propagation_matrix",
-    "calc_propagation_matrices", "transformer", "transformer_trace_back", "ErrorException", "InexactError",
+    "calc_propagation_matrices", "transformer", "transformer_batch", "transformer_trace_back", "ErrorException", "InexactError",
     "gauss_profile", "unnormalized_match",
 ]

@@ -244,6 +244,58 @@ def transformer(bdry, coords, modes, f=10.0e9, velocity_x=0, prop=propagator, pr
     return boost if reflect is None else (boost, refl)
 
 
+def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=propagator, diskR=0.15, emits=None,
+                      precision="fp64"):
+    """The disk-position sweep of examples/transformer_optim_utilities.jl:105,121,202
+    (calc_propagation_matrices, then transformer(...; propagation_matrices=...), per
+    candidate geometry) in one device call.
+
+    bdrys: list of SetupBoundaries (region counts may differ); modesets: a Modes or
+    a list of Modes; geometry c uses modesets[mode_set_of[c]] (default 0).  emits:
+    one axion mode vector per mode set (default: axion_induced_modes with a uniform
+    field).  Returns a list with one (nfreq, n) boost array per geometry (n of its
+    mode set), equal to transformer(bdrys[c], coords, modes, f=freqs, ...).
+    Like transformer, NaN permittivities of each setup are replaced by 1e30 in place."""
+    if not isinstance(modesets, (list, tuple)):
+        modesets = [modesets]
+    nconf, nsets = len(bdrys), len(modesets)
+    if mode_set_of is None:
+        mode_set_of = np.zeros(nconf, dtype=np.int32)
+    mode_set_of = np.ascontiguousarray(mode_set_of, dtype=np.int32)
+    if mode_set_of.shape != (nconf,):
+        raise L.DimensionMismatch(f"mode_set_of has shape {mode_set_of.shape}, expected {(nconf,)}")
+    if np.any(mode_set_of < 0) or np.any(mode_set_of >= nsets):
+        raise L.ArgumentError("mode_set_of entries must index modesets")
+    freqs = np.atleast_1d(np.asarray(f, dtype=np.float64))
+    if emits is None:
+        emits = [axion_induced_modes(coords, md, B=None, diskR=diskR, f=float(freqs[0])) for md in modesets]
+    if len(emits) != nsets:
+        raise L.DimensionMismatch(f"{len(emits)} emits for {nsets} mode sets")
+    m = L.Marshalled()
+    c = m.coords(coords)
+    for b in bdrys:
+        _nan_to_1e30(b)
+    setups = (L.Setup * max(nconf, 1))(*[m.setup(b, coords.ny, coords.nx) for b in bdrys])
+    mds = (L.ModesS * nsets)(*[_modes_struct(m, md) for md in modesets])
+    em = []
+    for md, e in zip(modesets, emits):
+        a = m.hold(L.as_c128(np.asarray(e).reshape(-1)))
+        if len(a) != md.n:
+            raise L.DimensionMismatch(f"emit of length {len(a)} for a mode set of {md.n} modes")
+        em.append(a.ctypes.data)
+    emp = (ctypes.c_void_p * nsets)(*em)
+    fr = m.hold(L.as_f64(freqs))
+    nf = len(fr)
+    sizes = np.array([modesets[s].n * nf for s in mode_set_of], dtype=np.int64)
+    offs = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64))
+    out = np.zeros(int(sizes.sum()), dtype=np.complex128)
+    ctx = L.context()
+    L.check(L.lib().bf_transformer_batch(ctx.handle, nconf, setups, ctypes.byref(c), nsets, mds, emp,
+                                         L.cptr(mode_set_of), L.cptr(fr), nf, prop_kind(prop), float(diskR),
+                                         precision_code(precision), L.cptr(offs), L.cptr(out)), ctx.handle)
+    return [out[o:o + sz].reshape(nf, -1) for o, sz in zip(offs, sizes)]
+
+
 def transformer_trace_back(reflected_beam, input_beam, bdry, coords, modes, inlcudes_axion=False, f=10.0e9,
                            velocity_x=0, prop=propagator, propagation_matrices=None, diskR=0.15, emit=None):
     """src/boost_fractor_transformer.jl:372-421 -> fields_regions (Nreg, 2, n)."""

@@ -417,6 +417,33 @@ int bf_transformer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, 
     });
 }
 
+int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, const bf_coords* coords, int32_t nsets,
+                         const bf_modes* modesets, const double* const* emits, const int32_t* mode_set_of,
+                         const double* freqs, int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                         const int64_t* boost_offset, double* boost) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nconf >= 0 && nsets >= 1, BF_ERR_ARG, "nconf must be >= 0 and nsets >= 1");
+    BF_REQUIRE(ctx, setups && modesets && emits && mode_set_of && boost_offset && boost, BF_ERR_ARG,
+               "setups, modesets, emits, mode_set_of, boost_offset and boost must not be NULL");
+    for (int32_t m = 0; m < nsets; m++) {
+        TRY(check_modes(ctx, &modesets[m]));
+        BF_REQUIRE(ctx, emits[m] != nullptr, BF_ERR_ARG, "emits[m] is NULL");
+    }
+    for (int32_t c = 0; c < nconf; c++) {
+        TRY(check_setup(ctx, &setups[c], 2, false));
+        BF_REQUIRE(ctx, mode_set_of[c] >= 0 && mode_set_of[c] < nsets, BF_ERR_ARG, "mode_set_of[c] out of range");
+        BF_REQUIRE(ctx, boost_offset[c] >= 0, BF_ERR_ARG, "boost_offset[c] is negative");
+    }
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::transformer_batch_dev(ctx, nconf, setups, coords, nsets, modesets, emits, mode_set_of, freqs, nfreq,
+                                  prop_kind, diskR, boost_offset, boost);
+    });
+}
+
 int bf_transformer_trace_back(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* setup,
                               const bf_coords* coords, const bf_modes* modes, double f, int32_t prop_kind,
                               const double* prop_mats, double diskR, int32_t includes_axion, const double* emit,

@@ -206,6 +206,10 @@ void propagation_matrices_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* 
 void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* md, const double* freqs,
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl);
+void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
+                           const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
+                           const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
+                           double* boost);
 void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* st,
                     const bf_coords* co, const bf_modes* md, double f, int kind, const double* prop_mats,
                     double diskR, int includes_axion, const double* emit, double* fields);

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace bf {
@@ -542,13 +543,25 @@ struct ChainStep {
     int pad;
 };
 
-__global__ void k_chain(const ChainStep* __restrict__ steps, int nsteps, const zd* __restrict__ P,
-                        const zd* __restrict__ Pinv, int nreg, int n, const zd* __restrict__ e0,
+// One chain per block: the steps [step0, step0 + nsteps), propagation matrices
+// P[pbase + reg], the axion source e0 + e0off and the outputs at slot `out`
+// (boost[out * n ...]).  A single geometry has one job per frequency; a batch
+// of geometries with the same mode count has one job per (geometry, frequency).
+struct ChainJob {
+    int64_t step0, pbase, out, e0off;
+    int nsteps, pad;
+};
+
+__global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __restrict__ steps,
+                        const zd* __restrict__ P, const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
                         const zd* __restrict__ reflect, zd* Tbuf, zd* work, zd* boost, zd* refl) {
     __shared__ double sval[256];
     __shared__ int sidx[256];
     __shared__ int piv[BF_MAX_MODES];
     const int f = blockIdx.x;
+    const ChainJob job = jobs[f];
+    const int nsteps = job.nsteps;
+    const zd* e0 = e0_all + job.e0off;
     const int N2 = 2 * n;
     const size_t nn = (size_t)n * n, NN = (size_t)N2 * N2;
     zd* T = Tbuf + (size_t)f * 2 * NN;
@@ -560,7 +573,7 @@ __global__ void k_chain(const ChainStep* __restrict__ steps, int nsteps, const z
     for (int i = threadIdx.x; i < n; i += blockDim.x) ab[i] = cmk<double>(0.0, 0.0);
     __syncthreads();
     for (int q = 0; q < nsteps; q++) {
-        const ChainStep st = steps[(size_t)f * nsteps + q];
+        const ChainStep st = steps[job.step0 + q];
         // axion_beam += S * (T21*e0 + T22*e0)
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
@@ -571,8 +584,8 @@ __global__ void k_chain(const ChainStep* __restrict__ steps, int nsteps, const z
             ab[i] = cadd(ab[i], cmul(st.S, cadd(s21, s22)));
         }
         // T <- T * [a P, b Pinv; b P, a Pinv]
-        const zd* Pm = P + ((size_t)f * nreg + st.reg) * nn;
-        const zd* Pi = Pinv + ((size_t)f * nreg + st.reg) * nn;
+        const zd* Pm = P + (size_t)(job.pbase + st.reg) * nn;
+        const zd* Pi = Pinv + (size_t)(job.pbase + st.reg) * nn;
         for (size_t e = threadIdx.x; e < NN; e += blockDim.x) {
             const int i = (int)(e % N2), j = (int)(e / N2);
             const bool left = j < n;
@@ -600,7 +613,7 @@ __global__ void k_chain(const ChainStep* __restrict__ steps, int nsteps, const z
     if (threadIdx.x == 0) {
         for (int i = 0; i < n; i++) tmp[i] = ab[i];
         lu_solve_col(M, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)f * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             for (int i = 0; i < n; i++) {
                 zd s = cmk<double>(0.0, 0.0);
@@ -608,7 +621,7 @@ __global__ void k_chain(const ChainStep* __restrict__ steps, int nsteps, const z
                 tmp[n + i] = s;
             }
             lu_solve_col(M, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)f * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -765,10 +778,19 @@ struct Gram {
 
 // All propagation matrices P[f][reg] for the regions in `regs` (0-based), at frequencies fr[0..nf).
 // eps: the permittivities to use (the transformer passes NaN-fixed values).
+// Frequency- and geometry-independent pieces of one mode set: the Gram matrix of
+// the patterns (flat untilted disks, 1D propagator) and the patterns' FFTs
+// (Parseval air regions).  Built on first use, reused by every later geometry.
+struct ModeCache {
+    DevBuf G, Fhat, FhatM;
+    bool haveG = false, haveF = false;
+};
+
 static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR,
                                       int nreg, const double* distance, const zd* eps, const double* tx,
                                       const double* ty, const zd* surfaces_dev, const std::vector<bool>& flat,
-                                      const double* fr, int nf, const std::vector<int>& regs, zd* P /* [nf][nreg][n][n] */) {
+                                      const double* fr, int nf, const std::vector<int>& regs, zd* P /* [nf][nreg][n][n] */,
+                                      ModeCache* cache = nullptr) {
     hipStream_t s = ctx->stream;
     const int n = md.n;
     const size_t K = g.K(), Ktot = md.Ktot, nn = (size_t)n * n;
@@ -786,13 +808,18 @@ static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev
     bool need_gram = false;
     for (int r : disks) if (flat[r] && tx[r] == 0.0 && ty[r] == 0.0) need_gram = true;
     if (!onedim.empty()) need_gram = true;
-    DevBuf G, Fhat, FhatM;
-    if (need_gram) {
+    ModeCache local;
+    ModeCache& mc = cache ? *cache : local;
+    DevBuf& G = mc.G;
+    DevBuf& Fhat = mc.Fhat;
+    DevBuf& FhatM = mc.FhatM;
+    if (need_gram && !mc.haveG) {
         G.alloc(sizeof(zd) * nn);
         GemmProb p{md.pat.as<zd>(), md.pat.as<zd>(), nullptr, 0, 0};
         gram.run({p}, G.as<zd>());
+        mc.haveG = true;
     }
-    if (!parseval.empty()) {
+    if (!parseval.empty() && !mc.haveF) {
         Fhat.alloc(sizeof(zd) * n * Ktot);
         FhatM.alloc(sizeof(zd) * n * Ktot);
         hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, Fhat.as<zd>(), md.pat.as<zd>(), n * Ktot);
@@ -800,6 +827,7 @@ static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev
         BF_CHECK_LAUNCH();
         fft_fields(ctx, g, Fhat.as<zd>(), n * md.C, false, diskR);
         fft_fields(ctx, g, FhatM.as<zd>(), n * md.C, true, diskR);
+        mc.haveF = true;
     }
     // frequency-dependent pieces, in frequency batches bounded by table memory
     const size_t per_f = std::max<size_t>(1, parseval.size() + disks.size()) * K * sizeof(zd);
@@ -913,11 +941,9 @@ static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev
     BF_HIP(hipStreamSynchronize(s));
 }
 
-static void setup_common(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, Grid& g, DevBuf& surf,
-                         std::vector<bool>& flat) {
+// surface maps on the device and which regions are flat (all-zero map)
+static void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat) {
     hipStream_t s = ctx->stream;
-    g.upload(co, s);
-    const size_t K = g.K();
     flat.assign(st->nreg, true);
     if (st->surfaces) {
         const zd* sf = reinterpret_cast<const zd*>(st->surfaces);
@@ -930,6 +956,12 @@ static void setup_common(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, G
         surf.alloc(sizeof(zd) * K * st->nreg);
         upload(surf.p, st->surfaces, sizeof(zd) * K * st->nreg, s);
     }
+}
+
+static void setup_common(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, Grid& g, DevBuf& surf,
+                         std::vector<bool>& flat) {
+    g.upload(co, ctx->stream);
+    upload_surfaces(ctx, st, g.K(), surf, flat);
 }
 
 void seed_modes_dev(bf_ctx* ctx, const bf_coords* coords, int M, int L, double diskR, double* patterns, double* kt) {
@@ -1028,6 +1060,58 @@ static std::vector<zd> eps_fixed(const bf_setup* st) {
     return v;
 }
 
+// chain steps of one geometry at one frequency (regions Nreg-1 .. 1, src/boost_fractor_transformer.jl:311-341)
+static void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainStep>& steps) {
+    for (int q = 0; q < nreg - 1; q++) {
+        const int sidx = nreg - 1 - q;                      // s in the reference loop
+        const int ir = nreg - sidx, ir1 = nreg - sidx - 1;  // idx_reg(s)-1, idx_reg(s+1)-1
+        const zd n1 = csqrt_j(eps[ir1]), n0 = csqrt_j(eps[ir]);
+        const zd one = cmk<double>(1.0, 0.0);
+        const zd S = rmul(0.5, csub(cdiv_h(one, cmul(n1, n1)), cdiv_h(one, cmul(n0, n0))));
+        // get_boundary_matrix(n_left = n0, n_right = n1): G = 1/(2 nr) [(nr+nl) I, (nr-nl) I; ...]
+        const zd sc = cdiv_h(one, rmul(2.0, n1));
+        ChainStep c;
+        c.S = S;
+        c.a = cmul(sc, cadd(n1, n0));
+        c.b = cmul(sc, csub(n1, n0));
+        c.reg = ir;
+        c.pad = 0;
+        steps.push_back(c);
+    }
+}
+
+// inverses of the selected matrices, then one chain per job; outputs boost (and refl) [njobs][n]
+static void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
+                               const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
+                               const zd* reflect, zd* boost, zd* refl) {
+    hipStream_t s = ctx->stream;
+    const size_t nn = (size_t)n * n;
+    const int N2 = 2 * n;
+    DevBuf dsel(sizeof(int) * sel.size()), work(sizeof(zd) * nn * sel.size());
+    upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
+    {
+        TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
+        hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P, Pinv, n, work.as<zd>(),
+                           dsel.as<int>());
+        BF_CHECK_LAUNCH();
+    }
+    const size_t nj = jobs.size();
+    DevBuf djobs(sizeof(ChainJob) * nj), dsteps(sizeof(ChainStep) * steps.size()),
+        Tb(sizeof(zd) * 2 * (size_t)N2 * N2 * nj), cw(sizeof(zd) * (nn + 3 * (size_t)n) * nj);
+    upload(djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
+    upload(dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
+    double flops = 0.0;
+    for (const ChainJob& j : jobs) flops += 64.0 * (double)n * n * n * j.nsteps;
+    {
+        // per region: T (2n x 2n) times the structured boundary-propagation matrix, 64 n^3 flop
+        TimedLaunch tl(ctx, "chain", flops);
+        hipLaunchKernelGGL(k_chain, dim3((unsigned)nj), dim3(256), 0, s, djobs.as<ChainJob>(), dsteps.as<ChainStep>(),
+                           P, Pinv, n, e0, reflect, Tb.as<zd>(), cw.as<zd>(), boost, refl);
+        BF_CHECK_LAUNCH();
+    }
+    BF_HIP(hipStreamSynchronize(s));
+}
+
 void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* modes, const double* freqs,
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl) {
@@ -1039,7 +1123,7 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
     setup_common(ctx, st, co, g, surf, flat);
     ModesDev md;
     md.upload(modes, g.K(), s);
-    const int n = md.n, nreg = st->nreg, N2 = 2 * n;
+    const int n = md.n, nreg = st->nreg;
     const size_t nn = (size_t)n * n;
     std::vector<zd> eps = eps_fixed(st);
     DevBuf P(sizeof(zd) * nn * nreg * nfreq), Pinv(sizeof(zd) * nn * nreg * nfreq);
@@ -1051,59 +1135,115 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
         propagation_matrices_core(ctx, g, md, kind, diskR, nreg, st->distance, eps.data(), st->tilt_x, st->tilt_y,
                                   surf.as<zd>(), flat, freqs, nfreq, regs, P.as<zd>());
     }
-    // inverses of every used P
     std::vector<int> sel;
-    for (int f = 0; f < nfreq; f++)
+    std::vector<ChainJob> jobs;
+    std::vector<ChainStep> steps;
+    for (int f = 0; f < nfreq; f++) {
         for (int r = 1; r < nreg; r++) sel.push_back(f * nreg + r);
-    DevBuf dsel(sizeof(int) * sel.size()), work(sizeof(zd) * nn * sel.size());
-    upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
-    {
-        TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
-        hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P.as<zd>(),
-                           Pinv.as<zd>(), n, work.as<zd>(), dsel.as<int>());
-        BF_CHECK_LAUNCH();
+        ChainJob j;
+        j.step0 = (int64_t)steps.size();
+        j.nsteps = nreg - 1;
+        j.pbase = (int64_t)f * nreg;
+        j.out = f;
+        j.e0off = 0;
+        j.pad = 0;
+        jobs.push_back(j);
+        append_chain_steps(eps, nreg, steps);
     }
-    // chain scalars per frequency: s = Nreg-1 .. 1
-    const int nsteps = nreg - 1;
-    std::vector<ChainStep> steps((size_t)nfreq * nsteps);
-    for (int f = 0; f < nfreq; f++)
-        for (int q = 0; q < nsteps; q++) {
-            const int sidx = nreg - 1 - q;  // s in the reference loop
-            const int ir = nreg - sidx, ir1 = nreg - sidx - 1;  // idx_reg(s)-1, idx_reg(s+1)-1
-            const zd n1 = csqrt_j(eps[ir1]), n0 = csqrt_j(eps[ir]);
-            const zd one = cmk<double>(1.0, 0.0);
-            const zd S = rmul(0.5, csub(cdiv_h(one, cmul(n1, n1)), cdiv_h(one, cmul(n0, n0))));
-            // get_boundary_matrix(n_left = n0, n_right = n1): G = 1/(2 nr) [(nr+nl) I, (nr-nl) I; ...]
-            const zd sc = cdiv_h(one, rmul(2.0, n1));
-            ChainStep c;
-            c.S = S;
-            c.a = cmul(sc, cadd(n1, n0));
-            c.b = cmul(sc, csub(n1, n0));
-            c.reg = ir;
-            c.pad = 0;
-            steps[(size_t)f * nsteps + q] = c;
-        }
-    DevBuf dsteps(sizeof(ChainStep) * steps.size()), Tb(sizeof(zd) * 2 * (size_t)N2 * N2 * nfreq),
-        cw(sizeof(zd) * (nn + 3 * (size_t)n) * nfreq), de0(sizeof(zd) * n), dref, dboost(sizeof(zd) * n * nfreq),
-        drefl;
-    upload(dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
+    DevBuf de0(sizeof(zd) * n), dref, dboost(sizeof(zd) * n * nfreq), drefl;
     upload(de0.p, emit, sizeof(zd) * n, s);
     if (reflect) {
         dref.alloc(sizeof(zd) * n);
         drefl.alloc(sizeof(zd) * n * nfreq);
         upload(dref.p, reflect, sizeof(zd) * n, s);
     }
-    {
-        // per region: T (2n x 2n) times the structured boundary-propagation matrix, 64 n^3 flop
-        TimedLaunch tl(ctx, "chain", 64.0 * (double)n * n * n * (double)nsteps * nfreq);
-        hipLaunchKernelGGL(k_chain, dim3(nfreq), dim3(256), 0, s, dsteps.as<ChainStep>(), nsteps, P.as<zd>(),
-                           Pinv.as<zd>(), nreg, n, de0.as<zd>(), dref.as<zd>(), Tb.as<zd>(), cw.as<zd>(),
-                           dboost.as<zd>(), drefl.as<zd>());
-        BF_CHECK_LAUNCH();
-    }
+    inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, de0.as<zd>(), dref.as<zd>(),
+                       dboost.as<zd>(), drefl.as<zd>());
     download(boost, dboost.p, sizeof(zd) * n * nfreq, s);
     if (reflect) download(refl, drefl.p, sizeof(zd) * n * nfreq, s);
     BF_HIP(hipStreamSynchronize(s));
+}
+
+// Many geometries (each its own setup and mode set) at the same frequencies.
+// Geometries are taken mode set by mode set, in chunks whose propagation
+// matrices fit a memory budget; every chunk runs its matrices' inverses and
+// all its (geometry, frequency) chains in one launch each.  The mode set's
+// patterns, pattern FFTs and flat-disk Gram matrix are built once.
+void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
+                           const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
+                           const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
+                           double* boost) {
+    hipStream_t s = ctx->stream;
+    Grid g;
+    g.upload(co, s);
+    const size_t K = g.K();
+    // bytes of P + Pinv per chunk (BF_BATCH_BYTES overrides, e.g. to exercise chunking on small cases)
+    size_t budget = (size_t)2 << 30;
+    if (const char* e = std::getenv("BF_BATCH_BYTES")) {
+        const long long v = std::atoll(e);
+        if (v > 0) budget = (size_t)v;
+    }
+    for (int set = 0; set < nsets; set++) {
+        std::vector<int> confs;
+        for (int c = 0; c < nconf; c++)
+            if (set_of[c] == set) confs.push_back(c);
+        if (confs.empty()) continue;
+        ModesDev md;
+        md.upload(&modesets[set], K, s);
+        ModeCache cache;
+        const int n = md.n;
+        const size_t nn = (size_t)n * n;
+        DevBuf de0(sizeof(zd) * n);
+        upload(de0.p, emits[set], sizeof(zd) * n, s);
+        for (size_t c0 = 0; c0 < confs.size();) {
+            // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry)
+            size_t c1 = c0, mats = 0;
+            while (c1 < confs.size()) {
+                const size_t m = (size_t)setups[confs[c1]].nreg * nfreq;
+                if (c1 > c0 && (mats + m) * nn * sizeof(zd) * 2 > budget) break;
+                mats += m;
+                c1++;
+            }
+            DevBuf P(sizeof(zd) * nn * mats), Pinv(sizeof(zd) * nn * mats), dboost(sizeof(zd) * n * nfreq * (c1 - c0));
+            std::vector<int> sel;
+            std::vector<ChainJob> jobs;
+            std::vector<ChainStep> steps;
+            size_t base = 0;
+            for (size_t q = c0; q < c1; q++) {
+                const bf_setup* st = &setups[confs[q]];
+                const int nreg = st->nreg;
+                DevBuf surf;
+                std::vector<bool> flat;
+                upload_surfaces(ctx, st, K, surf, flat);
+                std::vector<zd> eps = eps_fixed(st);
+                std::vector<int> regs;
+                for (int r = 1; r < nreg; r++) regs.push_back(r);
+                propagation_matrices_core(ctx, g, md, kind, diskR, nreg, st->distance, eps.data(), st->tilt_x,
+                                          st->tilt_y, surf.as<zd>(), flat, freqs, nfreq, regs,
+                                          P.as<zd>() + base * nn, &cache);
+                for (int f = 0; f < nfreq; f++) {
+                    for (int r = 1; r < nreg; r++) sel.push_back((int)(base + (size_t)f * nreg + r));
+                    ChainJob j;
+                    j.step0 = (int64_t)steps.size();
+                    j.nsteps = nreg - 1;
+                    j.pbase = (int64_t)(base + (size_t)f * nreg);
+                    j.out = (int64_t)((q - c0) * nfreq + f);
+                    j.e0off = 0;
+                    j.pad = 0;
+                    jobs.push_back(j);
+                    append_chain_steps(eps, nreg, steps);
+                }
+                base += (size_t)nreg * nfreq;
+            }
+            inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, de0.as<zd>(), nullptr,
+                               dboost.as<zd>(), nullptr);
+            for (size_t q = c0; q < c1; q++)
+                download(boost + 2 * boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
+                         sizeof(zd) * n * nfreq, s);
+            BF_HIP(hipStreamSynchronize(s));
+            c0 = c1;
+        }
+    }
 }
 
 // transformer_trace_back (:372-421): one block walks the regions applying G_s P_s

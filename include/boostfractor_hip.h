@@ -215,6 +215,21 @@ int bf_transformer(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, 
                    const double *freqs, int32_t nfreq, int32_t prop_kind, const double *prop_mats, double diskR,
                    const double *emit, const double *reflect, int32_t precision, double *boost, double *refl);
 
+/* transformer (src/boost_fractor_transformer.jl:285-359) for a batch of geometries
+ * at the same nfreq frequencies: the disk-position sweep of
+ * examples/transformer_optim_utilities.jl:105,121,202 (calc_propagation_matrices
+ * then transformer(...; propagation_matrices=...) per candidate geometry) in one
+ * call.  Geometry c uses mode set mode_set_of[c] (0 <= . < nsets) with axion
+ * source emits[mode_set_of[c]] (complex [n of that set]); setups may differ in
+ * region count, spacing, tilt and surfaces.  Propagation matrices are computed
+ * on the device (prop_kind); precision as for bf_transformer.
+ * boost: complex, geometry c's [nfreq][n] block starts at complex element
+ * boost_offset[c].  No reflect/refl outputs (the sweep uses the boost only). */
+int bf_transformer_batch(bf_ctx *ctx, int32_t nconf, const bf_setup *setups, const bf_coords *coords,
+                         int32_t nsets, const bf_modes *modesets, const double *const *emits,
+                         const int32_t *mode_set_of, const double *freqs, int32_t nfreq, int32_t prop_kind,
+                         double diskR, int32_t precision, const int64_t *boost_offset, double *boost);
+
 /* transformer_trace_back (src/boost_fractor_transformer.jl:372-421) at one frequency.
  * reflected, input: complex [n]; emit: complex [n] (used when includes_axion);
  * prop_mats: complex [nreg][n][n] or NULL.  fields: complex [nreg][2][n]. */

@@ -340,3 +340,60 @@ def test_propagation_matrices_grouped_batches(M, Lm):
         want = _oracle_mats(sb, coords, modes, f, O.PROP_PROPAGATOR)
         for r in range(1, len(sb.distance)):
             assert rel(got[q][r], want[r]) < TOL, (q, r, rel(got[q][r], want[r]))
+
+
+def _batch_geometries(N=32):
+    """uneven sweep: 1..4 disks, flat, tilted and rough, two mode sets"""
+    coords = W.grid_coords(N)
+    geoms = []
+    for c, nd in enumerate([1, 3, 2, 4, 1, 2]):
+        sb = W.booster(coords, ndisk=nd, seed=1200 + c, jitter=0.5e-3)
+        if c == 2:
+            _, rb = _rough_booster(N=N, ndisk=nd, seed=1300)
+            sb = rb
+        geoms.append(sb)
+    return coords, geoms
+
+
+@pytest.mark.parametrize("chunk_bytes", [None, "60000"])
+def test_transformer_batch_matches_single_geometry(monkeypatch, chunk_bytes):
+    """transformer_batch over geometries with uneven region counts and two mode
+    sets (n = 6 and 9) equals transformer per geometry; with a small matrix
+    budget the batch runs in several chunks per mode set"""
+    if chunk_bytes:
+        monkeypatch.setenv("BF_BATCH_BYTES", chunk_bytes)
+    coords, geoms = _batch_geometries()
+    sets = [_lg_like_modes(coords, 2, 1), _lg_like_modes(coords, 3, 1, seed=5)]
+    set_of = [0, 1, 1, 0, 1, 0]
+    emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
+    freqs = [19.6e9, 20.0e9, 20.4e9]
+    got = B.transformer_batch([g.copy() for g in geoms], coords, sets, f=freqs, mode_set_of=set_of, diskR=DISKR,
+                              emits=emits)
+    assert len(got) == len(geoms)
+    for c, sb in enumerate(geoms):
+        md = sets[set_of[c]]
+        assert got[c].shape == (len(freqs), md.n)
+        want = B.transformer(sb.copy(), coords, md, f=freqs, diskR=DISKR, emit=emits[set_of[c]])
+        eps = sb.eps.copy()
+        eps[np.isnan(eps.real)] = 1e30
+        for q, f in enumerate(freqs):
+            err = rel(got[c][q], want[q])
+            if err > TOL:
+                sens = chain_sensitivity(_oracle_mats(sb, coords, md, f, O.PROP_PROPAGATOR, eps), eps,
+                                         emits[set_of[c]])
+                assert err < 10 * sens, (c, q, err, sens)
+
+
+def test_transformer_batch_matches_oracle():
+    """two geometries of a batch against the CPU oracle (matrices and chain)"""
+    coords, geoms = _batch_geometries()
+    md = _lg_like_modes(coords, 2, 1)
+    emit = B.axion_induced_modes(coords, md, diskR=DISKR)
+    pick = [geoms[1], geoms[2]]
+    got = B.transformer_batch([g.copy() for g in pick], coords, md, f=[20.1e9], diskR=DISKR, emits=[emit])
+    for c, sb in enumerate(pick):
+        want = _otransformer(sb, coords, md, 20.1e9, O.PROP_PROPAGATOR, emit)
+        eps = sb.eps.copy()
+        eps[np.isnan(eps.real)] = 1e30
+        sens = chain_sensitivity(_oracle_mats(sb, coords, md, 20.1e9, O.PROP_PROPAGATOR, eps), eps, emit)
+        assert rel(got[c][0], want) < max(TOL, 10 * sens), (c, rel(got[c][0], want), sens)
