@@ -31,6 +31,10 @@ class InexactError(ValueError):
     """Julia's InexactError (a complex value stored into a Float64 array)"""
 
 
+class BoundsError(IndexError):
+    """Julia's BoundsError"""
+
+
 class HipError(RuntimeError):
     pass
 
@@ -91,6 +95,14 @@ _SIGS = {
     "bf_transformer_batch": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _I32,
                                             ctypes.POINTER(ModesS), ctypes.POINTER(ctypes.c_void_p), _P, _P, _I32,
                                             _I32, _D, _I32, _P, _P]),
+    "bf_propagation_matrices_grid": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords),
+                                                    ctypes.POINTER(ModesS), _P, _I32, _P, _I32, _P, _I32, _P, _I32,
+                                                    _I32, _D, _P]),
+    "bf_spacing_itp_create": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _D, _D, ctypes.POINTER(_P)]),
+    "bf_spacing_itp_bounds": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "bf_spacing_itp_matrices": (ctypes.c_int, [_P, _I32, _P, _P]),
+    "bf_spacing_itp_transformer": (ctypes.c_int, [_P, _I32, _P, _P, _P, _P, _P, _P]),
+    "bf_spacing_itp_destroy": (ctypes.c_int, [_P]),
     "bf_transformer_trace_back": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(Setup), ctypes.POINTER(Coords),
                                                  ctypes.POINTER(ModesS), _D, _I32, _P, _D, _I32, _P, _P]),
 }

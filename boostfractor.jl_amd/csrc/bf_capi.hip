@@ -3,7 +3,7 @@
 // Every entry point validates its arguments on the host first (so the error
 // paths are testable without a device), then runs the engine, converting any
 // internal bf::Error into a status code and a message for bf_last_error().
-#include "bf_internal.h"
+#include "bf_modematch.h"
 
 #include <cmath>
 #include <cstdio>
@@ -442,6 +442,74 @@ int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, con
         bf::transformer_batch_dev(ctx, nconf, setups, coords, nsets, modesets, emits, mode_set_of, freqs, nfreq,
                                   prop_kind, diskR, boost_offset, boost);
     });
+}
+
+int bf_propagation_matrices_grid(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, const bf_modes* modes,
+                                 const double* spacing_grid, int32_t nspacing, const double* tilt_x_grid,
+                                 int32_t ntilt_x, const double* tilt_y_grid, int32_t ntilt_y, const double* freqs,
+                                 int32_t nfreq, int32_t prop_kind, double diskR, double* out) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, false));
+    TRY(check_modes(ctx, modes));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, setup->nreg % 2 == 0, BF_ERR_DIM,
+               "the setup must have an even region count (mirror, gap, disk, ..., gap): "
+               "examples/transformer_optim_utilities.jl:21-22");
+    BF_REQUIRE(ctx, spacing_grid && tilt_x_grid && tilt_y_grid && out, BF_ERR_ARG,
+               "spacing_grid, tilt_x_grid, tilt_y_grid and out must not be NULL");
+    BF_REQUIRE(ctx, nspacing >= 1 && ntilt_x >= 1 && ntilt_y >= 1, BF_ERR_ARG, "grid lengths must be >= 1");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::propagation_matrices_grid_dev(ctx, setup, coords, modes, spacing_grid, nspacing, tilt_x_grid, ntilt_x,
+                                          tilt_y_grid, ntilt_y, freqs, nfreq, prop_kind, diskR, out);
+    });
+}
+
+int bf_spacing_itp_create(bf_ctx* ctx, const double* grid, int32_t nreg, int32_t nfreq, int32_t nspacing, int32_t n,
+                          double spacing_first, double spacing_step, bf_spacing_itp** out) {
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    BF_REQUIRE(ctx, grid != nullptr, BF_ERR_ARG, "grid is NULL");
+    BF_REQUIRE(ctx, nreg >= 2 && nreg % 2 == 0, BF_ERR_DIM, "nreg must be even and >= 2");
+    BF_REQUIRE(ctx, nfreq >= 1 && n >= 1 && n <= BF_MAX_MODES, BF_ERR_ARG, "nfreq >= 1 and 1 <= n <= 128 required");
+    BF_REQUIRE(ctx, nspacing >= 2, BF_ERR_ARG, "the cubic spacing interpolation needs at least 2 grid points");
+    BF_REQUIRE(ctx, std::isfinite(spacing_first) && std::isfinite(spacing_step) && spacing_step > 0.0, BF_ERR_ARG,
+               "the spacing grid must be finite and increasing");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        *out = bf::spacing_itp_create(ctx, grid, nreg, nfreq, nspacing, n, spacing_first, spacing_step);
+    });
+}
+
+int bf_spacing_itp_bounds(const bf_spacing_itp* itp, double* lo, double* hi) {
+    BF_REQUIRE(nullptr, itp && lo && hi, BF_ERR_ARG, "itp, lo and hi must not be NULL");
+    bf::spacing_itp_bounds(itp, lo, hi);
+    return BF_OK;
+}
+
+int bf_spacing_itp_matrices(bf_spacing_itp* itp, int32_t ncand, const double* dist_shift, double* out) {
+    BF_REQUIRE(nullptr, itp != nullptr, BF_ERR_ARG, "itp is NULL");
+    bf_ctx* ctx = itp->ctx;
+    BF_REQUIRE(ctx, ncand >= 0, BF_ERR_ARG, "ncand must be >= 0");
+    BF_REQUIRE(ctx, ncand == 0 || (out && (dist_shift || itp->nreg == 2)), BF_ERR_ARG, "dist_shift or out is NULL");
+    return guarded(ctx, [&] { bf::spacing_itp_matrices(itp, ncand, dist_shift, out); });
+}
+
+int bf_spacing_itp_transformer(bf_spacing_itp* itp, int32_t ncand, const double* dist_shift, const double* eps,
+                               const double* emit, const double* reflect, double* boost, double* refl) {
+    BF_REQUIRE(nullptr, itp != nullptr, BF_ERR_ARG, "itp is NULL");
+    bf_ctx* ctx = itp->ctx;
+    BF_REQUIRE(ctx, ncand >= 0, BF_ERR_ARG, "ncand must be >= 0");
+    BF_REQUIRE(ctx, eps && emit && boost && (dist_shift || itp->nreg == 2 || ncand == 0), BF_ERR_ARG,
+               "dist_shift, eps, emit and boost must not be NULL");
+    BF_REQUIRE(ctx, !reflect || refl, BF_ERR_ARG, "refl output is NULL while reflect is given");
+    return guarded(ctx, [&] { bf::spacing_itp_transformer(itp, ncand, dist_shift, eps, emit, reflect, boost, refl); });
+}
+
+int bf_spacing_itp_destroy(bf_spacing_itp* itp) {
+    bf::spacing_itp_destroy(itp);
+    return BF_OK;
 }
 
 int bf_transformer_trace_back(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* setup,

@@ -210,6 +210,16 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                            const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
                            double* boost);
+void propagation_matrices_grid_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* modes,
+                                   const double* spacing, int ns, const double* tgx, int ntx, const double* tgy,
+                                   int nty, const double* freqs, int nf, int kind, double diskR, double* out);
+bf_spacing_itp* spacing_itp_create(bf_ctx* ctx, const double* grid, int nreg, int nf, int ns, int n, double first,
+                                   double step);
+void spacing_itp_destroy(bf_spacing_itp* it);
+void spacing_itp_bounds(const bf_spacing_itp* it, double* lo, double* hi);
+void spacing_itp_matrices(bf_spacing_itp* it, int ncand, const double* shifts, double* out);
+void spacing_itp_transformer(bf_spacing_itp* it, int ncand, const double* shifts, const double* eps,
+                             const double* emit, const double* reflect, double* boost, double* refl);
 void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* st,
                     const bf_coords* co, const bf_modes* md, double f, int kind, const double* prop_mats,
                     double diskR, int includes_axion, const double* emit, double* fields);

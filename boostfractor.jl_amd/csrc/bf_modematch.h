@@ -1,0 +1,83 @@
+// bf_modematch.h -- mode-matching pieces shared by the transformer
+// (bf_transformer.hip) and its disk-position sweep caller (bf_sweep.hip).
+#pragma once
+
+#include "bf_internal.h"
+
+#include <vector>
+
+namespace bf {
+
+// mode patterns [n][C][ny][nx] and transverse wavenumbers [n] on the device
+struct ModesDev {
+    int n = 0, C = 0;
+    size_t K = 0, Ktot = 0;
+    DevBuf pat, kt;
+    void upload(const bf_modes* m, size_t K_, hipStream_t s) {
+        n = m->n;
+        C = m->ncomp;
+        K = K_;
+        Ktot = (size_t)C * K;
+        pat.alloc(sizeof(zd) * n * Ktot);
+        kt.alloc(sizeof(zd) * n);
+        bf::upload(pat.p, m->patterns, sizeof(zd) * n * Ktot, s);
+        bf::upload(kt.p, m->mode_kt, sizeof(zd) * n, s);
+    }
+};
+
+// Frequency- and geometry-independent pieces of one mode set: the Gram matrix of
+// the patterns (flat untilted disks, 1D propagator) and the patterns' FFTs
+// (Parseval air regions).  Built on first use, reused by every later geometry.
+struct ModeCache {
+    DevBuf G, Fhat, FhatM;
+    bool haveG = false, haveF = false;
+};
+
+// one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
+struct ChainStep {
+    zd S;     // axion_S_factor(sqrt(eps[idx(s+1)]), sqrt(eps[idx(s)]))
+    zd a, b;  // G = [a I, b I; b I, a I]
+    int reg;  // 0-based region index idx_reg(s) - 1
+    int pad;
+};
+
+// One chain per block: the steps [step0, step0 + nsteps), propagation matrices
+// P[pbase + reg], the axion source e0 + e0off and the outputs at slot `out`
+// (boost[out * n ...]).  A single geometry has one job per frequency; a batch
+// of geometries with the same mode count has one job per (geometry, frequency).
+struct ChainJob {
+    int64_t step0, pbase, out, e0off;
+    int nsteps, pad;
+};
+
+// propagation_matrix (src/boost_fractor_transformer.jl:174-227) of the regions
+// `regs` at nf frequencies into P [nf][nreg][n][n] (column-major matrices).
+// `cache` (optional) carries the mode set's frequency-independent pieces across calls.
+void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR, int nreg,
+                               const double* distance, const zd* eps, const double* tx, const double* ty,
+                               const zd* surfaces_dev, const std::vector<bool>& flat, const double* fr, int nf,
+                               const std::vector<int>& regs, zd* P, ModeCache* cache = nullptr);
+
+// surface maps on the device and which regions are flat (all-zero map)
+void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat);
+
+// permittivities with NaN replaced by 1e30 (src/boost_fractor_transformer.jl:289)
+std::vector<zd> eps_fixed(const bf_setup* st);
+
+// chain steps of one geometry at one frequency (regions Nreg-1 .. 1, :311-341)
+void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainStep>& steps);
+
+// inv(P) of the matrices `sel`, then one chain per job; boost (and refl) [njobs][n]
+void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
+                        const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
+                        const zd* reflect, zd* boost, zd* refl);
+
+}  // namespace bf
+
+// cubic B-spline coefficients of a propagation-matrix grid over the spacing axis
+struct bf_spacing_itp {
+    bf_ctx* ctx = nullptr;
+    int nreg = 0, nf = 0, ns = 0, n = 0;
+    double first = 0.0, step = 0.0;
+    bf::DevBuf coef;  // [ns + 2][nf][nreg][n][n]
+};

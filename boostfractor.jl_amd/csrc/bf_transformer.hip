@@ -20,7 +20,7 @@
 // LAPACK's izamax) for inv(P) and for T22 x = b, and one block per frequency
 // walking the regions, using the block structure G*[P 0; 0 inv(P)] =
 // [a P, b inv(P); b P, a inv(P)].
-#include "bf_internal.h"
+#include "bf_modematch.h"
 
 #include <algorithm>
 #include <cmath>
@@ -534,24 +534,8 @@ __global__ void k_batched_inverse(const zd* __restrict__ P, zd* Pinv, int n, zd*
 }
 
 // ---------------------------------------------------------------------------
-// transfer-matrix chain, one block per frequency (src/boost_fractor_transformer.jl:326-357)
+// transfer-matrix chain, one block per job (src/boost_fractor_transformer.jl:326-357)
 // ---------------------------------------------------------------------------
-struct ChainStep {
-    zd S;     // axion_S_factor(sqrt(eps[idx(s+1)]), sqrt(eps[idx(s)]))
-    zd a, b;  // G = [a I, b I; b I, a I]
-    int reg;  // 0-based region index idx_reg(s) - 1
-    int pad;
-};
-
-// One chain per block: the steps [step0, step0 + nsteps), propagation matrices
-// P[pbase + reg], the axion source e0 + e0off and the outputs at slot `out`
-// (boost[out * n ...]).  A single geometry has one job per frequency; a batch
-// of geometries with the same mode count has one job per (geometry, frequency).
-struct ChainJob {
-    int64_t step0, pbase, out, e0off;
-    int nsteps, pad;
-};
-
 __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __restrict__ steps,
                         const zd* __restrict__ P, const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
                         const zd* __restrict__ reflect, zd* Tbuf, zd* work, zd* boost, zd* refl) {
@@ -669,21 +653,6 @@ static zd cdiv_h(zd a, zd b) {
     return cmk<double>((a.re * r + a.im) / d, (a.im * r - a.re) / d);
 }
 
-struct ModesDev {
-    int n = 0, C = 0;
-    size_t K = 0, Ktot = 0;
-    DevBuf pat, kt;
-    void upload(const bf_modes* m, size_t K_, hipStream_t s) {
-        n = m->n;
-        C = m->ncomp;
-        K = K_;
-        Ktot = (size_t)C * K;
-        pat.alloc(sizeof(zd) * n * Ktot);
-        kt.alloc(sizeof(zd) * n);
-        bf::upload(pat.p, m->patterns, sizeof(zd) * n * Ktot, s);
-        bf::upload(kt.p, m->mode_kt, sizeof(zd) * n, s);
-    }
-};
 
 // Weighted Gram launcher: problems share n, Ktot and the weight period Kw.
 struct Gram {
@@ -778,19 +747,11 @@ struct Gram {
 
 // All propagation matrices P[f][reg] for the regions in `regs` (0-based), at frequencies fr[0..nf).
 // eps: the permittivities to use (the transformer passes NaN-fixed values).
-// Frequency- and geometry-independent pieces of one mode set: the Gram matrix of
-// the patterns (flat untilted disks, 1D propagator) and the patterns' FFTs
-// (Parseval air regions).  Built on first use, reused by every later geometry.
-struct ModeCache {
-    DevBuf G, Fhat, FhatM;
-    bool haveG = false, haveF = false;
-};
 
-static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR,
-                                      int nreg, const double* distance, const zd* eps, const double* tx,
-                                      const double* ty, const zd* surfaces_dev, const std::vector<bool>& flat,
-                                      const double* fr, int nf, const std::vector<int>& regs, zd* P /* [nf][nreg][n][n] */,
-                                      ModeCache* cache = nullptr) {
+void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR, int nreg,
+                               const double* distance, const zd* eps, const double* tx, const double* ty,
+                               const zd* surfaces_dev, const std::vector<bool>& flat,
+                               const double* fr, int nf, const std::vector<int>& regs, zd* P, ModeCache* cache) {
     hipStream_t s = ctx->stream;
     const int n = md.n;
     const size_t K = g.K(), Ktot = md.Ktot, nn = (size_t)n * n;
@@ -941,8 +902,7 @@ static void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev
     BF_HIP(hipStreamSynchronize(s));
 }
 
-// surface maps on the device and which regions are flat (all-zero map)
-static void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat) {
+void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat) {
     hipStream_t s = ctx->stream;
     flat.assign(st->nreg, true);
     if (st->surfaces) {
@@ -1052,7 +1012,7 @@ void propagation_matrices_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* 
     BF_HIP(hipStreamSynchronize(s));
 }
 
-static std::vector<zd> eps_fixed(const bf_setup* st) {
+std::vector<zd> eps_fixed(const bf_setup* st) {
     const zd* e = reinterpret_cast<const zd*>(st->eps);
     std::vector<zd> v(e, e + st->nreg);
     for (auto& z : v)
@@ -1060,8 +1020,7 @@ static std::vector<zd> eps_fixed(const bf_setup* st) {
     return v;
 }
 
-// chain steps of one geometry at one frequency (regions Nreg-1 .. 1, src/boost_fractor_transformer.jl:311-341)
-static void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainStep>& steps) {
+void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainStep>& steps) {
     for (int q = 0; q < nreg - 1; q++) {
         const int sidx = nreg - 1 - q;                      // s in the reference loop
         const int ir = nreg - sidx, ir1 = nreg - sidx - 1;  // idx_reg(s)-1, idx_reg(s+1)-1
@@ -1080,10 +1039,9 @@ static void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector
     }
 }
 
-// inverses of the selected matrices, then one chain per job; outputs boost (and refl) [njobs][n]
-static void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
-                               const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
-                               const zd* reflect, zd* boost, zd* refl) {
+void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
+                        const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
+                        const zd* reflect, zd* boost, zd* refl) {
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n;
     const int N2 = 2 * n;

@@ -230,6 +230,41 @@ int bf_transformer_batch(bf_ctx *ctx, int32_t nconf, const bf_setup *setups, con
                          const int32_t *mode_set_of, const double *freqs, int32_t nfreq, int32_t prop_kind,
                          double diskR, int32_t precision, const int64_t *boost_offset, double *boost);
 
+/* ---- disk-position sweep (examples/transformer_optim_utilities.jl) ------ */
+/* calc_propagation_matrices_grid (:11-52): the setup split into its gaps (0-based
+ * regions 1, 3, ..., nreg-1) and solids (0, 2, ..., nreg-2; nreg even); every gap
+ * distance shifted by spacing_grid[s] and every gap tilt by tilt_x_grid[tx],
+ * tilt_y_grid[ty]; solids computed once per frequency.
+ * out: complex [nspacing][ntilt_x][ntilt_y][nfreq][nreg][n][n] (column-major n x n). */
+int bf_propagation_matrices_grid(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords,
+                                 const bf_modes *modes, const double *spacing_grid, int32_t nspacing,
+                                 const double *tilt_x_grid, int32_t ntilt_x, const double *tilt_y_grid,
+                                 int32_t ntilt_y, const double *freqs, int32_t nfreq, int32_t prop_kind,
+                                 double diskR, double *out);
+
+/* construct_prop_matrix_interpolation (:58-74): cubic B-spline (Interpolations.jl
+ * BSpline(Cubic(Natural(OnCell())))) over a uniform spacing grid spacing_first +
+ * k*spacing_step, k < nspacing (>= 2), kept on the device.
+ * grid: complex [nspacing][nfreq][nreg][n][n] (the tilt-index-(1,1) slice of
+ * bf_propagation_matrices_grid's output). */
+typedef struct bf_spacing_itp bf_spacing_itp;
+int bf_spacing_itp_create(bf_ctx *ctx, const double *grid, int32_t nreg, int32_t nfreq, int32_t nspacing,
+                          int32_t n, double spacing_first, double spacing_step, bf_spacing_itp **out);
+/* Interpolations.bounds along the spacing axis: first - step/2, last + step/2 */
+int bf_spacing_itp_bounds(const bf_spacing_itp *itp, double *lo, double *hi);
+/* interpolate_prop_matrix (:79-91) for ncand candidates; dist_shift: [ncand][(nreg-2)/2]
+ * (one shift per gap except the last; solids and the last gap are evaluated at 0).
+ * A shift outside the bounds is an error (BF_ERR_ARG, Interpolations' BoundsError).
+ * out: complex [ncand][nfreq][nreg][n][n]. */
+int bf_spacing_itp_matrices(bf_spacing_itp *itp, int32_t ncand, const double *dist_shift, double *out);
+/* calc_boostfactor_modes / calc_modes (:99-125) on the interpolated matrices of
+ * ncand candidates: transformer (src/boost_fractor_transformer.jl:285-359) with
+ * propagation_matrices given.  eps: complex [nreg] (NaN read as 1e30); emit:
+ * complex [n]; reflect: complex [n] or NULL.  boost, refl: complex [ncand][nfreq][n]. */
+int bf_spacing_itp_transformer(bf_spacing_itp *itp, int32_t ncand, const double *dist_shift, const double *eps,
+                               const double *emit, const double *reflect, double *boost, double *refl);
+int bf_spacing_itp_destroy(bf_spacing_itp *itp);
+
 /* transformer_trace_back (src/boost_fractor_transformer.jl:372-421) at one frequency.
  * reflected, input: complex [n]; emit: complex [n] (used when includes_axion);
  * prop_mats: complex [nreg][n][n] or NULL.  fields: complex [nreg][2][n]. */
