@@ -204,7 +204,7 @@ def transformer_main(args):
         boost = B.transformer(sb, coords, modes, f=freqs, diskR=DISKR)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "inverse", "chain")}
+    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
     ctx.timing(False)
     ms, launches, flops = kt_["wgram"]
     achieved = flops / (ms * 1e-3) / 1e12
@@ -244,6 +244,85 @@ def transformer_main(args):
     print(json.dumps(out), flush=True)
 
 
+# disk-position sweep: seeded boosters with 10..80 disks (spacings 8 mm +- 0.5 mm),
+# 9/15/25 Bessel modes, 256^2, 100 frequencies on 19-21 GHz per geometry
+N5, NF5, NCONF5 = 256, 100, 4096
+MODESETS5 = [(3, 1), (5, 1), (5, 2)]  # (M, L): n = M (2L + 1) = 9, 15, 25
+
+
+def sweep_geometries(coords, nconf, seed=5000):
+    """the first `nconf` of the sweep's seeded geometries: disk count uniform in
+    10..80, mode set uniform over MODESETS5"""
+    from BoostFractor import workloads as W
+    rng = W.SplitMix64(seed)
+    geoms, set_of = [], []
+    for _ in range(nconf):
+        nd = 10 + int(rng.uniform() * 71)
+        ms = int(rng.uniform() * len(MODESETS5))
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp)))
+        set_of.append(ms)
+    return geoms, np.array(set_of, dtype=np.int32)
+
+
+def sweep_main(args):
+    import copy
+    import BoostFractor as B
+    from BoostFractor import workloads as W
+    coords = W.grid_coords(N5)
+    nconf = args.nconf
+    geoms, set_of = sweep_geometries(coords, nconf)
+    sets = [B.SeedModes(coords, ThreeDim=True, Mmax=M, Lmax=L, diskR=DISKR) for M, L in MODESETS5]
+    freqs = np.linspace(19e9, 21e9, NF5)
+    ctx = B.context()
+    ndisks = [(len(g.distance) - 2) // 2 for g in geoms]
+    print(f"[bench] sweep: {nconf} geometries (disks {min(ndisks)}..{max(ndisks)}), {N5}x{N5}, {NF5} freqs",
+          file=sys.stderr, flush=True)
+    emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
+    for _ in range(args.warmup):
+        B.transformer_batch(copy.deepcopy(geoms[:3]), coords, sets, f=freqs[:4], mode_set_of=set_of[:3],
+                            diskR=DISKR, emits=emits)
+    ctx.synchronize()
+    ctx.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        boost = B.transformer_batch(copy.deepcopy(geoms), coords, sets, f=freqs, mode_set_of=set_of, diskR=DISKR,
+                                    emits=emits)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
+    ctx.timing(False)
+    ms, launches, flops = kt_["wgram"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    pts = nconf * NF5 * args.steps / dt
+    out = {
+        "metric": "disk-position sweep boost-factor points/s on 1 MI355X (transformer, FP64)",
+        "value": round(pts, 2),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 geometries; Bessel modes)",
+        "config": {"workload": f"disk-position sweep, first {nconf} of {NCONF5} seeded boosters with 10-80 disks, "
+                               f"{N5}x{N5} grid over 1 m, 9/15/25 Bessel modes, {NF5} freqs 19-21 GHz",
+                   "geometries": nconf, "grid": N5, "frequencies": NF5},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4),
+                     "kernel": "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)",
+                     "avg_launch_ms": round(ms / max(launches, 1), 3),
+                     "alg_flops_per_launch": flops / max(launches, 1)},
+        "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
+        "beta2": {"min": float(min(np.sum(np.abs(b) ** 2, axis=-1).min() for b in boost)),
+                  "max": float(max(np.sum(np.abs(b) ** 2, axis=-1).max() for b in boost))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,8 +335,10 @@ def main():
                     help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
-    ap.add_argument("--workload", choices=("dancer", "transformer"), default="dancer",
-                    help="dancer: the headline sweep (default); transformer: the mode-matching sweep")
+    ap.add_argument("--workload", choices=("dancer", "transformer", "sweep"), default="dancer",
+                    help="dancer: the headline sweep (default); transformer: the mode-matching sweep; "
+                         "sweep: the many-geometry disk-position sweep")
+    ap.add_argument("--nconf", type=int, default=32, help="geometries of the sweep workload")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
         if args.summarize_trace:
@@ -272,6 +353,9 @@ def main():
         sys.exit(2)
     if args.workload == "transformer":
         transformer_main(args)
+        return
+    if args.workload == "sweep":
+        sweep_main(args)
         return
 
     import BoostFractor as B

@@ -137,6 +137,7 @@ int bf_ctx_destroy(bf_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     bf::clear_timers(ctx);
+    bf::release_blas(ctx);
     ctx->twiddles.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

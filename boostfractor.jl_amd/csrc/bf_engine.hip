@@ -830,6 +830,8 @@ void Grid::upload(const bf_coords* c, hipStream_t s) {
     bf::upload(Y.p, c->Y, sizeof(double) * ny, s);
     bf::upload(kX.p, c->kX, sizeof(double) * nx, s);
     bf::upload(kY.p, c->kY, sizeof(double) * ny, s);
+    hkX.assign(c->kX, c->kX + nx);
+    hkY.assign(c->kY, c->kY + ny);
 }
 
 template <typename T>

@@ -84,6 +84,7 @@ struct bf_ctx {
     std::map<std::pair<int, int>, bf::DevBuf> twiddles;
     bool timing = false;
     std::map<std::string, bf_kernel_timer> timers;
+    void* blas = nullptr;  // rocBLAS handle on `stream`, created on first use
 };
 
 namespace bf {
@@ -118,6 +119,7 @@ void clear_timers(bf_ctx* ctx);
 struct Grid {
     int nx = 0, ny = 0;
     DevBuf X, Y, kX, kY;  // double
+    std::vector<double> hkX, hkY;  // host copies of kX, kY
     void upload(const bf_coords* c, hipStream_t s);
     size_t K() const { return (size_t)nx * ny; }
 };
@@ -206,6 +208,7 @@ void propagation_matrices_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* 
 void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* md, const double* freqs,
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl);
+void release_blas(bf_ctx* ctx);
 void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
                            const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,

@@ -31,7 +31,31 @@ struct ModesDev {
 struct ModeCache {
     DevBuf G, Fhat, FhatM;
     bool haveG = false, haveF = false;
+    // Parseval regions by k-bin group: bins with equal (kx^2, ky^2) (and, on a
+    // square grid with kX == kY, the swapped pair) share one weight, so
+    // P = sum_g w_g H_g with H_g = sum_{k in g} conj(Fhat_i(k)) FhatM_j(k).
+    bool haveH = false;
+    int ngroups = 0;
+    DevBuf H;         // [ngroups][n][n] (column-major)
+    DevBuf gkx, gky;  // representative kX, kY of each group (double)
 };
+
+// Parseval weight of one (region, frequency): exp(-1im*kz*dz), kz from k0^2
+struct PwDesc {
+    zd k0sq;
+    double dz;
+    int pad0, pad1;
+};
+
+// Parseval problems of one mode set waiting for one grouped GEMM: their weights
+// and the device address of each output matrix
+struct ParsevalQueue {
+    std::vector<PwDesc> pd;
+    std::vector<zd*> dst;
+};
+
+// P = sum_g W_g H_g for every queued problem (GEMMs in chunks bounded by memory); empties the queue
+void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q);
 
 // one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
 struct ChainStep {
@@ -56,7 +80,8 @@ struct ChainJob {
 void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR, int nreg,
                                const double* distance, const zd* eps, const double* tx, const double* ty,
                                const zd* surfaces_dev, const std::vector<bool>& flat, const double* fr, int nf,
-                               const std::vector<int>& regs, zd* P, ModeCache* cache = nullptr);
+                               const std::vector<int>& regs, zd* P, ModeCache* cache = nullptr,
+                               ParsevalQueue* queue = nullptr);
 
 // surface maps on the device and which regions are flat (all-zero map)
 void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat);

@@ -22,6 +22,8 @@
 // [a P, b inv(P); b P, a inv(P)].
 #include "bf_modematch.h"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -610,17 +612,20 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
     }
 }
 
-// disk-region diagonal propagation: P[f][reg] = diag(exp(-1im*kz_m*dz)) * P
-__global__ void k_disk_diag(zd* P, const int* __restrict__ slots, const zd* __restrict__ k0sq, const double* dz,
+// disk-region diagonal propagation: P[f][reg] = diag(exp(-1im*kz_m*dz)) * X with
+// X = the shared Gram matrix G (flat untilted disk, from_gram[q] != 0) or P[f][reg] itself
+__global__ void k_disk_diag(zd* P, const int* __restrict__ slots, const int* __restrict__ from_gram,
+                            const zd* __restrict__ G, const zd* __restrict__ k0sq, const double* dz,
                             const zd* __restrict__ kt, int n) {
     const int q = blockIdx.x;
     const zd k2 = k0sq[q];
     zd* Pm = P + (size_t)slots[q] * n * n;
+    const zd* X = from_gram[q] ? G : Pm;
     for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
         const int i = e % n;
         const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
         const zd ph = cexp_j(rmul(dz[q], neg_i(kz)));
-        Pm[e] = cmul(ph, Pm[e]);
+        Pm[e] = cmul(ph, X[e]);
     }
 }
 
@@ -629,6 +634,49 @@ __global__ void k_scaled_copy(zd* P, const int* __restrict__ slots, const zd* __
                               int nn) {
     const int q = blockIdx.x;
     for (int e = threadIdx.x; e < nn; e += blockDim.x) P[(size_t)slots[q] * nn + e] = cmul(G[e], s[q]);
+}
+
+// Parseval regions by k-bin group.  H[g] = sum over the group's bins k and the
+// components c of conj(A_i(c,k)) B_j(c,k), column-major n x n (i + n j).
+__global__ void k_group_gram(const zd* __restrict__ A, const zd* __restrict__ B, size_t K, int C, int n,
+                             const int* __restrict__ off, const int* __restrict__ bins, zd* H) {
+    const int g = blockIdx.x;
+    const int nn = n * n;
+    const int b0 = off[g], b1 = off[g + 1];
+    const size_t Ktot = (size_t)C * K;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        const int i = e % n, j = e / n;
+        double re = 0.0, im = 0.0;
+        for (int q = b0; q < b1; q++) {
+            const size_t k = (size_t)bins[q];
+            for (int c = 0; c < C; c++) {
+                const zd a = A[(size_t)i * Ktot + (size_t)c * K + k], b = B[(size_t)j * Ktot + (size_t)c * K + k];
+                re += a.re * b.re + a.im * b.im;
+                im += a.re * b.im - a.im * b.re;
+            }
+        }
+        H[(size_t)g * nn + e] = cmk<double>(re, im);
+    }
+}
+
+// W[m][g] = exp(-1im*kz*dz) * scale with kz = sqrt(k0^2 - kx^2 - ky^2) at the group's
+// representative bin (the same phase as the per-bin tables of the propagator)
+__global__ void k_parseval_w(zd* W, const PwDesc* __restrict__ d, const double* __restrict__ gkx,
+                             const double* __restrict__ gky, int G, double scale) {
+    const int m = blockIdx.y;
+    const PwDesc dd = d[m];
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        const zd ph = kz_phase(dd.k0sq, gkx[g], gky[g], dd.dz);
+        W[(size_t)m * G + g] = cmk<double>(ph.re * scale, ph.im * scale);
+    }
+}
+
+// *dsts[m] = Cm[m] (n x n each)
+__global__ void k_scatter_mats(zd* const* __restrict__ dsts, const zd* __restrict__ Cm, int nn) {
+    const int m = blockIdx.y;
+    zd* dst = dsts[m];
+    const zd* src = Cm + (size_t)m * nn;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += gridDim.x * blockDim.x) dst[e] = src[e];
 }
 
 __global__ void k_copy_z(zd* dst, const zd* __restrict__ src, size_t n) {
@@ -653,6 +701,75 @@ static zd cdiv_h(zd a, zd b) {
     return cmk<double>((a.re * r + a.im) / d, (a.im * r - a.re) / d);
 }
 
+
+static rocblas_handle blas_of(bf_ctx* ctx) {
+    if (!ctx->blas) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(BF_ERR_HIP, "rocblas_create_handle failed");
+        ctx->blas = h;
+        if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success)
+            throw Error(BF_ERR_HIP, "rocblas_set_stream failed");
+    }
+    return static_cast<rocblas_handle>(ctx->blas);
+}
+
+void release_blas(bf_ctx* ctx) {
+    if (ctx->blas) {
+        (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+        ctx->blas = nullptr;
+    }
+}
+
+// Groups of k bins sharing a Parseval weight and their Gram matrices H_g.
+// Bins are keyed by (kx^2, ky^2) of the fftshifted entry they multiply; on a
+// square grid with kX == kY the pair is unordered (the weight's two
+// subtractions then run in the representative's order, a rounding-level
+// difference).  Returns false (no grouping) for non-finite k grids.
+static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md, ModeCache& mc) {
+    hipStream_t s = ctx->stream;
+    const int nx = g.nx, ny = g.ny, n = md.n;
+    const size_t K = g.K(), nn = (size_t)n * n;
+    for (double v : g.hkX) if (!std::isfinite(v)) return false;
+    for (double v : g.hkY) if (!std::isfinite(v)) return false;
+    const bool sym = nx == ny && std::memcmp(g.hkX.data(), g.hkY.data(), sizeof(double) * nx) == 0;
+    std::vector<std::pair<std::pair<double, double>, int>> keyed(K);
+    for (int my = 0; my < ny; my++)
+        for (int mx = 0; mx < nx; mx++) {
+            const int jx = (mx + nx / 2) % nx, jy = (my + ny / 2) % ny;
+            double a = g.hkX[jx] * g.hkX[jx], b = g.hkY[jy] * g.hkY[jy];
+            if (sym && b < a) std::swap(a, b);
+            keyed[(size_t)my * nx + mx] = {{a, b}, my * nx + mx};
+        }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int> off, bins(K);
+    std::vector<double> gx, gy;
+    for (size_t q = 0; q < K; q++) {
+        if (q == 0 || keyed[q].first != keyed[q - 1].first) {
+            off.push_back((int)q);
+            const int bin = keyed[q].second;  // lowest bin index of the group
+            gx.push_back(g.hkX[((bin % nx) + nx / 2) % nx]);
+            gy.push_back(g.hkY[((bin / nx) + ny / 2) % ny]);
+        }
+        bins[q] = keyed[q].second;
+    }
+    off.push_back((int)K);
+    const int G = (int)gx.size();
+    DevBuf doff(sizeof(int) * off.size()), dbins(sizeof(int) * K);
+    upload(doff.p, off.data(), sizeof(int) * off.size(), s);
+    upload(dbins.p, bins.data(), sizeof(int) * K, s);
+    mc.gkx.alloc(sizeof(double) * G);
+    mc.gky.alloc(sizeof(double) * G);
+    upload(mc.gkx.p, gx.data(), sizeof(double) * G, s);
+    upload(mc.gky.p, gy.data(), sizeof(double) * G, s);
+    mc.H.alloc(sizeof(zd) * nn * G);
+    hipLaunchKernelGGL(k_group_gram, dim3(G), dim3(256), 0, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(), K, md.C, n,
+                       doff.as<int>(), dbins.as<int>(), mc.H.as<zd>());
+    BF_CHECK_LAUNCH();
+    BF_HIP(hipStreamSynchronize(s));
+    mc.ngroups = G;
+    mc.haveH = true;
+    return true;
+}
 
 // Weighted Gram launcher: problems share n, Ktot and the weight period Kw.
 struct Gram {
@@ -751,7 +868,8 @@ struct Gram {
 void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR, int nreg,
                                const double* distance, const zd* eps, const double* tx, const double* ty,
                                const zd* surfaces_dev, const std::vector<bool>& flat,
-                               const double* fr, int nf, const std::vector<int>& regs, zd* P, ModeCache* cache) {
+                               const double* fr, int nf, const std::vector<int>& regs, zd* P, ModeCache* cache,
+                               ParsevalQueue* queue) {
     hipStream_t s = ctx->stream;
     const int n = md.n;
     const size_t K = g.K(), Ktot = md.Ktot, nn = (size_t)n * n;
@@ -790,15 +908,34 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         fft_fields(ctx, g, FhatM.as<zd>(), n * md.C, true, diskR);
         mc.haveF = true;
     }
+    // Parseval regions through k-bin groups (BF_PARSEVAL_GROUPS=0: per-bin weights)
+    bool grouped = false;
+    if (!parseval.empty()) {
+        const char* e = std::getenv("BF_PARSEVAL_GROUPS");
+        if (!(e && e[0] == '0')) grouped = mc.haveH || build_parseval_groups(ctx, g, md, mc);
+    }
+    const size_t NG = grouped ? (size_t)mc.ngroups : 0;
     // frequency-dependent pieces, in frequency batches bounded by table memory
-    const size_t per_f = std::max<size_t>(1, parseval.size() + disks.size()) * K * sizeof(zd);
+    const size_t per_f = std::max<size_t>(1, (grouped ? 0 : parseval.size()) + disks.size()) * K * sizeof(zd);
     const int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nf, (512u << 20) / per_f));
-    DevBuf ktabs, rtabs, scratch, prop, scr2, dslots, dk0, ddz;
+    DevBuf ktabs, rtabs, scratch, prop, scr2, dslots, dflag, dk0, ddz;
+    ParsevalQueue local_q;
+    ParsevalQueue& pq = queue ? *queue : local_q;
     for (int f0 = 0; f0 < nf; f0 += fb) {
         const int nb = std::min(fb, nf - f0);
         // Parseval air regions: w = exp(-i kz dz)/K in unshifted bins
         std::vector<GemmProb> probs;
-        if (!parseval.empty()) {
+        if (grouped) {
+            // queued: P[slot] = sum_g W[m][g] H_g, one GEMM for all of them
+            for (int f = 0; f < nb; f++)
+                for (int r : parseval) {
+                    zd e = eps[r];
+                    if (kind == BF_PROP_PROPAGATOR && e.im == 0.0) e.im = -0.0;
+                    const zd k0 = k0_of(e, 299792458.0 / fr[f0 + f]);
+                    pq.pd.push_back(PwDesc{cmul(k0, k0), distance[r], 0, 0});
+                    pq.dst.push_back(P + ((size_t)f * nreg + r) * nn);
+                }
+        } else if (!parseval.empty()) {
             std::vector<KTabDesc> kd;
             for (int f = 0; f < nb; f++)
                 for (int r : parseval) {
@@ -818,7 +955,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
                 }
         }
         // disk regions: weights = tilt*surface phase, or the shared Gram matrix
-        std::vector<int> slots;
+        std::vector<int> slots, from_gram;
         std::vector<zd> k0sq;
         std::vector<double> dzs;
         {
@@ -831,11 +968,8 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
                     slots.push_back(slot);
                     k0sq.push_back(cmul(k0, k0));
                     dzs.push_back(distance[r]);
-                    if (flat[r] && tx[r] == 0.0 && ty[r] == 0.0) {
-                        hipLaunchKernelGGL(k_copy_z, dim3(g1(nn, 64)), dim3(256), 0, s, P + (size_t)slot * nn,
-                                           G.as<zd>(), nn);
-                        BF_CHECK_LAUNCH();
-                    } else {
+                    from_gram.push_back(flat[r] && tx[r] == 0.0 && ty[r] == 0.0);
+                    if (!from_gram.back()) {
                         rd.push_back(RTabDesc{neg_i(k0), tx[r], ty[r], flat[r] ? -1 : (int64_t)(K * r)});
                         rd_slot.push_back({slot, (int)rd.size() - 1});
                     }
@@ -852,13 +986,15 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         // diag(exp(-i kz dz)) from the left for the disk regions
         if (!slots.empty()) {
             dslots.alloc(sizeof(int) * slots.size());
+            dflag.alloc(sizeof(int) * slots.size());
             dk0.alloc(sizeof(zd) * slots.size());
             ddz.alloc(sizeof(double) * slots.size());
             upload(dslots.p, slots.data(), sizeof(int) * slots.size(), s);
+            upload(dflag.p, from_gram.data(), sizeof(int) * slots.size(), s);
             upload(dk0.p, k0sq.data(), sizeof(zd) * slots.size(), s);
             upload(ddz.p, dzs.data(), sizeof(double) * slots.size(), s);
             hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)slots.size()), dim3(256), 0, s, P, dslots.as<int>(),
-                               dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n);
+                               dflag.as<int>(), G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n);
             BF_CHECK_LAUNCH();
             BF_HIP(hipStreamSynchronize(s));
         }
@@ -899,7 +1035,47 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         }
         P += (size_t)nb * nreg * nn;
     }
+    if (!queue) parseval_flush(ctx, mc, n, 1.0 / (double)((int64_t)g.nx * g.ny), local_q);
     BF_HIP(hipStreamSynchronize(s));
+}
+
+void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q) {
+    if (q.pd.empty()) return;
+    hipStream_t s = ctx->stream;
+    const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, total = q.pd.size();
+    // problems per GEMM: W and the GEMM output within 4 GB (BF_PARSEVAL_BYTES overrides), grid.y <= 65535
+    size_t budget = (size_t)4 << 30;
+    if (const char* e = std::getenv("BF_PARSEVAL_BYTES")) {
+        const long long v = std::atoll(e);
+        if (v > 0) budget = (size_t)v;
+    }
+    const size_t mc_ = std::max<size_t>(1, std::min<size_t>({total, (size_t)65535, budget / ((NG + nn) * sizeof(zd))}));
+    DevBuf Wt(sizeof(zd) * mc_ * NG), Cm(sizeof(zd) * mc_ * nn), dpw(sizeof(PwDesc) * mc_), ddst(sizeof(zd*) * mc_);
+    for (size_t m0 = 0; m0 < total; m0 += mc_) {
+        const size_t M = std::min(mc_, total - m0);
+        upload(dpw.p, q.pd.data() + m0, sizeof(PwDesc) * M, s);
+        upload(ddst.p, q.dst.data() + m0, sizeof(zd*) * M, s);
+        hipLaunchKernelGGL(k_parseval_w, dim3(g1(NG, 64), (unsigned)M), dim3(256), 0, s, Wt.as<zd>(),
+                           dpw.as<PwDesc>(), mc.gkx.as<double>(), mc.gky.as<double>(), (int)NG, scale);
+        BF_CHECK_LAUNCH();
+        {
+            TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
+            const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+            if (rocblas_zgemm(blas_of(ctx), rocblas_operation_none, rocblas_operation_none, (rocblas_int)nn,
+                              (rocblas_int)M, (rocblas_int)NG, &one,
+                              reinterpret_cast<const rocblas_double_complex*>(mc.H.as<zd>()), (rocblas_int)nn,
+                              reinterpret_cast<const rocblas_double_complex*>(Wt.as<zd>()), (rocblas_int)NG, &zero,
+                              reinterpret_cast<rocblas_double_complex*>(Cm.as<zd>()),
+                              (rocblas_int)nn) != rocblas_status_success)
+                throw Error(BF_ERR_HIP, "rocblas_zgemm failed");
+        }
+        hipLaunchKernelGGL(k_scatter_mats, dim3(g1(nn, 64), (unsigned)M), dim3(256), 0, s, ddst.as<zd*>(),
+                           Cm.as<zd>(), (int)nn);
+        BF_CHECK_LAUNCH();
+        BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next chunk's uploads
+    }
+    q.pd.clear();
+    q.dst.clear();
 }
 
 void upload_surfaces(bf_ctx* ctx, const bf_setup* st, size_t K, DevBuf& surf, std::vector<bool>& flat) {
@@ -1167,6 +1343,7 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
             std::vector<ChainJob> jobs;
             std::vector<ChainStep> steps;
             size_t base = 0;
+            ParsevalQueue pq;
             for (size_t q = c0; q < c1; q++) {
                 const bf_setup* st = &setups[confs[q]];
                 const int nreg = st->nreg;
@@ -1178,7 +1355,7 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                 for (int r = 1; r < nreg; r++) regs.push_back(r);
                 propagation_matrices_core(ctx, g, md, kind, diskR, nreg, st->distance, eps.data(), st->tilt_x,
                                           st->tilt_y, surf.as<zd>(), flat, freqs, nfreq, regs,
-                                          P.as<zd>() + base * nn, &cache);
+                                          P.as<zd>() + base * nn, &cache, &pq);
                 for (int f = 0; f < nfreq; f++) {
                     for (int r = 1; r < nreg; r++) sel.push_back((int)(base + (size_t)f * nreg + r));
                     ChainJob j;
@@ -1193,6 +1370,7 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                 }
                 base += (size_t)nreg * nfreq;
             }
+            parseval_flush(ctx, cache, n, 1.0 / (double)K, pq);
             inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, de0.as<zd>(), nullptr,
                                dboost.as<zd>(), nullptr);
             for (size_t q = c0; q < c1; q++)
