@@ -671,6 +671,82 @@ __global__ void k_parseval_w(zd* W, const PwDesc* __restrict__ d, const double* 
     }
 }
 
+// Tilted/rough disk regions over a band of frequencies.  The weight of region r
+// at frequency f is exp(a_f h(x)) with a_f = -1im*k0(f) and the height map
+// h = tx*x + ty*y + surface (src/boost_fractor.jl:145-147 as used by
+// propagation_matrix, src/boost_fractor_transformer.jl:212-221).  With a_c at
+// the band centre, exp(a_f h) = exp(a_c h) * sum_p ((a_f - a_c) h_s)^p / p! * (h/h_s)^p,
+// so P_f = sum_p c_{f,p} M_p with the frequency-independent moments
+// M_p = Phi^H diag(exp(a_c h) (h/h_s)^p) Phi; the series is cut where its
+// remainder bound falls below 1e-17 (the matrices are O(1)).
+struct MomDesc {
+    zd a;       // a_c = -1im*k0 at the band centre
+    double tx, ty;
+    int64_t surf;  // element offset of the surface map, -1: flat
+    double hinv;   // 1/h_s
+    int p, pad;
+};
+
+// max over the grid of |tx*x + ty*y + surface| for each region (one block per region)
+__global__ void k_height_max(const MomDesc* __restrict__ d, int nx, int ny, const double* __restrict__ X,
+                             const double* __restrict__ Y, const zd* __restrict__ surf, double* out) {
+    __shared__ double red[256];
+    const MomDesc dd = d[blockIdx.x];
+    const size_t K = (size_t)nx * ny;
+    double m = 0.0;
+    for (size_t i = threadIdx.x; i < K; i += blockDim.x) {
+        double hr = dd.tx * X[i % nx] + dd.ty * Y[i / nx], hi = 0.0;
+        if (dd.surf >= 0) {
+            const zd sv = surf[dd.surf + i];
+            hr += sv.re;
+            hi = sv.im;
+        }
+        m = fmax(m, sqrt(hr * hr + hi * hi));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// moment weights exp(a_c h) (h/h_s)^p, one table per descriptor
+__global__ void k_build_moments(zd* out, const MomDesc* __restrict__ d, int nx, int ny, const double* __restrict__ X,
+                                const double* __restrict__ Y, const zd* __restrict__ surf) {
+    const int tab = blockIdx.y;
+    const size_t K = (size_t)nx * ny;
+    const MomDesc dd = d[tab];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
+        zd h = cmk<double>(dd.tx * X[i % nx] + dd.ty * Y[i / nx], 0.0);
+        if (dd.surf >= 0) h = cadd(h, surf[dd.surf + i]);
+        zd v = cexp_j(cmul(dd.a, h));
+        const zd u = rmul(dd.hinv, h);
+        for (int q = 0; q < dd.p; q++) v = cmul(v, u);
+        out[(size_t)tab * K + i] = v;
+    }
+}
+
+struct CombJob {
+    int slot, q0, np, c0;  // output slot, first moment, moment count, first coefficient
+};
+
+// P[slot] = sum_p coef[c0 + p] * M[q0 + p]
+__global__ void k_combine_moments(zd* P, const zd* __restrict__ M, const CombJob* __restrict__ jobs,
+                                  const zd* __restrict__ coef, int nn) {
+    const CombJob jb = jobs[blockIdx.y];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += gridDim.x * blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int p = 0; p < jb.np; p++) {
+            const zd c = coef[jb.c0 + p], m = M[(size_t)(jb.q0 + p) * nn + e];
+            re += c.re * m.re - c.im * m.im;
+            im += c.re * m.im + c.im * m.re;
+        }
+        P[(size_t)jb.slot * nn + e] = cmk<double>(re, im);
+    }
+}
+
 // *dsts[m] = Cm[m] (n x n each)
 __global__ void k_scatter_mats(zd* const* __restrict__ dsts, const zd* __restrict__ Cm, int nn) {
     const int m = blockIdx.y;
@@ -916,8 +992,80 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     }
     const size_t NG = grouped ? (size_t)mc.ngroups : 0;
     // frequency-dependent pieces, in frequency batches bounded by table memory
-    const size_t per_f = std::max<size_t>(1, (grouped ? 0 : parseval.size()) + disks.size()) * K * sizeof(zd);
-    const int fb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nf, (512u << 20) / per_f));
+    // tilted/rough disk regions: moments of the frequency expansion (BF_DISK_EXPANSION=0: per-frequency tables)
+    std::vector<int> exp_q0(nreg, -1), exp_np(nreg, 0);
+    std::vector<double> exp_hs(nreg, 0.0);
+    std::vector<zd> exp_ac(nreg);
+    DevBuf Mom;
+    size_t n_direct_disks = 0;
+    {
+        std::vector<int> rough;
+        for (int r : disks)
+            if (!(flat[r] && tx[r] == 0.0 && ty[r] == 0.0)) rough.push_back(r);
+        const char* e = std::getenv("BF_DISK_EXPANSION");
+        if (!rough.empty() && nf > 2 && !(e && e[0] == '0')) {
+            std::vector<MomDesc> hd;
+            for (int r : rough)
+                hd.push_back(MomDesc{cmk<double>(0.0, 0.0), tx[r], ty[r], flat[r] ? -1 : (int64_t)(K * r), 0.0, 0, 0});
+            DevBuf dhd(sizeof(MomDesc) * hd.size()), dhm(sizeof(double) * hd.size());
+            upload(dhd.p, hd.data(), sizeof(MomDesc) * hd.size(), s);
+            hipLaunchKernelGGL(k_height_max, dim3((unsigned)hd.size()), dim3(256), 0, s, dhd.as<MomDesc>(), g.nx, g.ny,
+                               g.X.as<double>(), g.Y.as<double>(), surfaces_dev, dhm.as<double>());
+            BF_CHECK_LAUNCH();
+            std::vector<double> hmax(hd.size());
+            download(hmax.data(), dhm.p, sizeof(double) * hd.size(), s);
+            BF_HIP(hipStreamSynchronize(s));
+            const double fmin = *std::min_element(fr, fr + nf), fmax = *std::max_element(fr, fr + nf);
+            const double fc = 0.5 * (fmin + fmax);
+            std::vector<MomDesc> md_;
+            for (size_t t = 0; t < rough.size(); t++) {
+                const int r = rough[t];
+                const double hs = hmax[t];
+                if (!std::isfinite(hs)) continue;
+                const zd kc = k0_of(eps[r], 299792458.0 / fc);
+                const zd ac = neg_i(kc);
+                const zd d1 = csub(k0_of(eps[r], 299792458.0 / fmin), kc), d2 = csub(k0_of(eps[r], 299792458.0 / fmax), kc);
+                const double delta = std::max(std::hypot(d1.re, d1.im), std::hypot(d2.re, d2.im)) * hs;
+                const double growth = std::exp(std::hypot(ac.re, ac.im) * hs + delta);
+                int P = 0;
+                double tail = growth * delta;  // delta^(P+1)/(P+1)! * e^delta * |exp(a_c h)| bound
+                while (tail >= 1e-17 && P < 24) {
+                    P++;
+                    tail *= delta / (P + 1);
+                }
+                if (tail >= 1e-17 || P + 1 >= nf) continue;  // the series does not pay (or does not converge fast)
+                exp_q0[r] = (int)md_.size();
+                exp_np[r] = P + 1;
+                exp_hs[r] = hs;
+                exp_ac[r] = ac;
+                for (int p = 0; p <= P; p++)
+                    md_.push_back(MomDesc{ac, tx[r], ty[r], flat[r] ? -1 : (int64_t)(K * r), hs > 0.0 ? 1.0 / hs : 0.0,
+                                          p, 0});
+            }
+            if (!md_.empty()) {
+                Mom.alloc(sizeof(zd) * nn * md_.size());
+                const size_t chunk = std::max<size_t>(1, (512u << 20) / (K * sizeof(zd)));
+                DevBuf tabs(sizeof(zd) * K * std::min(chunk, md_.size())), dmd(sizeof(MomDesc) * md_.size());
+                upload(dmd.p, md_.data(), sizeof(MomDesc) * md_.size(), s);
+                for (size_t q0 = 0; q0 < md_.size(); q0 += chunk) {
+                    const size_t nq = std::min(chunk, md_.size() - q0);
+                    hipLaunchKernelGGL(k_build_moments, dim3(g1(K, 1024), (unsigned)nq), dim3(256), 0, s, tabs.as<zd>(),
+                                       dmd.as<MomDesc>() + q0, g.nx, g.ny, g.X.as<double>(), g.Y.as<double>(),
+                                       surfaces_dev);
+                    BF_CHECK_LAUNCH();
+                    std::vector<GemmProb> mp;
+                    for (size_t q = 0; q < nq; q++)
+                        mp.push_back(GemmProb{md.pat.as<zd>(), md.pat.as<zd>(), tabs.as<zd>() + q * K, (int)(q0 + q), 0});
+                    gram.run(mp, Mom.as<zd>());
+                }
+            }
+        }
+        for (int r : rough)
+            if (exp_np[r] == 0) n_direct_disks++;
+    }
+    const size_t per_f = std::max<size_t>(1, (grouped ? 0 : parseval.size()) + n_direct_disks) * K * sizeof(zd);
+    const int fb = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)nf, (512u << 20) / per_f, 65535 / std::max<size_t>(1, disks.size())}));
     DevBuf ktabs, rtabs, scratch, prop, scr2, dslots, dflag, dk0, ddz;
     ParsevalQueue local_q;
     ParsevalQueue& pq = queue ? *queue : local_q;
@@ -958,6 +1106,8 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         std::vector<int> slots, from_gram;
         std::vector<zd> k0sq;
         std::vector<double> dzs;
+        std::vector<CombJob> comb;
+        std::vector<zd> coefs;
         {
             std::vector<RTabDesc> rd;
             std::vector<std::pair<int, int>> rd_slot;
@@ -969,7 +1119,16 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
                     k0sq.push_back(cmul(k0, k0));
                     dzs.push_back(distance[r]);
                     from_gram.push_back(flat[r] && tx[r] == 0.0 && ty[r] == 0.0);
-                    if (!from_gram.back()) {
+                    if (!from_gram.back() && exp_np[r] > 0) {
+                        // c_p = ((a_f - a_c) h_s)^p / p!
+                        const zd dah = rmul(exp_hs[r], csub(neg_i(k0), exp_ac[r]));
+                        comb.push_back(CombJob{slot, exp_q0[r], exp_np[r], (int)coefs.size()});
+                        zd c = cmk<double>(1.0, 0.0);
+                        for (int p = 0; p < exp_np[r]; p++) {
+                            coefs.push_back(c);
+                            c = rmul(1.0 / (p + 1), cmul(c, dah));
+                        }
+                    } else if (!from_gram.back()) {
                         rd.push_back(RTabDesc{neg_i(k0), tx[r], ty[r], flat[r] ? -1 : (int64_t)(K * r)});
                         rd_slot.push_back({slot, (int)rd.size() - 1});
                     }
@@ -983,6 +1142,15 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             }
         }
         gram.run(probs, P);
+        if (!comb.empty()) {
+            DevBuf dcomb(sizeof(CombJob) * comb.size()), dcoef(sizeof(zd) * coefs.size());
+            upload(dcomb.p, comb.data(), sizeof(CombJob) * comb.size(), s);
+            upload(dcoef.p, coefs.data(), sizeof(zd) * coefs.size(), s);
+            hipLaunchKernelGGL(k_combine_moments, dim3(g1(nn, 64), (unsigned)comb.size()), dim3(256), 0, s, P,
+                               Mom.as<zd>(), dcomb.as<CombJob>(), dcoef.as<zd>(), (int)nn);
+            BF_CHECK_LAUNCH();
+            BF_HIP(hipStreamSynchronize(s));
+        }
         // diag(exp(-i kz dz)) from the left for the disk regions
         if (!slots.empty()) {
             dslots.alloc(sizeof(int) * slots.size());

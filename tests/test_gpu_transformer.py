@@ -397,3 +397,25 @@ def test_transformer_batch_matches_oracle():
         eps[np.isnan(eps.real)] = 1e30
         sens = chain_sensitivity(_oracle_mats(sb, coords, md, 20.1e9, O.PROP_PROPAGATOR, eps), eps, emit)
         assert rel(got[c][0], want) < max(TOL, 10 * sens), (c, rel(got[c][0], want), sens)
+
+
+@pytest.mark.parametrize("lossy", [False, True])
+def test_propagation_matrices_disk_frequency_expansion(monkeypatch, lossy):
+    """tilted/rough disks over a 24-frequency band take the frequency expansion
+    (moments of the height map): every region against the oracle at 1e-10 and
+    against per-frequency weight tables; a lossy disk has a complex k0"""
+    coords, sb = _rough_booster(N=32, ndisk=2)
+    if lossy:
+        sb.eps[2] = 24.0 - 0.5j
+    modes = _lg_like_modes(coords, 2, 1)
+    freqs = np.linspace(19.5e9, 20.5e9, 24)
+    got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    monkeypatch.setenv("BF_DISK_EXPANSION", "0")
+    direct = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    for q in range(len(freqs)):
+        for r in range(1, len(sb.distance)):
+            assert rel(got[q][r], direct[q][r]) < 1e-13, (q, r, rel(got[q][r], direct[q][r]))
+    for q in (0, 9, 23):
+        want = _oracle_mats(sb, coords, modes, freqs[q], O.PROP_PROPAGATOR)
+        for r in range(1, len(sb.distance)):
+            assert rel(got[q][r], want[r]) < TOL, (q, r, rel(got[q][r], want[r]))
