@@ -246,7 +246,9 @@ class Marshalled:
         ty = self.hold(as_f64(np.broadcast_to(b.relative_tilt_y, dist.shape)))
         surf = None
         s = np.asarray(b.relative_surfaces)
-        if s.size and np.any(s != 0):
+        # any nonzero entry (region by region: stops at the first rough region, no temporaries)
+        if s.size and any(np.asarray(s[r]).view(np.float64 if s.dtype == np.complex128 else s.dtype).any()
+                          for r in range(s.shape[0])):
             if s.shape != (len(dist), ny, nx):
                 raise DimensionMismatch(
                     f"relative_surfaces has shape {s.shape}, expected {(len(dist), ny, nx)}")

@@ -535,6 +535,84 @@ __global__ void k_batched_inverse(const zd* __restrict__ P, zd* Pinv, int n, zd*
     }
 }
 
+// The same inverses with one wave per matrix and the matrix and the solutions in
+// LDS (n <= INV_WAVE_MAX): the pivot is a wave argmax (largest |re|+|im|, lowest
+// row on ties, as above), every arithmetic step is the one of lu_factor_block /
+// lu_solve_col in the same order, so the results are identical.
+constexpr int INV_WAVE_MAX = 48;
+
+__global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restrict__ P, zd* Pinv, int n,
+                                                           const int* __restrict__ sel) {
+    extern __shared__ unsigned char smem_raw[];
+    zd* M = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
+    zd* X = M + n * n;                        // X[i * n + j]: row i of solution column j
+    __shared__ int piv[INV_WAVE_MAX];
+    const int lane = threadIdx.x;
+    const int b = sel ? sel[blockIdx.x] : blockIdx.x;
+    const int nn = n * n;
+    const zd* src = P + (size_t)b * nn;
+    for (int e = lane; e < nn; e += 64) M[e] = src[e];
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + lane; i < n; i += 64) {
+            const zd v = M[i + n * k];
+            const double a = fabs(v.re) + fabs(v.im);
+            if (a > best) { best = a; bi = i; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int p = bi;
+        if (p != k)
+            for (int j = lane; j < n; j += 64) {
+                const zd t = M[k + n * j];
+                M[k + n * j] = M[p + n * j];
+                M[p + n * j] = t;
+            }
+        if (lane == 0) piv[k] = p;
+        __syncthreads();
+        const zd d = M[k + n * k];
+        for (int i = k + 1 + lane; i < n; i += 64) {
+            const zd v = M[i + n * k];
+            const double den = d.re * d.re + d.im * d.im;
+            M[i + n * k] = cmk<double>((v.re * d.re + v.im * d.im) / den, (v.im * d.re - v.re * d.im) / den);
+        }
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int e = lane; e < m * m; e += 64) {
+            const int i = k + 1 + e % m, j = k + 1 + e / m;
+            M[i + n * j] = csub(M[i + n * j], cmul(M[i + n * k], M[k + n * j]));
+        }
+        __syncthreads();
+    }
+    for (int j = lane; j < n; j += 64) {
+        for (int i = 0; i < n; i++) X[i * n + j] = cmk<double>(i == j ? 1.0 : 0.0, 0.0);
+        for (int k = 0; k < n; k++)
+            if (piv[k] != k) {
+                const zd t = X[k * n + j];
+                X[k * n + j] = X[piv[k] * n + j];
+                X[piv[k] * n + j] = t;
+            }
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < i; k++) X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
+        for (int i = n - 1; i >= 0; i--) {
+            zd x = X[i * n + j];
+            for (int k = i + 1; k < n; k++) x = csub(x, cmul(M[i + n * k], X[k * n + j]));
+            const zd d = M[i + n * i];
+            const double den = d.re * d.re + d.im * d.im;
+            X[i * n + j] = cmk<double>((x.re * d.re + x.im * d.im) / den, (x.im * d.re - x.re * d.im) / den);
+        }
+    }
+    __syncthreads();
+    zd* dst = Pinv + (size_t)b * nn;
+    for (int e = lane; e < nn; e += 64) dst[e] = X[(e % n) * n + e / n];
+}
+
 // ---------------------------------------------------------------------------
 // transfer-matrix chain, one block per job (src/boost_fractor_transformer.jl:326-357)
 // ---------------------------------------------------------------------------
@@ -778,6 +856,23 @@ static zd cdiv_h(zd a, zd b) {
 }
 
 
+// inv(P[sel[q]]) -> Pinv[sel[q]] for q < count (dsel on the device)
+static void launch_inverse(bf_ctx* ctx, const zd* P, zd* Pinv, int n, const int* dsel, size_t count) {
+    hipStream_t s = ctx->stream;
+    if (count == 0) return;
+    if (n <= INV_WAVE_MAX) {
+        const size_t smem = 2 * sizeof(zd) * (size_t)n * n;
+        allow_lds((const void*)k_batched_inverse_wave, smem);
+        hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)count), dim3(64), smem, s, P, Pinv, n, dsel);
+        BF_CHECK_LAUNCH();
+    } else {
+        DevBuf work(sizeof(zd) * (size_t)n * n * count);
+        hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)count), dim3(256), 0, s, P, Pinv, n, work.as<zd>(), dsel);
+        BF_CHECK_LAUNCH();
+        BF_HIP(hipStreamSynchronize(s));  // `work` is released on return
+    }
+}
+
 static rocblas_handle blas_of(bf_ctx* ctx) {
     if (!ctx->blas) {
         rocblas_handle h = nullptr;
@@ -797,23 +892,38 @@ void release_blas(bf_ctx* ctx) {
 }
 
 // Groups of k bins sharing a Parseval weight and their Gram matrices H_g.
-// Bins are keyed by (kx^2, ky^2) of the fftshifted entry they multiply; on a
-// square grid with kX == kY the pair is unordered (the weight's two
-// subtractions then run in the representative's order, a rounding-level
-// difference).  Returns false (no grouping) for non-finite k grids.
+// A bin's weight depends on (kx^2, ky^2) of the fftshifted entry it multiplies.
+// Squares that agree to 1e-14 relative form one class (grids built from
+// floating-point ranges are symmetric only to a few ulp), and the pair of
+// classes is unordered.  The group's weight is evaluated at its lowest bin; for
+// the others the phase argument then differs at the rounding level (at most
+// ~dz*sqrt(|d(k^2)|) ~ 1e-7 rad for a bin sitting exactly on the evanescent
+// threshold, far less elsewhere).  Returns false (no grouping) for non-finite k grids.
 static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md, ModeCache& mc) {
     hipStream_t s = ctx->stream;
     const int nx = g.nx, ny = g.ny, n = md.n;
     const size_t K = g.K(), nn = (size_t)n * n;
     for (double v : g.hkX) if (!std::isfinite(v)) return false;
     for (double v : g.hkY) if (!std::isfinite(v)) return false;
-    const bool sym = nx == ny && std::memcmp(g.hkX.data(), g.hkY.data(), sizeof(double) * nx) == 0;
-    std::vector<std::pair<std::pair<double, double>, int>> keyed(K);
+    // classes of squared wavenumbers
+    std::vector<double> sq;
+    for (double v : g.hkX) sq.push_back(v * v);
+    for (double v : g.hkY) sq.push_back(v * v);
+    std::sort(sq.begin(), sq.end());
+    std::vector<double> cls;  // first (smallest) value of each class
+    for (double v : sq)
+        if (cls.empty() || v > cls.back() * (1.0 + 1e-14)) cls.push_back(v);
+    auto class_of = [&](double v) {
+        return (int)(std::upper_bound(cls.begin(), cls.end(), v) - cls.begin()) - 1;
+    };
+    std::vector<int> cx(nx), cy(ny);
+    for (int j = 0; j < nx; j++) cx[j] = class_of(g.hkX[j] * g.hkX[j]);
+    for (int j = 0; j < ny; j++) cy[j] = class_of(g.hkY[j] * g.hkY[j]);
+    std::vector<std::pair<std::pair<int, int>, int>> keyed(K);
     for (int my = 0; my < ny; my++)
         for (int mx = 0; mx < nx; mx++) {
-            const int jx = (mx + nx / 2) % nx, jy = (my + ny / 2) % ny;
-            double a = g.hkX[jx] * g.hkX[jx], b = g.hkY[jy] * g.hkY[jy];
-            if (sym && b < a) std::swap(a, b);
+            int a = cx[(mx + nx / 2) % nx], b = cy[(my + ny / 2) % ny];
+            if (b < a) std::swap(a, b);
             keyed[(size_t)my * nx + mx] = {{a, b}, my * nx + mx};
         }
     std::sort(keyed.begin(), keyed.end());
@@ -990,7 +1100,6 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         const char* e = std::getenv("BF_PARSEVAL_GROUPS");
         if (!(e && e[0] == '0')) grouped = mc.haveH || build_parseval_groups(ctx, g, md, mc);
     }
-    const size_t NG = grouped ? (size_t)mc.ngroups : 0;
     // frequency-dependent pieces, in frequency batches bounded by table memory
     // tilted/rough disk regions: moments of the frequency expansion (BF_DISK_EXPANSION=0: per-frequency tables)
     std::vector<int> exp_q0(nreg, -1), exp_np(nreg, 0);
@@ -1389,13 +1498,11 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n;
     const int N2 = 2 * n;
-    DevBuf dsel(sizeof(int) * sel.size()), work(sizeof(zd) * nn * sel.size());
+    DevBuf dsel(sizeof(int) * sel.size());
     upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
     {
         TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
-        hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P, Pinv, n, work.as<zd>(),
-                           dsel.as<int>());
-        BF_CHECK_LAUNCH();
+        launch_inverse(ctx, P, Pinv, n, dsel.as<int>(), sel.size());
     }
     const size_t nj = jobs.size();
     DevBuf djobs(sizeof(ChainJob) * nj), dsteps(sizeof(ChainStep) * steps.size()),
@@ -1611,11 +1718,9 @@ void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, c
     }
     std::vector<int> sel;
     for (int r = 1; r < nreg; r++) sel.push_back(r);
-    DevBuf dsel(sizeof(int) * sel.size()), work(sizeof(zd) * nn * sel.size());
+    DevBuf dsel(sizeof(int) * sel.size());
     upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
-    hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)sel.size()), dim3(256), 0, s, P.as<zd>(), Pinv.as<zd>(), n,
-                       work.as<zd>(), dsel.as<int>());
-    BF_CHECK_LAUNCH();
+    launch_inverse(ctx, P.as<zd>(), Pinv.as<zd>(), n, dsel.as<int>(), sel.size());
     // steps s = 1 .. nreg-1: region idx_reg(s) = nreg-s+1 (1-based)
     const int nsteps = nreg - 1;
     std::vector<ChainStep> steps(nsteps);
