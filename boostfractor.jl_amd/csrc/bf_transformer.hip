@@ -690,6 +690,118 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
     }
 }
 
+// The chain in LDS for n <= CHAIN_LDS_MAX.  Only the bottom block row [T21 T22]
+// of T feeds the axion sum and the solves, and it updates by itself:
+//     T21' = (a T21 + b T22) P,    T22' = (b T21 + a T22) inv(P)
+// (16 n^3 flop per region instead of 64 n^3 for the whole 2n x 2n product).
+// Every value is formed with the same operations in the same order as in
+// k_chain.  LDS: T21, T22, P, inv(P) (4 n^2 complex).
+constexpr int CHAIN_LDS_MAX = 48;
+constexpr int CHAIN_NJ = CHAIN_LDS_MAX / 4;  // output columns per wave (4 waves)
+
+__global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ jobs,
+                                                 const ChainStep* __restrict__ steps, const zd* __restrict__ P,
+                                                 const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
+                                                 const zd* __restrict__ reflect, zd* boost, zd* refl) {
+    extern __shared__ unsigned char smem_raw[];
+    const int nn = n * n;
+    zd* T21 = reinterpret_cast<zd*>(smem_raw);  // column-major n x n: [i + n k]
+    zd* T22 = T21 + nn;
+    zd* Pl = T22 + nn;
+    zd* Pil = Pl + nn;
+    __shared__ zd ab[CHAIN_LDS_MAX], e0[CHAIN_LDS_MAX], tmp[2 * CHAIN_LDS_MAX];
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    __shared__ int piv[CHAIN_LDS_MAX];
+    const ChainJob job = jobs[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int e = tid; e < nn; e += 256) {
+        T21[e] = cmk<double>(0.0, 0.0);
+        T22[e] = cmk<double>((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+    }
+    for (int i = tid; i < n; i += 256) {
+        ab[i] = cmk<double>(0.0, 0.0);
+        e0[i] = e0_all[job.e0off + i];
+    }
+    __syncthreads();
+    for (int q = 0; q < job.nsteps; q++) {
+        const ChainStep st = steps[job.step0 + q];
+        // axion_beam += S * (T21*e0 + T22*e0)
+        for (int i = tid; i < n; i += 256) {
+            zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++) {
+                s21 = cadd(s21, cmul(T21[i + n * k], e0[k]));
+                s22 = cadd(s22, cmul(T22[i + n * k], e0[k]));
+            }
+            ab[i] = cadd(ab[i], cmul(st.S, cadd(s21, s22)));
+        }
+        const zd* Pm = P + (size_t)(job.pbase + st.reg) * nn;
+        const zd* Pi = Pinv + (size_t)(job.pbase + st.reg) * nn;
+        for (int e = tid; e < nn; e += 256) {
+            Pl[e] = Pm[e];
+            Pil[e] = Pi[e];
+        }
+        __syncthreads();
+        // U = a T21 + b T22 (into T21), V = b T21 + a T22 (into T22)
+        for (int e = tid; e < nn; e += 256) {
+            const zd t21 = T21[e], t22 = T22[e];
+            T21[e] = cadd(cmul(st.a, t21), cmul(st.b, t22));
+            T22[e] = cadd(cmul(st.b, t21), cmul(st.a, t22));
+        }
+        __syncthreads();
+        // T21' = U P, T22' = V inv(P): lane = row, wave w takes columns w, w + 4, ...
+        zd o21[CHAIN_NJ], o22[CHAIN_NJ];
+#pragma unroll
+        for (int t = 0; t < CHAIN_NJ; t++) {
+            o21[t] = cmk<double>(0.0, 0.0);
+            o22[t] = cmk<double>(0.0, 0.0);
+        }
+        if (lane < n) {
+            for (int k = 0; k < n; k++) {
+                const zd u = T21[lane + n * k], v = T22[lane + n * k];
+#pragma unroll
+                for (int t = 0; t < CHAIN_NJ; t++) {
+                    const int j = w + 4 * t;
+                    if (j < n) {
+                        o21[t] = cadd(o21[t], cmul(u, Pl[k + n * j]));
+                        o22[t] = cadd(o22[t], cmul(v, Pil[k + n * j]));
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (lane < n) {
+#pragma unroll
+            for (int t = 0; t < CHAIN_NJ; t++) {
+                const int j = w + 4 * t;
+                if (j < n) {
+                    T21[lane + n * j] = o21[t];
+                    T22[lane + n * j] = o22[t];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
+    if (reflect)
+        for (int i = tid; i < n; i += 256) {
+            zd s = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++) s = cadd(s, cmul(T21[i + n * k], reflect[k]));
+            tmp[n + i] = s;
+        }
+    __syncthreads();
+    lu_factor_block(T22, n, piv, sval, sidx);
+    if (tid == 0) {
+        for (int i = 0; i < n; i++) tmp[i] = ab[i];
+        lu_solve_col(T22, n, piv, tmp);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        if (reflect) {
+            lu_solve_col(T22, n, piv, tmp + n);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+        }
+    }
+}
+
 // disk-region diagonal propagation: P[f][reg] = diag(exp(-1im*kz_m*dz)) * X with
 // X = the shared Gram matrix G (flat untilted disk, from_gram[q] != 0) or P[f][reg] itself
 __global__ void k_disk_diag(zd* P, const int* __restrict__ slots, const int* __restrict__ from_gram,
@@ -1505,17 +1617,29 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         launch_inverse(ctx, P, Pinv, n, dsel.as<int>(), sel.size());
     }
     const size_t nj = jobs.size();
-    DevBuf djobs(sizeof(ChainJob) * nj), dsteps(sizeof(ChainStep) * steps.size()),
-        Tb(sizeof(zd) * 2 * (size_t)N2 * N2 * nj), cw(sizeof(zd) * (nn + 3 * (size_t)n) * nj);
+    const bool lds = n <= CHAIN_LDS_MAX;
+    DevBuf djobs(sizeof(ChainJob) * nj), dsteps(sizeof(ChainStep) * steps.size()), Tb, cw;
+    if (!lds) {
+        Tb.alloc(sizeof(zd) * 2 * (size_t)N2 * N2 * nj);
+        cw.alloc(sizeof(zd) * (nn + 3 * (size_t)n) * nj);
+    }
     upload(djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
     upload(dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
     double flops = 0.0;
-    for (const ChainJob& j : jobs) flops += 64.0 * (double)n * n * n * j.nsteps;
+    for (const ChainJob& j : jobs) flops += (lds ? 16.0 : 64.0) * (double)n * n * n * j.nsteps;
     {
-        // per region: T (2n x 2n) times the structured boundary-propagation matrix, 64 n^3 flop
+        // per region: the bottom block row (16 n^3 flop) or the whole 2n x 2n product (64 n^3 flop)
         TimedLaunch tl(ctx, "chain", flops);
-        hipLaunchKernelGGL(k_chain, dim3((unsigned)nj), dim3(256), 0, s, djobs.as<ChainJob>(), dsteps.as<ChainStep>(),
-                           P, Pinv, n, e0, reflect, Tb.as<zd>(), cw.as<zd>(), boost, refl);
+        if (lds) {
+            const size_t smem = 4 * sizeof(zd) * nn;
+            allow_lds((const void*)k_chain_lds, smem);
+            hipLaunchKernelGGL(k_chain_lds, dim3((unsigned)nj), dim3(256), smem, s, djobs.as<ChainJob>(),
+                               dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, boost, refl);
+        } else {
+            hipLaunchKernelGGL(k_chain, dim3((unsigned)nj), dim3(256), 0, s, djobs.as<ChainJob>(),
+                               dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, Tb.as<zd>(), cw.as<zd>(), boost,
+                               refl);
+        }
         BF_CHECK_LAUNCH();
     }
     BF_HIP(hipStreamSynchronize(s));
