@@ -217,21 +217,32 @@ def transformer_main(args):
     ctx = B.context()
     print(f"[bench] transformer: {NDISK}-disk rough booster, {N3}x{N3}, {n} LG modes, {NF3} freqs",
           file=sys.stderr, flush=True)
+    # inputs resident: the plan uploads setup and modes once; each step recomputes the sweep
+    plan = B.TransformerPlan(sb0, coords, modes, freqs, diskR=DISKR)
     for _ in range(args.warmup):
-        B.transformer(copy.deepcopy(sb0), coords, modes, f=freqs[:8], diskR=DISKR)
+        plan.run()
     ctx.synchronize()
-    setups = [copy.deepcopy(sb0) for _ in range(args.steps)]  # transformer mutates eps (NaN -> 1e30)
-    ctx.timing(True)
     t0 = time.perf_counter()
-    for sb in setups:
-        boost = B.transformer(sb, coords, modes, f=freqs, diskR=DISKR)
+    for _ in range(args.steps):
+        plan.run()
     ctx.synchronize()
     dt = time.perf_counter() - t0
+    ctx.timing(True)  # one more, instrumented step for the kernel means
+    plan.run()
+    ctx.synchronize()
     kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
     ctx.timing(False)
+    boost = plan.fetch()
+    plan.close()
+    # the same sweep through the host-buffer API (setup and modes copied in, boost out)
+    t1 = time.perf_counter()
+    B.transformer(copy.deepcopy(sb0), coords, modes, f=freqs, diskR=DISKR)
+    ctx.synchronize()
+    host_dt = time.perf_counter() - t1
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)  # also the byte-counter calibration of profiler runs
     alg = (len(sb0.distance) - 1) * 8.0 * n * n * N3 * N3  # survey: sum over regions 2..Nreg of 8 n^2 K
     roof = mm_roofline(kt_, "zgemm", TRAFFIC_PROFILE_TR, alg * NF3 * args.steps / dt)
+    roof["kernel_ms_from"] = "one instrumented step after the timed ones"
     roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)
     beta2 = np.sum(np.abs(boost) ** 2, axis=-1)
     out = {
@@ -247,7 +258,8 @@ def transformer_main(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 booster, tilts N(0,50 urad), 10 um roughness; LG modes w0=0.1 m)",
-        "note": "host buffers in and out: each step includes copying modes and surfaces to the device",
+        "note": "inputs resident (TransformerPlan); the host-buffer API (transformer) on the same sweep: "
+                f"{NF3 / host_dt:.1f} points/s",
         "config": {"workload": f"transformer, 20-disk booster with per-disk tilt and roughness, {N3}x{N3} grid over "
                                f"1 m, {n} Laguerre-Gauss modes (p<=4, |l|<=4), {NF3} freqs 19.5-20.5 GHz",
                    "grid": N3, "modes": n, "frequencies": NF3},

@@ -244,6 +244,56 @@ def transformer(bdry, coords, modes, f=10.0e9, velocity_x=0, prop=propagator, pr
     return boost if reflect is None else (boost, refl)
 
 
+class TransformerPlan:
+    """Device-resident frequency sweep of `transformer` over one geometry
+    (src/boost_fractor_transformer.jl:285-359): the setup, the modes and
+    emit/reflect are uploaded once; every `run()` recomputes the sweep from them
+    on the device; `fetch()` returns boost (nfreq, n) [and refl].  NaN
+    permittivities are read as 1e30 (the caller's setup is not modified)."""
+
+    def __init__(self, bdry, coords, modes, f, prop=propagator, diskR=0.15, emit=None, reflect=None, ctx=None):
+        self.ctx = ctx or L.context()
+        self.freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(f, dtype=np.float64)))
+        self.n = modes.n
+        if emit is None:
+            emit = axion_induced_modes(coords, modes, B=None, diskR=diskR, f=float(self.freqs[0]))
+        m = L.Marshalled()
+        c = m.coords(coords)
+        s = m.setup(bdry, coords.ny, coords.nx)
+        md = _modes_struct(m, modes)
+        em = m.hold(L.as_c128(np.asarray(emit).reshape(-1)))
+        if len(em) != self.n:
+            raise L.DimensionMismatch(f"emit of length {len(em)} for {self.n} modes")
+        rf = m.hold(L.as_c128(np.broadcast_to(reflect, (self.n,)))) if reflect is not None else None
+        self.has_reflect = reflect is not None
+        h = ctypes.c_void_p()
+        L.check(L.lib().bf_transformer_plan_create(self.ctx.handle, ctypes.byref(s), ctypes.byref(c),
+                                                   ctypes.byref(md), L.cptr(self.freqs), len(self.freqs),
+                                                   prop_kind(prop), float(diskR), L.cptr(em), L.cptr(rf),
+                                                   ctypes.byref(h)), self.ctx.handle)
+        self.handle = h
+
+    def run(self):
+        L.check(L.lib().bf_transformer_plan_run(self.handle), self.ctx.handle)
+
+    def fetch(self):
+        boost = np.zeros((len(self.freqs), self.n), dtype=np.complex128)
+        refl = np.zeros_like(boost) if self.has_reflect else None
+        L.check(L.lib().bf_transformer_plan_fetch(self.handle, L.cptr(boost), L.cptr(refl)), self.ctx.handle)
+        return boost if refl is None else (boost, refl)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().bf_transformer_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=propagator, diskR=0.15, emits=None,
                       precision="fp64"):
     """The disk-position sweep of examples/transformer_optim_utilities.jl:105,121,202

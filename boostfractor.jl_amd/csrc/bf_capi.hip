@@ -418,6 +418,39 @@ int bf_transformer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, 
     });
 }
 
+int bf_transformer_plan_create(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, const bf_modes* modes,
+                               const double* freqs, int32_t nfreq, int32_t prop_kind, double diskR, const double* emit,
+                               const double* reflect, bf_transformer_plan** out) {
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_coords(ctx, coords));
+    TRY(check_setup(ctx, setup, 2, false));
+    TRY(check_modes(ctx, modes));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, emit != nullptr, BF_ERR_ARG, "emit is NULL");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        *out = bf::transformer_plan_create(ctx, setup, coords, modes, freqs, nfreq, prop_kind, diskR, emit, reflect);
+    });
+}
+
+int bf_transformer_plan_run(bf_transformer_plan* plan) {
+    BF_REQUIRE(nullptr, plan != nullptr, BF_ERR_ARG, "plan is NULL");
+    return guarded(plan->ctx, [&] { bf::transformer_plan_run(plan); });
+}
+
+int bf_transformer_plan_fetch(bf_transformer_plan* plan, double* boost, double* refl) {
+    BF_REQUIRE(nullptr, plan != nullptr, BF_ERR_ARG, "plan is NULL");
+    BF_REQUIRE(plan->ctx, !refl || plan->has_reflect, BF_ERR_ARG, "refl requested but the plan has no reflect input");
+    return guarded(plan->ctx, [&] { bf::transformer_plan_fetch(plan, boost, refl); });
+}
+
+int bf_transformer_plan_destroy(bf_transformer_plan* plan) {
+    if (plan) bf::transformer_plan_destroy(plan);
+    return BF_OK;
+}
+
 int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, const bf_coords* coords, int32_t nsets,
                          const bf_modes* modesets, const double* const* emits, const int32_t* mode_set_of,
                          const double* freqs, int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,

@@ -209,6 +209,12 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl);
 void release_blas(bf_ctx* ctx);
+bf_transformer_plan* transformer_plan_create(bf_ctx* ctx, const bf_setup* st, const bf_coords* co,
+                                             const bf_modes* modes, const double* freqs, int nfreq, int kind,
+                                             double diskR, const double* emit, const double* reflect);
+void transformer_plan_run(bf_transformer_plan* pl);
+void transformer_plan_fetch(bf_transformer_plan* pl, double* boost, double* refl);
+void transformer_plan_destroy(bf_transformer_plan* pl);
 void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
                            const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,

@@ -35,9 +35,14 @@ struct ModeCache {
     // square grid with kX == kY, the swapped pair) share one weight, so
     // P = sum_g w_g H_g with H_g = sum_{k in g} conj(Fhat_i(k)) FhatM_j(k).
     bool haveH = false;
+    DevBuf H;  // [ngroups][n][n] (column-major)
+    // the grouping itself depends on the grid only
+    bool haveIdx = false;
     int ngroups = 0;
-    DevBuf H;         // [ngroups][n][n] (column-major)
-    DevBuf gkx, gky;  // representative kX, kY of each group (double)
+    DevBuf gkx, gky;    // representative kX, kY of each group (double)
+    DevBuf goff, gbin;  // group g holds bins gbin[goff[g] .. goff[g+1])
+    // forget everything computed from the patterns (keeps the grouping)
+    void reset_data() { haveG = haveF = haveH = false; }
 };
 
 // Parseval weight of one (region, frequency): exp(-1im*kz*dz), kz from k0^2
@@ -98,6 +103,24 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
                         const zd* reflect, zd* boost, zd* refl);
 
 }  // namespace bf
+
+// A resident transformer sweep: inputs uploaded once; every run recomputes
+// everything from them (pattern transforms, group Grams, propagation matrices,
+// inverses, chains) on the device.  Only the grouping of k bins (a function of
+// the grid alone) is kept between runs.
+struct bf_transformer_plan {
+    bf_ctx* ctx = nullptr;
+    bf::Grid g;
+    bf::ModesDev md;
+    bf::DevBuf surf, P, Pinv, de0, dref, dboost, drefl;
+    std::vector<bool> flat;
+    std::vector<double> distance, tx, ty, freqs;
+    std::vector<zd> eps;
+    int nreg = 0, nf = 0, kind = 0;
+    double diskR = 0.0;
+    bool has_reflect = false;
+    bf::ModeCache mc;
+};
 
 // cubic B-spline coefficients of a propagation-matrix grid over the spacing axis
 struct bf_spacing_itp {

@@ -1015,56 +1015,61 @@ static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md
     hipStream_t s = ctx->stream;
     const int nx = g.nx, ny = g.ny, n = md.n;
     const size_t K = g.K(), nn = (size_t)n * n;
-    for (double v : g.hkX) if (!std::isfinite(v)) return false;
-    for (double v : g.hkY) if (!std::isfinite(v)) return false;
-    // classes of squared wavenumbers
-    std::vector<double> sq;
-    for (double v : g.hkX) sq.push_back(v * v);
-    for (double v : g.hkY) sq.push_back(v * v);
-    std::sort(sq.begin(), sq.end());
-    std::vector<double> cls;  // first (smallest) value of each class
-    for (double v : sq)
-        if (cls.empty() || v > cls.back() * (1.0 + 1e-14)) cls.push_back(v);
-    auto class_of = [&](double v) {
-        return (int)(std::upper_bound(cls.begin(), cls.end(), v) - cls.begin()) - 1;
-    };
-    std::vector<int> cx(nx), cy(ny);
-    for (int j = 0; j < nx; j++) cx[j] = class_of(g.hkX[j] * g.hkX[j]);
-    for (int j = 0; j < ny; j++) cy[j] = class_of(g.hkY[j] * g.hkY[j]);
-    std::vector<std::pair<std::pair<int, int>, int>> keyed(K);
-    for (int my = 0; my < ny; my++)
-        for (int mx = 0; mx < nx; mx++) {
-            int a = cx[(mx + nx / 2) % nx], b = cy[(my + ny / 2) % ny];
-            if (b < a) std::swap(a, b);
-            keyed[(size_t)my * nx + mx] = {{a, b}, my * nx + mx};
+    if (!mc.haveIdx) {
+        for (double v : g.hkX) if (!std::isfinite(v)) return false;
+        for (double v : g.hkY) if (!std::isfinite(v)) return false;
+        // classes of squared wavenumbers
+        std::vector<double> sq;
+        for (double v : g.hkX) sq.push_back(v * v);
+        for (double v : g.hkY) sq.push_back(v * v);
+        std::sort(sq.begin(), sq.end());
+        std::vector<double> cls;  // first (smallest) value of each class
+        for (double v : sq)
+            if (cls.empty() || v > cls.back() * (1.0 + 1e-14)) cls.push_back(v);
+        auto class_of = [&](double v) {
+            return (int)(std::upper_bound(cls.begin(), cls.end(), v) - cls.begin()) - 1;
+        };
+        std::vector<int> cx(nx), cy(ny);
+        for (int j = 0; j < nx; j++) cx[j] = class_of(g.hkX[j] * g.hkX[j]);
+        for (int j = 0; j < ny; j++) cy[j] = class_of(g.hkY[j] * g.hkY[j]);
+        std::vector<std::pair<std::pair<int, int>, int>> keyed(K);
+        for (int my = 0; my < ny; my++)
+            for (int mx = 0; mx < nx; mx++) {
+                int a = cx[(mx + nx / 2) % nx], b = cy[(my + ny / 2) % ny];
+                if (b < a) std::swap(a, b);
+                keyed[(size_t)my * nx + mx] = {{a, b}, my * nx + mx};
+            }
+        std::sort(keyed.begin(), keyed.end());
+        std::vector<int> off, bins(K);
+        std::vector<double> gx, gy;
+        for (size_t q = 0; q < K; q++) {
+            if (q == 0 || keyed[q].first != keyed[q - 1].first) {
+                off.push_back((int)q);
+                const int bin = keyed[q].second;  // lowest bin index of the group
+                gx.push_back(g.hkX[((bin % nx) + nx / 2) % nx]);
+                gy.push_back(g.hkY[((bin / nx) + ny / 2) % ny]);
+            }
+            bins[q] = keyed[q].second;
         }
-    std::sort(keyed.begin(), keyed.end());
-    std::vector<int> off, bins(K);
-    std::vector<double> gx, gy;
-    for (size_t q = 0; q < K; q++) {
-        if (q == 0 || keyed[q].first != keyed[q - 1].first) {
-            off.push_back((int)q);
-            const int bin = keyed[q].second;  // lowest bin index of the group
-            gx.push_back(g.hkX[((bin % nx) + nx / 2) % nx]);
-            gy.push_back(g.hkY[((bin / nx) + ny / 2) % ny]);
-        }
-        bins[q] = keyed[q].second;
+        off.push_back((int)K);
+        const int G = (int)gx.size();
+        mc.goff.alloc(sizeof(int) * off.size());
+        mc.gbin.alloc(sizeof(int) * K);
+        upload(mc.goff.p, off.data(), sizeof(int) * off.size(), s);
+        upload(mc.gbin.p, bins.data(), sizeof(int) * K, s);
+        mc.gkx.alloc(sizeof(double) * G);
+        mc.gky.alloc(sizeof(double) * G);
+        upload(mc.gkx.p, gx.data(), sizeof(double) * G, s);
+        upload(mc.gky.p, gy.data(), sizeof(double) * G, s);
+        BF_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+        mc.ngroups = G;
+        mc.haveIdx = true;
     }
-    off.push_back((int)K);
-    const int G = (int)gx.size();
-    DevBuf doff(sizeof(int) * off.size()), dbins(sizeof(int) * K);
-    upload(doff.p, off.data(), sizeof(int) * off.size(), s);
-    upload(dbins.p, bins.data(), sizeof(int) * K, s);
-    mc.gkx.alloc(sizeof(double) * G);
-    mc.gky.alloc(sizeof(double) * G);
-    upload(mc.gkx.p, gx.data(), sizeof(double) * G, s);
-    upload(mc.gky.p, gy.data(), sizeof(double) * G, s);
-    mc.H.alloc(sizeof(zd) * nn * G);
+    const int G = mc.ngroups;
+    if (mc.H.bytes < sizeof(zd) * nn * G) mc.H.alloc(sizeof(zd) * nn * G);
     hipLaunchKernelGGL(k_group_gram, dim3(G), dim3(256), 0, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(), K, md.C, n,
-                       doff.as<int>(), dbins.as<int>(), mc.H.as<zd>());
+                       mc.goff.as<int>(), mc.gbin.as<int>(), mc.H.as<zd>());
     BF_CHECK_LAUNCH();
-    BF_HIP(hipStreamSynchronize(s));
-    mc.ngroups = G;
     mc.haveH = true;
     return true;
 }
@@ -1191,14 +1196,14 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     DevBuf& Fhat = mc.Fhat;
     DevBuf& FhatM = mc.FhatM;
     if (need_gram && !mc.haveG) {
-        G.alloc(sizeof(zd) * nn);
+        if (G.bytes < sizeof(zd) * nn) G.alloc(sizeof(zd) * nn);
         GemmProb p{md.pat.as<zd>(), md.pat.as<zd>(), nullptr, 0, 0};
         gram.run({p}, G.as<zd>());
         mc.haveG = true;
     }
     if (!parseval.empty() && !mc.haveF) {
-        Fhat.alloc(sizeof(zd) * n * Ktot);
-        FhatM.alloc(sizeof(zd) * n * Ktot);
+        if (Fhat.bytes < sizeof(zd) * n * Ktot) Fhat.alloc(sizeof(zd) * n * Ktot);
+        if (FhatM.bytes < sizeof(zd) * n * Ktot) FhatM.alloc(sizeof(zd) * n * Ktot);
         hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, Fhat.as<zd>(), md.pat.as<zd>(), n * Ktot);
         hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, FhatM.as<zd>(), md.pat.as<zd>(), n * Ktot);
         BF_CHECK_LAUNCH();
@@ -1696,6 +1701,80 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
     if (reflect) download(refl, drefl.p, sizeof(zd) * n * nfreq, s);
     BF_HIP(hipStreamSynchronize(s));
 }
+
+}  // namespace bf
+
+namespace bf {
+
+bf_transformer_plan* transformer_plan_create(bf_ctx* ctx, const bf_setup* st, const bf_coords* co,
+                                             const bf_modes* modes, const double* freqs, int nfreq, int kind,
+                                             double diskR, const double* emit, const double* reflect) {
+    hipStream_t s = ctx->stream;
+    auto* pl = new bf_transformer_plan;
+    try {
+        pl->ctx = ctx;
+        setup_common(ctx, st, co, pl->g, pl->surf, pl->flat);
+        pl->md.upload(modes, pl->g.K(), s);
+        pl->nreg = st->nreg;
+        pl->nf = nfreq;
+        pl->kind = kind;
+        pl->diskR = diskR;
+        pl->distance.assign(st->distance, st->distance + st->nreg);
+        pl->tx.assign(st->tilt_x, st->tilt_x + st->nreg);
+        pl->ty.assign(st->tilt_y, st->tilt_y + st->nreg);
+        pl->freqs.assign(freqs, freqs + nfreq);
+        pl->eps = eps_fixed(st);
+        const int n = pl->md.n;
+        const size_t nn = (size_t)n * n;
+        pl->P.alloc(sizeof(zd) * nn * pl->nreg * nfreq);
+        pl->Pinv.alloc(sizeof(zd) * nn * pl->nreg * nfreq);
+        pl->de0.alloc(sizeof(zd) * n);
+        upload(pl->de0.p, emit, sizeof(zd) * n, s);
+        pl->dboost.alloc(sizeof(zd) * n * nfreq);
+        pl->has_reflect = reflect != nullptr;
+        if (reflect) {
+            pl->dref.alloc(sizeof(zd) * n);
+            pl->drefl.alloc(sizeof(zd) * n * nfreq);
+            upload(pl->dref.p, reflect, sizeof(zd) * n, s);
+        }
+        BF_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete pl;
+        throw;
+    }
+    return pl;
+}
+
+void transformer_plan_run(bf_transformer_plan* pl) {
+    bf_ctx* ctx = pl->ctx;
+    const int n = pl->md.n, nreg = pl->nreg, nf = pl->nf;
+    pl->mc.reset_data();
+    std::vector<int> regs;
+    for (int r = 1; r < nreg; r++) regs.push_back(r);
+    propagation_matrices_core(ctx, pl->g, pl->md, pl->kind, pl->diskR, nreg, pl->distance.data(), pl->eps.data(),
+                              pl->tx.data(), pl->ty.data(), pl->surf.as<zd>(), pl->flat, pl->freqs.data(), nf, regs,
+                              pl->P.as<zd>(), &pl->mc);
+    std::vector<int> sel;
+    std::vector<ChainJob> jobs;
+    std::vector<ChainStep> steps;
+    append_chain_steps(pl->eps, nreg, steps);  // the same region sequence at every frequency
+    for (int f = 0; f < nf; f++) {
+        for (int r = 1; r < nreg; r++) sel.push_back(f * nreg + r);
+        jobs.push_back(ChainJob{0, (int64_t)f * nreg, f, 0, nreg - 1, 0});
+    }
+    inverse_and_chains(ctx, n, pl->P.as<zd>(), pl->Pinv.as<zd>(), sel, jobs, steps, pl->de0.as<zd>(),
+                       pl->dref.as<zd>(), pl->dboost.as<zd>(), pl->drefl.as<zd>());
+}
+
+void transformer_plan_fetch(bf_transformer_plan* pl, double* boost, double* refl) {
+    hipStream_t s = pl->ctx->stream;
+    const size_t bytes = sizeof(zd) * pl->md.n * pl->nf;
+    if (boost) download(boost, pl->dboost.p, bytes, s);
+    if (refl && pl->has_reflect) download(refl, pl->drefl.p, bytes, s);
+    BF_HIP(hipStreamSynchronize(s));
+}
+
+void transformer_plan_destroy(bf_transformer_plan* pl) { delete pl; }
 
 // Many geometries (each its own setup and mode set) at the same frequencies.
 // Geometries are taken mode set by mode set, in chunks whose propagation

@@ -215,6 +215,21 @@ int bf_transformer(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, 
                    const double *freqs, int32_t nfreq, int32_t prop_kind, const double *prop_mats, double diskR,
                    const double *emit, const double *reflect, int32_t precision, double *boost, double *refl);
 
+/* A resident transformer sweep (no reference counterpart; the callers' frequency
+ * sweeps over one geometry, e.g. examples/transformer_optim_utilities.jl:99-109):
+ * create uploads the setup, the modes and emit/reflect once; every run
+ * recomputes transformer (src/boost_fractor_transformer.jl:285-359) at the
+ * plan's frequencies from them on the device; fetch copies boost (and refl)
+ * out, complex [nfreq][n].  NaN permittivities are read as 1e30; the caller's
+ * setup is not modified. */
+typedef struct bf_transformer_plan bf_transformer_plan;
+int bf_transformer_plan_create(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, const bf_modes *modes,
+                               const double *freqs, int32_t nfreq, int32_t prop_kind, double diskR,
+                               const double *emit, const double *reflect, bf_transformer_plan **out);
+int bf_transformer_plan_run(bf_transformer_plan *plan);
+int bf_transformer_plan_fetch(bf_transformer_plan *plan, double *boost, double *refl);
+int bf_transformer_plan_destroy(bf_transformer_plan *plan);
+
 /* transformer (src/boost_fractor_transformer.jl:285-359) for a batch of geometries
  * at the same nfreq frequencies: the disk-position sweep of
  * examples/transformer_optim_utilities.jl:105,121,202 (calc_propagation_matrices

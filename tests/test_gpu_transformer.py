@@ -419,3 +419,22 @@ def test_propagation_matrices_disk_frequency_expansion(monkeypatch, lossy):
         want = _oracle_mats(sb, coords, modes, freqs[q], O.PROP_PROPAGATOR)
         for r in range(1, len(sb.distance)):
             assert rel(got[q][r], want[r]) < TOL, (q, r, rel(got[q][r], want[r]))
+
+
+def test_transformer_plan_matches_transformer():
+    """the resident sweep gives transformer's results run after run (rough booster:
+    Parseval air, frequency-series disks, reflectivity)"""
+    coords, sb = _rough_booster(N=32, ndisk=3)
+    modes = _lg_like_modes(coords, 2, 1)
+    emit = B.axion_induced_modes(coords, modes, diskR=DISKR)
+    refl_in = np.linspace(1, 2, modes.n) + 0.1j
+    freqs = np.linspace(19.6e9, 20.4e9, 9)
+    want_b, want_r = B.transformer(sb.copy(), coords, modes, f=freqs, diskR=DISKR, emit=emit, reflect=refl_in)
+    eps0 = sb.eps.copy()
+    plan = B.TransformerPlan(sb, coords, modes, freqs, diskR=DISKR, emit=emit, reflect=refl_in)
+    assert np.array_equal(sb.eps, eps0, equal_nan=True)  # the plan leaves the setup alone
+    for _ in range(2):
+        plan.run()
+        b, r = plan.fetch()
+        assert rel(b, want_b) < 1e-12 and rel(r, want_r) < 1e-12
+    plan.close()
