@@ -504,6 +504,44 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
     const int sx = (fl & CF_INSHIFT) ? (mx + nx / 2) % nx : mx;
     cplx<S>* s = smem + (size_t)c * LDC;
     cplx<C> v[E];
+    constexpr uint32_t PROPF = CF_FWD | CF_KMID | CF_INV;
+    if constexpr (FL == PROPF) {
+        // The dancer's propagation (no shifts): element m of this thread is row
+        // y = t + TT m.  A resource over the band rows only answers rows outside
+        // [ylo, yhi) with zero: their offsets, (y - ylo) * nx + mx elements,
+        // wrap to >= 2^31 below the band and reach past its end above it, so
+        // every offset is the lane base plus a uniform multiple of 16 rows.
+        const uint32_t rowb = (uint32_t)nx * (uint32_t)sizeof(cplx<S>), stepb = (uint32_t)TT * rowb;
+        const uint32_t bandb = (uint32_t)(a.yhi - a.ylo) * rowb;
+        const auto rsb = buf_rsrc(a.src + d.off + (size_t)a.ylo * nx, bandb);
+        const uint32_t lband = ok ? (uint32_t)((t - a.ylo) * nx + mx) * (uint32_t)sizeof(cplx<S>) : BUF_OOB;
+        cplx<S> raw[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) raw[m] = buf_load<S>(rsb, lband + (uint32_t)m * stepb);
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
+        fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+        if (has_kt) {
+            const uint32_t krow = (uint32_t)nx * (uint32_t)sizeof(cplx<C>), kstep = (uint32_t)TT * krow;
+            const uint32_t lk = ok ? (uint32_t)(t * nx + mx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
+            constexpr int KCH = sizeof(C) == 8 ? 8 : E;
+#pragma unroll
+            for (int m0 = 0; m0 < E; m0 += KCH) {
+                cplx<C> k[KCH];
+#pragma unroll
+                for (int q = 0; q < KCH; q++) k[q] = buf_load<C>(rk, lk + (uint32_t)(m0 + q) * kstep);
+#pragma unroll
+                for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
+            }
+        }
+        fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+        // band rows, or every row of a field stored whole
+        const auto rdb = buf_rsrc(a.dst + d.off + (whole ? 0 : (size_t)a.ylo * nx), whole ? K * (uint32_t)sizeof(cplx<S>) : bandb);
+        const uint32_t lst = ok ? (uint32_t)((t - (whole ? 0 : a.ylo)) * nx + mx) * (uint32_t)sizeof(cplx<S>) : BUF_OOB;
+#pragma unroll
+        for (int m = 0; m < E; m++) buf_store<S>(rdb, lst + (uint32_t)m * stepb, ccast<S>(v[m]));
+        return;
+    }
     {
         uint32_t off[E];
 #pragma unroll
@@ -578,15 +616,21 @@ __global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
     const uint32_t K = (uint32_t)N * (uint32_t)a.ny;
     cplx<S>* s = smem + (size_t)li * LD;
     cplx<S>* sp = smem + (size_t)(li ^ 1) * LD;  // partner line, same y
-    const auto rf = buf_rsrc(a.fields + (((size_t)f * a.R + (exists ? reg : 0)) * 2 + slot) * K,
-                             K * (uint32_t)sizeof(cplx<S>));
+    // A wave spans several lines (different regions and slots), so the line
+    // address is per lane: plain global accesses under one per-line condition
+    // (a buffer resource per line would not be wave-uniform).
+    cplx<S>* line = a.fields + (((size_t)f * a.R + (exists ? reg : 0)) * 2 + slot) * K + (size_t)yc * N + t;
     const uint32_t o0 = (uint32_t)yc * N + t;
     cplx<C> v[E];
     {
         cplx<S> raw[E];
+        if (exists) {
 #pragma unroll
-        for (int m = 0; m < E; m++)
-            raw[m] = buf_load<S>(rf, exists ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB);
+            for (int m = 0; m < E; m++) raw[m] = line[TT * m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; m++) raw[m] = cmk<S>(0, 0);
+        }
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
     }
@@ -595,14 +639,14 @@ __global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
     fft_reg<C, S, N, E, true>(v, s, t, a.tw);
     if (a.rtab) {
         const int ti = exists ? a.rtab_idx[f * a.R + reg] : -1;
-        const auto rt = buf_rsrc(a.rtab + (size_t)(ti >= 0 ? ti : 0) * K, K * (uint32_t)sizeof(cplx<C>));
-        cplx<C> tab[E];
+        if (ti >= 0) {
+            const cplx<C>* tl = a.rtab + (size_t)ti * K + o0;
+            cplx<C> tab[E];
 #pragma unroll
-        for (int m = 0; m < E; m++)
-            tab[m] = buf_load<C>(rt, ti >= 0 ? (o0 + TT * m) * (uint32_t)sizeof(cplx<C>) : BUF_OOB);
-        if (ti >= 0)
+            for (int m = 0; m < E; m++) tab[m] = tl[TT * m];
 #pragma unroll
             for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
+        }
     }
     // exchange with the partner line through LDS (the FFT's buffers are free now)
     __syncthreads();
@@ -624,21 +668,20 @@ __global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
     }
     if (b == a.R) {  // exit boundary (uniform over the block): Eout += (1+r) * A on the A lines
         const bool eo_line = exists && side == 0;
-        const auto re = buf_rsrc(a.Eout + (size_t)f * a.eout_fstride + (size_t)a.slice * K, K * (uint32_t)sizeof(zd));
         const bool assign = (a.flags & DF_EOUT_ASSIGN) != 0;
         const zd opr = cmk<double>(1.0 + r.re, r.im);
-        constexpr int ECH = 4;  // exit blocks are few: keep their extra registers low
+        if (eo_line) {
+            zd* eo = a.Eout + (size_t)f * a.eout_fstride + (size_t)a.slice * K + o0;
+            constexpr int ECH = 4;  // exit blocks are few: keep their extra registers low
 #pragma unroll
-        for (int m0 = 0; m0 < E; m0 += ECH) {
-            zd prev[ECH];
+            for (int m0 = 0; m0 < E; m0 += ECH) {
+                zd prev[ECH];
 #pragma unroll
-            for (int q = 0; q < ECH; q++)
-                prev[q] = buf_load<double>(re, (eo_line && !assign) ? (o0 + TT * (m0 + q)) * (uint32_t)sizeof(zd)
-                                                                    : BUF_OOB);
+                for (int q = 0; q < ECH; q++) prev[q] = assign ? cmk<double>(0.0, 0.0) : eo[TT * (m0 + q)];
 #pragma unroll
-            for (int q = 0; q < ECH; q++)
-                buf_store<double>(re, eo_line ? (o0 + TT * (m0 + q)) * (uint32_t)sizeof(zd) : BUF_OOB,
-                                  cadd(prev[q], cmul(opr, ccast<double>(v[m0 + q]))));
+                for (int q = 0; q < ECH; q++)
+                    eo[TT * (m0 + q)] = cadd(prev[q], cmul(opr, ccast<double>(v[m0 + q])));
+            }
         }
     }
     cplx<S> wp[E];  // partner values, all LDS reads issued together
@@ -665,9 +708,9 @@ __global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
     }
     if (!(a.flags & DF_FWD) || exit_only) return;  // only Eout is needed
     fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+    if (exists)
 #pragma unroll
-    for (int m = 0; m < E; m++)
-        buf_store<S>(rf, exists ? (o0 + TT * m) * (uint32_t)sizeof(cplx<S>) : BUF_OOB, ccast<S>(v[m]));
+        for (int m = 0; m < E; m++) line[TT * m] = ccast<S>(v[m]);
 }
 
 // pointwise bounce update for the unfused propagators (propagator1D, momentum space)
