@@ -16,6 +16,7 @@ from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
 from .transformer import (ErrorException, Modes, SeedModes, axion_induced_modes, calc_propagation_matrices,
                           e_field_dimensions, field2modes, index, mode, modes2field, propagation_matrix, transformer,
                           TransformerPlan, transformer_batch, transformer_trace_back)
+from . import analytical1d as Analytical1D  # noqa: N812  (the reference module's name, src/Analytical1D.py)
 from .sweep import (SpacingInterpolation, calc_boostfactor_cost, calc_boostfactor_cost_batch, calc_boostfactor_modes,
                     calc_modes, calc_propagation_matrices_grid, construct_prop_matrix_interpolation,
                     copy_setup_boundaries, interpolate_prop_matrix, p_norm, soft_box_penalty)
@@ -30,4 +31,5 @@ __all__ = [
     "gauss_profile", "unnormalized_match", "BoundsError", "calc_propagation_matrices_grid", "SpacingInterpolation",
     "construct_prop_matrix_interpolation", "interpolate_prop_matrix", "calc_boostfactor_modes", "calc_modes",
     "calc_boostfactor_cost", "calc_boostfactor_cost_batch", "soft_box_penalty", "p_norm", "copy_setup_boundaries",
+    "Analytical1D",
 ]

@@ -98,6 +98,8 @@ _SIGS = {
     "bf_transformer_plan_run": (ctypes.c_int, [_P]),
     "bf_transformer_plan_fetch": (ctypes.c_int, [_P, _P, _P]),
     "bf_transformer_plan_destroy": (ctypes.c_int, [_P]),
+    "bf_disk_system_1d": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _P, _D, _P, _P, _I32, _P, _P, _P]),
+    "bf_transform_surface_1d": (ctypes.c_int, [_P, _P, _I32, _P, _P, _P, _P, _I32, _D, _D, _I32, _P]),
     "bf_transformer_batch": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _I32,
                                             ctypes.POINTER(ModesS), ctypes.POINTER(ctypes.c_void_p), _P, _P, _I32,
                                             _I32, _D, _I32, _P, _P]),

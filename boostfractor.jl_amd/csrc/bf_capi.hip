@@ -478,6 +478,35 @@ int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, con
     });
 }
 
+int bf_disk_system_1d(bf_ctx* ctx, const double* freqs, int32_t nfreq, int32_t ncand, int32_t num_disk,
+                      const double* spacings, double disk_thickness, const double* eps_disk, const double* eps_gap,
+                      int32_t mirror, const double* surfaceloss, double* gamma, double* axion) {
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, ncand >= 1 && num_disk >= 0, BF_ERR_ARG, "ncand >= 1 and num_disk >= 0 required");
+    BF_REQUIRE(ctx, spacings && eps_disk && eps_gap && gamma && axion, BF_ERR_ARG,
+               "spacings, eps_disk, eps_gap, gamma and axion must not be NULL");
+    BF_REQUIRE(ctx, (int64_t)ncand * nfreq <= ((int64_t)1 << 31), BF_ERR_UNSUPPORTED, "more than 2^31 points");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::disk_system_1d_dev(ctx, freqs, nfreq, ncand, num_disk, spacings, disk_thickness, eps_disk, eps_gap, mirror,
+                               surfaceloss, gamma, axion);
+    });
+}
+
+int bf_transform_surface_1d(bf_ctx* ctx, const double* freqs, int32_t n, const double* gamma_in, double* axion,
+                            const double* eps_i, const double* eps_m, int32_t eps_m_real, double l,
+                            double surfaceloss, int32_t use_ray, double* gamma_out) {
+    TRY(check_freqs(ctx, freqs, n));
+    BF_REQUIRE(ctx, gamma_in && eps_i && eps_m && gamma_out, BF_ERR_ARG,
+               "gamma_in, eps_i, eps_m and gamma_out must not be NULL");
+    BF_REQUIRE(ctx, use_ray || !axion, BF_ERR_ARG, "Need to use ray approx. (src/Analytical1D.py:44-45)");
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        bf::transform_surface_1d_dev(ctx, freqs, n, gamma_in, axion, eps_i, eps_m, eps_m_real, l, surfaceloss, use_ray,
+                                     gamma_out);
+    });
+}
+
 int bf_propagation_matrices_grid(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, const bf_modes* modes,
                                  const double* spacing_grid, int32_t nspacing, const double* tilt_x_grid,
                                  int32_t ntilt_x, const double* tilt_y_grid, int32_t ntilt_y, const double* freqs,

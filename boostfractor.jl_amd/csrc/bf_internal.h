@@ -209,6 +209,12 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl);
 void release_blas(bf_ctx* ctx);
+void disk_system_1d_dev(bf_ctx* ctx, const double* freqs, int nfreq, int ncand, int num_disk, const double* spacings,
+                        double thickness, const double* eps_disk, const double* eps_gap, int mirror,
+                        const double* surfaceloss, double* gamma, double* axion);
+void transform_surface_1d_dev(bf_ctx* ctx, const double* freqs, int n, const double* gamma_in, double* axion,
+                              const double* eps_i, const double* eps_m, int m_real, double l, double surfaceloss,
+                              int ray, double* gamma_out);
 bf_transformer_plan* transformer_plan_create(bf_ctx* ctx, const bf_setup* st, const bf_coords* co,
                                              const bf_modes* modes, const double* freqs, int nfreq, int kind,
                                              double diskR, const double* emit, const double* reflect);

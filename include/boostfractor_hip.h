@@ -245,6 +245,26 @@ int bf_transformer_batch(bf_ctx *ctx, int32_t nconf, const bf_setup *setups, con
                          const int32_t *mode_set_of, const double *freqs, int32_t nfreq, int32_t prop_kind,
                          double diskR, int32_t precision, const int64_t *boost_offset, double *boost);
 
+/* ---- 1D analytical model (src/Analytical1D.py) ------------------------- */
+/* disk_system (src/Analytical1D.py:82-131), ray-tracing form, for ncand geometries
+ * at nfreq frequencies in one launch: mirror (eps 1e20, 0.1 m) when `mirror`, then
+ * gap 0, (disk, gap i+1) x num_disk; spacings: [ncand][num_disk+1] (m); eps_disk,
+ * eps_gap: complex (as lossy_eps gives them); surfaceloss: [2 num_disk + 1] or NULL
+ * (non_uniform_surfaceloss).  gamma (amplitude reflectivity) and axion (amplitude
+ * boost factor): complex [ncand][nfreq]. */
+int bf_disk_system_1d(bf_ctx *ctx, const double *freqs, int32_t nfreq, int32_t ncand, int32_t num_disk,
+                      const double *spacings, double disk_thickness, const double *eps_disk, const double *eps_gap,
+                      int32_t mirror, const double *surfaceloss, double *gamma, double *axion);
+/* transform_surface (src/Analytical1D.py:5-67) over n frequency points: gamma_in,
+ * gamma_out complex [n]; axion complex [n] updated in place, or NULL; eps_i, eps_m
+ * complex; eps_m_real != 0 when the caller's eps_m is a real number (the
+ * propagation phase is then formed in real arithmetic, as numpy does); use_ray 0
+ * selects the impedance transformation, which (as in the reference) cannot carry
+ * an axion field: BF_ERR_ARG with axion != NULL. */
+int bf_transform_surface_1d(bf_ctx *ctx, const double *freqs, int32_t n, const double *gamma_in, double *axion,
+                            const double *eps_i, const double *eps_m, int32_t eps_m_real, double l,
+                            double surfaceloss, int32_t use_ray, double *gamma_out);
+
 /* ---- disk-position sweep (examples/transformer_optim_utilities.jl) ------ */
 /* calc_propagation_matrices_grid (:11-52): the setup split into its gaps (0-based
  * regions 1, 3, ..., nreg-1) and solids (0, 2, ..., nreg-2; nreg even); every gap
