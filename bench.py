@@ -362,6 +362,50 @@ def sweep_main(args):
     print(json.dumps(out), flush=True)
 
 
+def model1d_main(args):
+    """1D analytical model sweep: 4096 spacing candidates of a 20-disk booster x
+    1000 frequencies in one launch (disk_system_batch)"""
+    import BoostFractor as B
+    from BoostFractor import workloads as W
+    ncand, nfreq, nd = 4096, 1000, NDISK
+    rng = W.SplitMix64(7000)
+    sp = 8e-3 + (2.0 * rng.uniforms(ncand * (nd + 1)) - 1.0).reshape(ncand, nd + 1) * 0.3e-3
+    freqs = np.linspace(19.5e9, 20.5e9, nfreq)
+    ctx = B.context()
+    A1 = B.Analytical1D
+    for _ in range(args.warmup):
+        A1.disk_system_batch(freqs, sp[:64], disk_epsilon=24)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g, a = A1.disk_system_batch(freqs, sp, disk_epsilon=24)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    pts = ncand * nfreq * args.steps / dt
+    out = {
+        "metric": "1D analytical model boost-factor points/s on 1 MI355X (FP64)",
+        "value": round(pts, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded SplitMix64 spacings)",
+        "note": "host buffers in and out (65 MB of results per step)",
+        "config": {"workload": f"disk_system (src/Analytical1D.py), {nd} disks eps=24, {ncand} spacing candidates "
+                               f"8 mm +- 0.3 mm x {nfreq} freqs 19.5-20.5 GHz", "candidates": ncand,
+                   "frequencies": nfreq},
+        "boost_max": float(np.max(np.abs(a) ** 2)),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import analytic1d as OA
+        t1 = time.perf_counter()
+        nc = 8
+        for c_ in range(nc):
+            OA.disk_system(freqs, 24 + 0j, 1 + 0j, sp[c_], nd)
+        cdt = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(nc * nfreq / cdt, 1), "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"{nc} candidates x {nfreq} frequencies with the numpy restatement of "
+                                         "src/Analytical1D.py (vectorised over frequencies, one thread)"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,9 +418,9 @@ def main():
                     help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
-    ap.add_argument("--workload", choices=("dancer", "transformer", "sweep"), default="dancer",
+    ap.add_argument("--workload", choices=("dancer", "transformer", "sweep", "model1d"), default="dancer",
                     help="dancer: the headline sweep (default); transformer: the mode-matching sweep; "
-                         "sweep: the many-geometry disk-position sweep")
+                         "sweep: the many-geometry disk-position sweep; model1d: the 1D analytical model")
     ap.add_argument("--nconf", type=int, default=32, help="geometries of the sweep workload")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
@@ -395,6 +439,9 @@ def main():
         return
     if args.workload == "sweep":
         sweep_main(args)
+        return
+    if args.workload == "model1d":
+        model1d_main(args)
         return
 
     import BoostFractor as B
