@@ -438,3 +438,20 @@ def test_transformer_plan_matches_transformer():
         b, r = plan.fetch()
         assert rel(b, want_b) < 1e-12 and rel(r, want_r) < 1e-12
     plan.close()
+
+
+def test_transformer_more_than_48_modes_uses_global_chain():
+    """n = 49 modes: the inverses and the chain take the global-memory kernels
+    (the LDS ones hold n <= 48); boost and reflectivity against the oracle"""
+    coords, sb = _flat_booster(N=32, ndisk=1)
+    modes = _lg_like_modes(coords, 7, 3)
+    assert modes.n == 49
+    emit = B.axion_induced_modes(coords, modes, diskR=DISKR)
+    refl_in = np.linspace(1, 2, modes.n) + 0.3j
+    boost, refl = B.transformer(sb.copy(), coords, modes, f=20.0e9, diskR=DISKR, emit=emit, reflect=refl_in)
+    want_b, want_r = _otransformer(sb, coords, modes, 20.0e9, O.PROP_PROPAGATOR, emit, refl_in)
+    eps = sb.eps.copy()
+    eps[np.isnan(eps.real)] = 1e30
+    sens = chain_sensitivity(_oracle_mats(sb, coords, modes, 20.0e9, O.PROP_PROPAGATOR, eps), eps, emit)
+    tol = max(TOL, 10 * sens)
+    assert rel(boost, want_b) < tol and rel(refl, want_r) < tol, (rel(boost, want_b), rel(refl, want_r), sens)
