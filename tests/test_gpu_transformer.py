@@ -455,3 +455,19 @@ def test_transformer_more_than_48_modes_uses_global_chain():
     sens = chain_sensitivity(_oracle_mats(sb, coords, modes, 20.0e9, O.PROP_PROPAGATOR, eps), eps, emit)
     tol = max(TOL, 10 * sens)
     assert rel(boost, want_b) < tol and rel(refl, want_r) < tol, (rel(boost, want_b), rel(refl, want_r), sens)
+
+
+def test_parseval_groups_match_per_bin_weights(monkeypatch):
+    """flat air regions: the k-bin group GEMM against per-bin weight tables in the
+    Gram kernel (BF_PARSEVAL_GROUPS=0) and against the oracle"""
+    coords, sb = _flat_booster(N=48, ndisk=2)
+    modes = _lg_like_modes(coords, 3, 1)
+    freqs = [19.8e9, 20.2e9]
+    got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    monkeypatch.setenv("BF_PARSEVAL_GROUPS", "0")
+    per_bin = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    for q, f in enumerate(freqs):
+        want = _oracle_mats(sb, coords, modes, f, O.PROP_PROPAGATOR)
+        for r in range(1, len(sb.distance)):
+            assert rel(got[q][r], per_bin[q][r]) < 1e-12, (q, r)
+            assert rel(per_bin[q][r], want[r]) < TOL and rel(got[q][r], want[r]) < TOL, (q, r)
