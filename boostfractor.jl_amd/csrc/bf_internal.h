@@ -85,7 +85,23 @@ struct bf_ctx {
     bool timing = false;
     std::map<std::string, bf_kernel_timer> timers;
     void* blas = nullptr;  // rocBLAS handle on `stream`, created on first use
+    // grow-only device buffers reused by consecutive calls on this context
+    // (work arrays of the mode-matching batches), released with the context
+    std::map<int, bf::DevBuf> scratch;
 };
+
+namespace bf {
+// scratch slot `slot` of the context with at least `bytes` bytes (contents undefined)
+inline DevBuf& scratch_buf(bf_ctx* ctx, int slot, size_t bytes) {
+    DevBuf& b = ctx->scratch[slot];
+    if (b.bytes < bytes) b.alloc(bytes);
+    return b;
+}
+enum ScratchSlot : int {
+    SB_W = 1, SB_CM, SB_PW, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
+    SB_SLOTS, SB_FLAG, SB_K0, SB_DZ
+};
+}  // namespace bf
 
 namespace bf {
 

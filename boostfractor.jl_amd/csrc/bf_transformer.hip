@@ -1292,7 +1292,11 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     const size_t per_f = std::max<size_t>(1, (grouped ? 0 : parseval.size()) + n_direct_disks) * K * sizeof(zd);
     const int fb = (int)std::max<size_t>(
         1, std::min<size_t>({(size_t)nf, (512u << 20) / per_f, 65535 / std::max<size_t>(1, disks.size())}));
-    DevBuf ktabs, rtabs, scratch, prop, scr2, dslots, dflag, dk0, ddz;
+    DevBuf ktabs, rtabs, scratch, prop, scr2;
+    DevBuf& dslots = ctx->scratch[SB_SLOTS];
+    DevBuf& dflag = ctx->scratch[SB_FLAG];
+    DevBuf& dk0 = ctx->scratch[SB_K0];
+    DevBuf& ddz = ctx->scratch[SB_DZ];
     ParsevalQueue local_q;
     ParsevalQueue& pq = queue ? *queue : local_q;
     for (int f0 = 0; f0 < nf; f0 += fb) {
@@ -1379,10 +1383,10 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         }
         // diag(exp(-i kz dz)) from the left for the disk regions
         if (!slots.empty()) {
-            dslots.alloc(sizeof(int) * slots.size());
-            dflag.alloc(sizeof(int) * slots.size());
-            dk0.alloc(sizeof(zd) * slots.size());
-            ddz.alloc(sizeof(double) * slots.size());
+            if (dslots.bytes < sizeof(int) * slots.size()) dslots.alloc(sizeof(int) * slots.size());
+            if (dflag.bytes < sizeof(int) * slots.size()) dflag.alloc(sizeof(int) * slots.size());
+            if (dk0.bytes < sizeof(zd) * slots.size()) dk0.alloc(sizeof(zd) * slots.size());
+            if (ddz.bytes < sizeof(double) * slots.size()) ddz.alloc(sizeof(double) * slots.size());
             upload(dslots.p, slots.data(), sizeof(int) * slots.size(), s);
             upload(dflag.p, from_gram.data(), sizeof(int) * slots.size(), s);
             upload(dk0.p, k0sq.data(), sizeof(zd) * slots.size(), s);
@@ -1403,8 +1407,8 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
                     sl.push_back((int)((size_t)f * nreg + r));
                     ph.push_back(cexp_j(cmk<double>(distance[r] * e.re, distance[r] * e.im)));
                 }
-            dslots.alloc(sizeof(int) * sl.size());
-            dk0.alloc(sizeof(zd) * sl.size());
+            if (dslots.bytes < sizeof(int) * sl.size()) dslots.alloc(sizeof(int) * sl.size());
+            if (dk0.bytes < sizeof(zd) * sl.size()) dk0.alloc(sizeof(zd) * sl.size());
             upload(dslots.p, sl.data(), sizeof(int) * sl.size(), s);
             upload(dk0.p, ph.data(), sizeof(zd) * sl.size(), s);
             hipLaunchKernelGGL(k_scaled_copy, dim3((unsigned)sl.size()), dim3(256), 0, s, P, dslots.as<int>(),
@@ -1444,7 +1448,10 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
         if (v > 0) budget = (size_t)v;
     }
     const size_t mc_ = std::max<size_t>(1, std::min<size_t>({total, (size_t)65535, budget / ((NG + nn) * sizeof(zd))}));
-    DevBuf Wt(sizeof(zd) * mc_ * NG), Cm(sizeof(zd) * mc_ * nn), dpw(sizeof(PwDesc) * mc_), ddst(sizeof(zd*) * mc_);
+    DevBuf& Wt = scratch_buf(ctx, SB_W, sizeof(zd) * mc_ * NG);
+    DevBuf& Cm = scratch_buf(ctx, SB_CM, sizeof(zd) * mc_ * nn);
+    DevBuf& dpw = scratch_buf(ctx, SB_PW, sizeof(PwDesc) * mc_);
+    DevBuf& ddst = scratch_buf(ctx, SB_DST, sizeof(zd*) * mc_);
     for (size_t m0 = 0; m0 < total; m0 += mc_) {
         const size_t M = std::min(mc_, total - m0);
         upload(dpw.p, q.pd.data() + m0, sizeof(PwDesc) * M, s);
@@ -1615,7 +1622,7 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n;
     const int N2 = 2 * n;
-    DevBuf dsel(sizeof(int) * sel.size());
+    DevBuf& dsel = scratch_buf(ctx, SB_SEL, sizeof(int) * sel.size());
     upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
     {
         TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
@@ -1623,11 +1630,10 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     }
     const size_t nj = jobs.size();
     const bool lds = n <= CHAIN_LDS_MAX;
-    DevBuf djobs(sizeof(ChainJob) * nj), dsteps(sizeof(ChainStep) * steps.size()), Tb, cw;
-    if (!lds) {
-        Tb.alloc(sizeof(zd) * 2 * (size_t)N2 * N2 * nj);
-        cw.alloc(sizeof(zd) * (nn + 3 * (size_t)n) * nj);
-    }
+    DevBuf& djobs = scratch_buf(ctx, SB_JOBS, sizeof(ChainJob) * nj);
+    DevBuf& dsteps = scratch_buf(ctx, SB_STEPS, sizeof(ChainStep) * steps.size());
+    DevBuf& Tb = scratch_buf(ctx, SB_TB, lds ? 0 : sizeof(zd) * 2 * (size_t)N2 * N2 * nj);
+    DevBuf& cw = scratch_buf(ctx, SB_CW, lds ? 0 : sizeof(zd) * (nn + 3 * (size_t)n) * nj);
     upload(djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
     upload(dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
     double flops = 0.0;
@@ -1816,7 +1822,9 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                 mats += m;
                 c1++;
             }
-            DevBuf P(sizeof(zd) * nn * mats), Pinv(sizeof(zd) * nn * mats), dboost(sizeof(zd) * n * nfreq * (c1 - c0));
+            DevBuf& P = scratch_buf(ctx, SB_P, sizeof(zd) * nn * mats);
+            DevBuf& Pinv = scratch_buf(ctx, SB_PINV, sizeof(zd) * nn * mats);
+            DevBuf& dboost = scratch_buf(ctx, SB_BOOST, sizeof(zd) * n * nfreq * (c1 - c0));
             std::vector<int> sel;
             std::vector<ChainJob> jobs;
             std::vector<ChainStep> steps;
