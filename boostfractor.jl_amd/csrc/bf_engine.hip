@@ -274,6 +274,7 @@ struct DancerRowArgs {
     const double* Y;
     double R2;
     const uint32_t* keep_tab;  // [ny][nx/16] aperture bits of the register path, or null
+    int keep_band;             // band class of keep_tab: no bit outside [keep_band, 15 - keep_band]
     uint32_t flags;
     zd* Eout;  // [nslices][ny][nx] per frequency slot, stride eout_fstride
     size_t eout_fstride;
@@ -482,7 +483,12 @@ __global__ void __launch_bounds__(512) k_rows_reg(RowArgs<T> a) {
 }
 
 // FL != 0: the flags are fixed at compile time (the dancer's FWD|KMID|INV).
-template <typename T, int N, uint32_t FL>
+// BM > 0 (dancer propagation only): every row of the band [ylo, yhi) is t + TT m
+// with BM <= m <= 15 - BM, so the forward transform skips the butterflies of the
+// other (zero) elements and the inverse transform forms the band elements
+// first; a field stored whole then forms the others from the same last-pass
+// input, still in LDS.
+template <typename T, int N, uint32_t FL, int BM = 0>
 __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
     BF_TYPES(T);
     extern __shared__ unsigned char smem_raw[];
@@ -517,12 +523,16 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
         const uint32_t bandb = (uint32_t)(a.yhi - a.ylo) * rowb;
         const auto rsb = buf_rsrc(a.src + d.off + (size_t)a.ylo * nx, bandb);
         const uint32_t lband = ok ? (uint32_t)((t - a.ylo) * nx + mx) * (uint32_t)sizeof(cplx<S>) : BUF_OOB;
+        constexpr int MLO = BM, MHI = E - 1 - BM;
         cplx<S> raw[E];
 #pragma unroll
-        for (int m = 0; m < E; m++) raw[m] = buf_load<S>(rsb, lband + (uint32_t)m * stepb);
+        for (int m = MLO; m <= MHI; m++) raw[m] = buf_load<S>(rsb, lband + (uint32_t)m * stepb);
 #pragma unroll
-        for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
-        fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+        for (int m = 0; m < E; m++) v[m] = (m >= MLO && m <= MHI) ? ccast<C>(raw[m]) : cmk<C>(0, 0);
+        if constexpr (BM == 0)
+            fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+        else
+            fft_reg_m<C, S, N, E, false, band_mask16(BM), 0xFFFF>(v, s, t, a.tw);
         if (has_kt) {
             const uint32_t krow = (uint32_t)nx * (uint32_t)sizeof(cplx<C>), kstep = (uint32_t)TT * krow;
             const uint32_t lk = ok ? (uint32_t)(t * nx + mx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
@@ -536,12 +546,23 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
                 for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
             }
         }
-        fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+        if constexpr (BM == 0)
+            fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+        else
+            fft_reg_m<C, S, N, E, true, 0xFFFF, band_mask16(BM)>(v, s, t, a.tw);
         // band rows, or every row of a field stored whole
         const auto rdb = buf_rsrc(a.dst + d.off + (whole ? 0 : (size_t)a.ylo * nx), whole ? K * (uint32_t)sizeof(cplx<S>) : bandb);
         const uint32_t lst = ok ? (uint32_t)((t - (whole ? 0 : a.ylo)) * nx + mx) * (uint32_t)sizeof(cplx<S>) : BUF_OOB;
 #pragma unroll
-        for (int m = 0; m < E; m++) buf_store<S>(rdb, lst + (uint32_t)m * stepb, ccast<S>(v[m]));
+        for (int m = MLO; m <= MHI; m++) buf_store<S>(rdb, lst + (uint32_t)m * stepb, ccast<S>(v[m]));
+        if constexpr (BM > 0) {
+            if (whole) {  // uniform over the block
+                fft_reg_last_m<C, S, N, E, true, 0xFFFF & ~band_mask16(BM)>(v, s, t, a.tw);
+#pragma unroll
+                for (int m = 0; m < E; m++)
+                    if (m < MLO || m > MHI) buf_store<S>(rdb, lst + (uint32_t)m * stepb, ccast<S>(v[m]));
+            }
+        }
         return;
     }
     {
@@ -594,7 +615,11 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
     }
 }
 
-template <typename T, int N>
+// OBM > 0: only elements m in [OBM, 15 - OBM] of the inverse transform are
+// formed, updated and exchanged (blocks whose update needs no other element:
+// not the exit boundary, no field maximum).  IBM > 0: the aperture keeps no
+// element outside [IBM, 15 - IBM], so the forward transform skips the others.
+template <typename T, int N, int OBM, int IBM>
 __device__ __forceinline__ void dancer_rows_reg_body(const DancerRowArgs<T>& a) {
     BF_TYPES(T);
     extern __shared__ unsigned char smem_raw[];
@@ -639,22 +664,27 @@ __device__ __forceinline__ void dancer_rows_reg_body(const DancerRowArgs<T>& a) 
     // next bounce's aperture (bit m: keep point t + TT m)
     const uint32_t keep = (a.flags & DF_FWD) ? (a.keep_tab ? a.keep_tab[yc * TT + t] : aperture_bits<E, TT>(a.X, a.Y[yc], a.R2, t))
                                              : ~0u;
-    fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    // elements formed by the inverse transform
+    constexpr int OLO = OBM, OHI = E - 1 - OBM;
+    if constexpr (OBM == 0)
+        fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    else
+        fft_reg_m<C, S, N, E, true, 0xFFFF, band_mask16(OBM)>(v, s, t, a.tw);
     if (a.rtab) {
         const int ti = exists ? a.rtab_idx[f * a.R + reg] : -1;
         if (ti >= 0) {
             const cplx<C>* tl = a.rtab + (size_t)ti * K + o0;
             cplx<C> tab[E];
 #pragma unroll
-            for (int m = 0; m < E; m++) tab[m] = tl[TT * m];
+            for (int m = OLO; m <= OHI; m++) tab[m] = tl[TT * m];
 #pragma unroll
-            for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
+            for (int m = OLO; m <= OHI; m++) v[m] = cmul(v[m], tab[m]);
         }
     }
     // exchange with the partner line through LDS (the FFT's buffers are free now)
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < E; m++) s[t + TT * m] = ccast<S>(v[m]);
+    for (int m = OLO; m <= OHI; m++) s[t + TT * m] = ccast<S>(v[m]);
     __syncthreads();
     // boundary update (src/boost_fractor_recurprop.jl:90-119) as own*c_own + partner*c_w:
     //   side 0 (A):  b < R: A' = r A + (1-r) B;    b = R: A' = r A, Eout += (1+r) A
@@ -669,7 +699,7 @@ __device__ __forceinline__ void dancer_rows_reg_body(const DancerRowArgs<T>& a) 
         cown = cmk<C>((C)(-r.re), (C)(-r.im));
         cw = (b == 0) ? cmk<C>(0, 0) : cmk<C>((C)(1.0 + r.re), (C)r.im);
     }
-    if (b == a.R) {  // exit boundary (uniform over the block): Eout += (1+r) * A on the A lines
+    if (OBM == 0 && b == a.R) {  // exit boundary (uniform over the block): Eout += (1+r) * A on the A lines
         const bool eo_line = exists && side == 0;
         const bool assign = (a.flags & DF_EOUT_ASSIGN) != 0;
         const zd opr = cmk<double>(1.0 + r.re, r.im);
@@ -689,17 +719,21 @@ __device__ __forceinline__ void dancer_rows_reg_body(const DancerRowArgs<T>& a) 
     }
     cplx<S> wp[E];  // partner values, all LDS reads issued together
 #pragma unroll
-    for (int m = 0; m < E; m++) wp[m] = sp[t + TT * m];
+    for (int m = OLO; m <= OHI; m++) wp[m] = sp[t + TT * m];
     double vmax = 0.0;
 #pragma unroll
     for (int m = 0; m < E; m++) {
+        if (m < OLO || m > OHI) {
+            v[m] = cmk<C>(0, 0);  // outside the aperture: zeroed by the mask
+            continue;
+        }
         const cplx<C> w = ccast<C>(wp[m]);
         cplx<C> nv = exists ? cadd(cmul(v[m], cown), cmul(cw, w)) : cmk<C>(0, 0);
-        if (a.flags & DF_AMAX) vmax = fmax(vmax, hypot((double)nv.re, (double)nv.im));
+        if (OBM == 0 && (a.flags & DF_AMAX)) vmax = fmax(vmax, hypot((double)nv.re, (double)nv.im));
         if (!((keep >> m) & 1u)) nv = cmk<C>(0, 0);
         v[m] = nv;
     }
-    if (a.flags & DF_AMAX) {
+    if (OBM == 0 && (a.flags & DF_AMAX)) {
         for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o));
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vmax;
         __syncthreads();
@@ -710,23 +744,43 @@ __device__ __forceinline__ void dancer_rows_reg_body(const DancerRowArgs<T>& a) 
         }
     }
     if (!(a.flags & DF_FWD) || exit_only) return;  // only Eout is needed
-    fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+    if constexpr (IBM == 0)
+        fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+    else
+        fft_reg_m<C, S, N, E, false, band_mask16(IBM), 0xFFFF>(v, s, t, a.tw);
     if (exists)
 #pragma unroll
         for (int m = 0; m < E; m++) line[TT * m] = ccast<S>(v[m]);
 }
 
-template <typename T, int N>
+// XB > 0: the aperture bits keep no element outside [XB, 15 - XB] (the launcher
+// checks the table and that the forward transform runs)
+template <typename T, int N, int XB>
+__device__ __forceinline__ void dancer_rows_reg_dispatch(const DancerRowArgs<T>& a) {
+    if constexpr (XB == 0) {
+        dancer_rows_reg_body<T, N, 0, 0>(a);
+    } else {
+        int b, tile;
+        bool exit_only;
+        dancer_block(a, blockIdx.x, b, tile, exit_only);
+        if (b == a.R || (a.flags & DF_AMAX))  // uniform over the block
+            dancer_rows_reg_body<T, N, 0, XB>(a);
+        else
+            dancer_rows_reg_body<T, N, XB, XB>(a);
+    }
+}
+
+template <typename T, int N, int XB = 0>
 __global__ void __launch_bounds__(512) k_dancer_rows_reg(DancerRowArgs<T> a) {
-    dancer_rows_reg_body<T, N>(a);
+    dancer_rows_reg_dispatch<T, N, XB>(a);
 }
 
 // The complex64-storage instantiation capped at 128 VGPRs (4 waves per SIMD
 // instead of 3, at the price of 4 spilled registers): measured 7% faster; the
 // FP64 one would spill far more and is left uncapped.
-template <int N>
+template <int N, int XB = 0>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_dancer_rows_reg4(DancerRowArgs<float> a) {
-    dancer_rows_reg_body<float, N>(a);
+    dancer_rows_reg_dispatch<float, N, XB>(a);
 }
 
 // pointwise bounce update for the unfused propagators (propagator1D, momentum space)
@@ -966,6 +1020,16 @@ struct RowLaunch {
     }
 };
 
+// Band class of the pruned register transforms: the largest BM in {5, 3} with
+// every possibly nonzero element m of a thread in [BM, E-1-BM] (elements mlo..mhi),
+// else 0 (no pruning).  BF_PRUNE=0 switches the pruned kernels off.
+static int prune_class(int mlo, int mhi, int E) {
+    const char* v = getenv("BF_PRUNE");
+    if ((v && std::string(v) == "0") || mhi < mlo) return 0;
+    const int bm = std::min(mlo, E - 1 - mhi);
+    return bm >= 5 ? 5 : bm >= 3 ? 3 : 0;
+}
+
 template <typename T, int N>
 struct ColLaunch {
     static void run(ColArgs<T> a, int tx_req, hipStream_t s) {
@@ -979,9 +1043,24 @@ struct ColLaunch {
                 const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)tx * C::LDC;
                 constexpr uint32_t PROP = CF_FWD | CF_KMID | CF_INV;
                 if (a.flags == PROP) {
-                    allow_lds((const void*)k_cols_reg<T, N, PROP>, smem);
-                    hipLaunchKernelGGL((k_cols_reg<T, N, PROP>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * C::TT),
-                                       smem, s, a);
+                    const int bm = N >= 128 ? prune_class(a.ylo / C::TT, (a.yhi - 1) / C::TT, C::E) : 0;
+                    const dim3 grid((a.nx + tx - 1) / tx, a.nf), block(tx * C::TT);
+                    bool pruned = false;
+                    if constexpr (N >= 128) {
+                        if (bm == 5) {
+                            allow_lds((const void*)k_cols_reg<T, N, PROP, 5>, smem);
+                            hipLaunchKernelGGL((k_cols_reg<T, N, PROP, 5>), grid, block, smem, s, a);
+                            pruned = true;
+                        } else if (bm == 3) {
+                            allow_lds((const void*)k_cols_reg<T, N, PROP, 3>, smem);
+                            hipLaunchKernelGGL((k_cols_reg<T, N, PROP, 3>), grid, block, smem, s, a);
+                            pruned = true;
+                        }
+                    }
+                    if (!pruned) {
+                        allow_lds((const void*)k_cols_reg<T, N, PROP>, smem);
+                        hipLaunchKernelGGL((k_cols_reg<T, N, PROP>), grid, block, smem, s, a);
+                    }
                 } else {
                     allow_lds((const void*)k_cols_reg<T, N, 0>, smem);
                     hipLaunchKernelGGL((k_cols_reg<T, N, 0>), dim3((a.nx + tx - 1) / tx, a.nf), dim3(tx * C::TT),
@@ -1019,6 +1098,18 @@ static unsigned dancer_tiles(DancerRowArgs<T>& a, int rows) {
 
 template <typename T, int N>
 struct DancerRowLaunch {
+    template <int XB>
+    static void launch_rows(const DancerRowArgs<T>& a, dim3 grid, dim3 block, size_t smem, hipStream_t s) {
+        if constexpr (XB > 0 && N < 128) {
+            launch_rows<0>(a, grid, block, smem, s);
+        } else if constexpr (std::is_same<T, float>::value) {
+            allow_lds((const void*)k_dancer_rows_reg4<N, XB>, smem);
+            hipLaunchKernelGGL((k_dancer_rows_reg4<N, XB>), grid, block, smem, s, a);
+        } else {
+            allow_lds((const void*)k_dancer_rows_reg<T, N, XB>, smem);
+            hipLaunchKernelGGL((k_dancer_rows_reg<T, N, XB>), grid, block, smem, s, a);
+        }
+    }
     static void run(DancerRowArgs<T> a, int fb, int rows_req, hipStream_t s) {
         if constexpr (N >= 16) {
             if (use_reg_fft()) {
@@ -1027,13 +1118,14 @@ struct DancerRowLaunch {
                 rows = std::max(1, std::min(rows, 256 / C::TT));
                 const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * (2 * rows);
                 const unsigned nblk = dancer_tiles(a, rows);
-                if constexpr (std::is_same<T, float>::value) {
-                    allow_lds((const void*)k_dancer_rows_reg4<N>, smem);
-                    hipLaunchKernelGGL((k_dancer_rows_reg4<N>), dim3(nblk, fb), dim3(2 * rows * C::TT), smem, s, a);
+                const dim3 grid(nblk, fb), block(2 * rows * C::TT);
+                const int xb = (a.keep_tab && (a.flags & DF_FWD)) ? a.keep_band : 0;
+                if (xb == 5) {
+                    launch_rows<5>(a, grid, block, smem, s);
+                } else if (xb == 3) {
+                    launch_rows<3>(a, grid, block, smem, s);
                 } else {
-                    allow_lds((const void*)k_dancer_rows_reg<T, N>, smem);
-                    hipLaunchKernelGGL((k_dancer_rows_reg<T, N>), dim3(nblk, fb), dim3(2 * rows * C::TT), smem, s,
-                                       a);
+                    launch_rows<0>(a, grid, block, smem, s);
                 }
                 BF_CHECK_LAUNCH();
                 return;
@@ -1315,6 +1407,7 @@ struct DancerPlan : DancerPlanBase {
     bool has_surf = false;
     DevBuf surf, coef, fields, scratch, Eout, hist, power, active, amax, count, fdesc, fdesc_col, ridx, ax, emit,
         reflect, ph1d, keep_tab;
+    int keep_band = 0;  // band class of keep_tab (see prune_class)
     std::unique_ptr<PropEngine<T>> pe;
     DancerOptions opt;
     int nslices;
@@ -1467,6 +1560,9 @@ struct DancerPlan : DancerPlanBase {
                     for (int m = 0; m < 16; m++) bits |= (in_aperture(co->X[t + TT * m], co->Y[y], R2) ? 1u : 0u) << m;
                     kb[(size_t)y * TT + t] = bits;
                 }
+            uint32_t any = 0;
+            for (uint32_t w : kb) any |= w;
+            keep_band = any ? prune_class(__builtin_ctz(any), 31 - __builtin_clz(any), 16) : 0;
             keep_tab.alloc(sizeof(uint32_t) * kb.size());
             upload(keep_tab.p, kb.data(), sizeof(uint32_t) * kb.size(), s);
         }
@@ -1570,6 +1666,7 @@ struct DancerPlan : DancerPlanBase {
                     a.coef = coef.as<zd>();
                     a.X = g.X.as<double>(); a.Y = g.Y.as<double>(); a.R2 = diskR * diskR;
                     a.keep_tab = keep_tab.as<uint32_t>();
+                    a.keep_band = keep_band;
                     a.flags = (n < nmax ? DF_FWD : 0u) | (amin != 0.0 ? DF_AMAX : 0u) |
                               (nslices > 1 ? DF_EOUT_ASSIGN : 0u);
                     a.Eout = eo_target;

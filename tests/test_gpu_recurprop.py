@@ -99,8 +99,8 @@ def test_cheerleader_golden():
     assert np.max(np.abs(np.array(got) / g - 1)) < TOL
 
 
-def _small_booster(N=64, ndisk=3, seed=1001, tilt=True):
-    coords = W.grid_coords(N)
+def _small_booster(N=64, ndisk=3, seed=1001, tilt=True, span=1.0):
+    coords = W.grid_coords(N, span)
     sb = W.booster(coords, ndisk=ndisk, seed=seed)
     if tilt:
         rng = W.SplitMix64(seed + 7)
@@ -247,3 +247,43 @@ def test_dancer_row_band_is_exact(monkeypatch, N, prec):
     assert np.count_nonzero(np.any(XX ** 2 + YY ** 2 < DISKR ** 2, axis=1)) < N // 2
     assert np.array_equal(out["0"][0], out["1"][0])
     assert np.array_equal(out["0"][1], out["1"][1])
+
+
+# (N, span): the aperture's rows and columns fall in elements [5, 10] of a
+# thread's 16 (N/16 apart) on a 1 m grid, in [3, 12] on 0.5 m and in [4, 11]
+# on 0.6 m, so the column and row passes run their band-pruned transforms.
+@pytest.mark.parametrize("N,span,prec", [(128, 1.0, "fp64"), (128, 1.0, "mixed"), (128, 1.0, "fp32"),
+                                         (128, 0.5, "fp64"), (128, 0.6, "mixed"), (256, 1.0, "mixed"),
+                                         (256, 1.0, "fp64")])
+def test_dancer_pruned_transforms_match_unpruned(monkeypatch, N, span, prec):
+    """Band-pruned register transforms (butterflies of zero elements skipped,
+    elements that are not stored not formed) against the unpruned ones: summed and
+    history outputs, tilted and rough disks (tables applied in the row pass) and
+    the field maximum of the amin criterion.  The arithmetic is the same up to FMA
+    contraction (a product fused where the pruned code uses it once), so the
+    results agree to rounding: 1e-12 of the field maximum in FP64, 1e-5 with
+    complex64 storage (mixed), 1e-4 in FP32 arithmetic."""
+    coords, sb = _small_booster(N=N, ndisk=3, seed=1003, span=span)
+    freqs = [19.8e9, 20.1e9]
+    out = {}
+    for prune in ("0", "1"):
+        monkeypatch.setenv("BF_PRUNE", prune)
+        e = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+        h = B.dancer(0, 6, sb, coords, f=20e9, diskR=DISKR, precision=prec, returnsum=False, immediatesum=False)
+        a = B.dancer(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
+        out[prune] = (e, h, a)
+    tol = {"fp64": 1e-12, "mixed": 1e-5, "fp32": 1e-4}[prec]
+    assert np.any(out["1"][0] != 0)
+    for q in range(3):
+        assert rel(out["1"][q], out["0"][q]) < tol, (q, rel(out["1"][q], out["0"][q]))
+
+
+def test_dancer_pruned_matches_oracle():
+    """The pruned path (128 x 128, tilted and rough disks) against the oracle."""
+    coords, sb = _small_booster(N=128, ndisk=2, seed=1004)
+    freqs = [19.9e9, 20.2e9]
+    got = B.dancer(0, 15, sb, coords, f=freqs, diskR=DISKR)
+    os_, oc = osetup(sb), ocoords(coords)
+    want = pmap(lambda f: O.dancer(os_, oc, 0, 15, f, O.PROP_PROPAGATOR, diskR=DISKR), freqs)
+    for q in range(len(freqs)):
+        assert rel(got[q], want[q]) < TOL, (q, rel(got[q], want[q]))
