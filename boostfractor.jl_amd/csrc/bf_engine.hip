@@ -58,14 +58,20 @@ __global__ void k_twiddles(cplx<T>* out, int n) {
     out[m] = cmk<T>((T)c, (T)(-s));
 }
 
+// quarter != 0 (unshifted tables of even-sized grids whose k range is mirror
+// symmetric, k[j] = -k[N-1-j] as -dk(N-1)/2 : dk : dk(N-1)/2 is): only bins
+// mx < nx/2, my < ny/2 are stored, as [ny/2][nx/2]; bin nx-1-mx reads entry mx
+// (and likewise in y), whose k differs from its own by the rounding of the range.
 template <typename T>
 __global__ void k_build_ktab(cplx<T>* out, const KTabDesc* __restrict__ d, int nx, int ny,
-                             const double* __restrict__ kX, const double* __restrict__ kY, double scale) {
+                             const double* __restrict__ kX, const double* __restrict__ kY, double scale,
+                             int quarter = 0) {
     const int tab = blockIdx.y;
-    const size_t K = (size_t)nx * ny;
+    const int wx = quarter ? nx / 2 : nx;
+    const size_t K = (size_t)wx * (quarter ? ny / 2 : ny);
     const KTabDesc dd = d[tab];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (size_t)gridDim.x * blockDim.x) {
-        int mx = (int)(i % nx), my = (int)(i / nx);
+        int mx = (int)(i % wx), my = (int)(i / wx);
         int jx = mx, jy = my;
         if (!dd.shifted) {  // unshifted bin m multiplies the fftshifted entry (m + floor(N/2)) mod N
             jx = (mx + nx / 2) % nx;
@@ -186,6 +192,7 @@ struct ColArgs {
     // not loaded) and are not needed on output (not stored), except for the
     // fields whose descriptor carries FD_WHOLE, stored whole.
     int ylo, yhi;
+    int kq;  // 0: full k-space tables; else quarter tables of row width kq = nx/2 (dancer propagation)
 };
 
 template <typename T>
@@ -507,7 +514,8 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
     const auto rs = buf_rsrc(a.src + d.off, K * (uint32_t)sizeof(cplx<S>));
     const auto rd = buf_rsrc(a.dst + d.off, K * (uint32_t)sizeof(cplx<S>));
     const bool has_kt = d.ktab >= 0;
-    const auto rk = buf_rsrc(a.ktab + (size_t)(has_kt ? d.ktab : 0) * K, K * (uint32_t)sizeof(cplx<C>));
+    const uint32_t KT = a.kq ? (uint32_t)a.kq * (uint32_t)(N / 2) : K;  // elements per table
+    const auto rk = buf_rsrc(a.ktab + (size_t)(has_kt ? d.ktab : 0) * KT, KT * (uint32_t)sizeof(cplx<C>));
     const bool whole = store_whole(d);
     const int sx = (fl & CF_INSHIFT) ? (mx + nx / 2) % nx : mx;
     cplx<S>* s = smem + (size_t)c * LDC;
@@ -534,16 +542,36 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
         else
             fft_reg_m<C, S, N, E, false, band_mask16(BM), 0xFFFF>(v, s, t, a.tw);
         if (has_kt) {
-            const uint32_t krow = (uint32_t)nx * (uint32_t)sizeof(cplx<C>), kstep = (uint32_t)TT * krow;
-            const uint32_t lk = ok ? (uint32_t)(t * nx + mx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
             constexpr int KCH = sizeof(C) == 8 ? 8 : E;
+            if (a.kq) {
+                // quarter table: column min(mx, nx-1-mx); row y = t + TT m is < N/2 for
+                // m < E/2 (entry y) and >= N/2 otherwise (entry N-1-y = TT-1-t + TT (15-m))
+                const uint32_t krow = (uint32_t)a.kq * (uint32_t)sizeof(cplx<C>), kstep = (uint32_t)TT * krow;
+                const int qx = mx < nx / 2 ? mx : nx - 1 - mx;
+                const uint32_t lo = ok ? (uint32_t)(t * a.kq + qx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
+                const uint32_t hi = ok ? (uint32_t)((TT - 1 - t) * a.kq + qx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
 #pragma unroll
-            for (int m0 = 0; m0 < E; m0 += KCH) {
-                cplx<C> k[KCH];
+                for (int m0 = 0; m0 < E; m0 += KCH) {
+                    cplx<C> k[KCH];
 #pragma unroll
-                for (int q = 0; q < KCH; q++) k[q] = buf_load<C>(rk, lk + (uint32_t)(m0 + q) * kstep);
+                    for (int q = 0; q < KCH; q++) {
+                        const int m = m0 + q;
+                        k[q] = buf_load<C>(rk, m < E / 2 ? lo + (uint32_t)m * kstep : hi + (uint32_t)(E - 1 - m) * kstep);
+                    }
 #pragma unroll
-                for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
+                    for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
+                }
+            } else {
+                const uint32_t krow = (uint32_t)nx * (uint32_t)sizeof(cplx<C>), kstep = (uint32_t)TT * krow;
+                const uint32_t lk = ok ? (uint32_t)(t * nx + mx) * (uint32_t)sizeof(cplx<C>) : BUF_OOB;
+#pragma unroll
+                for (int m0 = 0; m0 < E; m0 += KCH) {
+                    cplx<C> k[KCH];
+#pragma unroll
+                    for (int q = 0; q < KCH; q++) k[q] = buf_load<C>(rk, lk + (uint32_t)(m0 + q) * kstep);
+#pragma unroll
+                    for (int q = 0; q < KCH; q++) v[m0 + q] = cmul(v[m0 + q], k[q]);
+                }
             }
         }
         if constexpr (BM == 0)
@@ -1033,6 +1061,8 @@ static int prune_class(int mlo, int mhi, int E) {
 template <typename T, int N>
 struct ColLaunch {
     static void run(ColArgs<T> a, int tx_req, hipStream_t s) {
+        if (a.kq && !(N >= 16 && use_reg_fft() && a.flags == (CF_FWD | CF_KMID | CF_INV) && a.nx == N))
+            throw std::runtime_error("quarter k-space tables are read only by the register propagation pass");
         if constexpr (N >= 16) {
             if (use_reg_fft()) {
                 using C = RegE<T, N>;
@@ -1171,6 +1201,10 @@ struct PropEngine {
     std::vector<int> ktab_of;  // per region entry: k-space table index
     std::vector<int> rtab_of;  // per region entry: rtab index or -1
     int col_tx = 0;            // autotuned column tile (0: default)
+    // Quarter k-space tables (see k_build_ktab): set by the fused dancer with
+    // complex64 fields on even power-of-two grids; only its register column
+    // pass reads them.
+    bool quarter = false;
 
     PropEngine(bf_ctx* c, const Grid* gr, int k, double diskR, const zd* surf)
         : ctx(c), g(gr), kind(k), R2(diskR * diskR), surfaces(surf) {
@@ -1229,12 +1263,13 @@ struct PropEngine {
         ntab = (int)kd.size();
         if (kind == BF_PROP_1D) return;
         const double scale = 1.0 / (double)((int64_t)g->nx * g->ny);  // ifft normalisation folded in
-        if (ktab.bytes < sizeof(cplx<C>) * K * ntab) ktab.alloc(sizeof(cplx<C>) * K * ntab);
+        const size_t KT = quarter ? (size_t)(g->nx / 2) * (g->ny / 2) : K;
+        if (ktab.bytes < sizeof(cplx<C>) * KT * ntab) ktab.alloc(sizeof(cplx<C>) * KT * ntab);
         if (kdesc.bytes < sizeof(KTabDesc) * ntab) kdesc.alloc(sizeof(KTabDesc) * ntab);
         upload(kdesc.p, kd.data(), sizeof(KTabDesc) * ntab, ctx->stream);
-        hipLaunchKernelGGL(k_build_ktab<C>, dim3(grid1d(K, 256, 1024), ntab), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_build_ktab<C>, dim3(grid1d(KT, 256, 1024), ntab), dim3(256), 0, ctx->stream,
                            ktab.as<cplx<C>>(), kdesc.as<KTabDesc>(), g->nx, g->ny, g->kX.as<double>(),
-                           g->kY.as<double>(), scale);
+                           g->kY.as<double>(), scale, quarter ? 1 : 0);
         BF_CHECK_LAUNCH();
         if (!rd.empty()) {
             if (rtab.bytes < sizeof(cplx<C>) * K * rd.size()) rtab.alloc(sizeof(cplx<C>) * K * rd.size());
@@ -1262,6 +1297,7 @@ struct PropEngine {
         ColArgs<T> a;
         a.src = src; a.dst = dst; a.fd = fd; a.nf = nf; a.nx = g->nx; a.ny = g->ny; a.tw = twy; a.flags = flags;
         a.ktab = ktab.as<cplx<C>>(); a.tx = 0; a.active = active;
+        a.kq = quarter ? g->nx / 2 : 0;
         a.ylo = ylo; a.yhi = yhi < 0 ? g->ny : yhi;
         dispatch_n<ColLaunch, T>(g->ny, a, col_tx, ctx->stream);
     }
@@ -1504,8 +1540,27 @@ struct DancerPlan : DancerPlanBase {
             band_lo = lo < hi ? lo : 0;
             band_hi = lo < hi ? hi : 0;
         }
+        bool qtab;
+        {
+            // Quarter k-space tables for complex64 fields: the phase of a mirrored bin
+            // comes from its mirror's k, equal up to the rounding of the k range
+            // (~1e-15 rad per bounce), far inside what complex64 storage rounds
+            // away; the FP64 path keeps full tables.  BF_QTAB=0 switches it off.
+            const char* q = getenv("BF_QTAB");
+            const int n = g.nx;
+            auto mirrored = [](const double* k, int len) {
+                double mx = 0.0;
+                for (int j = 0; j < len; j++) mx = std::max(mx, std::fabs(k[j]));
+                for (int j = 0; j < len; j++)
+                    if (!(std::fabs(k[j] + k[len - 1 - j]) <= 1e-12 * mx)) return false;
+                return true;
+            };
+            qtab = !std::is_same<T, double>::value && !(q && std::string(q) == "0") && use_reg_fft() &&
+                          (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS) && g.nx == g.ny && n >= 16 &&
+                          n <= 1024 && (n & (n - 1)) == 0 && mirrored(co->kX, g.nx) && mirrored(co->kY, g.ny);
+        }
         // Frequency batch: what one bounce of a batch touches (the band rows of every
-        // field, plus one FP64 k-space table per distinct (eps, dz)) is kept near
+        // field, plus one k-space table per distinct (eps, dz)) is kept near
         // 192 MB, the size of the Infinity Cache; the frequencies are then spread
         // evenly over the batches.  Measured on the 20-disk booster at 256^2:
         // 5 frequencies (complex64 fields) and 4 (complex128) are the fastest.
@@ -1522,7 +1577,7 @@ struct DancerPlan : DancerPlanBase {
             std::sort(keys.begin(), keys.end());
             const size_t ntab = (size_t)(std::unique(keys.begin(), keys.end()) - keys.begin());
             const size_t touched = sizeof(cplx<S>) * (size_t)(band_hi - band_lo) * g.nx * 2 * R +
-                                   ntab * K * sizeof(cplx<C>);
+                                   ntab * (qtab ? K / 4 : K) * sizeof(cplx<C>);
             const size_t budget = (size_t)192 << 20;
             fb = (int)std::max<size_t>(1, budget / std::max<size_t>(touched, 1));
         }
@@ -1572,6 +1627,7 @@ struct DancerPlan : DancerPlanBase {
         }
         pe.reset(new PropEngine<T>(ctx, &g, kind, diskR, surf.as<zd>()));
         pe->col_tx = opt.col_tx;
+        pe->quarter = qtab;
         // aperture bits of the register row pass (thread t of a line holds x = t + (nx/16) m, m < 16)
         if (g.nx >= 16 && g.nx % 16 == 0) {
             const int TT = g.nx / 16;

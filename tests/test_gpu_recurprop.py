@@ -287,3 +287,20 @@ def test_dancer_pruned_matches_oracle():
     want = pmap(lambda f: O.dancer(os_, oc, 0, 15, f, O.PROP_PROPAGATOR, diskR=DISKR), freqs)
     for q in range(len(freqs)):
         assert rel(got[q], want[q]) < TOL, (q, rel(got[q], want[q]))
+
+
+@pytest.mark.parametrize("N,prec", [(64, "mixed"), (128, "mixed"), (128, "fp32"), (256, "mixed")])
+def test_dancer_quarter_tables_match_full(monkeypatch, N, prec):
+    """Complex64-field paths read quarter k-space tables ([N/2+1]^2 entries, a
+    mirrored bin reads its mirror's phase, equal up to the rounding of the k
+    range): against full tables within the storage rounding (1e-5 of the field
+    maximum with FP64 arithmetic, 1e-4 in FP32), tilted and rough disks."""
+    coords, sb = _small_booster(N=N, ndisk=3, seed=1005)
+    freqs = [19.8e9, 20.1e9]
+    out = {}
+    for q in ("0", "1"):
+        monkeypatch.setenv("BF_QTAB", q)
+        out[q] = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+    assert np.any(out["1"] != 0)
+    tol = {"mixed": 1e-5, "fp32": 1e-4}[prec]
+    assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
