@@ -15,7 +15,7 @@ from .propagators import propagator, propagator1D, propagatorMomentumSpace, prop
 from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
 from .transformer import (ErrorException, Modes, SeedModes, axion_induced_modes, calc_propagation_matrices,
                           e_field_dimensions, field2modes, index, mode, modes2field, propagation_matrix, transformer,
-                          TransformerPlan, transformer_batch, transformer_trace_back)
+                          TransformerPlan, TransformerBatchPlan, transformer_batch, transformer_trace_back)
 from . import analytical1d as Analytical1D  # noqa: N812  (the reference module's name, src/Analytical1D.py)
 from .sweep import (SpacingInterpolation, calc_boostfactor_cost, calc_boostfactor_cost_batch, calc_boostfactor_modes,
                     calc_modes, calc_propagation_matrices_grid, construct_prop_matrix_interpolation,
@@ -27,7 +27,7 @@ __all__ = [
     "context", "device_count", "propagator", "propagatorNoTilts", "propagatorMomentumSpace", "propagator1D",
     "dancer", "dance_intro", "cheerleader", "DancerPlan", "Modes", "SeedModes", "mode", "index",
     "e_field_dimensions", "axion_induced_modes", "field2modes", "modes2field", "propagation_matrix",
-    "calc_propagation_matrices", "transformer", "TransformerPlan", "transformer_batch", "transformer_trace_back", "ErrorException", "InexactError",
+    "calc_propagation_matrices", "transformer", "TransformerPlan", "TransformerBatchPlan", "transformer_batch", "transformer_trace_back", "ErrorException", "InexactError",
     "gauss_profile", "unnormalized_match", "BoundsError", "calc_propagation_matrices_grid", "SpacingInterpolation",
     "construct_prop_matrix_interpolation", "interpolate_prop_matrix", "calc_boostfactor_modes", "calc_modes",
     "calc_boostfactor_cost", "calc_boostfactor_cost_batch", "soft_box_penalty", "p_norm", "copy_setup_boundaries",

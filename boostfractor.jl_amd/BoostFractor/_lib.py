@@ -103,6 +103,12 @@ _SIGS = {
     "bf_transformer_batch": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _I32,
                                             ctypes.POINTER(ModesS), ctypes.POINTER(ctypes.c_void_p), _P, _P, _I32,
                                             _I32, _D, _I32, _P, _P]),
+    "bf_transformer_batch_plan_create": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, ctypes.POINTER(ModesS),
+                                                        ctypes.POINTER(ctypes.c_void_p), _P, _I32, _I32, _D, _I32,
+                                                        ctypes.POINTER(_P)]),
+    "bf_transformer_batch_plan_run": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), _P, _P]),
+    "bf_transformer_batch_plan_fetch": (ctypes.c_int, [_P, _P]),
+    "bf_transformer_batch_plan_destroy": (ctypes.c_int, [_P]),
     "bf_propagation_matrices_grid": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords),
                                                     ctypes.POINTER(ModesS), _P, _I32, _P, _I32, _P, _I32, _P, _I32,
                                                     _I32, _D, _P]),

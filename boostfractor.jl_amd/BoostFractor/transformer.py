@@ -294,6 +294,95 @@ class TransformerPlan:
             pass
 
 
+def _batch_layout(nconf, modesets, mode_set_of, nf):
+    """mode_set_of as int32 (default all 0), checked; per-geometry output offsets and sizes"""
+    nsets = len(modesets)
+    if mode_set_of is None:
+        mode_set_of = np.zeros(nconf, dtype=np.int32)
+    mode_set_of = np.ascontiguousarray(mode_set_of, dtype=np.int32)
+    if mode_set_of.shape != (nconf,):
+        raise L.DimensionMismatch(f"mode_set_of has shape {mode_set_of.shape}, expected {(nconf,)}")
+    if np.any(mode_set_of < 0) or np.any(mode_set_of >= nsets):
+        raise L.ArgumentError("mode_set_of entries must index modesets")
+    sizes = np.array([modesets[s].n * nf for s in mode_set_of], dtype=np.int64)
+    offs = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64))
+    return mode_set_of, offs, sizes
+
+
+def _batch_sources(m, coords, modesets, emits, diskR, f0):
+    """marshalled mode sets and one axion source per set (default: axion_induced_modes, uniform field)"""
+    nsets = len(modesets)
+    if emits is None:
+        emits = [axion_induced_modes(coords, md, B=None, diskR=diskR, f=f0) for md in modesets]
+    if len(emits) != nsets:
+        raise L.DimensionMismatch(f"{len(emits)} emits for {nsets} mode sets")
+    mds = (L.ModesS * nsets)(*[_modes_struct(m, md) for md in modesets])
+    em = []
+    for md, e in zip(modesets, emits):
+        a = m.hold(L.as_c128(np.asarray(e).reshape(-1)))
+        if len(a) != md.n:
+            raise L.DimensionMismatch(f"emit of length {len(a)} for a mode set of {md.n} modes")
+        em.append(a.ctypes.data)
+    return mds, (ctypes.c_void_p * nsets)(*em)
+
+
+class TransformerBatchPlan:
+    """Device-resident disk-position sweep (transformer_batch) for an optimiser
+    that evaluates many candidate geometries against fixed modes
+    (examples/transformer_optim_utilities.jl:105,121,202): the grid, the mode sets,
+    their axion sources (emits) and the frequencies are uploaded once; each
+    `run(bdrys, mode_set_of)` takes the candidate setups from the host and
+    computes every boost on the device; `fetch()` returns one (nfreq, n) array per
+    geometry of the last run.  Like transformer, NaN permittivities of each setup
+    are replaced by 1e30 in place."""
+
+    def __init__(self, coords, modesets, f, prop=propagator, diskR=0.15, emits=None, precision="fp64", ctx=None):
+        self.ctx = ctx or L.context()
+        self.modesets = list(modesets) if isinstance(modesets, (list, tuple)) else [modesets]
+        self.freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(f, dtype=np.float64)))
+        self.coords = coords
+        m = L.Marshalled()
+        c = m.coords(coords)
+        mds, emp = _batch_sources(m, coords, self.modesets, emits, diskR, float(self.freqs[0]))
+        h = ctypes.c_void_p()
+        L.check(L.lib().bf_transformer_batch_plan_create(self.ctx.handle, ctypes.byref(c), len(self.modesets), mds,
+                                                         emp, L.cptr(self.freqs), len(self.freqs), prop_kind(prop),
+                                                         float(diskR), precision_code(precision), ctypes.byref(h)),
+                self.ctx.handle)
+        self.handle = h
+        self._layout = None
+
+    def run(self, bdrys, mode_set_of=None):
+        nconf = len(bdrys)
+        mode_set_of, offs, sizes = _batch_layout(nconf, self.modesets, mode_set_of, len(self.freqs))
+        m = L.Marshalled()
+        for b in bdrys:
+            _nan_to_1e30(b)
+        setups = (L.Setup * max(nconf, 1))(*[m.setup(b, self.coords.ny, self.coords.nx) for b in bdrys])
+        L.check(L.lib().bf_transformer_batch_plan_run(self.handle, nconf, setups, L.cptr(mode_set_of), L.cptr(offs)),
+                self.ctx.handle)
+        self._layout = (offs, sizes)
+
+    def fetch(self):
+        if self._layout is None:
+            raise L.ArgumentError("fetch() before run()")
+        offs, sizes = self._layout
+        out = np.zeros(int(sizes.sum()), dtype=np.complex128)
+        L.check(L.lib().bf_transformer_batch_plan_fetch(self.handle, L.cptr(out)), self.ctx.handle)
+        return [out[o:o + sz].reshape(len(self.freqs), -1) for o, sz in zip(offs, sizes)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().bf_transformer_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=propagator, diskR=0.15, emits=None,
                       precision="fp64"):
     """The disk-position sweep of examples/transformer_optim_utilities.jl:105,121,202
@@ -309,35 +398,16 @@ def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=
     if not isinstance(modesets, (list, tuple)):
         modesets = [modesets]
     nconf, nsets = len(bdrys), len(modesets)
-    if mode_set_of is None:
-        mode_set_of = np.zeros(nconf, dtype=np.int32)
-    mode_set_of = np.ascontiguousarray(mode_set_of, dtype=np.int32)
-    if mode_set_of.shape != (nconf,):
-        raise L.DimensionMismatch(f"mode_set_of has shape {mode_set_of.shape}, expected {(nconf,)}")
-    if np.any(mode_set_of < 0) or np.any(mode_set_of >= nsets):
-        raise L.ArgumentError("mode_set_of entries must index modesets")
     freqs = np.atleast_1d(np.asarray(f, dtype=np.float64))
-    if emits is None:
-        emits = [axion_induced_modes(coords, md, B=None, diskR=diskR, f=float(freqs[0])) for md in modesets]
-    if len(emits) != nsets:
-        raise L.DimensionMismatch(f"{len(emits)} emits for {nsets} mode sets")
+    mode_set_of, offs, sizes = _batch_layout(nconf, modesets, mode_set_of, len(freqs))
     m = L.Marshalled()
     c = m.coords(coords)
+    mds, emp = _batch_sources(m, coords, modesets, emits, diskR, float(freqs[0]))
     for b in bdrys:
         _nan_to_1e30(b)
     setups = (L.Setup * max(nconf, 1))(*[m.setup(b, coords.ny, coords.nx) for b in bdrys])
-    mds = (L.ModesS * nsets)(*[_modes_struct(m, md) for md in modesets])
-    em = []
-    for md, e in zip(modesets, emits):
-        a = m.hold(L.as_c128(np.asarray(e).reshape(-1)))
-        if len(a) != md.n:
-            raise L.DimensionMismatch(f"emit of length {len(a)} for a mode set of {md.n} modes")
-        em.append(a.ctypes.data)
-    emp = (ctypes.c_void_p * nsets)(*em)
     fr = m.hold(L.as_f64(freqs))
     nf = len(fr)
-    sizes = np.array([modesets[s].n * nf for s in mode_set_of], dtype=np.int64)
-    offs = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64))
     out = np.zeros(int(sizes.sum()), dtype=np.complex128)
     ctx = L.context()
     L.check(L.lib().bf_transformer_batch(ctx.handle, nconf, setups, ctypes.byref(c), nsets, mds, emp,

@@ -478,6 +478,55 @@ int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, con
     });
 }
 
+int bf_transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* coords, int32_t nsets, const bf_modes* modesets,
+                                     const double* const* emits, const double* freqs, int32_t nfreq,
+                                     int32_t prop_kind, double diskR, int32_t precision,
+                                     bf_transformer_batch_plan** out) {
+    BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nsets >= 1, BF_ERR_ARG, "nsets must be >= 1");
+    BF_REQUIRE(ctx, modesets && emits, BF_ERR_ARG, "modesets and emits must not be NULL");
+    for (int32_t m = 0; m < nsets; m++) {
+        TRY(check_modes(ctx, &modesets[m]));
+        BF_REQUIRE(ctx, emits[m] != nullptr, BF_ERR_ARG, "emits[m] is NULL");
+    }
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        *out = bf::transformer_batch_plan_create(ctx, coords, nsets, modesets, emits, freqs, nfreq, prop_kind, diskR);
+    });
+}
+
+int bf_transformer_batch_plan_run(bf_transformer_batch_plan* plan, int32_t nconf, const bf_setup* setups,
+                                  const int32_t* mode_set_of, const int64_t* boost_offset) {
+    BF_REQUIRE(nullptr, plan != nullptr, BF_ERR_ARG, "plan is NULL");
+    bf_ctx* ctx = plan->ctx;
+    const int32_t nsets = (int32_t)plan->md.size();
+    BF_REQUIRE(ctx, nconf >= 0, BF_ERR_ARG, "nconf must be >= 0");
+    BF_REQUIRE(ctx, nconf == 0 || (setups && mode_set_of && boost_offset), BF_ERR_ARG,
+               "setups, mode_set_of and boost_offset must not be NULL");
+    for (int32_t c = 0; c < nconf; c++) {
+        TRY(check_setup(ctx, &setups[c], 2, false));
+        BF_REQUIRE(ctx, mode_set_of[c] >= 0 && mode_set_of[c] < nsets, BF_ERR_ARG, "mode_set_of[c] out of range");
+        BF_REQUIRE(ctx, boost_offset[c] >= 0, BF_ERR_ARG, "boost_offset[c] is negative");
+    }
+    return guarded(ctx, [&] { bf::transformer_batch_plan_run(plan, nconf, setups, mode_set_of, boost_offset); });
+}
+
+int bf_transformer_batch_plan_fetch(bf_transformer_batch_plan* plan, double* boost) {
+    BF_REQUIRE(nullptr, plan != nullptr, BF_ERR_ARG, "plan is NULL");
+    BF_REQUIRE(plan->ctx, boost != nullptr || plan->blocks.empty(), BF_ERR_ARG, "boost is NULL");
+    return guarded(plan->ctx, [&] { bf::transformer_batch_plan_fetch(plan, boost); });
+}
+
+int bf_transformer_batch_plan_destroy(bf_transformer_batch_plan* plan) {
+    if (plan) bf::transformer_batch_plan_destroy(plan);
+    return BF_OK;
+}
+
 int bf_disk_system_1d(bf_ctx* ctx, const double* freqs, int32_t nfreq, int32_t ncand, int32_t num_disk,
                       const double* spacings, double disk_thickness, const double* eps_disk, const double* eps_gap,
                       int32_t mirror, const double* surfaceloss, double* gamma, double* axion) {

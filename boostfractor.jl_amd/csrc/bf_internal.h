@@ -241,6 +241,13 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                            const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
                            double* boost);
+bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int nsets,
+                                                         const bf_modes* modesets, const double* const* emits,
+                                                         const double* freqs, int nfreq, int kind, double diskR);
+void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,
+                                const int32_t* set_of, const int64_t* boost_off);
+void transformer_batch_plan_fetch(bf_transformer_batch_plan* pl, double* boost);
+void transformer_batch_plan_destroy(bf_transformer_batch_plan* pl);
 void propagation_matrices_grid_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* modes,
                                    const double* spacing, int ns, const double* tgx, int ntx, const double* tgy,
                                    int nty, const double* freqs, int nf, int kind, double diskR, double* out);

@@ -57,10 +57,18 @@ struct PwDesc {
 struct ParsevalQueue {
     std::vector<PwDesc> pd;
     std::vector<zd*> dst;
+    // flat untilted disks waiting for one k_disk_diag launch: matrix slot (counted
+    // in n*n blocks from diag_base), from_gram flag, k0^2 and thickness
+    zd* diag_base = nullptr;
+    std::vector<int> dslot, dfrom;
+    std::vector<zd> dk0sq;
+    std::vector<double> ddz;
 };
 
 // P = sum_g W_g H_g for every queued problem (GEMMs in chunks bounded by memory); empties the queue
 void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q);
+// the queued disk regions of `q` (diag(exp(-i kz dz)) from the left) in one launch
+void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q);
 
 // one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
 struct ChainStep {
@@ -120,6 +128,23 @@ struct bf_transformer_plan {
     double diskR = 0.0;
     bool has_reflect = false;
     bf::ModeCache mc;
+};
+
+// Device-resident disk-position sweep: grid, mode sets (patterns), their axion
+// sources and the frequencies stay on the device; each run takes the candidate
+// geometries from the host and leaves every boost vector in `dboost`.
+struct bf_transformer_batch_plan {
+    bf_ctx* ctx = nullptr;
+    bf::Grid g;
+    std::vector<bf::ModesDev> md;  // per mode set
+    std::vector<bf::DevBuf> de0;   // per mode set: axion source, complex [n]
+    std::vector<bf::ModeCache> mc; // per mode set: the k-bin grouping (grid only) kept across runs
+    std::vector<double> freqs;
+    int kind = 0;
+    double diskR = 0.0;
+    bf::DevBuf dboost;  // complex, the last run's results at the caller's offsets
+    size_t nboost = 0;  // extent of the last run's results (complex elements)
+    std::vector<std::pair<size_t, size_t>> blocks;  // the last run's (offset, length) per geometry
 };
 
 // cubic B-spline coefficients of a propagation-matrix grid over the spacing axis

@@ -24,6 +24,8 @@
 
 #include <rocblas/rocblas.h>
 
+#include <memory>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -1382,7 +1384,15 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             BF_HIP(hipStreamSynchronize(s));
         }
         // diag(exp(-i kz dz)) from the left for the disk regions
-        if (!slots.empty()) {
+        if (!slots.empty() && queue && queue->diag_base) {
+            const int64_t b = (int64_t)((P - queue->diag_base) / (ptrdiff_t)nn);
+            for (size_t q = 0; q < slots.size(); q++) {
+                queue->dslot.push_back((int)(b + slots[q]));
+                queue->dfrom.push_back(from_gram[q]);
+                queue->dk0sq.push_back(k0sq[q]);
+                queue->ddz.push_back(dzs[q]);
+            }
+        } else if (!slots.empty()) {
             if (dslots.bytes < sizeof(int) * slots.size()) dslots.alloc(sizeof(int) * slots.size());
             if (dflag.bytes < sizeof(int) * slots.size()) dflag.alloc(sizeof(int) * slots.size());
             if (dk0.bytes < sizeof(zd) * slots.size()) dk0.alloc(sizeof(zd) * slots.size());
@@ -1434,7 +1444,31 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         P += (size_t)nb * nreg * nn;
     }
     if (!queue) parseval_flush(ctx, mc, n, 1.0 / (double)((int64_t)g.nx * g.ny), local_q);
-    BF_HIP(hipStreamSynchronize(s));
+    // local buffers are released on return: wait for the work that reads them (with a
+    // queue and none of them, the caller's flush and chains follow in stream order)
+    if (!queue || Mom.p || ktabs.p || rtabs.p || scratch.p || prop.p || scr2.p) BF_HIP(hipStreamSynchronize(s));
+}
+
+void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q) {
+    if (q.dslot.empty()) return;
+    hipStream_t s = ctx->stream;
+    const size_t m = q.dslot.size();
+    DevBuf& dslots = scratch_buf(ctx, SB_SLOTS, sizeof(int) * m);
+    DevBuf& dflag = scratch_buf(ctx, SB_FLAG, sizeof(int) * m);
+    DevBuf& dk0 = scratch_buf(ctx, SB_K0, sizeof(zd) * m);
+    DevBuf& ddz = scratch_buf(ctx, SB_DZ, sizeof(double) * m);
+    upload(dslots.p, q.dslot.data(), sizeof(int) * m, s);
+    upload(dflag.p, q.dfrom.data(), sizeof(int) * m, s);
+    upload(dk0.p, q.dk0sq.data(), sizeof(zd) * m, s);
+    upload(ddz.p, q.ddz.data(), sizeof(double) * m, s);
+    hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)m), dim3(256), 0, s, q.diag_base, dslots.as<int>(), dflag.as<int>(),
+                       mc.G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), md.n);
+    BF_CHECK_LAUNCH();
+    BF_HIP(hipStreamSynchronize(s));  // the queue's host arrays are cleared next
+    q.dslot.clear();
+    q.dfrom.clear();
+    q.dk0sq.clear();
+    q.ddz.clear();
 }
 
 void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q) {
@@ -1785,34 +1819,72 @@ void transformer_plan_destroy(bf_transformer_plan* pl) { delete pl; }
 // Many geometries (each its own setup and mode set) at the same frequencies.
 // Geometries are taken mode set by mode set, in chunks whose propagation
 // matrices fit a memory budget; every chunk runs its matrices' inverses and
-// all its (geometry, frequency) chains in one launch each.  The mode set's
-// patterns, pattern FFTs and flat-disk Gram matrix are built once.
-void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
-                           const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
-                           const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
-                           double* boost) {
+// all its (geometry, frequency) chains in one launch each.  The plan keeps the
+// grid, the mode sets' patterns, their axion sources and the frequencies on
+// the device, and the k-bin grouping of the grid once built; every run
+// recomputes the pattern FFTs, group Grams and flat-disk Gram matrix of each
+// mode set it uses (once per set and run).
+bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int nsets,
+                                                         const bf_modes* modesets, const double* const* emits,
+                                                         const double* freqs, int nfreq, int kind, double diskR) {
     hipStream_t s = ctx->stream;
-    Grid g;
-    g.upload(co, s);
-    const size_t K = g.K();
+    auto* pl = new bf_transformer_batch_plan;
+    try {
+        pl->ctx = ctx;
+        pl->g.upload(co, s);
+        const size_t K = pl->g.K();
+        pl->md.resize(nsets);
+        pl->de0.resize(nsets);
+        pl->mc.resize(nsets);
+        for (int set = 0; set < nsets; set++) {
+            pl->md[set].upload(&modesets[set], K, s);
+            pl->de0[set].alloc(sizeof(zd) * pl->md[set].n);
+            upload(pl->de0[set].p, emits[set], sizeof(zd) * pl->md[set].n, s);
+        }
+        pl->freqs.assign(freqs, freqs + nfreq);
+        pl->kind = kind;
+        pl->diskR = diskR;
+        BF_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete pl;
+        throw;
+    }
+    return pl;
+}
+
+void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,
+                                const int32_t* set_of, const int64_t* boost_off) {
+    bf_ctx* ctx = pl->ctx;
+    hipStream_t s = ctx->stream;
+    const Grid& g = pl->g;
+    const int nfreq = (int)pl->freqs.size();
+    const double* freqs = pl->freqs.data();
+    // results: geometry c's [nfreq][n] block at complex element boost_off[c]
+    size_t total = 0;
+    pl->blocks.clear();
+    for (int c = 0; c < nconf; c++) {
+        const size_t len = (size_t)pl->md[set_of[c]].n * nfreq;
+        pl->blocks.push_back({(size_t)boost_off[c], len});
+        total = std::max(total, (size_t)boost_off[c] + len);
+    }
+    if (pl->dboost.bytes < sizeof(zd) * total) pl->dboost.alloc(sizeof(zd) * total);
+    pl->nboost = total;
     // bytes of P + Pinv per chunk (BF_BATCH_BYTES overrides, e.g. to exercise chunking on small cases)
     size_t budget = (size_t)2 << 30;
     if (const char* e = std::getenv("BF_BATCH_BYTES")) {
         const long long v = std::atoll(e);
         if (v > 0) budget = (size_t)v;
     }
-    for (int set = 0; set < nsets; set++) {
+    for (int set = 0; set < (int)pl->md.size(); set++) {
         std::vector<int> confs;
         for (int c = 0; c < nconf; c++)
             if (set_of[c] == set) confs.push_back(c);
         if (confs.empty()) continue;
-        ModesDev md;
-        md.upload(&modesets[set], K, s);
-        ModeCache cache;
+        const ModesDev& md = pl->md[set];
+        ModeCache& cache = pl->mc[set];
+        cache.reset_data();  // pattern FFTs and Gram tables are rebuilt every run
         const int n = md.n;
         const size_t nn = (size_t)n * n;
-        DevBuf de0(sizeof(zd) * n);
-        upload(de0.p, emits[set], sizeof(zd) * n, s);
         for (size_t c0 = 0; c0 < confs.size();) {
             // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry)
             size_t c1 = c0, mats = 0;
@@ -1830,17 +1902,18 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
             std::vector<ChainStep> steps;
             size_t base = 0;
             ParsevalQueue pq;
+            pq.diag_base = P.as<zd>();
             for (size_t q = c0; q < c1; q++) {
                 const bf_setup* st = &setups[confs[q]];
                 const int nreg = st->nreg;
                 DevBuf surf;
                 std::vector<bool> flat;
-                upload_surfaces(ctx, st, K, surf, flat);
+                upload_surfaces(ctx, st, g.K(), surf, flat);
                 std::vector<zd> eps = eps_fixed(st);
                 std::vector<int> regs;
                 for (int r = 1; r < nreg; r++) regs.push_back(r);
-                propagation_matrices_core(ctx, g, md, kind, diskR, nreg, st->distance, eps.data(), st->tilt_x,
-                                          st->tilt_y, surf.as<zd>(), flat, freqs, nfreq, regs,
+                propagation_matrices_core(ctx, g, md, pl->kind, pl->diskR, nreg, st->distance, eps.data(),
+                                          st->tilt_x, st->tilt_y, surf.as<zd>(), flat, freqs, nfreq, regs,
                                           P.as<zd>() + base * nn, &cache, &pq);
                 for (int f = 0; f < nfreq; f++) {
                     for (int r = 1; r < nreg; r++) sel.push_back((int)(base + (size_t)f * nreg + r));
@@ -1856,16 +1929,50 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                 }
                 base += (size_t)nreg * nfreq;
             }
-            parseval_flush(ctx, cache, n, 1.0 / (double)K, pq);
-            inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, de0.as<zd>(), nullptr,
+            parseval_flush(ctx, cache, n, 1.0 / (double)g.K(), pq);
+            disk_diag_flush(ctx, cache, md, pq);
+            inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, pl->de0[set].as<zd>(), nullptr,
                                dboost.as<zd>(), nullptr);
             for (size_t q = c0; q < c1; q++)
-                download(boost + 2 * boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
-                         sizeof(zd) * n * nfreq, s);
+                BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
+                                      sizeof(zd) * n * nfreq, hipMemcpyDeviceToDevice, s));
+            // the chunk's host-side job lists and the scratch buffers are reused by the next chunk
             BF_HIP(hipStreamSynchronize(s));
             c0 = c1;
         }
     }
+}
+
+// copies the last run's geometry blocks to the same offsets of `boost` (the
+// elements between blocks are left as they are); runs of adjacent blocks go
+// in one copy
+void transformer_batch_plan_fetch(bf_transformer_batch_plan* pl, double* boost) {
+    hipStream_t s = pl->ctx->stream;
+    std::vector<std::pair<size_t, size_t>> b = pl->blocks;
+    std::sort(b.begin(), b.end());
+    for (size_t i = 0; i < b.size();) {
+        size_t lo = b[i].first, hi = b[i].first + b[i].second;
+        size_t j = i + 1;
+        while (j < b.size() && b[j].first <= hi) {
+            hi = std::max(hi, b[j].first + b[j].second);
+            j++;
+        }
+        if (hi > lo) download(boost + 2 * lo, pl->dboost.as<zd>() + lo, sizeof(zd) * (hi - lo), s);
+        i = j;
+    }
+    BF_HIP(hipStreamSynchronize(s));
+}
+
+void transformer_batch_plan_destroy(bf_transformer_batch_plan* pl) { delete pl; }
+
+void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const bf_coords* co, int nsets,
+                           const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
+                           const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
+                           double* boost) {
+    std::unique_ptr<bf_transformer_batch_plan> pl(
+        transformer_batch_plan_create(ctx, co, nsets, modesets, emits, freqs, nfreq, kind, diskR));
+    transformer_batch_plan_run(pl.get(), nconf, setups, set_of, boost_off);
+    transformer_batch_plan_fetch(pl.get(), boost);
 }
 
 // transformer_trace_back (:372-421): one block walks the regions applying G_s P_s

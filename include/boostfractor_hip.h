@@ -244,6 +244,25 @@ int bf_transformer_batch(bf_ctx *ctx, int32_t nconf, const bf_setup *setups, con
                          int32_t nsets, const bf_modes *modesets, const double *const *emits,
                          const int32_t *mode_set_of, const double *freqs, int32_t nfreq, int32_t prop_kind,
                          double diskR, int32_t precision, const int64_t *boost_offset, double *boost);
+/* bf_transformer_batch with the grid, the mode sets, their axion sources and the
+ * frequencies kept on the device (the same sweep, examples/
+ * transformer_optim_utilities.jl:105,121,202, for an optimiser that evaluates
+ * many candidate geometries against fixed modes): create uploads them once;
+ * each run takes the candidate setups (and their mode set and output offset, as
+ * bf_transformer_batch) from the host and computes every boost vector on the
+ * device, recomputing the mode sets' pattern FFTs and Gram tables it needs;
+ * fetch copies the last run's geometry blocks to the same offsets of `boost`
+ * (elements between blocks are not written).  Results equal
+ * bf_transformer_batch's. */
+typedef struct bf_transformer_batch_plan bf_transformer_batch_plan;
+int bf_transformer_batch_plan_create(bf_ctx *ctx, const bf_coords *coords, int32_t nsets, const bf_modes *modesets,
+                                     const double *const *emits, const double *freqs, int32_t nfreq,
+                                     int32_t prop_kind, double diskR, int32_t precision,
+                                     bf_transformer_batch_plan **out);
+int bf_transformer_batch_plan_run(bf_transformer_batch_plan *plan, int32_t nconf, const bf_setup *setups,
+                                  const int32_t *mode_set_of, const int64_t *boost_offset);
+int bf_transformer_batch_plan_fetch(bf_transformer_batch_plan *plan, double *boost);
+int bf_transformer_batch_plan_destroy(bf_transformer_batch_plan *plan);
 
 /* ---- 1D analytical model (src/Analytical1D.py) ------------------------- */
 /* disk_system (src/Analytical1D.py:82-131), ray-tracing form, for ncand geometries

@@ -384,6 +384,32 @@ def test_transformer_batch_matches_single_geometry(monkeypatch, chunk_bytes):
                 assert err < 10 * sens, (c, q, err, sens)
 
 
+@pytest.mark.parametrize("chunk_bytes", [None, "60000"])
+def test_transformer_batch_plan_matches_batch(monkeypatch, chunk_bytes):
+    """TransformerBatchPlan (mode sets, sources and frequencies resident) equals
+    transformer_batch bit for bit, on two runs in a row, the second over fewer
+    geometries in another order and mode-set assignment; fetch() returns the last
+    run's geometries only; with a small matrix budget every run is chunked"""
+    if chunk_bytes:
+        monkeypatch.setenv("BF_BATCH_BYTES", chunk_bytes)
+    coords, geoms = _batch_geometries()
+    sets = [_lg_like_modes(coords, 2, 1), _lg_like_modes(coords, 3, 1, seed=5)]
+    emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
+    freqs = [19.6e9, 20.0e9, 20.4e9]
+    plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
+    for pick, set_of in [(list(range(6)), [0, 1, 1, 0, 1, 0]), ([4, 1, 5], [1, 0, 1])]:
+        sel = [geoms[i].copy() for i in pick]
+        want = B.transformer_batch([g.copy() for g in sel], coords, sets, f=freqs, mode_set_of=set_of,
+                                   diskR=DISKR, emits=emits)
+        plan.run(sel, mode_set_of=set_of)
+        got = plan.fetch()
+        assert len(got) == len(pick)
+        for c in range(len(pick)):
+            assert got[c].shape == want[c].shape
+            assert np.array_equal(got[c], want[c]), c
+    plan.close()
+
+
 def test_transformer_batch_matches_oracle():
     """two geometries of a batch against the CPU oracle (matrices and chain)"""
     coords, geoms = _batch_geometries()
