@@ -219,7 +219,8 @@ int bf_unnormalized_match(bf_ctx* ctx, int32_t nfield, int64_t n, const double* 
 
 // Launch-shape and band overrides for tuning and A/B checks: BF_COL_TX
 // (columns per column-pass block), BF_ROWS (rows per dancer row block),
-// BF_BAND=0 (process every row instead of the aperture's row band).
+// BF_BAND=0 (process every row instead of the aperture's row band), BF_GRAPH=0
+// (plans launch every bounce instead of replaying captured graphs).
 static bf::DancerOptions env_options() {
     bf::DancerOptions o;
     if (const char* v = getenv("BF_COL_TX")) o.col_tx = atoi(v);
@@ -293,6 +294,8 @@ int bf_dancer_plan_create(bf_ctx* ctx, const bf_setup* setup, const bf_coords* c
     return guarded(ctx, [&] {
         bf::DancerOptions o = env_options();
         o.freq_batch = freq_batch;
+        const char* gv = getenv("BF_GRAPH");
+        o.graph = (gv && std::string(gv) == "0") ? 0 : 1;  // a plan is run many times
         auto* p = new bf_dancer_plan();
         try {
             p->impl = bf::make_dancer_plan(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, diskR, precision, o);

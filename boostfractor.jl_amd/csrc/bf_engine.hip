@@ -1434,6 +1434,16 @@ void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double*
 template <typename T>
 struct DancerPlan : DancerPlanBase {
     BF_TYPES(T);
+    // The fused bounce loop of one frequency batch (2 (nmax+1) launches) captured
+    // once and replayed by later runs.  Every buffer its kernels read belongs to the
+    // plan and keeps its address, except the engine's phase/tilt tables, which grow
+    // by reallocation: a graph is recaptured when their addresses change.
+    struct BatchGraph {
+        hipGraphExec_t exec = nullptr;
+        const void* ktab = nullptr;
+        const void* rtab = nullptr;
+    };
+    std::vector<BatchGraph> graphs;
     Grid g;
     int kind, R, nmax, nfreq, fb, K_;
     double amin, diskR;
@@ -1457,6 +1467,8 @@ struct DancerPlan : DancerPlanBase {
 
     ~DancerPlan() override {
         for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto& bg : graphs)
+            if (bg.exec) (void)hipGraphExecDestroy(bg.exec);
     }
 
     void tmark(int n, int k, int which) {
@@ -1648,6 +1660,51 @@ struct DancerPlan : DancerPlanBase {
         BF_HIP(hipStreamSynchronize(s));
     }
 
+    // The fused bounces of one frequency batch: column pass, row pass (boundary
+    // update, Eout, next aperture) and the amin test, nmax + 1 times.  Only kernel
+    // launches (no allocation, copy or synchronisation), so a stream can capture it.
+    void bounce_loop(int f0, int nb, zd* eo_target) {
+        hipStream_t s = ctx->stream;
+        const size_t K = g.K();
+        int rm = 0;
+        for (int n = 0; n <= nmax; n++) {
+            tmark(n, 1, 0);
+            // every slot of the table-ordered list (inactive slots return at once)
+            const int ncol = (int)fd_host.size();
+            pe->cols(fields.as<cplx<S>>(), fields.as<cplx<S>>(), fdesc_col.as<FieldDesc>() + (size_t)rm * ncol,
+                     ncol, CF_FWD | CF_KMID | CF_INV, active.as<int>(), band_lo, band_hi);
+            tmark(n, 1, 1);
+            DancerRowArgs<T> a;
+            a.fields = fields.as<cplx<S>>();
+            a.R = R; a.nx = g.nx; a.ny = g.ny; a.rm = rm; a.tw = pe->twx;
+            a.rtab = pe->has_rtab() ? pe->rtab.template as<cplx<C>>() : nullptr;
+            a.rtab_idx = ridx.as<int>();
+            a.coef = coef.as<zd>();
+            a.X = g.X.as<double>(); a.Y = g.Y.as<double>(); a.R2 = diskR * diskR;
+            a.keep_tab = keep_tab.as<uint32_t>();
+            a.keep_band = keep_band;
+            a.flags = (n < nmax ? DF_FWD : 0u) | (amin != 0.0 ? DF_AMAX : 0u) |
+                      (nslices > 1 ? DF_EOUT_ASSIGN : 0u);
+            a.Eout = eo_target;
+            a.eout_fstride = K * nslices;
+            a.slice = nslices > 1 ? n : 0;
+            a.amax = amax.as<double>();
+            a.active = active.as<int>();
+            a.count = count.as<int>() + f0;
+            a.ylo = band_lo;
+            a.yhi = band_hi;
+            tmark(n, 0, 0);
+            dispatch_n<DancerRowLaunch, T>(g.nx, a, nb, opt.rows, s);
+            tmark(n, 0, 1);
+            if (amin != 0.0) {
+                hipLaunchKernelGGL(k_amin_check, dim3(1), dim3(64), 0, s, active.as<int>(), amax.as<double>(),
+                                   amin, nb);
+                BF_CHECK_LAUNCH();
+            }
+            rm = 1 - rm;
+        }
+    }
+
     void run() override {
         hipStream_t s = ctx->stream;
         const size_t K = g.K();
@@ -1730,43 +1787,36 @@ struct DancerPlan : DancerPlanBase {
             int rm = 0;
             if (fused) {
                 pe->rows(fields.as<cplx<S>>(), fd, nf, RF_MASK | RF_FWD, active.as<int>());
-                for (int n = 0; n <= nmax; n++) {
-                    tmark(n, 1, 0);
-                    // every slot of the table-ordered list (inactive slots return at once)
-                    const int ncol = (int)fd_host.size();
-                    pe->cols(fields.as<cplx<S>>(), fields.as<cplx<S>>(), fdesc_col.as<FieldDesc>() + (size_t)rm * ncol,
-                             ncol, CF_FWD | CF_KMID | CF_INV, active.as<int>(), band_lo, band_hi);
-                    tmark(n, 1, 1);
-                    DancerRowArgs<T> a;
-                    a.fields = fields.as<cplx<S>>();
-                    a.R = R; a.nx = g.nx; a.ny = g.ny; a.rm = rm; a.tw = pe->twx;
-                    a.rtab = pe->has_rtab() ? pe->rtab.template as<cplx<C>>() : nullptr;
-                    a.rtab_idx = ridx.as<int>();
-                    a.coef = coef.as<zd>();
-                    a.X = g.X.as<double>(); a.Y = g.Y.as<double>(); a.R2 = diskR * diskR;
-                    a.keep_tab = keep_tab.as<uint32_t>();
-                    a.keep_band = keep_band;
-                    a.flags = (n < nmax ? DF_FWD : 0u) | (amin != 0.0 ? DF_AMAX : 0u) |
-                              (nslices > 1 ? DF_EOUT_ASSIGN : 0u);
-                    a.Eout = eo_target;
-                    a.eout_fstride = K * nslices;
-                    a.slice = nslices > 1 ? n : 0;
-                    a.amax = amax.as<double>();
-                    a.active = active.as<int>();
-                    a.count = count.as<int>() + f0;
-                    a.ylo = band_lo;
-                    a.yhi = band_hi;
-                    tmark(n, 0, 0);
-                    dispatch_n<DancerRowLaunch, T>(g.nx, a, nb, opt.rows, s);
-                    tmark(n, 0, 1);
-                    if (amin != 0.0) {
-                        hipLaunchKernelGGL(k_amin_check, dim3(1), dim3(64), 0, s, active.as<int>(), amax.as<double>(),
-                                           amin, nb);
-                        BF_CHECK_LAUNCH();
+                if (opt.graph && !timing) {
+                    const size_t bi = (size_t)(f0 / fb);
+                    if (graphs.size() <= bi) graphs.resize(bi + 1);
+                    BatchGraph& bg = graphs[bi];
+                    if (bg.exec && (bg.ktab != pe->ktab.p || bg.rtab != pe->rtab.p)) {
+                        BF_HIP(hipGraphExecDestroy(bg.exec));
+                        bg.exec = nullptr;
                     }
-                    rm = 1 - rm;
+                    if (!bg.exec) {
+                        BF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                        hipGraph_t gr = nullptr;
+                        try {
+                            bounce_loop(f0, nb, eo_target);
+                        } catch (...) {
+                            (void)hipStreamEndCapture(s, &gr);
+                            if (gr) (void)hipGraphDestroy(gr);
+                            throw;
+                        }
+                        BF_HIP(hipStreamEndCapture(s, &gr));
+                        const hipError_t e = hipGraphInstantiate(&bg.exec, gr, nullptr, nullptr, 0);
+                        (void)hipGraphDestroy(gr);
+                        BF_HIP(e);
+                        bg.ktab = pe->ktab.p;
+                        bg.rtab = pe->rtab.p;
+                    }
+                    BF_HIP(hipGraphLaunch(bg.exec, s));
+                } else {
+                    bounce_loop(f0, nb, eo_target);
+                    tcollect(nmax + 1);
                 }
-                tcollect(nmax + 1);
             } else {
                 for (int n = 0; n <= nmax; n++) {
                     pe->propagate(fields.as<cplx<S>>(), fd, nf, ph1d.as<zd>(), scratch.as<cplx<S>>(),

@@ -182,6 +182,7 @@ struct DancerOptions {
     int freq_batch = 0;
     int col_tx = 0, rows = 0;  // launch shapes (0: defaults)
     int band = 1;              // skip the rows the aperture zeroes (0: process every row)
+    int graph = 0;             // replay each frequency batch's bounce loop from a captured HIP graph
 };
 
 struct DancerPlanBase {

@@ -304,3 +304,35 @@ def test_dancer_quarter_tables_match_full(monkeypatch, N, prec):
     assert np.any(out["1"] != 0)
     tol = {"mixed": 1e-5, "fp32": 1e-4}[prec]
     assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
+
+
+@pytest.mark.parametrize("N,prec,amin", [(64, "fp64", 0.0), (128, "mixed", 0.0), (64, "mixed", 0.05)])
+def test_dancer_plan_graph_replays_match_eager(monkeypatch, N, prec, amin):
+    """Plans replay each frequency batch's bounce loop from a HIP graph captured by
+    the first run: three runs in a row, with eager work and synchronisations
+    between them, each equal bit for bit to the plan launched bounce by bounce;
+    5 frequencies in batches of 2 (three graphs, the last batch partial); the
+    instrumented (timing) run takes the launch-by-launch path and agrees too."""
+    coords, sb = _small_booster(N=N, ndisk=3, seed=1006)
+    freqs = [19.7e9, 19.9e9, 20.0e9, 20.2e9, 20.4e9]
+    monkeypatch.setenv("BF_GRAPH", "0")
+    eager = B.DancerPlan(amin, 30, sb, coords, freqs, diskR=DISKR, precision=prec, freq_batch=2)
+    eager.run()
+    want_E, want_p = eager.fetch()
+    eager.close()
+    monkeypatch.setenv("BF_GRAPH", "1")
+    plan = B.DancerPlan(amin, 30, sb, coords, freqs, diskR=DISKR, precision=prec, freq_batch=2)
+    for rep in range(3):
+        plan.run()
+        E, p = plan.fetch()
+        assert np.any(E != 0)
+        assert np.array_equal(E, want_E), rep
+        assert np.array_equal(p, want_p), rep
+        B.dancer(0, 4, sb, coords, f=20e9, diskR=DISKR)  # eager work on the same stream in between
+        B.context().synchronize()
+    plan.timing(True)
+    plan.run()
+    plan.timing(False)
+    E, p = plan.fetch()
+    assert np.array_equal(E, want_E)
+    plan.close()
