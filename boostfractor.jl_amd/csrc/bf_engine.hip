@@ -1787,7 +1787,7 @@ struct DancerPlan : DancerPlanBase {
             int rm = 0;
             if (fused) {
                 pe->rows(fields.as<cplx<S>>(), fd, nf, RF_MASK | RF_FWD, active.as<int>());
-                if (opt.graph && !timing) {
+                if (opt.graph && !timing && nmax >= 0) {
                     const size_t bi = (size_t)(f0 / fb);
                     if (graphs.size() <= bi) graphs.resize(bi + 1);
                     BatchGraph& bg = graphs[bi];

@@ -158,3 +158,23 @@ def test_oracle_not_imported_by_product():
         for f in files:
             if f.endswith((".py", ".hip", ".h", ".cpp", "Makefile")) or f == "Makefile":
                 assert not pat.search(open(os.path.join(dirpath, f)).read()), f
+
+
+def test_batch_plan_validation_without_device():
+    """the resident sweep plan's entry points reject bad arguments on the host"""
+    lib = L.lib()
+    m, c = _coords_struct()
+    fr = np.array([20e9])
+    h = ctypes.c_void_p()
+    # no mode set
+    rc = lib.bf_transformer_batch_plan_create(None, ctypes.byref(c), 0, None, None, L.cptr(fr), 1, 0, 0.15, 0,
+                                              ctypes.byref(h))
+    assert rc == L.BF_ERR_ARG and b"nsets" in lib.bf_last_error(None)
+    assert not h.value
+    # NULL output handle
+    rc = lib.bf_transformer_batch_plan_create(None, ctypes.byref(c), 1, None, None, L.cptr(fr), 1, 0, 0.15, 0, None)
+    assert rc == L.BF_ERR_ARG
+    # NULL plan
+    assert lib.bf_transformer_batch_plan_run(None, 0, None, None, None) == L.BF_ERR_ARG
+    assert lib.bf_transformer_batch_plan_fetch(None, None) == L.BF_ERR_ARG
+    assert lib.bf_transformer_batch_plan_destroy(None) == L.BF_OK
