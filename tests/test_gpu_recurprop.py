@@ -278,9 +278,11 @@ def test_dancer_pruned_transforms_match_unpruned(monkeypatch, N, span, prec):
         assert rel(out["1"][q], out["0"][q]) < tol, (q, rel(out["1"][q], out["0"][q]))
 
 
-def test_dancer_pruned_matches_oracle():
-    """The pruned path (128 x 128, tilted and rough disks) against the oracle."""
-    coords, sb = _small_booster(N=128, ndisk=2, seed=1004)
+@pytest.mark.parametrize("N", [128, 256])
+def test_dancer_pruned_matches_oracle(N):
+    """The pruned path (tilted and rough disks; 256 x 256 is the benchmark's grid)
+    against the oracle."""
+    coords, sb = _small_booster(N=N, ndisk=2, seed=1004)
     freqs = [19.9e9, 20.2e9]
     got = B.dancer(0, 15, sb, coords, f=freqs, diskR=DISKR)
     os_, oc = osetup(sb), ocoords(coords)
