@@ -121,11 +121,13 @@ def timed_steps(plan, ctx, steps, warmup, timing):
     for _ in range(warmup):
         plan.run()
     ctx.synchronize()
+    print(f"[bench]   {warmup} warmup run(s) done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.run()
     ctx.synchronize()
     dt = time.perf_counter() - t0
+    print(f"[bench]   {steps} timed run(s) done", file=sys.stderr, flush=True)
     kt = None
     if timing:
         plan.timing(True)
@@ -420,6 +422,7 @@ def model1d_main(args):
 
 
 def main():
+    global N, NFREQ, NMAX  # the dancer512 workload resizes the dancer sweep
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -431,9 +434,12 @@ def main():
                     help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
-    ap.add_argument("--workload", choices=("dancer", "transformer", "sweep", "model1d"), default="dancer",
-                    help="dancer: the headline sweep (default); transformer: the mode-matching sweep; "
-                         "sweep: the many-geometry disk-position sweep; model1d: the 1D analytical model")
+    ap.add_argument("--workload", choices=("dancer", "dancer512", "transformer", "sweep", "model1d"),
+                    default="dancer",
+                    help="dancer: the headline sweep (default); dancer512: the same booster at 512x512, 100 "
+                         "frequencies, 3000 bounces (run it with --steps 1 --warmup 0: ~1 min per sweep); "
+                         "transformer: the mode-matching sweep; sweep: the many-geometry disk-position sweep; "
+                         "model1d: the 1D analytical model")
     ap.add_argument("--nconf", type=int, default=32, help="geometries of the sweep workload")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
@@ -456,6 +462,10 @@ def main():
     if args.workload == "model1d":
         model1d_main(args)
         return
+    if args.workload == "dancer512":
+        N, NFREQ = 512, 100
+        if args.nmax == NMAX:
+            args.nmax = NMAX = 3000
 
     import BoostFractor as B
     from BoostFractor import workloads as W
@@ -504,7 +514,8 @@ def main():
     nb_per_launch = NFREQ * (nmax + 1) / launches  # frequency-bounces per launch (one instrumented run)
     alg_bytes = nb_per_launch * 2 * F * K * 8  # every field read once and written once, complex64
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile("k_cols_reg", fb32) if dom == "col_pass" else None
+    # the committed counter summary covers the 256 x 256 workload only
+    traffic = traffic_from_profile("k_cols_reg", fb32) if (dom == "col_pass" and N == 256) else None
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=10)
     stream_gbs = 2 * (1 << 30) / (bw_ms * 1e-3) / 1e9
 
@@ -522,8 +533,8 @@ def main():
         "dtype": "f64",
         "storage": "complex64 fields in HBM (mixed precision: every FFT, phase and boundary update in f64)",
         "data": "synthetic (seeded SplitMix64 booster geometry, axion source from dance_intro)",
-        "config": {"workload": f"dancer, 20-disk booster eps=24, 256x256 grid over 1 m, 50 freqs 19.5-20.5 GHz, "
-                               f"nmax={nmax}, FFT propagator, one batched device-resident sweep",
+        "config": {"workload": f"dancer, 20-disk booster eps=24, {N}x{N} grid over 1 m, {NFREQ} freqs "
+                               f"19.5-20.5 GHz, nmax={nmax}, FFT propagator, one batched device-resident sweep",
                    "disks": NDISK, "grid": N, "frequencies": NFREQ, "nmax": nmax, "freq_batch": fb32},
         "fp64": fp64,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
