@@ -497,3 +497,30 @@ def test_parseval_groups_match_per_bin_weights(monkeypatch):
         for r in range(1, len(sb.distance)):
             assert rel(got[q][r], per_bin[q][r]) < 1e-12, (q, r)
             assert rel(per_bin[q][r], want[r]) < TOL and rel(got[q][r], want[r]) < TOL, (q, r)
+
+
+def test_transformer_batch_plan_matches_single_geometry_256():
+    """The sweep benchmark's shape (256 x 256, Bessel modes n = 9 and 25, boosters
+    of 12 and 24 disks with spacings 8 mm +- 0.5 mm): the resident batch plan
+    against transformer per geometry, per point within 1e-10 of the boost vector
+    (or the chain's own sensitivity, whichever is larger)."""
+    coords = W.grid_coords(256)
+    rng = W.SplitMix64(5200)
+    geoms = []
+    for nd in (12, 24):
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp)))
+    sets = [B.SeedModes(coords, ThreeDim=True, Mmax=3, Lmax=1, diskR=DISKR),
+            B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=2, diskR=DISKR)]
+    set_of = [1, 0]
+    emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
+    freqs = [19.3e9, 20.0e9, 20.7e9]
+    plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
+    plan.run([g.copy() for g in geoms], mode_set_of=set_of)
+    got = plan.fetch()
+    plan.close()
+    for c, sb in enumerate(geoms):
+        md = sets[set_of[c]]
+        want = B.transformer(sb.copy(), coords, md, f=freqs, diskR=DISKR, emit=emits[set_of[c]])
+        for q in range(len(freqs)):
+            assert rel(got[c][q], want[q]) < 1e-10, (c, q, rel(got[c][q], want[q]))
