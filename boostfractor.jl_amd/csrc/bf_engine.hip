@@ -129,6 +129,8 @@ struct RowArgs {
     const double* Y;
     double R2;
     const int* active;
+    // rows outside [ylo, yhi) are skipped (not read, not written) when yhi > ylo
+    int ylo, yhi;
 };
 
 template <typename T, int N>
@@ -149,6 +151,7 @@ __global__ void __launch_bounds__(1024) k_rows(RowArgs<T> a) {
     const int y = valid ? (int)(line % a.ny) : 0;
     const FieldDesc d = a.fd[j];
     if (valid && a.active && !a.active[d.group]) valid = false;
+    if (a.yhi > a.ylo && (y < a.ylo || y >= a.yhi)) valid = false;
     cplx<S>* row = a.base + d.off + (size_t)y * a.nx;
     for (int x = t; x < n; x += TPL) s[x] = valid ? ccast<C>(row[x]) : cmk<C>(0, 0);
     __syncthreads();
@@ -193,6 +196,7 @@ struct ColArgs {
     // fields whose descriptor carries FD_WHOLE, stored whole.
     int ylo, yhi;
     int kq;  // 0: full k-space tables; else quarter tables of row width kq = nx/2 (dancer propagation)
+    int all_whole;  // every field is stored whole (the band limits the loads only)
 };
 
 template <typename T>
@@ -221,7 +225,7 @@ __global__ void __launch_bounds__(1024) k_cols(ColArgs<T> a) {
     cplx<S>* dst = a.dst + d.off;
     const cplx<C>* kt = (d.ktab >= 0) ? a.ktab + (size_t)d.ktab * ny * nx : nullptr;
     const uint32_t fl = a.flags;
-    const bool whole = store_whole(d);
+    const bool whole = store_whole(d) || a.all_whole;
     for (int i = threadIdx.x; i < tx * n; i += blockDim.x) {
         const int c = i % tx, my = i / tx, mx = x0 + c;
         cplx<C> v = cmk<C>(0, 0);
@@ -457,6 +461,7 @@ __global__ void __launch_bounds__(512) k_rows_reg(RowArgs<T> a) {
     const int y = valid ? (int)(line % a.ny) : 0;
     const FieldDesc d = a.fd[j];
     if (valid && a.active && !a.active[d.group]) valid = false;
+    if (a.yhi > a.ylo && (y < a.ylo || y >= a.yhi)) valid = false;
     const uint32_t K = (uint32_t)N * (uint32_t)a.ny;
     const auto rf = buf_rsrc(a.base + d.off, K * (uint32_t)sizeof(cplx<S>));
     const uint32_t o0 = (uint32_t)y * N + t;
@@ -516,7 +521,7 @@ __global__ void __launch_bounds__(512) k_cols_reg(ColArgs<T> a) {
     const bool has_kt = d.ktab >= 0;
     const uint32_t KT = a.kq ? (uint32_t)a.kq * (uint32_t)(N / 2) : K;  // elements per table
     const auto rk = buf_rsrc(a.ktab + (size_t)(has_kt ? d.ktab : 0) * KT, KT * (uint32_t)sizeof(cplx<C>));
-    const bool whole = store_whole(d);
+    const bool whole = store_whole(d) || a.all_whole;
     const int sx = (fl & CF_INSHIFT) ? (mx + nx / 2) % nx : mx;
     cplx<S>* s = smem + (size_t)c * LDC;
     cplx<C> v[E];
@@ -1201,10 +1206,11 @@ struct PropEngine {
     std::vector<int> ktab_of;  // per region entry: k-space table index
     std::vector<int> rtab_of;  // per region entry: rtab index or -1
     int col_tx = 0;            // autotuned column tile (0: default)
-    // Quarter k-space tables (see k_build_ktab): set by the fused dancer with
-    // complex64 fields on even power-of-two grids; only its register column
-    // pass reads them.
+    // Quarter k-space tables (see k_build_ktab): set for complex64 fields on
+    // power-of-two grids; only the register propagation pass reads them.
     bool quarter = false;
+    // aperture row band [blo, bhi) of propagate() (bhi <= blo: every row)
+    int blo = 0, bhi = 0;
 
     PropEngine(bf_ctx* c, const Grid* gr, int k, double diskR, const zd* surf)
         : ctx(c), g(gr), kind(k), R2(diskR * diskR), surfaces(surf) {
@@ -1285,19 +1291,23 @@ struct PropEngine {
     bool has_rtab() const { return rtab.p != nullptr && !rtab_of.empty() &&
                                    std::any_of(rtab_of.begin(), rtab_of.end(), [](int v) { return v >= 0; }); }
 
-    void rows(cplx<S>* base, const FieldDesc* fd, int nf, uint32_t flags, const int* active) {
+    // ylo/yhi: only rows in [ylo, yhi) when yhi > ylo
+    void rows(cplx<S>* base, const FieldDesc* fd, int nf, uint32_t flags, const int* active, int ylo = 0,
+              int yhi = 0) {
         RowArgs<T> a;
         a.base = base; a.fd = fd; a.nf = nf; a.nx = g->nx; a.ny = g->ny; a.tw = twx; a.flags = flags;
         a.rtab = rtab.as<cplx<C>>(); a.X = g->X.as<double>(); a.Y = g->Y.as<double>(); a.R2 = R2; a.active = active;
+        a.ylo = ylo; a.yhi = yhi;
         dispatch_n<RowLaunch, T>(g->nx, a, ctx->stream);
     }
-    // ylo/yhi: row band (yhi < 0: all rows)
+    // ylo/yhi: row band (yhi < 0: all rows); all_whole: store every row of every field
     void cols(const cplx<S>* src, cplx<S>* dst, const FieldDesc* fd, int nf, uint32_t flags, const int* active,
-              int ylo = 0, int yhi = -1) {
+              int ylo = 0, int yhi = -1, bool all_whole = false) {
         ColArgs<T> a;
         a.src = src; a.dst = dst; a.fd = fd; a.nf = nf; a.nx = g->nx; a.ny = g->ny; a.tw = twy; a.flags = flags;
         a.ktab = ktab.as<cplx<C>>(); a.tx = 0; a.active = active;
         a.kq = quarter ? g->nx / 2 : 0;
+        a.all_whole = all_whole ? 1 : 0;
         a.ylo = ylo; a.yhi = yhi < 0 ? g->ny : yhi;
         dispatch_n<ColLaunch, T>(g->ny, a, col_tx, ctx->stream);
     }
@@ -1309,8 +1319,16 @@ struct PropEngine {
         switch (kind) {
             case BF_PROP_PROPAGATOR:
             case BF_PROP_NOTILTS:
-                rows(base, fd, nf, RF_MASK | RF_FWD, active);
-                cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active);
+                if (bhi > blo) {
+                    // the mask zeroes every row outside the aperture band: the masked row
+                    // FFT skips them and the column pass reads only the band, storing
+                    // every row of the propagated field
+                    rows(base, fd, nf, RF_MASK | RF_FWD, active, blo, bhi);
+                    cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active, blo, bhi, true);
+                } else {
+                    rows(base, fd, nf, RF_MASK | RF_FWD, active);
+                    cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active);
+                }
                 rows(base, fd, nf, RF_INV | (kind == BF_PROP_PROPAGATOR ? RF_RTAB : 0u), active);
                 break;
             case BF_PROP_MOMENTUM:
@@ -1431,6 +1449,43 @@ void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double*
 // ---------------------------------------------------------------------------
 // dancer plan
 // ---------------------------------------------------------------------------
+// Quarter k-space tables for complex64 fields: the phase of a mirrored bin comes
+// from its mirror's k, equal up to the rounding of the k range (~1e-15 rad per
+// propagation), far inside what complex64 storage rounds away; the FP64 path
+// keeps full tables.  Needs the register propagation pass (square power-of-two
+// grid) and a mirror-symmetric k range.  BF_QTAB=0 switches it off.
+template <typename T>
+static bool quarter_tables_ok(const bf_coords* co, int kind) {
+    const char* q = getenv("BF_QTAB");
+    const int n = co->nx;
+    auto mirrored = [](const double* k, int len) {
+        double mx = 0.0;
+        for (int j = 0; j < len; j++) mx = std::max(mx, std::fabs(k[j]));
+        for (int j = 0; j < len; j++)
+            if (!(std::fabs(k[j] + k[len - 1 - j]) <= 1e-12 * mx)) return false;
+        return true;
+    };
+    return !std::is_same<T, double>::value && !(q && std::string(q) == "0") && use_reg_fft() &&
+           (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS) && co->nx == co->ny && n >= 16 && n <= 1024 &&
+           (n & (n - 1)) == 0 && mirrored(co->kX, co->nx) && mirrored(co->kY, co->ny);
+}
+
+// Rows y with Y[y]^2 + min_x X[x]^2 < diskR^2 (a relative margin of 1e-9 keeps
+// every row the device mask could keep): [lo, hi), or [0, 0) when none.
+static void aperture_band(const bf_coords* co, double diskR, int& lo, int& hi) {
+    double minx2 = INFINITY;
+    for (int x = 0; x < co->nx; x++) minx2 = std::min(minx2, co->X[x] * co->X[x]);
+    const double lim = diskR * diskR * (1.0 + 1e-9);
+    lo = co->ny;
+    hi = 0;
+    for (int y = 0; y < co->ny; y++)
+        if (co->Y[y] * co->Y[y] + minx2 < lim) {
+            lo = std::min(lo, y);
+            hi = y + 1;
+        }
+    if (lo >= hi) lo = hi = 0;
+}
+
 template <typename T>
 struct DancerPlan : DancerPlanBase {
     BF_TYPES(T);
@@ -1539,38 +1594,9 @@ struct DancerPlan : DancerPlanBase {
         // criterion reads every row, so it keeps the full range.
         band_lo = 0;
         band_hi = g.ny;
-        if (opt.band && amin == 0.0 && (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS)) {
-            double minx2 = INFINITY;
-            for (int x = 0; x < g.nx; x++) minx2 = std::min(minx2, co->X[x] * co->X[x]);
-            const double lim = diskR * diskR * (1.0 + 1e-9);
-            int lo = g.ny, hi = 0;
-            for (int y = 0; y < g.ny; y++)
-                if (co->Y[y] * co->Y[y] + minx2 < lim) {
-                    lo = std::min(lo, y);
-                    hi = y + 1;
-                }
-            band_lo = lo < hi ? lo : 0;
-            band_hi = lo < hi ? hi : 0;
-        }
-        bool qtab;
-        {
-            // Quarter k-space tables for complex64 fields: the phase of a mirrored bin
-            // comes from its mirror's k, equal up to the rounding of the k range
-            // (~1e-15 rad per bounce), far inside what complex64 storage rounds
-            // away; the FP64 path keeps full tables.  BF_QTAB=0 switches it off.
-            const char* q = getenv("BF_QTAB");
-            const int n = g.nx;
-            auto mirrored = [](const double* k, int len) {
-                double mx = 0.0;
-                for (int j = 0; j < len; j++) mx = std::max(mx, std::fabs(k[j]));
-                for (int j = 0; j < len; j++)
-                    if (!(std::fabs(k[j] + k[len - 1 - j]) <= 1e-12 * mx)) return false;
-                return true;
-            };
-            qtab = !std::is_same<T, double>::value && !(q && std::string(q) == "0") && use_reg_fft() &&
-                          (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS) && g.nx == g.ny && n >= 16 &&
-                          n <= 1024 && (n & (n - 1)) == 0 && mirrored(co->kX, g.nx) && mirrored(co->kY, g.ny);
-        }
+        if (opt.band && amin == 0.0 && (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS))
+            aperture_band(co, diskR, band_lo, band_hi);
+        const bool qtab = quarter_tables_ok<T>(co, kind);
         // Frequency batch: what one bounce of a batch touches (the band rows of every
         // field, plus one k-space table per distinct (eps, dz)) is kept near
         // 192 MB, the size of the Infinity Cache; the frequencies are then spread
@@ -2017,6 +2043,16 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         amx(sizeof(double) * fb), dfd, dph;
     if (kind == BF_PROP_MOMENTUM) scr.alloc(per * fb);
     PropEngine<T> pe(ctx, &g, kind, diskR, surf.as<zd>());
+    pe.quarter = quarter_tables_ok<T>(co, kind);
+    {
+        // every propagation starts by masking: its masked row FFT and column pass
+        // need only the aperture's row band (register path; BF_BAND=0 disables)
+        const char* b = getenv("BF_BAND");
+        auto pow2 = [](int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; };
+        if (!(b && std::string(b) == "0") && use_reg_fft() && pow2(g.nx) && pow2(g.ny) &&
+            (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS))
+            aperture_band(co, diskR, pe.blo, pe.bhi);
+    }
     // descriptors: per region i, fields (f, i) for all f; pairs (f, i), (f, i+1)
     std::vector<FieldDesc> fd;
     std::vector<int> single_off(Nd), pair_off(Nd);

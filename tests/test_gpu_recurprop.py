@@ -338,3 +338,32 @@ def test_dancer_plan_graph_replays_match_eager(monkeypatch, N, prec, amin):
     E, p = plan.fetch()
     assert np.array_equal(E, want_E)
     plan.close()
+
+
+def test_cheerleader_band_matches_oracle():
+    """The cheerleader's propagations on the aperture band (masked row FFT and
+    column loads on band rows only, band-pruned column transforms at 128 x 128),
+    tilted and rough disks, both outputs, against the oracle."""
+    coords, sb = _small_booster(N=128, ndisk=2, seed=1007)
+    freqs = [19.9e9, 20.2e9]
+    left, right = B.cheerleader(0, 12, sb, coords, f=freqs, diskR=DISKR, returnboth=True)
+    os_, oc = osetup(sb), ocoords(coords)
+    for q, f in enumerate(freqs):
+        wl, wr = O.cheerleader(os_, oc, 0, 12, f, O.PROP_PROPAGATOR, diskR=DISKR, returnboth=True)
+        assert rel(right[q], wr) < TOL and rel(left[q], wl) < TOL, (q, rel(right[q], wr), rel(left[q], wl))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cheerleader_band_and_quarter_tables_match_full(monkeypatch, prec):
+    """Band-limited propagations (and, with complex64 fields, quarter phase
+    tables) against every row and full tables: 1e-12 of the field maximum in
+    FP64, 1e-5 with complex64 fields."""
+    coords, sb = _small_booster(N=128, ndisk=3, seed=1008)
+    out = {}
+    for on in ("0", "1"):
+        monkeypatch.setenv("BF_BAND", on)
+        monkeypatch.setenv("BF_QTAB", on)
+        out[on] = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec)
+    assert np.any(out["1"] != 0)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
