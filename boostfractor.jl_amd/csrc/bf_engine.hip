@@ -1314,21 +1314,27 @@ struct PropEngine {
 
     // Full propagation of nf fields (field j uses tables fd[j].ktab / fd[j].rtab).
     // `scr` must hold as many elements as `base` spans (momentum-space kind only).
+    // the masked forward half of a propagation (propagator and propagatorNoTilts):
+    // row FFT of the masked fields and the column pass; the inverse row pass follows
+    void propagate_head(cplx<S>* base, const FieldDesc* fd, int nf, const int* active) {
+        if (bhi > blo) {
+            // the mask zeroes every row outside the aperture band: the masked row
+            // FFT skips them and the column pass reads only the band, storing
+            // every row of the propagated field
+            rows(base, fd, nf, RF_MASK | RF_FWD, active, blo, bhi);
+            cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active, blo, bhi, true);
+        } else {
+            rows(base, fd, nf, RF_MASK | RF_FWD, active);
+            cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active);
+        }
+    }
+
     void propagate(cplx<S>* base, const FieldDesc* fd, int nf, const zd* ph1d, cplx<S>* scr, const int* active) {
         const size_t K = g->K();
         switch (kind) {
             case BF_PROP_PROPAGATOR:
             case BF_PROP_NOTILTS:
-                if (bhi > blo) {
-                    // the mask zeroes every row outside the aperture band: the masked row
-                    // FFT skips them and the column pass reads only the band, storing
-                    // every row of the propagated field
-                    rows(base, fd, nf, RF_MASK | RF_FWD, active, blo, bhi);
-                    cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active, blo, bhi, true);
-                } else {
-                    rows(base, fd, nf, RF_MASK | RF_FWD, active);
-                    cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active);
-                }
+                propagate_head(base, fd, nf, active);
                 rows(base, fd, nf, RF_INV | (kind == BF_PROP_PROPAGATOR ? RF_RTAB : 0u), active);
                 break;
             case BF_PROP_MOMENTUM:
@@ -1970,6 +1976,90 @@ __global__ void k_ch_pair(cplx<typename Storage<T>::type>* F, int Nd, size_t K, 
         Fi[p] = ccast<S>(cadd(cmul(a, rr), cmul(tl, c)));
     }
 }
+// The pair step of the cheerleader in one pass over rows: the propagation's
+// last row pass (inverse FFT along x, tilt/surface table) for the region pair
+// (i, i+1) and then the boundary update of k_ch_pair on the same row.  Row y is
+// two lines of a block (field i, field i+1) that exchange values through LDS;
+// both values are rounded to the storage precision before the update, as the
+// separate passes round them in HBM.
+template <typename T>
+struct ChPairArgs {
+    cplx<typename Storage<T>::type>* F;  // [fb][Nd][K]
+    const FieldDesc* fd;                 // the pair's descriptors, [f][2]
+    int Nd, ny, i, rows;
+    const cplx<typename Compute<T>::type>* tw;
+    const cplx<typename Compute<T>::type>* rtab;  // null: no tilt/surface tables
+    zd rl, tr, rr, tl;
+    const int* active;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
+    BF_TYPES(T);
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int E = RegE<T, N>::E, TT = RegE<T, N>::TT, LD = RegE<T, N>::LD;
+    cplx<S>* smem = reinterpret_cast<cplx<S>*>(smem_raw);
+    const int f = blockIdx.y;
+    if (a.active && !a.active[f]) return;  // uniform over the block
+    const int li = threadIdx.x / TT, t = threadIdx.x % TT;
+    const int side = li & 1;  // 0: field i, 1: field i+1
+    const int y = blockIdx.x * a.rows + (li >> 1);
+    const bool ok = y < a.ny;
+    const int yc = ok ? y : a.ny - 1;
+    const size_t K = (size_t)N * a.ny;
+    cplx<S>* s = smem + (size_t)li * LD;
+    const cplx<S>* sp = smem + (size_t)(li ^ 1) * LD;  // partner line, same y
+    const FieldDesc d = a.fd[2 * f + side];
+    cplx<S>* line = a.F + ((size_t)f * a.Nd + a.i + side) * K + (size_t)yc * N + t;
+    cplx<C> v[E];
+    {
+        cplx<S> raw[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) raw[m] = line[TT * m];
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = ccast<C>(raw[m]);
+    }
+    fft_reg<C, S, N, E, true>(v, s, t, a.tw);
+    if (a.rtab && d.rtab >= 0) {
+        const cplx<C>* tl = a.rtab + (size_t)d.rtab * K + (size_t)yc * N + t;
+        cplx<C> tab[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) tab[m] = tl[TT * m];
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
+    }
+    __syncthreads();  // the transform's LDS reads are done
+#pragma unroll
+    for (int m = 0; m < E; m++) s[t + TT * m] = ccast<S>(v[m]);
+    __syncthreads();
+    const cplx<C> rl = cmk<C>((C)a.rl.re, (C)a.rl.im), tr = cmk<C>((C)a.tr.re, (C)a.tr.im);
+    const cplx<C> rr = cmk<C>((C)a.rr.re, (C)a.rr.im), tl = cmk<C>((C)a.tl.re, (C)a.tl.im);
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const cplx<C> own = ccast<C>(ccast<S>(v[m])), other = ccast<C>(sp[t + TT * m]);
+        // k_ch_pair: Fn = c rl + tr a, Fi = a rr + tl c (a: field i, c: field i+1)
+        const cplx<C> nv = side == 0 ? cadd(cmul(own, rr), cmul(tl, other)) : cadd(cmul(own, rl), cmul(tr, other));
+        if (ok) line[TT * m] = ccast<S>(nv);
+    }
+}
+
+template <typename T, int N>
+struct ChPairLaunch {
+    static void run(ChPairArgs<T> a, int nb, hipStream_t s) {
+        if constexpr (N >= 16) {
+            using C = RegE<T, N>;
+            a.rows = std::max(1, 128 / C::TT);
+            const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * 2 * a.rows;
+            allow_lds((const void*)k_ch_pair_rows<T, N>, smem);
+            hipLaunchKernelGGL((k_ch_pair_rows<T, N>), dim3((a.ny + a.rows - 1) / a.rows, nb), dim3(2 * a.rows * C::TT),
+                               smem, s, a);
+            BF_CHECK_LAUNCH();
+        } else {
+            throw Error(BF_ERR_UNSUPPORTED, "fused cheerleader rows need a power-of-two grid of at least 16");
+        }
+    }
+};
+
 // emission of the left-moving fields (:290-300)
 template <typename T>
 __global__ void k_ch_emit_left(cplx<typename Storage<T>::type>* F, const cplx<typename Storage<T>::type>* __restrict__ E,
@@ -2044,6 +2134,9 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
     if (kind == BF_PROP_MOMENTUM) scr.alloc(per * fb);
     PropEngine<T> pe(ctx, &g, kind, diskR, surf.as<zd>());
     pe.quarter = quarter_tables_ok<T>(co, kind);
+    // the pair steps' inverse row pass fused with their boundary update (register
+    // path, propagator and propagatorNoTilts; BF_CHFUSE=0 runs them separately)
+    bool fuse_pair = false;
     {
         // every propagation starts by masking: its masked row FFT and column pass
         // need only the aperture's row band (register path; BF_BAND=0 disables)
@@ -2052,6 +2145,9 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         if (!(b && std::string(b) == "0") && use_reg_fft() && pow2(g.nx) && pow2(g.ny) &&
             (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS))
             aperture_band(co, diskR, pe.blo, pe.bhi);
+        const char* fz = getenv("BF_CHFUSE");
+        fuse_pair = !(fz && std::string(fz) == "0") && use_reg_fft() && pow2(g.nx) &&
+                    (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS);
     }
     // descriptors: per region i, fields (f, i) for all f; pairs (f, i), (f, i+1)
     std::vector<FieldDesc> fd;
@@ -2147,10 +2243,21 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
             hipLaunchKernelGGL(k_ch_first<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, tl[0], rl[0], A);
             BF_CHECK_LAUNCH();
             for (int i = 1; i <= Nd - 3; i++) {
-                prop_pair(i, A);
-                hipLaunchKernelGGL(k_ch_pair<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, i, rl[i], tr[i], rr[i],
-                                   tl[i], A);
-                BF_CHECK_LAUNCH();
+                if (fuse_pair) {
+                    pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A);
+                    ChPairArgs<T> pa;
+                    pa.F = Fp; pa.fd = D + pair_off[i]; pa.Nd = Nd; pa.ny = g.ny; pa.i = i; pa.rows = 0;
+                    pa.tw = pe.twx;
+                    pa.rtab = (kind == BF_PROP_PROPAGATOR && pe.has_rtab()) ? pe.rtab.template as<cplx<C>>() : nullptr;
+                    pa.rl = rl[i]; pa.tr = tr[i]; pa.rr = rr[i]; pa.tl = tl[i];
+                    pa.active = A;
+                    dispatch_n<ChPairLaunch, T>(g.nx, pa, nb, s);
+                } else {
+                    prop_pair(i, A);
+                    hipLaunchKernelGGL(k_ch_pair<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, i, rl[i], tr[i], rr[i],
+                                       tl[i], A);
+                    BF_CHECK_LAUNCH();
+                }
             }
             prop_single(Nd - 2, A);
             hipLaunchKernelGGL(k_ch_push<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, Nd - 2, tr[Nd - 2], rr[Nd - 2],
