@@ -548,7 +548,9 @@ def main():
                      "measured_stream_copy_GBs": round(stream_gbs, 1)},
         "work_reductions": {"row_band": list(info32["band"]), "rows": N,
                             "note": "rows outside the aperture band skipped by both passes (exit field kept whole); "
-                                    "identical k-space tables shared; not folded into roofline.achieved"},
+                                    "band-pruned register FFTs (zero elements and unstored outputs skipped); "
+                                    "identical k-space tables shared, quarter tables for complex64 fields; "
+                                    "not folded into roofline.achieved"},
         "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt32.items()},
     }
     if not args.no_cpu_baseline:
