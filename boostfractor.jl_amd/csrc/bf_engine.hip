@@ -2021,10 +2021,10 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
     }
     fft_reg<C, S, N, E, true>(v, s, t, a.tw);
     if (a.rtab && d.rtab >= 0) {
-        const cplx<C>* tl = a.rtab + (size_t)d.rtab * K + (size_t)yc * N + t;
+        const cplx<C>* tp = a.rtab + (size_t)d.rtab * K + (size_t)yc * N + t;
         cplx<C> tab[E];
 #pragma unroll
-        for (int m = 0; m < E; m++) tab[m] = tl[TT * m];
+        for (int m = 0; m < E; m++) tab[m] = tp[TT * m];
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = cmul(v[m], tab[m]);
     }
