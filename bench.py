@@ -377,6 +377,60 @@ def sweep_main(args):
     print(json.dumps(out), flush=True)
 
 
+def cheerleader_main(args):
+    """cheerleader (Gauss-Seidel recursive Fourier solver) on the 20-disk booster,
+    256x256, 50 frequencies, 100 iterations (the reference test's nmax), mixed
+    precision with the FP64 sweep beside it"""
+    import BoostFractor as B
+    coords, sb, freqs = workload()
+    ctx = B.context()
+    nmax = 100
+    dx = coords.X[1] - coords.X[0]
+    print(f"[bench] cheerleader: {NDISK}-disk booster, {N}x{N}, {NFREQ} freqs, nmax={nmax}", file=sys.stderr,
+          flush=True)
+
+    def timed(prec, steps, warmup):
+        for _ in range(warmup):
+            B.cheerleader(0, nmax, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            E = B.cheerleader(0, nmax, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        print(f"[bench] cheerleader {prec}: {dt / steps:.3f} s/step", file=sys.stderr, flush=True)
+        return dt, np.sum(np.abs(E) ** 2, axis=(1, 2)) * dx * dx / (math.pi * DISKR ** 2)
+
+    dt, b32 = timed("mixed", args.steps, args.warmup)
+    dt64, b64 = timed("fp64", max(1, args.fp64_steps), 1)
+    err = np.abs(b32 / b64 - 1.0)
+    pts, pts64 = NFREQ * args.steps / dt, NFREQ * max(1, args.fp64_steps) / dt64
+    out = {
+        "metric": "cheerleader boost-factor freq points/s on 1 MI355X: mixed-prec batched vs plain FP64",
+        "value": round(pts, 3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage": "complex64 fields (mixed precision: FFT, phase and update arithmetic in f64)",
+        "data": "synthetic (seeded SplitMix64 booster geometry, axion source from dance_intro)",
+        "config": {"workload": f"cheerleader, 20-disk booster eps=24, {N}x{N} grid over 1 m, {NFREQ} freqs "
+                               f"19.5-20.5 GHz, nmax={nmax}, FFT propagator, host buffers in and out",
+                   "disks": NDISK, "grid": N, "frequencies": NFREQ, "nmax": nmax},
+        "fp64": {"value": round(pts64, 3), "unit": "points/s", "mixed_over_fp64": round(pts / pts64, 3),
+                 "beta2_rel_err_vs_fp64": {"max": float(np.max(err)), "median": float(np.median(err))}},
+        "roofline": None,
+        "note": "the Gauss-Seidel sweep is a sequence of small launches (one region pair per step); "
+                "per-kernel rooflines are reported for the dancer",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def model1d_main(args):
     """1D analytical model sweep: 4096 spacing candidates of a 20-disk booster x
     1000 frequencies in one launch (disk_system_batch)"""
@@ -434,12 +488,13 @@ def main():
                     help="bounces - 1 (default: the workload's 2000; smaller only for profiler runs)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
-    ap.add_argument("--workload", choices=("dancer", "dancer512", "transformer", "sweep", "model1d"),
+    ap.add_argument("--workload", choices=("dancer", "dancer512", "cheerleader", "transformer", "sweep", "model1d"),
                     default="dancer",
                     help="dancer: the headline sweep (default); dancer512: the same booster at 512x512, 100 "
                          "frequencies, 3000 bounces (run it with --steps 1 --warmup 0: ~1 min per sweep); "
-                         "transformer: the mode-matching sweep; sweep: the many-geometry disk-position sweep; "
-                         "model1d: the 1D analytical model")
+                         "cheerleader: the Gauss-Seidel solver on the 20-disk booster; transformer: the "
+                         "mode-matching sweep; sweep: the many-geometry disk-position sweep; model1d: the 1D "
+                         "analytical model")
     ap.add_argument("--nconf", type=int, default=32, help="geometries of the sweep workload")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
@@ -461,6 +516,9 @@ def main():
         return
     if args.workload == "model1d":
         model1d_main(args)
+        return
+    if args.workload == "cheerleader":
+        cheerleader_main(args)
         return
     if args.workload == "dancer512":
         N, NFREQ = 512, 100
