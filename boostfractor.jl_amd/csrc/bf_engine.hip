@@ -1315,14 +1315,15 @@ struct PropEngine {
     // Full propagation of nf fields (field j uses tables fd[j].ktab / fd[j].rtab).
     // `scr` must hold as many elements as `base` spans (momentum-space kind only).
     // the masked forward half of a propagation (propagator and propagatorNoTilts):
-    // row FFT of the masked fields and the column pass; the inverse row pass follows
-    void propagate_head(cplx<S>* base, const FieldDesc* fd, int nf, const int* active) {
+    // row FFT of the masked fields and the column pass; the inverse row pass follows.
+    // band_out: the column pass stores the band rows only (the caller reads no other)
+    void propagate_head(cplx<S>* base, const FieldDesc* fd, int nf, const int* active, bool band_out = false) {
         if (bhi > blo) {
             // the mask zeroes every row outside the aperture band: the masked row
             // FFT skips them and the column pass reads only the band, storing
-            // every row of the propagated field
+            // every row of the propagated field (or the band only)
             rows(base, fd, nf, RF_MASK | RF_FWD, active, blo, bhi);
-            cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active, blo, bhi, true);
+            cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active, blo, bhi, !band_out);
         } else {
             rows(base, fd, nf, RF_MASK | RF_FWD, active);
             cols(base, base, fd, nf, CF_FWD | CF_KMID | CF_INV, active);
@@ -1981,7 +1982,9 @@ __global__ void k_ch_pair(cplx<typename Storage<T>::type>* F, int Nd, size_t K, 
 // (i, i+1) and then the boundary update of k_ch_pair on the same row.  Row y is
 // two lines of a block (field i, field i+1) that exchange values through LDS;
 // both values are rounded to the storage precision before the update, as the
-// separate passes round them in HBM.
+// separate passes round them in HBM.  Rows [ylo, yhi) only: the whole grid, or
+// the aperture band when nothing reads the pair's other rows before the mask
+// of their next propagation zeroes them.
 template <typename T>
 struct ChPairArgs {
     cplx<typename Storage<T>::type>* F;  // [fb][Nd][K]
@@ -1991,6 +1994,7 @@ struct ChPairArgs {
     const cplx<typename Compute<T>::type>* rtab;  // null: no tilt/surface tables
     zd rl, tr, rr, tl;
     const int* active;
+    int ylo, yhi;  // rows processed
 };
 
 template <typename T, int N>
@@ -2003,9 +2007,9 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
     if (a.active && !a.active[f]) return;  // uniform over the block
     const int li = threadIdx.x / TT, t = threadIdx.x % TT;
     const int side = li & 1;  // 0: field i, 1: field i+1
-    const int y = blockIdx.x * a.rows + (li >> 1);
-    const bool ok = y < a.ny;
-    const int yc = ok ? y : a.ny - 1;
+    const int y = a.ylo + blockIdx.x * a.rows + (li >> 1);
+    const bool ok = y < a.yhi;
+    const int yc = ok ? y : a.yhi - 1;
     const size_t K = (size_t)N * a.ny;
     cplx<S>* s = smem + (size_t)li * LD;
     const cplx<S>* sp = smem + (size_t)(li ^ 1) * LD;  // partner line, same y
@@ -2051,8 +2055,8 @@ struct ChPairLaunch {
             a.rows = std::max(1, 128 / C::TT);
             const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * 2 * a.rows;
             allow_lds((const void*)k_ch_pair_rows<T, N>, smem);
-            hipLaunchKernelGGL((k_ch_pair_rows<T, N>), dim3((a.ny + a.rows - 1) / a.rows, nb), dim3(2 * a.rows * C::TT),
-                               smem, s, a);
+            hipLaunchKernelGGL((k_ch_pair_rows<T, N>), dim3((a.yhi - a.ylo + a.rows - 1) / a.rows, nb),
+                               dim3(2 * a.rows * C::TT), smem, s, a);
             BF_CHECK_LAUNCH();
         } else {
             throw Error(BF_ERR_UNSUPPORTED, "fused cheerleader rows need a power-of-two grid of at least 16");
@@ -2149,6 +2153,11 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         fuse_pair = !(fz && std::string(fz) == "0") && use_reg_fft() && pow2(g.nx) &&
                     (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS);
     }
+    // The pair steps update the interior fields F[1..Nd-2] only; each is masked
+    // again before it is read (its next propagation), so the steps need the band
+    // rows only.  F[0] and F[Nd-1] (the outputs) come from the single steps, which
+    // stay whole; the amin criterion reads every row, so it keeps them whole too.
+    const bool pair_band = fuse_pair && pe.bhi > pe.blo && amin == 0.0;
     // descriptors: per region i, fields (f, i) for all f; pairs (f, i), (f, i+1)
     std::vector<FieldDesc> fd;
     std::vector<int> single_off(Nd), pair_off(Nd);
@@ -2244,9 +2253,11 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
             BF_CHECK_LAUNCH();
             for (int i = 1; i <= Nd - 3; i++) {
                 if (fuse_pair) {
-                    pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A);
+                    pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A, pair_band);
                     ChPairArgs<T> pa;
                     pa.F = Fp; pa.fd = D + pair_off[i]; pa.Nd = Nd; pa.ny = g.ny; pa.i = i; pa.rows = 0;
+                    pa.ylo = pair_band ? pe.blo : 0;
+                    pa.yhi = pair_band ? pe.bhi : g.ny;
                     pa.tw = pe.twx;
                     pa.rtab = (kind == BF_PROP_PROPAGATOR && pe.has_rtab()) ? pe.rtab.template as<cplx<C>>() : nullptr;
                     pa.rl = rl[i]; pa.tr = tr[i]; pa.rr = rr[i]; pa.tl = tl[i];

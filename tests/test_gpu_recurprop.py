@@ -369,12 +369,14 @@ def test_cheerleader_band_and_quarter_tables_match_full(monkeypatch, prec):
     assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
 
 
+@pytest.mark.parametrize("N", [64, 128])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cheerleader_fused_pair_rows_match_separate(monkeypatch, prec):
+def test_cheerleader_fused_pair_rows_match_separate(monkeypatch, prec, N):
     """The pair steps' inverse row pass fused with the boundary update against the
-    separate passes (tilted and rough disks, both outputs, amin criterion):
+    separate passes (tilted and rough disks, both outputs, amin criterion; with
+    amin = 0 the fused pair steps keep the aperture band's rows only):
     1e-12 of the field maximum in FP64, 1e-5 with complex64 fields."""
-    coords, sb = _small_booster(N=64, ndisk=3, seed=1009)
+    coords, sb = _small_booster(N=N, ndisk=3, seed=1009)
     out = {}
     for fz in ("0", "1"):
         monkeypatch.setenv("BF_CHFUSE", fz)
