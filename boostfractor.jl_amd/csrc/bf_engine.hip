@@ -454,14 +454,16 @@ __global__ void __launch_bounds__(512) k_rows_reg(RowArgs<T> a) {
     const int lpb = blockDim.x / TT;
     const int li = threadIdx.x / TT, t = threadIdx.x % TT;
     cplx<S>* s = smem + (size_t)li * LD;
-    const long nlines = (long)a.nf * a.ny;
+    // the grid covers the rows [ylo, yhi) of every field (all rows when yhi <= ylo)
+    const bool band = a.yhi > a.ylo;
+    const int nr = band ? a.yhi - a.ylo : a.ny, y0 = band ? a.ylo : 0;
+    const long nlines = (long)a.nf * nr;
     const long line = (long)blockIdx.x * lpb + li;
     bool valid = line < nlines;
-    const int j = valid ? (int)(line / a.ny) : 0;
-    const int y = valid ? (int)(line % a.ny) : 0;
+    const int j = valid ? (int)(line / nr) : 0;
+    const int y = valid ? y0 + (int)(line % nr) : 0;
     const FieldDesc d = a.fd[j];
     if (valid && a.active && !a.active[d.group]) valid = false;
-    if (a.yhi > a.ylo && (y < a.ylo || y >= a.yhi)) valid = false;
     const uint32_t K = (uint32_t)N * (uint32_t)a.ny;
     const auto rf = buf_rsrc(a.base + d.off, K * (uint32_t)sizeof(cplx<S>));
     const uint32_t o0 = (uint32_t)y * N + t;
@@ -1033,7 +1035,8 @@ struct RowLaunch {
             if (use_reg_fft()) {
                 using C = RegE<T, N>;
                 const int lpb = std::max(1, 256 / C::TT);
-                const long nlines = (long)a.nf * a.ny;
+                // lines of the row band only (k_rows_reg maps the grid onto [ylo, yhi))
+                const long nlines = (long)a.nf * (a.yhi > a.ylo ? a.yhi - a.ylo : a.ny);
                 const size_t smem = sizeof(cplx<typename Storage<T>::type>) * (size_t)C::LD * lpb;
                 allow_lds((const void*)k_rows_reg<T, N>, smem);
                 hipLaunchKernelGGL((k_rows_reg<T, N>), dim3((unsigned)((nlines + lpb - 1) / lpb)), dim3(lpb * C::TT),
