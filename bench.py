@@ -161,6 +161,29 @@ def cpu_baseline(coords, sb, freqs, budget_s=20.0):
                       f"(FP64 C restatement of the reference dancer), extrapolated linearly in bounces"}
 
 
+def cheerleader_cpu_baseline(coords, sb, freqs, nmax, budget_s=15.0):
+    """FP64 CPU restatement of the cheerleader (oracle, one thread) on a bounded
+    sample: one frequency point with nmax = 0 (set-up, prologue and one
+    iteration) and with a few iterations more, extrapolated linearly in
+    iterations to nmax + 1."""
+    from oracle import oracle as O
+    oc = O.Coords(coords.X, coords.Y, coords.kX, coords.kY)
+    os_ = O.Setup(sb.distance, sb.eps, sb.r)
+    t = time.perf_counter()
+    O.cheerleader(os_, oc, 0, 0, freqs[0], diskR=DISKR)
+    t0 = time.perf_counter() - t
+    n1 = max(1, min(nmax, int(budget_s / max(t0, 1e-6))))
+    t = time.perf_counter()
+    O.cheerleader(os_, oc, 0, n1, freqs[0], diskR=DISKR)
+    t1 = time.perf_counter() - t
+    per_iter = max(t1 - t0, 0.0) / n1
+    full = t0 + per_iter * nmax
+    return {"value": 1.0 / full, "unit": "points/s", "cores": 1, "kind": "port",
+            "sample": f"1 frequency point, {n1 + 1} of {nmax + 1} iterations (and 1 for the prologue time) on one "
+                      f"thread (FP64 C restatement of the reference cheerleader), extrapolated linearly in "
+                      f"iterations"}
+
+
 def transformer_cpu_baseline(coords, sb, pat, kt):
     """FP64 CPU restatement on a bounded sample: the propagation matrix of one
     air region and one disk region (single thread), scaled to the 21 air and
@@ -428,6 +451,9 @@ def cheerleader_main(args):
         "note": "the Gauss-Seidel sweep is a sequence of small launches (one region pair per step); "
                 "per-kernel rooflines are reported for the dancer",
     }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle cheerleader, bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cheerleader_cpu_baseline(coords, sb, freqs, nmax)
     print(json.dumps(out), flush=True)
 
 
