@@ -1987,7 +1987,9 @@ __global__ void k_ch_pair(cplx<typename Storage<T>::type>* F, int Nd, size_t K, 
 // both values are rounded to the storage precision before the update, as the
 // separate passes round them in HBM.  Rows [ylo, yhi) only: the whole grid, or
 // the aperture band when nothing reads the pair's other rows before the mask
-// of their next propagation zeroes them.
+// of their next propagation zeroes them.  With xout the kernel also starts
+// that next propagation: the updated rows are masked by the aperture and
+// transformed along x (the masked row FFT of k_rows_reg) before they are stored.
 template <typename T>
 struct ChPairArgs {
     cplx<typename Storage<T>::type>* F;  // [fb][Nd][K]
@@ -1998,6 +2000,10 @@ struct ChPairArgs {
     zd rl, tr, rr, tl;
     const int* active;
     int ylo, yhi;  // rows processed
+    int xout;      // store the masked row transform of the updated fields
+    const double* X;
+    const double* Y;
+    double R2;  // aperture radius squared
 };
 
 template <typename T, int N>
@@ -2046,8 +2052,20 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
         const cplx<C> own = ccast<C>(ccast<S>(v[m])), other = ccast<C>(sp[t + TT * m]);
         // k_ch_pair: Fn = c rl + tr a, Fi = a rr + tl c (a: field i, c: field i+1)
         const cplx<C> nv = side == 0 ? cadd(cmul(own, rr), cmul(tl, other)) : cadd(cmul(own, rl), cmul(tr, other));
-        if (ok) line[TT * m] = ccast<S>(nv);
+        v[m] = ccast<C>(ccast<S>(nv));  // the stored value
     }
+    if (a.xout) {  // uniform over the launch
+        const uint32_t keep = aperture_bits<E, TT>(a.X, a.Y[yc], a.R2, t);
+#pragma unroll
+        for (int m = 0; m < E; m++)
+            if (!((keep >> m) & 1u)) v[m] = cmk<C>(0, 0);
+        // the transform's first barrier orders the partner's reads of s above
+        // before this line's buffer is overwritten
+        fft_reg<C, S, N, E, false>(v, s, t, a.tw);
+    }
+#pragma unroll
+    for (int m = 0; m < E; m++)
+        if (ok) line[TT * m] = ccast<S>(v[m]);
 }
 
 template <typename T, int N>
@@ -2161,6 +2179,14 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
     // rows only.  F[0] and F[Nd-1] (the outputs) come from the single steps, which
     // stay whole; the amin criterion reads every row, so it keeps them whole too.
     const bool pair_band = fuse_pair && pe.bhi > pe.blo && amin == 0.0;
+    // With the band kept, the pair steps also store their fields as the next
+    // propagation's masked row transform (every reader of an interior field is
+    // a masked propagation).  xf[r]: F[r] holds that transform of its band rows,
+    // the same for every frequency of the batch (all stay active when amin is 0);
+    // BF_CHXF=0 stores them in real space.
+    const char* xz = getenv("BF_CHXF");
+    const bool pair_xform = pair_band && !(xz && std::string(xz) == "0");
+    std::vector<char> xf(Nd, 0);
     // descriptors: per region i, fields (f, i) for all f; pairs (f, i), (f, i+1)
     std::vector<FieldDesc> fd;
     std::vector<int> single_off(Nd), pair_off(Nd);
@@ -2238,6 +2264,19 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         auto prop_pair = [&](int i, const int* active) {
             pe.propagate(Fp, D + pair_off[i], 2 * nb, P1 + pair_off[i], Sp, active);
         };
+        const uint32_t inv_rows = RF_INV | (kind == BF_PROP_PROPAGATOR ? RF_RTAB : 0u);
+        // propagation of region i in the loop: a field that already holds its
+        // masked row transform goes straight to the column pass
+        auto prop_single_loop = [&](int i, const int* active) {
+            if (!xf[i]) {
+                prop_single(i, active);
+                return;
+            }
+            pe.cols(Fp, Fp, D + single_off[i], nb, CF_FWD | CF_KMID | CF_INV, active, pe.blo, pe.bhi, true);
+            pe.rows(Fp, D + single_off[i], nb, inv_rows, active);
+            xf[i] = 0;
+        };
+        std::fill(xf.begin(), xf.end(), 0);
         // prologue (:281-300), runs for every frequency slot
         for (int i = 1; i <= Nd - 2; i++) {
             prop_single(i, nullptr);
@@ -2251,12 +2290,25 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         BF_HIP(hipMemsetAsync(amx.p, 0, sizeof(double) * fb, s));
         const int* A = act.as<int>();
         for (int n = 0; n <= nmax; n++) {
-            prop_single(1, A);
+            prop_single_loop(1, A);
             hipLaunchKernelGGL(k_ch_first<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, tl[0], rl[0], A);
             BF_CHECK_LAUNCH();
+            xf[0] = xf[1] = 0;
             for (int i = 1; i <= Nd - 3; i++) {
                 if (fuse_pair) {
-                    pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A, pair_band);
+                    if (pair_xform) {
+                        // masked row FFT of the fields still in real space, then the column pass
+                        const uint32_t fwd = RF_MASK | RF_FWD;
+                        if (!xf[i] && !xf[i + 1])
+                            pe.rows(Fp, D + pair_off[i], 2 * nb, fwd, A, pe.blo, pe.bhi);
+                        else if (!xf[i])
+                            pe.rows(Fp, D + single_off[i], nb, fwd, A, pe.blo, pe.bhi);
+                        else if (!xf[i + 1])
+                            pe.rows(Fp, D + single_off[i + 1], nb, fwd, A, pe.blo, pe.bhi);
+                        pe.cols(Fp, Fp, D + pair_off[i], 2 * nb, CF_FWD | CF_KMID | CF_INV, A, pe.blo, pe.bhi, false);
+                    } else {
+                        pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A, pair_band);
+                    }
                     ChPairArgs<T> pa;
                     pa.F = Fp; pa.fd = D + pair_off[i]; pa.Nd = Nd; pa.ny = g.ny; pa.i = i; pa.rows = 0;
                     pa.ylo = pair_band ? pe.blo : 0;
@@ -2265,7 +2317,10 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
                     pa.rtab = (kind == BF_PROP_PROPAGATOR && pe.has_rtab()) ? pe.rtab.template as<cplx<C>>() : nullptr;
                     pa.rl = rl[i]; pa.tr = tr[i]; pa.rr = rr[i]; pa.tl = tl[i];
                     pa.active = A;
+                    pa.xout = pair_xform ? 1 : 0;
+                    pa.X = g.X.as<double>(); pa.Y = g.Y.as<double>(); pa.R2 = pe.R2;
                     dispatch_n<ChPairLaunch, T>(g.nx, pa, nb, s);
+                    xf[i] = xf[i + 1] = pair_xform ? 1 : 0;
                 } else {
                     prop_pair(i, A);
                     hipLaunchKernelGGL(k_ch_pair<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, i, rl[i], tr[i], rr[i],
@@ -2273,10 +2328,11 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
                     BF_CHECK_LAUNCH();
                 }
             }
-            prop_single(Nd - 2, A);
+            prop_single_loop(Nd - 2, A);
             hipLaunchKernelGGL(k_ch_push<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, Nd - 2, tr[Nd - 2], rr[Nd - 2],
                                A);
             BF_CHECK_LAUNCH();
+            xf[Nd - 2] = xf[Nd - 1] = 0;
             if (amin != 0.0) {
                 hipLaunchKernelGGL(k_amax_fields<T>, dim3(grid1d(K * Nd, 256, 1024), nb), dim3(256), 0, s, Fp,
                                    K * Nd, amx.as<double>(), A);

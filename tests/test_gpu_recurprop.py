@@ -374,16 +374,19 @@ def test_cheerleader_band_and_quarter_tables_match_full(monkeypatch, prec):
 def test_cheerleader_fused_pair_rows_match_separate(monkeypatch, prec, N):
     """The pair steps' inverse row pass fused with the boundary update against the
     separate passes (tilted and rough disks, both outputs, amin criterion; with
-    amin = 0 the fused pair steps keep the aperture band's rows only):
+    amin = 0 the fused pair steps keep the aperture band's rows only and store
+    the next propagation's masked row transform, BF_CHXF=0 stores real space):
     1e-12 of the field maximum in FP64, 1e-5 with complex64 fields."""
     coords, sb = _small_booster(N=N, ndisk=3, seed=1009)
     out = {}
-    for fz in ("0", "1"):
+    for fz, xf in (("0", "1"), ("1", "0"), ("1", "1")):
         monkeypatch.setenv("BF_CHFUSE", fz)
+        monkeypatch.setenv("BF_CHXF", xf)
         lr = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec, returnboth=True)
         am = B.cheerleader(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
-        out[fz] = (lr[0], lr[1], am)
+        out[fz + xf] = (lr[0], lr[1], am)
     tol = 1e-12 if prec == "fp64" else 1e-5
-    assert np.any(out["1"][1] != 0) and np.any(out["1"][2] != 0)  # (the mirror side's output is zero)
-    for q in range(3):
-        assert rel(out["1"][q], out["0"][q]) < tol, (q, rel(out["1"][q], out["0"][q]))
+    assert np.any(out["11"][1] != 0) and np.any(out["11"][2] != 0)  # (the mirror side's output is zero)
+    for key in ("10", "11"):
+        for q in range(3):
+            assert rel(out[key][q], out["01"][q]) < tol, (key, q, rel(out[key][q], out["01"][q]))
