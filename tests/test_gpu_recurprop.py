@@ -186,6 +186,22 @@ def test_cheerleader_amin_reflect():
     assert rel(got, O.cheerleader(os_, oc, 0, 10, 20e9, reflect=refl, diskR=DISKR)) < TOL
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", TOL), ("mixed", 1e-5)])
+def test_cheerleader_emit_matches_oracle(prec, tol):
+    """User-supplied emission fields (Nd x 2 x grid) instead of dance_intro, on a
+    power-of-two grid where the propagations run on the aperture band and the
+    pair steps store the next masked row transform; both outputs."""
+    coords, sb = _small_booster(N=64, ndisk=2, seed=1010)
+    Nd = len(sb.distance)
+    rng = np.random.default_rng(11)
+    emit = rng.standard_normal((Nd, 2, 64, 64)) + 1j * rng.standard_normal((Nd, 2, 64, 64))
+    left, right = B.cheerleader(0, 12, sb, coords, f=20.1e9, emit=emit, diskR=DISKR, precision=prec,
+                                returnboth=True)
+    wl, wr = O.cheerleader(osetup(sb), ocoords(coords), 0, 12, 20.1e9, emit=emit, diskR=DISKR, returnboth=True)
+    assert np.any(wr != 0) and np.any(wl != 0)
+    assert rel(right, wr) < tol and rel(left, wl) < tol, (rel(right, wr), rel(left, wl))
+
+
 def test_dancer_mixed_precision_beta2():
     """20-disk booster, 128^2, FP32 fields/FFT: beta^2 within 1e-4 of the FP64 engine."""
     coords = W.grid_coords(128)
