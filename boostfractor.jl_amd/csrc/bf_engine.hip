@@ -34,6 +34,7 @@ enum : uint32_t { RF_INV = 1, RF_RTAB = 2, RF_MASK = 4, RF_FWD = 8 };
 enum : uint32_t { CF_FWD = 1, CF_KMID = 2, CF_INV = 4, CF_KLOAD = 8, CF_INSHIFT = 16, CF_OUTSHIFT = 32 };
 enum : uint32_t { DF_FWD = 1, DF_AMAX = 2, DF_EOUT_ASSIGN = 4 };
 enum : int { FD_WHOLE = 1 };  // FieldDesc.pad: column pass stores every row of this field
+constexpr int FD_PAIR_SHIFT = 8;  // FieldDesc.pad >> FD_PAIR_SHIFT: region pair of a cheerleader pair step
 
 // Precision tags: `double` = FP64 storage and arithmetic; `float` = fields stored
 // in FP32 in HBM, all in-register arithmetic (FFTs, phases, boundary updates) in
@@ -1993,11 +1994,13 @@ __global__ void k_ch_pair(cplx<typename Storage<T>::type>* F, int Nd, size_t K, 
 template <typename T>
 struct ChPairArgs {
     cplx<typename Storage<T>::type>* F;  // [fb][Nd][K]
-    const FieldDesc* fd;                 // the pair's descriptors, [f][2]
-    int Nd, ny, i, rows;
+    // entry e (one frequency of one region pair (i, i+1)): fd[2e] field i,
+    // fd[2e+1] field i+1; i = fd[2e].pad >> FD_PAIR_SHIFT
+    const FieldDesc* fd;
+    int ny, rows;
     const cplx<typename Compute<T>::type>* tw;
     const cplx<typename Compute<T>::type>* rtab;  // null: no tilt/surface tables
-    zd rl, tr, rr, tl;
+    const zd* coef;  // [i][4]: rl, tr, rr, tl of boundary i
     const int* active;
     int ylo, yhi;  // rows processed
     int xout;      // store the masked row transform of the updated fields
@@ -2012,8 +2015,9 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
     extern __shared__ unsigned char smem_raw[];
     constexpr int E = RegE<T, N>::E, TT = RegE<T, N>::TT, LD = RegE<T, N>::LD;
     cplx<S>* smem = reinterpret_cast<cplx<S>*>(smem_raw);
-    const int f = blockIdx.y;
-    if (a.active && !a.active[f]) return;  // uniform over the block
+    const int e = blockIdx.y;
+    const FieldDesc d0 = a.fd[2 * e];
+    if (a.active && !a.active[d0.group]) return;  // uniform over the block
     const int li = threadIdx.x / TT, t = threadIdx.x % TT;
     const int side = li & 1;  // 0: field i, 1: field i+1
     const int y = a.ylo + blockIdx.x * a.rows + (li >> 1);
@@ -2022,8 +2026,8 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
     const size_t K = (size_t)N * a.ny;
     cplx<S>* s = smem + (size_t)li * LD;
     const cplx<S>* sp = smem + (size_t)(li ^ 1) * LD;  // partner line, same y
-    const FieldDesc d = a.fd[2 * f + side];
-    cplx<S>* line = a.F + ((size_t)f * a.Nd + a.i + side) * K + (size_t)yc * N + t;
+    const FieldDesc d = a.fd[2 * e + side];
+    cplx<S>* line = a.F + d.off + (size_t)yc * N + t;
     cplx<C> v[E];
     {
         cplx<S> raw[E];
@@ -2045,8 +2049,9 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
 #pragma unroll
     for (int m = 0; m < E; m++) s[t + TT * m] = ccast<S>(v[m]);
     __syncthreads();
-    const cplx<C> rl = cmk<C>((C)a.rl.re, (C)a.rl.im), tr = cmk<C>((C)a.tr.re, (C)a.tr.im);
-    const cplx<C> rr = cmk<C>((C)a.rr.re, (C)a.rr.im), tl = cmk<C>((C)a.tl.re, (C)a.tl.im);
+    const zd* cf = a.coef + 4 * (d0.pad >> FD_PAIR_SHIFT);
+    const cplx<C> rl = cmk<C>((C)cf[0].re, (C)cf[0].im), tr = cmk<C>((C)cf[1].re, (C)cf[1].im);
+    const cplx<C> rr = cmk<C>((C)cf[2].re, (C)cf[2].im), tl = cmk<C>((C)cf[3].re, (C)cf[3].im);
 #pragma unroll
     for (int m = 0; m < E; m++) {
         const cplx<C> own = ccast<C>(ccast<S>(v[m])), other = ccast<C>(sp[t + TT * m]);
@@ -2070,6 +2075,7 @@ __global__ void __launch_bounds__(512) k_ch_pair_rows(ChPairArgs<T> a) {
 
 template <typename T, int N>
 struct ChPairLaunch {
+    // nb: entries (region pair x frequency) in a.fd
     static void run(ChPairArgs<T> a, int nb, hipStream_t s) {
         if constexpr (N >= 16) {
             using C = RegE<T, N>;
@@ -2198,8 +2204,28 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         pair_off[i] = (int)fd.size();
         for (int f = 0; f < fb; f++)
             for (int q = 0; q < 2; q++)
-                fd.push_back(FieldDesc{(int64_t)(((size_t)f * Nd + i + q) * K), f * Nd + i + q, -1, f, 0});
+                fd.push_back(FieldDesc{(int64_t)(((size_t)f * Nd + i + q) * K), f * Nd + i + q, -1, f,
+                                       i << FD_PAIR_SHIFT});
     }
+    // boundary coefficients for the pair kernel, [i][4] = rl, tr, rr, tl
+    DevBuf dcoef(sizeof(zd) * 4 * Nd);
+    {
+        std::vector<zd> cf(4 * Nd);
+        for (int i = 0; i < Nd; i++) {
+            cf[4 * i] = rl[i]; cf[4 * i + 1] = tr[i]; cf[4 * i + 2] = rr[i]; cf[4 * i + 3] = tl[i];
+        }
+        upload(dcoef.p, cf.data(), sizeof(zd) * cf.size(), s);
+    }
+    // Wavefront order (amin = 0, pair steps fused and transformed): step j of
+    // iteration n (j = 0: first region, 1..Nd-3: pair (j, j+1), Nd-2: last
+    // region; step j updates fields j and j+1) runs at time 2n + j.  Steps that
+    // share a field keep their sequential order and the steps of one time touch
+    // disjoint fields, so each time's steps share launches and the results are
+    // those of the sequential sweep.  BF_CHWAVE=0 sweeps step by step.
+    const char* wz = getenv("BF_CHWAVE");
+    const bool wave = pair_xform && Nd >= 4 && !(wz && std::string(wz) == "0") &&
+                      (size_t)(Nd / 2 + 1) * 2 * fb <= 65535;  // fields per launch within grid.y
+    DevBuf dwave;
     dfd.alloc(sizeof(FieldDesc) * fd.size());
     dph.alloc(sizeof(zd) * fd.size());
     if (emit_h) {
@@ -2289,7 +2315,118 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
         upload(act.p, a0.data(), sizeof(int) * fb, s);
         BF_HIP(hipMemsetAsync(amx.p, 0, sizeof(double) * fb, s));
         const int* A = act.as<int>();
-        for (int n = 0; n <= nmax; n++) {
+        // fused pair step over `entries` (region pair, frequency) entries of fdp
+        auto pair_launch = [&](const FieldDesc* fdp, int entries) {
+            ChPairArgs<T> pa;
+            pa.F = Fp; pa.fd = fdp; pa.ny = g.ny; pa.rows = 0;
+            pa.ylo = pair_band ? pe.blo : 0;
+            pa.yhi = pair_band ? pe.bhi : g.ny;
+            pa.tw = pe.twx;
+            pa.rtab = (kind == BF_PROP_PROPAGATOR && pe.has_rtab()) ? pe.rtab.template as<cplx<C>>() : nullptr;
+            pa.coef = dcoef.as<zd>();
+            pa.active = A;
+            pa.xout = pair_xform ? 1 : 0;
+            pa.X = g.X.as<double>(); pa.Y = g.Y.as<double>(); pa.R2 = pe.R2;
+            dispatch_n<ChPairLaunch, T>(g.nx, pa, entries, s);
+        };
+        if (wave) {
+            // per time: masked row FFT of the fields in real space (r1), column pass
+            // of every propagated field (c; the single regions' fields whole), inverse
+            // row pass of the single regions (inv), their updates, the pair entries (p)
+            struct Step {
+                size_t r1, c, inv, p;
+                int nr1, nc, ninv, np;
+                bool first, push;
+            };
+            std::vector<FieldDesc> wl;
+            std::vector<Step> steps;
+            std::map<std::string, Step> seen;  // (step range, transformed regions) -> lists
+            std::vector<char> xs(Nd, 0);
+            auto add = [&](std::vector<FieldDesc>& v, int r, int pad) {
+                for (int f = 0; f < nb; f++) {
+                    FieldDesc d = fd[single_off[r] + f];
+                    d.pad = pad;
+                    v.push_back(d);
+                }
+            };
+            for (int tau = 0; tau <= 2 * nmax + Nd - 2; tau++) {
+                int jlo = std::max(0, tau - 2 * nmax), jhi = std::min(Nd - 2, tau);
+                if ((tau - jlo) & 1) jlo++;
+                if ((tau - jhi) & 1) jhi--;
+                if (jlo > jhi) continue;
+                std::string key((const char*)&jlo, sizeof(int));
+                key.append((const char*)&jhi, sizeof(int));
+                key.append(xs.data(), xs.size());
+                auto it = seen.find(key);
+                if (it != seen.end()) {
+                    steps.push_back(it->second);
+                    for (int j = jlo; j <= jhi; j += 2) {
+                        const char v = (j == 0 || j == Nd - 2) ? 0 : 1;
+                        xs[j] = xs[j + 1] = v;
+                    }
+                    continue;
+                }
+                std::vector<FieldDesc> r1, c, inv, p;
+                Step st{};
+                for (int j = jlo; j <= jhi; j += 2) {
+                    if (j == 0 || j == Nd - 2) {
+                        const int r = j == 0 ? 1 : Nd - 2;
+                        if (!xs[r]) add(r1, r, 0);
+                        add(c, r, FD_WHOLE);
+                        add(inv, r, 0);
+                        (j == 0 ? st.first : st.push) = true;
+                        xs[j] = xs[j + 1] = 0;
+                    } else {
+                        if (!xs[j]) add(r1, j, 0);
+                        if (!xs[j + 1]) add(r1, j + 1, 0);
+                        add(c, j, 0);
+                        add(c, j + 1, 0);
+                        for (int f = 0; f < nb; f++) {
+                            FieldDesc d0 = fd[single_off[j] + f], d1 = fd[single_off[j + 1] + f];
+                            d0.pad = d1.pad = j << FD_PAIR_SHIFT;
+                            p.push_back(d0);
+                            p.push_back(d1);
+                        }
+                        xs[j] = xs[j + 1] = 1;
+                    }
+                }
+                auto put = [&](const std::vector<FieldDesc>& v, size_t& off, int& n) {
+                    off = wl.size();
+                    n = (int)v.size();
+                    wl.insert(wl.end(), v.begin(), v.end());
+                };
+                put(r1, st.r1, st.nr1);
+                put(c, st.c, st.nc);
+                put(inv, st.inv, st.ninv);
+                put(p, st.p, st.np);
+                st.np /= 2;
+                seen.emplace(std::move(key), st);
+                steps.push_back(st);
+            }
+            // every descriptor addresses a field of F
+            for (const FieldDesc& d : wl)
+                if (d.off < 0 || (size_t)d.off + K > K * Nd * (size_t)nb)
+                    throw Error(BF_ERR_ARG, "cheerleader wavefront descriptor outside the field buffer");
+            dwave.alloc(sizeof(FieldDesc) * std::max<size_t>(wl.size(), 1));
+            upload(dwave.p, wl.data(), sizeof(FieldDesc) * wl.size(), s);
+            const FieldDesc* W = dwave.as<FieldDesc>();
+            for (const Step& st : steps) {
+                if (st.nr1) pe.rows(Fp, W + st.r1, st.nr1, RF_MASK | RF_FWD, A, pe.blo, pe.bhi);
+                pe.cols(Fp, Fp, W + st.c, st.nc, CF_FWD | CF_KMID | CF_INV, A, pe.blo, pe.bhi, false);
+                if (st.ninv) pe.rows(Fp, W + st.inv, st.ninv, inv_rows, A);
+                if (st.first) {
+                    hipLaunchKernelGGL(k_ch_first<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, tl[0], rl[0], A);
+                    BF_CHECK_LAUNCH();
+                }
+                if (st.push) {
+                    hipLaunchKernelGGL(k_ch_push<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, Nd - 2, tr[Nd - 2],
+                                       rr[Nd - 2], A);
+                    BF_CHECK_LAUNCH();
+                }
+                if (st.np) pair_launch(W + st.p, st.np);
+            }
+        }
+        for (int n = 0; n <= (wave ? -1 : nmax); n++) {
             prop_single_loop(1, A);
             hipLaunchKernelGGL(k_ch_first<T>, dim3(gb, nb), dim3(256), 0, s, Fp, Nd, K, tl[0], rl[0], A);
             BF_CHECK_LAUNCH();
@@ -2309,17 +2446,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
                     } else {
                         pe.propagate_head(Fp, D + pair_off[i], 2 * nb, A, pair_band);
                     }
-                    ChPairArgs<T> pa;
-                    pa.F = Fp; pa.fd = D + pair_off[i]; pa.Nd = Nd; pa.ny = g.ny; pa.i = i; pa.rows = 0;
-                    pa.ylo = pair_band ? pe.blo : 0;
-                    pa.yhi = pair_band ? pe.bhi : g.ny;
-                    pa.tw = pe.twx;
-                    pa.rtab = (kind == BF_PROP_PROPAGATOR && pe.has_rtab()) ? pe.rtab.template as<cplx<C>>() : nullptr;
-                    pa.rl = rl[i]; pa.tr = tr[i]; pa.rr = rr[i]; pa.tl = tl[i];
-                    pa.active = A;
-                    pa.xout = pair_xform ? 1 : 0;
-                    pa.X = g.X.as<double>(); pa.Y = g.Y.as<double>(); pa.R2 = pe.R2;
-                    dispatch_n<ChPairLaunch, T>(g.nx, pa, nb, s);
+                    pair_launch(D + pair_off[i], nb);
                     xf[i] = xf[i + 1] = pair_xform ? 1 : 0;
                 } else {
                     prop_pair(i, A);

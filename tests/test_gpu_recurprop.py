@@ -186,6 +186,23 @@ def test_cheerleader_amin_reflect():
     assert rel(got, O.cheerleader(os_, oc, 0, 10, 20e9, reflect=refl, diskR=DISKR)) < TOL
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("N,ndisk,nmax", [(64, 3, 20), (128, 2, 9), (64, 1, 7), (64, 2, 0)])
+def test_cheerleader_wavefront_matches_sweep(monkeypatch, prec, N, ndisk, nmax):
+    """Steps of several iterations sharing launches (wavefront order) against the
+    step-by-step sweep: the same kernels on the same fields in an order that
+    keeps every dependency, so both outputs agree bit for bit (Nd = 4 and
+    nmax = 0 included)."""
+    coords, sb = _small_booster(N=N, ndisk=ndisk, seed=1011 + N + ndisk)
+    out = {}
+    for wz in ("0", "1"):
+        monkeypatch.setenv("BF_CHWAVE", wz)
+        out[wz] = B.cheerleader(0, nmax, sb, coords, f=[19.9e9, 20.05e9, 20.2e9], diskR=DISKR, precision=prec,
+                                returnboth=True)
+    assert np.any(out["1"][1] != 0)
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", TOL), ("mixed", 1e-5)])
 def test_cheerleader_emit_matches_oracle(prec, tol):
     """User-supplied emission fields (Nd x 2 x grid) instead of dance_intro, on a
