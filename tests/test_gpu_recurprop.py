@@ -99,6 +99,21 @@ def test_cheerleader_golden():
     assert np.max(np.abs(np.array(got) / g - 1)) < TOL
 
 
+def test_cheerleader_fine_golden():
+    """test/runtests.jl:212 (commented out; Deps = 0:0.1:3pi), a spread of points
+    that are not on the :211 grid."""
+    coords = ref_coords()
+    sb = ref_setup_default(coords)
+    dx = ref_dx(coords)
+    Deps = np.arange(0, 3 * math.pi, 0.1)
+    g = np.array(GOLDEN["cheerleader_3d_deps_0_0.1_3pi"]["beta2"])
+    for i in [3, 17, 26, 48, 63, 79, 94]:
+        E = B.cheerleader(0, 100, with_distance(sb, deps_distance(Deps[i])), coords, prop=B.propagator, f=10e9,
+                          diskR=DISKR)
+        b = np.sum(np.abs(E) ** 2) * dx * dx / (math.pi * DISKR ** 2)
+        assert abs(b / g[i] - 1) < TOL, (i, b, g[i])
+
+
 def _small_booster(N=64, ndisk=3, seed=1001, tilt=True, span=1.0):
     coords = W.grid_coords(N, span)
     sb = W.booster(coords, ndisk=ndisk, seed=seed)
