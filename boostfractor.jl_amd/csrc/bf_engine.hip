@@ -2367,6 +2367,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
                     continue;
                 }
                 std::vector<FieldDesc> r1, c, inv, p;
+                std::vector<std::vector<FieldDesc>> pj;  // pair entries per step, ascending j
                 Step st{};
                 for (int j = jlo; j <= jhi; j += 2) {
                     if (j == 0 || j == Nd - 2) {
@@ -2381,15 +2382,20 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
                         if (!xs[j + 1]) add(r1, j + 1, 0);
                         add(c, j, 0);
                         add(c, j + 1, 0);
+                        pj.emplace_back();
                         for (int f = 0; f < nb; f++) {
                             FieldDesc d0 = fd[single_off[j] + f], d1 = fd[single_off[j + 1] + f];
                             d0.pad = d1.pad = j << FD_PAIR_SHIFT;
-                            p.push_back(d0);
-                            p.push_back(d1);
+                            pj.back().push_back(d0);
+                            pj.back().push_back(d1);
                         }
                         xs[j] = xs[j + 1] = 1;
                     }
                 }
+                // The column pass takes the steps in ascending j and the pair kernel in
+                // descending j: each launch then reads first what the launch before it
+                // wrote last, the part of its output still in the Infinity Cache.
+                for (auto it = pj.rbegin(); it != pj.rend(); ++it) p.insert(p.end(), it->begin(), it->end());
                 auto put = [&](const std::vector<FieldDesc>& v, size_t& off, int& n) {
                     off = wl.size();
                     n = (int)v.size();
