@@ -218,6 +218,23 @@ def test_cheerleader_wavefront_matches_sweep(monkeypatch, prec, N, ndisk, nmax):
     assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
 
 
+def test_cheerleader_wavefront_several_batches(monkeypatch):
+    """260 frequencies run as two batches (256 + 4): the wavefront lists of the
+    smaller last batch, against the step-by-step sweep bit for bit and two
+    points (one per batch) against the oracle."""
+    coords, sb = _small_booster(N=32, ndisk=1, seed=1013)
+    freqs = np.linspace(19.5e9, 20.5e9, 260)
+    out = {}
+    for wz in ("0", "1"):
+        monkeypatch.setenv("BF_CHWAVE", wz)
+        out[wz] = B.cheerleader(0, 6, sb, coords, f=freqs, diskR=DISKR)
+    assert np.array_equal(out["1"], out["0"])
+    os_, oc = osetup(sb), ocoords(coords)
+    for q in (3, 258):
+        want = O.cheerleader(os_, oc, 0, 6, freqs[q], diskR=DISKR)
+        assert rel(out["1"][q], want) < TOL, (q, rel(out["1"][q], want))
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", TOL), ("mixed", 1e-5)])
 def test_cheerleader_emit_matches_oracle(prec, tol):
     """User-supplied emission fields (Nd x 2 x grid) instead of dance_intro, on a
