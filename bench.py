@@ -290,6 +290,8 @@ def transformer_main(args):
                    "grid": N3, "modes": n, "frequencies": NF3},
         "roofline": roof,
         "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
+        # useful flops (8 n^2 per complex multiply-add of the product) over each class's HIP-event time
+        "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
         "work_reductions": "air regions by k-bin groups (8 n^2 G per region, G = distinct (kx^2, ky^2) pairs); "
                            "tilted/rough disks by a frequency series of moment Grams",
         "beta2": {"min": float(beta2.min()), "max": float(beta2.max())},
@@ -394,6 +396,8 @@ def sweep_main(args):
         "note": f"inputs resident (TransformerBatchPlan); the host-buffer API (transformer_batch) on the same "
                 f"sweep: {host_pts:.1f} points/s",
         "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
+        # useful flops (8 n^2 per complex multiply-add of the product) over each class's HIP-event time
+        "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
         "beta2": {"min": float(min(np.sum(np.abs(b) ** 2, axis=-1).min() for b in boost)),
                   "max": float(max(np.sum(np.abs(b) ** 2, axis=-1).max() for b in boost))},
     }
