@@ -3,6 +3,7 @@
 There is no CPU fallback: if the HIP library is missing or no device is
 visible, every engine call raises.
 """
+import contextlib
 import ctypes
 import os
 import threading
@@ -83,6 +84,8 @@ _SIGS = {
     "bf_dancer_plan_info": (ctypes.c_int, [_P, _P, _P, _P]),
     "bf_ctx_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "bf_ctx_kernel_time": (ctypes.c_int, [_P, ctypes.c_char_p, _P, _P, _P]),
+    "bf_ctx_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int64]),
+    "bf_ctx_get_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
     "bf_stream_copy_probe": (ctypes.c_int, [_P, ctypes.c_int64, _I32, ctypes.POINTER(_D)]),
     "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),
@@ -202,6 +205,27 @@ class Context:
         check(lib().bf_ctx_kernel_time(self.handle, kernel.encode(), ctypes.byref(ms), ctypes.byref(n),
                                        ctypes.byref(fl)), self.handle)
         return ms.value, n.value, fl.value
+
+    def set_option(self, name, value):
+        """set one engine option of this context (bf_ctx_set_option)"""
+        check(lib().bf_ctx_set_option(self.handle, name.encode(), int(value)), self.handle)
+
+    def get_option(self, name):
+        v = ctypes.c_int64()
+        check(lib().bf_ctx_get_option(self.handle, name.encode(), ctypes.byref(v)), self.handle)
+        return v.value
+
+    @contextlib.contextmanager
+    def options(self, **kw):
+        """engine options set for the body of a `with` block, restored afterwards"""
+        old = {k: self.get_option(k) for k in kw}
+        try:
+            for k, v in kw.items():
+                self.set_option(k, v)
+            yield self
+        finally:
+            for k, v in old.items():
+                self.set_option(k, v)
 
     def stream_copy_probe(self, nbytes, reps=20):
         """mean ms per device-to-device copy of nbytes with the engine's copy kernel"""

@@ -27,9 +27,20 @@ static int fail(const bf_ctx* ctx, int code, const std::string& msg) {
         if (!(cond)) return fail(ctx, code, msg); \
     } while (0)
 
+namespace bf {
+static thread_local const EngineOptions* t_opt = nullptr;
+const EngineOptions& opts() {
+    static const EngineOptions defaults;
+    return t_opt ? *t_opt : defaults;
+}
+OptionScope::OptionScope(const EngineOptions* o) : prev(t_opt) { t_opt = o; }
+OptionScope::~OptionScope() { t_opt = prev; }
+}  // namespace bf
+
 template <typename F>
 static int guarded(bf_ctx* ctx, F&& body) {
     try {
+        bf::OptionScope scope(ctx ? &ctx->opt : nullptr);
         if (ctx) BF_HIP(hipSetDevice(ctx->device));
         body();
         return BF_OK;
@@ -170,6 +181,52 @@ int bf_ctx_kernel_time(bf_ctx* ctx, const char* kernel, double* ms, int64_t* lau
     });
 }
 
+// name -> option field of bf::EngineOptions
+static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp, int*& ip) {
+    struct Int { const char* n; int bf::EngineOptions::*m; };
+    static const Int ints[] = {
+        {"fft_lds", &bf::EngineOptions::fft_lds}, {"prune", &bf::EngineOptions::prune},
+        {"qtab", &bf::EngineOptions::qtab}, {"band", &bf::EngineOptions::band},
+        {"chfuse", &bf::EngineOptions::chfuse}, {"chxf", &bf::EngineOptions::chxf},
+        {"chwave", &bf::EngineOptions::chwave}, {"graph", &bf::EngineOptions::graph},
+        {"col_tx", &bf::EngineOptions::col_tx}, {"rows", &bf::EngineOptions::rows},
+        {"parseval_groups", &bf::EngineOptions::parseval_groups},
+        {"disk_expansion", &bf::EngineOptions::disk_expansion},
+    };
+    ip = nullptr;
+    for (const Int& e : ints)
+        if (!strcmp(name, e.n)) {
+            ip = &(o.*e.m);
+            tmp = *ip;
+            return &tmp;
+        }
+    if (!strcmp(name, "parseval_bytes")) return &o.parseval_bytes;
+    if (!strcmp(name, "batch_bytes")) return &o.batch_bytes;
+    return nullptr;
+}
+
+int bf_ctx_set_option(bf_ctx* ctx, const char* name, int64_t value) {
+    BF_REQUIRE(ctx, ctx && name, BF_ERR_ARG, "ctx or name is NULL");
+    int64_t tmp = 0;
+    int* ip = nullptr;
+    int64_t* slot = option_slot(ctx->opt, name, tmp, ip);
+    BF_REQUIRE(ctx, slot != nullptr, BF_ERR_ARG, std::string("unknown option: ") + name);
+    BF_REQUIRE(ctx, value >= 0 && (!ip || value <= 1 << 30), BF_ERR_ARG, "option values must be >= 0");
+    if (ip) *ip = (int)value;
+    else *slot = value;
+    return BF_OK;
+}
+
+int bf_ctx_get_option(bf_ctx* ctx, const char* name, int64_t* value) {
+    BF_REQUIRE(ctx, ctx && name && value, BF_ERR_ARG, "ctx, name or value is NULL");
+    int64_t tmp = 0;
+    int* ip = nullptr;
+    int64_t* slot = option_slot(ctx->opt, name, tmp, ip);
+    BF_REQUIRE(ctx, slot != nullptr, BF_ERR_ARG, std::string("unknown option: ") + name);
+    *value = *slot;
+    return BF_OK;
+}
+
 int bf_ctx_synchronize(bf_ctx* ctx) {
     BF_REQUIRE(ctx, ctx, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] { BF_HIP(hipStreamSynchronize(ctx->stream)); });
@@ -217,15 +274,13 @@ int bf_unnormalized_match(bf_ctx* ctx, int32_t nfield, int64_t n, const double* 
     return guarded(ctx, [&] { bf::unnormalized_match_dev(ctx, nfield, n, Eout, Eantenna, dx, dy, out); });
 }
 
-// Launch-shape and band overrides for tuning and A/B checks: BF_COL_TX
-// (columns per column-pass block), BF_ROWS (rows per dancer row block),
-// BF_BAND=0 (process every row instead of the aperture's row band), BF_GRAPH=0
-// (plans launch every bounce instead of replaying captured graphs).
-static bf::DancerOptions env_options() {
+// Launch shapes and the row band from the context's options (bf_ctx_set_option).
+static bf::DancerOptions ctx_options() {
+    const bf::EngineOptions& e = bf::opts();
     bf::DancerOptions o;
-    if (const char* v = getenv("BF_COL_TX")) o.col_tx = atoi(v);
-    if (const char* v = getenv("BF_ROWS")) o.rows = atoi(v);
-    if (const char* v = getenv("BF_BAND")) o.band = atoi(v);
+    o.col_tx = e.col_tx;
+    o.rows = e.rows;
+    o.band = e.band;
     return o;
 }
 
@@ -242,7 +297,7 @@ int bf_dancer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, doubl
     BF_REQUIRE(ctx, Eout || power, BF_ERR_ARG, "Eout and power are both NULL");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
-        bf::DancerOptions o = env_options();
+        bf::DancerOptions o = ctx_options();
         o.reflect = reflect;  // reflect takes precedence over emit (src/boost_fractor_recurprop.jl:53-63)
         o.emit = reflect ? nullptr : emit;
         o.returnsum = returnsum ? 1 : 0;
@@ -262,6 +317,7 @@ int bf_cheerleader(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, 
     TRY(check_coords(ctx, coords));
     TRY(check_setup(ctx, setup, 3, true));
     TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nmax >= -1, BF_ERR_ARG, "nmax must be >= -1");
     BF_REQUIRE(ctx, out_right != nullptr, BF_ERR_ARG, "out_right is NULL");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
@@ -292,10 +348,9 @@ int bf_dancer_plan_create(bf_ctx* ctx, const bf_setup* setup, const bf_coords* c
     BF_REQUIRE(ctx, freq_batch >= 0, BF_ERR_ARG, "freq_batch must be >= 0");
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
-        bf::DancerOptions o = env_options();
+        bf::DancerOptions o = ctx_options();
         o.freq_batch = freq_batch;
-        const char* gv = getenv("BF_GRAPH");
-        o.graph = (gv && std::string(gv) == "0") ? 0 : 1;  // a plan is run many times
+        o.graph = bf::opts().graph ? 1 : 0;  // a plan is run many times
         auto* p = new bf_dancer_plan();
         try {
             p->impl = bf::make_dancer_plan(ctx, setup, coords, amin, nmax, freqs, nfreq, prop_kind, diskR, precision, o);

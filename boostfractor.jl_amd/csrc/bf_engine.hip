@@ -870,6 +870,11 @@ __global__ void k_amin_check(int* active, double* amax, double amin, int fb) {
     amax[f] = 0.0;
 }
 
+// one thread per frequency slot of the batch (batches hold up to 256 slots)
+static void launch_amin_check(hipStream_t s, int* active, double* amax, double amin, int nb) {
+    hipLaunchKernelGGL(k_amin_check, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, s, active, amax, amin, nb);
+}
+
 // dance_intro fill (src/boost_fractor_recurprop.jl:153-201): fields[f][n][s] = ax[n][s] * [x^2+y^2 <= R^2]
 // bfield: null -> the default real mask x^2 + y^2 > R^2 ? 0 : 1 (ax .* mask, complex x real);
 // else complex [nd+1][K]: region n's right-moving amplitude multiplies layer n, its
@@ -1020,14 +1025,7 @@ static void dispatch_n(int n, A&&... args) {
     }
 }
 
-static bool use_reg_fft() {
-    static int v = -1;
-    if (v < 0) {
-        const char* s = getenv("BF_FFT");
-        v = (s && std::string(s) == "lds") ? 0 : 1;
-    }
-    return v == 1;
-}
+static bool use_reg_fft() { return !opts().fft_lds; }
 
 template <typename T, int N>
 struct RowLaunch {
@@ -1059,10 +1057,9 @@ struct RowLaunch {
 
 // Band class of the pruned register transforms: the largest BM in {5, 3} with
 // every possibly nonzero element m of a thread in [BM, E-1-BM] (elements mlo..mhi),
-// else 0 (no pruning).  BF_PRUNE=0 switches the pruned kernels off.
+// else 0 (no pruning).  Option prune=0 switches the pruned kernels off.
 static int prune_class(int mlo, int mhi, int E) {
-    const char* v = getenv("BF_PRUNE");
-    if ((v && std::string(v) == "0") || mhi < mlo) return 0;
+    if (!opts().prune || mhi < mlo) return 0;
     const int bm = std::min(mlo, E - 1 - mhi);
     return bm >= 5 ? 5 : bm >= 3 ? 3 : 0;
 }
@@ -1464,10 +1461,9 @@ void dance_intro_dev(bf_ctx* ctx, const bf_coords* coords, int nd, const double*
 // from its mirror's k, equal up to the rounding of the k range (~1e-15 rad per
 // propagation), far inside what complex64 storage rounds away; the FP64 path
 // keeps full tables.  Needs the register propagation pass (square power-of-two
-// grid) and a mirror-symmetric k range.  BF_QTAB=0 switches it off.
+// grid) and a mirror-symmetric k range.  Option qtab=0 switches it off.
 template <typename T>
 static bool quarter_tables_ok(const bf_coords* co, int kind) {
-    const char* q = getenv("BF_QTAB");
     const int n = co->nx;
     auto mirrored = [](const double* k, int len) {
         double mx = 0.0;
@@ -1476,7 +1472,7 @@ static bool quarter_tables_ok(const bf_coords* co, int kind) {
             if (!(std::fabs(k[j] + k[len - 1 - j]) <= 1e-12 * mx)) return false;
         return true;
     };
-    return !std::is_same<T, double>::value && !(q && std::string(q) == "0") && use_reg_fft() &&
+    return !std::is_same<T, double>::value && opts().qtab && use_reg_fft() &&
            (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS) && co->nx == co->ny && n >= 16 && n <= 1024 &&
            (n & (n - 1)) == 0 && mirrored(co->kX, co->nx) && mirrored(co->kY, co->ny);
 }
@@ -1734,8 +1730,7 @@ struct DancerPlan : DancerPlanBase {
             dispatch_n<DancerRowLaunch, T>(g.nx, a, nb, opt.rows, s);
             tmark(n, 0, 1);
             if (amin != 0.0) {
-                hipLaunchKernelGGL(k_amin_check, dim3(1), dim3(64), 0, s, active.as<int>(), amax.as<double>(),
-                                   amin, nb);
+                launch_amin_check(s, active.as<int>(), amax.as<double>(), amin, nb);
                 BF_CHECK_LAUNCH();
             }
             rm = 1 - rm;
@@ -1865,8 +1860,7 @@ struct DancerPlan : DancerPlanBase {
                                        count.as<int>() + f0);
                     BF_CHECK_LAUNCH();
                     if (amin != 0.0) {
-                        hipLaunchKernelGGL(k_amin_check, dim3(1), dim3(64), 0, s, active.as<int>(), amax.as<double>(),
-                                           amin, nb);
+                        launch_amin_check(s, active.as<int>(), amax.as<double>(), amin, nb);
                         BF_CHECK_LAUNCH();
                     }
                     rm = 1 - rm;
@@ -2166,18 +2160,16 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
     PropEngine<T> pe(ctx, &g, kind, diskR, surf.as<zd>());
     pe.quarter = quarter_tables_ok<T>(co, kind);
     // the pair steps' inverse row pass fused with their boundary update (register
-    // path, propagator and propagatorNoTilts; BF_CHFUSE=0 runs them separately)
+    // path, propagator and propagatorNoTilts; option chfuse=0 runs them separately)
     bool fuse_pair = false;
     {
         // every propagation starts by masking: its masked row FFT and column pass
-        // need only the aperture's row band (register path; BF_BAND=0 disables)
-        const char* b = getenv("BF_BAND");
+        // need only the aperture's row band (register path; option band=0 disables)
         auto pow2 = [](int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; };
-        if (!(b && std::string(b) == "0") && use_reg_fft() && pow2(g.nx) && pow2(g.ny) &&
+        if (opts().band && use_reg_fft() && pow2(g.nx) && pow2(g.ny) &&
             (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS))
             aperture_band(co, diskR, pe.blo, pe.bhi);
-        const char* fz = getenv("BF_CHFUSE");
-        fuse_pair = !(fz && std::string(fz) == "0") && use_reg_fft() && pow2(g.nx) &&
+        fuse_pair = opts().chfuse && use_reg_fft() && pow2(g.nx) &&
                     (kind == BF_PROP_PROPAGATOR || kind == BF_PROP_NOTILTS);
     }
     // The pair steps update the interior fields F[1..Nd-2] only; each is masked
@@ -2189,9 +2181,8 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
     // propagation's masked row transform (every reader of an interior field is
     // a masked propagation).  xf[r]: F[r] holds that transform of its band rows,
     // the same for every frequency of the batch (all stay active when amin is 0);
-    // BF_CHXF=0 stores them in real space.
-    const char* xz = getenv("BF_CHXF");
-    const bool pair_xform = pair_band && !(xz && std::string(xz) == "0");
+    // option chxf=0 stores them in real space.
+    const bool pair_xform = pair_band && opts().chxf;
     std::vector<char> xf(Nd, 0);
     // descriptors: per region i, fields (f, i) for all f; pairs (f, i), (f, i+1)
     std::vector<FieldDesc> fd;
@@ -2221,9 +2212,8 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
     // region; step j updates fields j and j+1) runs at time 2n + j.  Steps that
     // share a field keep their sequential order and the steps of one time touch
     // disjoint fields, so each time's steps share launches and the results are
-    // those of the sequential sweep.  BF_CHWAVE=0 sweeps step by step.
-    const char* wz = getenv("BF_CHWAVE");
-    const bool wave = pair_xform && Nd >= 4 && !(wz && std::string(wz) == "0") &&
+    // those of the sequential sweep.  Option chwave=0 sweeps step by step.
+    const bool wave = pair_xform && Nd >= 4 && opts().chwave &&
                       (size_t)(Nd / 2 + 1) * 2 * fb <= 65535;  // fields per launch within grid.y
     DevBuf dwave;
     dfd.alloc(sizeof(FieldDesc) * fd.size());
@@ -2469,7 +2459,7 @@ void cheerleader_run(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, doubl
             if (amin != 0.0) {
                 hipLaunchKernelGGL(k_amax_fields<T>, dim3(grid1d(K * Nd, 256, 1024), nb), dim3(256), 0, s, Fp,
                                    K * Nd, amx.as<double>(), A);
-                hipLaunchKernelGGL(k_amin_check, dim3(1), dim3(64), 0, s, act.as<int>(), amx.as<double>(), amin, nb);
+                launch_amin_check(s, act.as<int>(), amx.as<double>(), amin, nb);
                 BF_CHECK_LAUNCH();
             }
         }

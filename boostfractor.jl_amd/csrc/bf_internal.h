@@ -75,8 +75,40 @@ struct bf_kernel_timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
 
+namespace bf {
+// Switches of the engine's work reductions, launch shapes and chunk budgets.
+// Each context holds its own set (bf_ctx_set_option); the defaults are the
+// product configuration, the other values exist for A/B checks and tuning.
+// No setting changes a result beyond the rounding the tests bound.
+struct EngineOptions {
+    int fft_lds = 0;          // 1: LDS FFT passes instead of the register-resident ones
+    int prune = 1;            // band-pruned register transforms
+    int qtab = 1;             // quarter k-space tables for complex64 fields
+    int band = 1;             // skip the rows the aperture zeroes
+    int chfuse = 1;           // cheerleader: pair steps fused with their boundary update
+    int chxf = 1;             // cheerleader: pair steps store the next masked row transform
+    int chwave = 1;           // cheerleader: wavefront order when amin is 0
+    int graph = 1;            // dancer plans replay captured HIP graphs
+    int col_tx = 0, rows = 0; // dancer launch shapes (0: tuned/default)
+    int parseval_groups = 1;  // flat air regions by k-bin groups
+    int disk_expansion = 1;   // tilted/rough disks by a frequency series of moment Grams
+    int64_t parseval_bytes = 0, batch_bytes = 0;  // chunk budgets (0: defaults)
+};
+// Options of the context whose call runs on this thread (defaults outside a call).
+const EngineOptions& opts();
+// Makes `o` the running call's options for the lifetime of the object.
+struct OptionScope {
+    const EngineOptions* prev;
+    explicit OptionScope(const EngineOptions* o);
+    ~OptionScope();
+    OptionScope(const OptionScope&) = delete;
+    OptionScope& operator=(const OptionScope&) = delete;
+};
+}  // namespace bf
+
 // Per-caller context: one device, one stream, cached twiddle tables.
 struct bf_ctx {
+    bf::EngineOptions opt;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;

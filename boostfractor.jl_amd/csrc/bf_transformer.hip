@@ -1213,14 +1213,11 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         fft_fields(ctx, g, FhatM.as<zd>(), n * md.C, true, diskR);
         mc.haveF = true;
     }
-    // Parseval regions through k-bin groups (BF_PARSEVAL_GROUPS=0: per-bin weights)
+    // Parseval regions through k-bin groups (option parseval_groups=0: per-bin weights)
     bool grouped = false;
-    if (!parseval.empty()) {
-        const char* e = std::getenv("BF_PARSEVAL_GROUPS");
-        if (!(e && e[0] == '0')) grouped = mc.haveH || build_parseval_groups(ctx, g, md, mc);
-    }
+    if (!parseval.empty() && opts().parseval_groups) grouped = mc.haveH || build_parseval_groups(ctx, g, md, mc);
     // frequency-dependent pieces, in frequency batches bounded by table memory
-    // tilted/rough disk regions: moments of the frequency expansion (BF_DISK_EXPANSION=0: per-frequency tables)
+    // tilted/rough disk regions: moments of the frequency expansion (option disk_expansion=0: per-frequency tables)
     std::vector<int> exp_q0(nreg, -1), exp_np(nreg, 0);
     std::vector<double> exp_hs(nreg, 0.0);
     std::vector<zd> exp_ac(nreg);
@@ -1230,8 +1227,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
         std::vector<int> rough;
         for (int r : disks)
             if (!(flat[r] && tx[r] == 0.0 && ty[r] == 0.0)) rough.push_back(r);
-        const char* e = std::getenv("BF_DISK_EXPANSION");
-        if (!rough.empty() && nf > 2 && !(e && e[0] == '0')) {
+        if (!rough.empty() && nf > 2 && opts().disk_expansion) {
             std::vector<MomDesc> hd;
             for (int r : rough)
                 hd.push_back(MomDesc{cmk<double>(0.0, 0.0), tx[r], ty[r], flat[r] ? -1 : (int64_t)(K * r), 0.0, 0, 0});
@@ -1475,12 +1471,8 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
     if (q.pd.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, total = q.pd.size();
-    // problems per GEMM: W and the GEMM output within 4 GB (BF_PARSEVAL_BYTES overrides), grid.y <= 65535
-    size_t budget = (size_t)4 << 30;
-    if (const char* e = std::getenv("BF_PARSEVAL_BYTES")) {
-        const long long v = std::atoll(e);
-        if (v > 0) budget = (size_t)v;
-    }
+    // problems per GEMM: W and the GEMM output within 4 GB (option parseval_bytes overrides), grid.y <= 65535
+    const size_t budget = opts().parseval_bytes > 0 ? (size_t)opts().parseval_bytes : (size_t)4 << 30;
     const size_t mc_ = std::max<size_t>(1, std::min<size_t>({total, (size_t)65535, budget / ((NG + nn) * sizeof(zd))}));
     DevBuf& Wt = scratch_buf(ctx, SB_W, sizeof(zd) * mc_ * NG);
     DevBuf& Cm = scratch_buf(ctx, SB_CM, sizeof(zd) * mc_ * nn);
@@ -1869,12 +1861,8 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
     }
     if (pl->dboost.bytes < sizeof(zd) * total) pl->dboost.alloc(sizeof(zd) * total);
     pl->nboost = total;
-    // bytes of P + Pinv per chunk (BF_BATCH_BYTES overrides, e.g. to exercise chunking on small cases)
-    size_t budget = (size_t)2 << 30;
-    if (const char* e = std::getenv("BF_BATCH_BYTES")) {
-        const long long v = std::atoll(e);
-        if (v > 0) budget = (size_t)v;
-    }
+    // bytes of P + Pinv per chunk (option batch_bytes overrides, e.g. to exercise chunking on small cases)
+    const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : (size_t)2 << 30;
     for (int set = 0; set < (int)pl->md.size(); set++) {
         std::vector<int> confs;
         for (int c = 0; c < nconf; c++)

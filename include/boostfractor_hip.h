@@ -102,6 +102,20 @@ int bf_ctx_synchronize(bf_ctx *ctx);
  * (transfer-matrix chain and solves, 64 n^3 flop per region and frequency). */
 int bf_ctx_timing(bf_ctx *ctx, int32_t enable);
 int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *launches, double *flops);
+/* Engine options of a context (no reference counterpart; the nearest is the
+ * planner flags FFTW's plan_fft takes, src/boost_fractor.jl:163-164).  The
+ * defaults are the product configuration; the library never reads the process
+ * environment.  Switches (0/1): "fft_lds" (LDS FFT passes instead of the
+ * register-resident ones), "prune" (band-pruned transforms), "qtab" (quarter
+ * k-space tables for complex64 fields), "band" (skip the rows the aperture
+ * zeroes), "chfuse", "chxf", "chwave" (cheerleader pair fusion, stored row
+ * transforms, wavefront order), "graph" (dancer plans replay HIP graphs),
+ * "parseval_groups", "disk_expansion" (transformer work reductions).  Shapes
+ * and budgets (0 = default): "col_tx", "rows" (dancer launch shapes),
+ * "parseval_bytes", "batch_bytes" (chunk budgets).  Unknown names and negative
+ * values return BF_ERR_ARG.  No option changes a result beyond rounding. */
+int bf_ctx_set_option(bf_ctx *ctx, const char *name, int64_t value);
+int bf_ctx_get_option(bf_ctx *ctx, const char *name, int64_t *value);
 
 /* ---- propagators --------------------------------------------------------
  * replaces prop(E0, dz, diskR, eps, tilt_x, tilt_y, surface, lambda, coords)

@@ -191,6 +191,23 @@ def test_cheerleader_matches_oracle(prop, okind):
         assert rel(right[q], wr) < TOL and rel(left[q], wl) < TOL, q
 
 
+def test_cheerleader_amin_more_than_64_frequencies():
+    """amin with 70 frequencies in one batch: the stop criterion covers every
+    frequency slot, including those at index 64 and above, against the oracle;
+    the criterion really stops those points early (their result differs from
+    the run to nmax)"""
+    coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
+    os_, oc = osetup(sb), ocoords(coords)
+    freqs = np.linspace(19.5e9, 20.5e9, 70)
+    got = B.cheerleader(1.5, 40, sb, coords, f=freqs, diskR=DISKR)
+    full = B.cheerleader(0, 40, sb, coords, f=freqs[64:], diskR=DISKR)
+    for q in (0, 63, 64, 66, 69):
+        want = O.cheerleader(os_, oc, 1.5, 40, freqs[q], diskR=DISKR)
+        assert rel(got[q], want) < TOL, (q, rel(got[q], want))
+        if q >= 64:
+            assert rel(got[q], full[q - 64]) > 0.1, q  # stopped early (the oracle stops these near 0.7 rel)
+
+
 def test_cheerleader_amin_reflect():
     coords, sb = _small_booster(N=32, ndisk=2, tilt=False)
     os_, oc = osetup(sb), ocoords(coords)
@@ -203,7 +220,7 @@ def test_cheerleader_amin_reflect():
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("N,ndisk,nmax", [(64, 3, 20), (128, 2, 9), (64, 1, 7), (64, 2, 0)])
-def test_cheerleader_wavefront_matches_sweep(monkeypatch, prec, N, ndisk, nmax):
+def test_cheerleader_wavefront_matches_sweep(prec, N, ndisk, nmax):
     """Steps of several iterations sharing launches (wavefront order) against the
     step-by-step sweep: the same kernels on the same fields in an order that
     keeps every dependency, so both outputs agree bit for bit (Nd = 4 and
@@ -211,14 +228,14 @@ def test_cheerleader_wavefront_matches_sweep(monkeypatch, prec, N, ndisk, nmax):
     coords, sb = _small_booster(N=N, ndisk=ndisk, seed=1011 + N + ndisk)
     out = {}
     for wz in ("0", "1"):
-        monkeypatch.setenv("BF_CHWAVE", wz)
-        out[wz] = B.cheerleader(0, nmax, sb, coords, f=[19.9e9, 20.05e9, 20.2e9], diskR=DISKR, precision=prec,
-                                returnboth=True)
+        with B.context().options(chwave=int(wz)):
+            out[wz] = B.cheerleader(0, nmax, sb, coords, f=[19.9e9, 20.05e9, 20.2e9], diskR=DISKR, precision=prec,
+                                    returnboth=True)
     assert np.any(out["1"][1] != 0)
     assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
 
 
-def test_cheerleader_wavefront_several_batches(monkeypatch):
+def test_cheerleader_wavefront_several_batches():
     """260 frequencies run as two batches (256 + 4): the wavefront lists of the
     smaller last batch, against the step-by-step sweep bit for bit and two
     points (one per batch) against the oracle."""
@@ -226,8 +243,8 @@ def test_cheerleader_wavefront_several_batches(monkeypatch):
     freqs = np.linspace(19.5e9, 20.5e9, 260)
     out = {}
     for wz in ("0", "1"):
-        monkeypatch.setenv("BF_CHWAVE", wz)
-        out[wz] = B.cheerleader(0, 6, sb, coords, f=freqs, diskR=DISKR)
+        with B.context().options(chwave=int(wz)):
+            out[wz] = B.cheerleader(0, 6, sb, coords, f=freqs, diskR=DISKR)
     assert np.array_equal(out["1"], out["0"])
     os_, oc = osetup(sb), ocoords(coords)
     for q in (3, 258):
@@ -293,7 +310,7 @@ def test_dancer_plan_matches_one_shot():
 
 
 @pytest.mark.parametrize("N,prec", [(64, "fp64"), (64, "mixed"), (64, "fp32"), (48, "fp64"), (40, "mixed")])
-def test_dancer_row_band_is_exact(monkeypatch, N, prec):
+def test_dancer_row_band_is_exact(N, prec):
     """Skipping the rows the aperture zeroes changes nothing: the fused bounce with
     the row band equals the bounce over every row, bit for bit (register and LDS
     FFT paths, all precisions, summed and history outputs)."""
@@ -302,9 +319,9 @@ def test_dancer_row_band_is_exact(monkeypatch, N, prec):
     freqs = [19.8e9, 20.1e9]
     out = {}
     for band in ("0", "1"):
-        monkeypatch.setenv("BF_BAND", band)
-        e = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
-        h = B.dancer(0, 9, sb, coords, f=20e9, diskR=DISKR, precision=prec, returnsum=False, immediatesum=False)
+        with B.context().options(band=int(band)):
+            e = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+            h = B.dancer(0, 9, sb, coords, f=20e9, diskR=DISKR, precision=prec, returnsum=False, immediatesum=False)
         out[band] = (e, h)
     assert np.any(out["1"][0] != 0)
     # the band really is narrower than the grid here
@@ -320,7 +337,7 @@ def test_dancer_row_band_is_exact(monkeypatch, N, prec):
 @pytest.mark.parametrize("N,span,prec", [(128, 1.0, "fp64"), (128, 1.0, "mixed"), (128, 1.0, "fp32"),
                                          (128, 0.5, "fp64"), (128, 0.6, "mixed"), (256, 1.0, "mixed"),
                                          (256, 1.0, "fp64")])
-def test_dancer_pruned_transforms_match_unpruned(monkeypatch, N, span, prec):
+def test_dancer_pruned_transforms_match_unpruned(N, span, prec):
     """Band-pruned register transforms (butterflies of zero elements skipped,
     elements that are not stored not formed) against the unpruned ones: summed and
     history outputs, tilted and rough disks (tables applied in the row pass) and
@@ -332,10 +349,10 @@ def test_dancer_pruned_transforms_match_unpruned(monkeypatch, N, span, prec):
     freqs = [19.8e9, 20.1e9]
     out = {}
     for prune in ("0", "1"):
-        monkeypatch.setenv("BF_PRUNE", prune)
-        e = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
-        h = B.dancer(0, 6, sb, coords, f=20e9, diskR=DISKR, precision=prec, returnsum=False, immediatesum=False)
-        a = B.dancer(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
+        with B.context().options(prune=int(prune)):
+            e = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+            h = B.dancer(0, 6, sb, coords, f=20e9, diskR=DISKR, precision=prec, returnsum=False, immediatesum=False)
+            a = B.dancer(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
         out[prune] = (e, h, a)
     tol = {"fp64": 1e-12, "mixed": 1e-5, "fp32": 1e-4}[prec]
     assert np.any(out["1"][0] != 0)
@@ -357,7 +374,7 @@ def test_dancer_pruned_matches_oracle(N):
 
 
 @pytest.mark.parametrize("N,prec", [(64, "mixed"), (128, "mixed"), (128, "fp32"), (256, "mixed")])
-def test_dancer_quarter_tables_match_full(monkeypatch, N, prec):
+def test_dancer_quarter_tables_match_full(N, prec):
     """Complex64-field paths read quarter k-space tables ([N/2+1]^2 entries, a
     mirrored bin reads its mirror's phase, equal up to the rounding of the k
     range): against full tables within the storage rounding (1e-5 of the field
@@ -366,15 +383,15 @@ def test_dancer_quarter_tables_match_full(monkeypatch, N, prec):
     freqs = [19.8e9, 20.1e9]
     out = {}
     for q in ("0", "1"):
-        monkeypatch.setenv("BF_QTAB", q)
-        out[q] = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
+        with B.context().options(qtab=int(q)):
+            out[q] = B.dancer(0, 20, sb, coords, f=freqs, diskR=DISKR, precision=prec)
     assert np.any(out["1"] != 0)
     tol = {"mixed": 1e-5, "fp32": 1e-4}[prec]
     assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
 
 
 @pytest.mark.parametrize("N,prec,amin", [(64, "fp64", 0.0), (128, "mixed", 0.0), (64, "mixed", 0.05)])
-def test_dancer_plan_graph_replays_match_eager(monkeypatch, N, prec, amin):
+def test_dancer_plan_graph_replays_match_eager(N, prec, amin):
     """Plans replay each frequency batch's bounce loop from a HIP graph captured by
     the first run: three runs in a row, with eager work and synchronisations
     between them, each equal bit for bit to the plan launched bounce by bounce;
@@ -382,12 +399,12 @@ def test_dancer_plan_graph_replays_match_eager(monkeypatch, N, prec, amin):
     instrumented (timing) run takes the launch-by-launch path and agrees too."""
     coords, sb = _small_booster(N=N, ndisk=3, seed=1006)
     freqs = [19.7e9, 19.9e9, 20.0e9, 20.2e9, 20.4e9]
-    monkeypatch.setenv("BF_GRAPH", "0")
-    eager = B.DancerPlan(amin, 30, sb, coords, freqs, diskR=DISKR, precision=prec, freq_batch=2)
-    eager.run()
-    want_E, want_p = eager.fetch()
-    eager.close()
-    monkeypatch.setenv("BF_GRAPH", "1")
+    with B.context().options(graph=0):
+        eager = B.DancerPlan(amin, 30, sb, coords, freqs, diskR=DISKR, precision=prec, freq_batch=2)
+        eager.run()
+        want_E, want_p = eager.fetch()
+        eager.close()
+    assert B.context().get_option("graph") == 1
     plan = B.DancerPlan(amin, 30, sb, coords, freqs, diskR=DISKR, precision=prec, freq_batch=2)
     for rep in range(3):
         plan.run()
@@ -419,16 +436,15 @@ def test_cheerleader_band_matches_oracle():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cheerleader_band_and_quarter_tables_match_full(monkeypatch, prec):
+def test_cheerleader_band_and_quarter_tables_match_full(prec):
     """Band-limited propagations (and, with complex64 fields, quarter phase
     tables) against every row and full tables: 1e-12 of the field maximum in
     FP64, 1e-5 with complex64 fields."""
     coords, sb = _small_booster(N=128, ndisk=3, seed=1008)
     out = {}
     for on in ("0", "1"):
-        monkeypatch.setenv("BF_BAND", on)
-        monkeypatch.setenv("BF_QTAB", on)
-        out[on] = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec)
+        with B.context().options(band=int(on), qtab=int(on)):
+            out[on] = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec)
     assert np.any(out["1"] != 0)
     tol = 1e-12 if prec == "fp64" else 1e-5
     assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
@@ -436,19 +452,18 @@ def test_cheerleader_band_and_quarter_tables_match_full(monkeypatch, prec):
 
 @pytest.mark.parametrize("N", [64, 128])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cheerleader_fused_pair_rows_match_separate(monkeypatch, prec, N):
+def test_cheerleader_fused_pair_rows_match_separate(prec, N):
     """The pair steps' inverse row pass fused with the boundary update against the
     separate passes (tilted and rough disks, both outputs, amin criterion; with
     amin = 0 the fused pair steps keep the aperture band's rows only and store
-    the next propagation's masked row transform, BF_CHXF=0 stores real space):
+    the next propagation's masked row transform, option chxf=0 stores real space):
     1e-12 of the field maximum in FP64, 1e-5 with complex64 fields."""
     coords, sb = _small_booster(N=N, ndisk=3, seed=1009)
     out = {}
     for fz, xf in (("0", "1"), ("1", "0"), ("1", "1")):
-        monkeypatch.setenv("BF_CHFUSE", fz)
-        monkeypatch.setenv("BF_CHXF", xf)
-        lr = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec, returnboth=True)
-        am = B.cheerleader(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
+        with B.context().options(chfuse=int(fz), chxf=int(xf)):
+            lr = B.cheerleader(0, 20, sb, coords, f=[19.8e9, 20.1e9], diskR=DISKR, precision=prec, returnboth=True)
+            am = B.cheerleader(0.05, 40, sb, coords, f=20e9, diskR=DISKR, precision=prec)
         out[fz + xf] = (lr[0], lr[1], am)
     tol = 1e-12 if prec == "fp64" else 1e-5
     assert np.any(out["11"][1] != 0) and np.any(out["11"][2] != 0)  # (the mirror side's output is zero)

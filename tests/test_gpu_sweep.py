@@ -80,11 +80,11 @@ def test_spacing_interpolation_matches_oracle_spline():
 
 
 @pytest.mark.parametrize("chunk_bytes", [None, "20000"])
-def test_spacing_interpolation_transformer(monkeypatch, chunk_bytes):
+def test_spacing_interpolation_transformer(engine_options, chunk_bytes):
     """chain on the interpolated matrices of several candidates (one launch, or
     several chunks) equals transformer(...; propagation_matrices=...) and the oracle"""
     if chunk_bytes:
-        monkeypatch.setenv("BF_BATCH_BYTES", chunk_bytes)
+        engine_options(batch_bytes=int(chunk_bytes))
     coords, sb, modes = _setup(ndisk=2)
     spacing = np.linspace(-2e-4, 2e-4, 5)
     freqs = [19.9e9, 20.1e9]

@@ -356,12 +356,12 @@ def _batch_geometries(N=32):
 
 
 @pytest.mark.parametrize("chunk_bytes", [None, "60000"])
-def test_transformer_batch_matches_single_geometry(monkeypatch, chunk_bytes):
+def test_transformer_batch_matches_single_geometry(engine_options, chunk_bytes):
     """transformer_batch over geometries with uneven region counts and two mode
     sets (n = 6 and 9) equals transformer per geometry; with a small matrix
     budget the batch runs in several chunks per mode set"""
     if chunk_bytes:
-        monkeypatch.setenv("BF_BATCH_BYTES", chunk_bytes)
+        engine_options(batch_bytes=int(chunk_bytes))
     coords, geoms = _batch_geometries()
     sets = [_lg_like_modes(coords, 2, 1), _lg_like_modes(coords, 3, 1, seed=5)]
     set_of = [0, 1, 1, 0, 1, 0]
@@ -385,13 +385,13 @@ def test_transformer_batch_matches_single_geometry(monkeypatch, chunk_bytes):
 
 
 @pytest.mark.parametrize("chunk_bytes", [None, "60000"])
-def test_transformer_batch_plan_matches_batch(monkeypatch, chunk_bytes):
+def test_transformer_batch_plan_matches_batch(engine_options, chunk_bytes):
     """TransformerBatchPlan (mode sets, sources and frequencies resident) equals
     transformer_batch bit for bit, on two runs in a row, the second over fewer
     geometries in another order and mode-set assignment; fetch() returns the last
     run's geometries only; with a small matrix budget every run is chunked"""
     if chunk_bytes:
-        monkeypatch.setenv("BF_BATCH_BYTES", chunk_bytes)
+        engine_options(batch_bytes=int(chunk_bytes))
     coords, geoms = _batch_geometries()
     sets = [_lg_like_modes(coords, 2, 1), _lg_like_modes(coords, 3, 1, seed=5)]
     emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
@@ -426,7 +426,7 @@ def test_transformer_batch_matches_oracle():
 
 
 @pytest.mark.parametrize("lossy", [False, True])
-def test_propagation_matrices_disk_frequency_expansion(monkeypatch, lossy):
+def test_propagation_matrices_disk_frequency_expansion(engine_options, lossy):
     """tilted/rough disks over a 24-frequency band take the frequency expansion
     (moments of the height map): every region against the oracle at 1e-10 and
     against per-frequency weight tables; a lossy disk has a complex k0"""
@@ -436,7 +436,7 @@ def test_propagation_matrices_disk_frequency_expansion(monkeypatch, lossy):
     modes = _lg_like_modes(coords, 2, 1)
     freqs = np.linspace(19.5e9, 20.5e9, 24)
     got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
-    monkeypatch.setenv("BF_DISK_EXPANSION", "0")
+    engine_options(disk_expansion=0)
     direct = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
     for q in range(len(freqs)):
         for r in range(1, len(sb.distance)):
@@ -483,14 +483,14 @@ def test_transformer_more_than_48_modes_uses_global_chain():
     assert rel(boost, want_b) < tol and rel(refl, want_r) < tol, (rel(boost, want_b), rel(refl, want_r), sens)
 
 
-def test_parseval_groups_match_per_bin_weights(monkeypatch):
+def test_parseval_groups_match_per_bin_weights(engine_options):
     """flat air regions: the k-bin group GEMM against per-bin weight tables in the
-    Gram kernel (BF_PARSEVAL_GROUPS=0) and against the oracle"""
+    Gram kernel (option parseval_groups=0) and against the oracle"""
     coords, sb = _flat_booster(N=48, ndisk=2)
     modes = _lg_like_modes(coords, 3, 1)
     freqs = [19.8e9, 20.2e9]
     got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
-    monkeypatch.setenv("BF_PARSEVAL_GROUPS", "0")
+    engine_options(parseval_groups=0)
     per_bin = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
     for q, f in enumerate(freqs):
         want = _oracle_mats(sb, coords, modes, f, O.PROP_PROPAGATOR)

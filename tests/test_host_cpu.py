@@ -130,10 +130,29 @@ def test_cabi_validation_without_device():
     assert rc == L.BF_ERR_ARG and b"InexactError" in lib.bf_last_error(None)
     with pytest.raises(B.InexactError):
         B.dance_intro(sb, W.grid_coords(8), velocity_x=1e-3)
+    # cheerleader: nmax below -1 is rejected like the dancer's (no device work)
+    rc = lib.bf_cheerleader(None, ctypes.byref(s), ctypes.byref(c), 0.0, -2, L.cptr(f), 1, 0, None, None, 0.15, 0,
+                            L.cptr(out), None)
+    assert rc == L.BF_ERR_ARG and b"nmax" in lib.bf_last_error(None)
+    rc = lib.bf_cheerleader(None, ctypes.byref(s), ctypes.byref(c), 0.0, -(2 ** 31), L.cptr(f), 1, 0, None, None,
+                            0.15, 0, L.cptr(out), None)
+    assert rc == L.BF_ERR_ARG
+    # engine options need a context and a known name
+    assert lib.bf_ctx_set_option(None, b"prune", 0) == L.BF_ERR_ARG
+    v = ctypes.c_int64()
+    assert lib.bf_ctx_get_option(None, b"prune", ctypes.byref(v)) == L.BF_ERR_ARG
     # gauss_profile / unnormalized_match argument checks
     g = np.zeros((8, 8), dtype=complex)
     assert lib.bf_gauss_profile(None, ctypes.byref(c), 1e-9, -0.1, 10e9, L.cptr(g)) == L.BF_ERR_ARG
     assert lib.bf_unnormalized_match(None, 0, 64, L.cptr(g), L.cptr(g), 0.01, 0.01, L.cptr(g)) == L.BF_ERR_ARG
+
+
+def test_library_reads_no_environment():
+    """engine switches are per-context options (bf_ctx_set_option), never
+    process-environment variables: no source of the library calls getenv"""
+    csrc = os.path.join(ROOT, "boostfractor.jl_amd", "csrc")
+    for f in os.listdir(csrc):
+        assert "getenv" not in open(os.path.join(csrc, f)).read(), f
 
 
 def test_mirror_rejects_user_propagators():
