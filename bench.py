@@ -305,24 +305,11 @@ def transformer_main(args):
 # disk-position sweep: seeded boosters with 10..80 disks (spacings 8 mm +- 0.5 mm),
 # 9/15/25 Bessel modes, 256^2, 100 frequencies on 19-21 GHz per geometry
 N5, NF5, NCONF5 = 256, 100, 4096
-MODESETS5 = [(3, 1), (5, 1), (5, 2)]  # (M, L): n = M (2L + 1) = 9, 15, 25
 
 
-def sweep_geometries(coords, nconf, seed=5000):
-    """the first `nconf` of the sweep's seeded geometries: disk count uniform in
-    10..80, mode set uniform over MODESETS5"""
+def sweep_geometries(coords, nconf, first=0):
     from BoostFractor import workloads as W
-    rng = W.SplitMix64(seed)
-    geoms, set_of = [], []
-    for _ in range(nconf):
-        nd = 10 + int(rng.uniform() * 71)
-        ms = int(rng.uniform() * len(MODESETS5))
-        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
-        g = W.booster(coords, ndisk=nd, spacings=list(sp))
-        g.relative_surfaces = np.zeros((0, coords.ny, coords.nx), dtype=np.complex128)  # flat disks
-        geoms.append(g)
-        set_of.append(ms)
-    return geoms, np.array(set_of, dtype=np.int32)
+    return W.sweep_geometries(coords, nconf, first=first)
 
 
 def _eps_copies(geoms):
@@ -341,7 +328,7 @@ def sweep_main(args):
     coords = W.grid_coords(N5)
     nconf = args.nconf
     geoms, set_of = sweep_geometries(coords, nconf)
-    sets = [B.SeedModes(coords, ThreeDim=True, Mmax=M, Lmax=L, diskR=DISKR) for M, L in MODESETS5]
+    sets = [B.SeedModes(coords, ThreeDim=True, Mmax=M, Lmax=L, diskR=DISKR) for M, L in W.SWEEP_MODESETS]
     freqs = np.linspace(19e9, 21e9, NF5)
     ctx = B.context()
     ndisks = [(len(g.distance) - 2) // 2 for g in geoms]

@@ -147,6 +147,30 @@ def rough_booster(coords, ndisk=20, seed=1002, tilt_sigma=50e-6, sigma=10e-6, co
     return sb
 
 
+# disk-position sweep: (M, L) of the Bessel mode sets, n = M (2L + 1) = 9, 15, 25
+SWEEP_MODESETS = [(3, 1), (5, 1), (5, 2)]
+
+
+def sweep_geometries(coords, nconf, seed=5000, first=0):
+    """geometries first .. first+nconf-1 of the disk-position sweep: disk count
+    uniform in 10..80, mode set uniform over SWEEP_MODESETS, spacings
+    8 mm + U(-0.5, 0.5) mm, eps 24, 1 mm disks, flat and untilted (one
+    SplitMix64 stream, so every slice of the sweep is the same geometries)"""
+    rng = SplitMix64(seed)
+    geoms, set_of = [], []
+    for c in range(first + nconf):
+        nd = 10 + int(rng.uniform() * 71)
+        ms = int(rng.uniform() * len(SWEEP_MODESETS))
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        if c < first:
+            continue
+        g = booster(coords, ndisk=nd, spacings=list(sp))
+        g.relative_surfaces = np.zeros((0, coords.ny, coords.nx), dtype=np.complex128)  # flat disks
+        geoms.append(g)
+        set_of.append(ms)
+    return geoms, np.array(set_of, dtype=np.int32)
+
+
 def beta2_dancer(E, coords, diskR):
     """Boost factor of an emitted field: sum|E|^2 dx dy / (pi R^2) (docs/src/3d_algorithms.md:22-36)."""
     dx = abs(coords.X[1] - coords.X[0]) if coords.nx > 1 else 1.0

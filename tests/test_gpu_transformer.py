@@ -12,8 +12,8 @@ import pytest
 
 import BoostFractor as B
 from BoostFractor import workloads as W
-from helpers import (DISKR, GOLDEN, O, deps_distance, ocoords, pmap, ref_coords, ref_setup_eps9, rel,
-                     with_distance)
+from helpers import (DISKR, GOLDEN, O, chain_sensitivity, deps_distance, ocoords, pmap, ref_coords, ref_setup_eps9,
+                     rel, with_distance)
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
@@ -171,32 +171,6 @@ def _oracle_mats(sb, coords, modes, f, okind, eps=None):
     return [O.propagation_matrix(sb.distance[r], DISKR, eps[r], sb.relative_tilt_x[r], sb.relative_tilt_y[r],
                                  sb.relative_surfaces[r], lam, ocoords(coords), pat, kt, okind)
             for r in range(len(sb.distance))]
-
-
-def chain_sensitivity(Ps, eps, emit, trials=3, seed=0):
-    """relative change of the boost vector when every propagation matrix is
-    perturbed at the 1e-15 level: the intrinsic FP64 sensitivity of the chain"""
-    n = len(emit)
-    N = len(eps)
-
-    def run(rng):
-        T_ = np.eye(2 * n, dtype=complex)
-        ab = np.zeros(n, complex)
-        for s in range(N - 1, 0, -1):
-            ir, ir1 = N - s, N - s - 1
-            n1, n0 = np.sqrt(eps[ir1]), np.sqrt(eps[ir])
-            ab += (1 / n1 ** 2 - 1 / n0 ** 2) / 2 * (T_[n:, :n] @ emit + T_[n:, n:] @ emit)
-            P = Ps[ir]
-            if rng is not None:
-                P = P * (1 + 1e-15 * (rng.standard_normal(P.shape) + 1j * rng.standard_normal(P.shape)))
-            a, b = (n1 + n0) / (2 * n1), (n1 - n0) / (2 * n1)
-            Pi = np.linalg.inv(P)
-            T_ = T_ @ np.block([[a * P, b * Pi], [b * P, a * Pi]])
-        return -np.linalg.solve(T_[n:, n:], ab)
-
-    b0 = run(None)
-    rng = np.random.default_rng(seed)
-    return max(np.max(np.abs(run(rng) - b0)) / np.max(np.abs(b0)) for _ in range(trials))
 
 
 KINDS = [(B.propagator, O.PROP_PROPAGATOR), (B.propagatorNoTilts, O.PROP_NOTILTS), (B.propagator1D, O.PROP_1D),
