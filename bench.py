@@ -2,27 +2,29 @@
 """Benchmark: boost-factor frequency points/s on one MI355X, batched mixed
 precision vs plain FP64.
 
-Workload (one "step"): a 20-disk booster (eps=24, 1 mm disks, spacings
-8 mm + U(-0.3, 0.3) mm from SplitMix64 seed 1000, disk radius 0.15 m) in front
-of a mirror, recursive-Fourier `dancer` with the FFT propagator on a 256 x 256
-grid spanning 1 m, 50 frequencies on 19.5-20.5 GHz, nmax = 2000 (2001 bounces),
-all frequencies as one batched, device-resident sweep.  `value` is the mixed
-precision sweep's points/s; the same sweep in FP64 is timed beside it and the
-relative beta^2 error of every point against FP64 is reported.
+Default workload (`--workload sweep`): the disk-position sweep, 4096 seeded
+boosters (disk counts uniform in 10..80, spacings 8 mm + U(-0.5, 0.5) mm,
+eps 24, 1 mm disks, radius 0.15 m, flat) in front of a mirror, `transformer`
+mode matching with 9, 15 or 25 Bessel modes per geometry on a 256 x 256 grid
+over 1 m, 100 frequencies on 19-21 GHz: 409,600 points.  The K timed steps
+split the geometries into K disjoint contiguous slices, so K steps cover the
+whole sweep once; `value` is the whole sweep's points over the timed time.
+The sweep has no reduced-precision path (its products run on FP64 MFMA), so
+`mixed_over_fp64` is 1.0.  The dancer workloads (`--workload dancer`,
+`dancer512`) time a complex64-storage path against FP64 and report the ratio
+and the beta^2 error of every point.
 
 Kernel timing: the timed steps run without instrumentation; one more step
-then records HIP events on the engine's stream around every launch of the two
-bounce kernels, which gives the kernels' mean launch times.
+then records HIP events on the engine's stream around every launch of the
+kernel classes, which gives the kernels' mean launch times.
 
-Roofline: `achieved` counts the algorithmic bytes of the dancer bounce, every
-field read once and written once per bounce and frequency (2 F K B_c with
-F = 82 fields, K = 256^2, B_c = 8 B for complex64 storage), times the
-frequency-bounces one launch of the dominant kernel processes, over that
-kernel's mean launch time.  Skipping the rows the aperture zeroes and sharing
-the phase tables reduce the bytes actually moved, which `traffic` reports (HBM
-bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the
-same counter's reading for a copy kernel of known size); they are never folded
-into `achieved`.
+Roofline: the dominant kernel's executed work per launch over its mean launch
+time (sweep and transformer: FP64 MFMA flops of the grouped-Parseval
+product; dancer: the bounce's algorithmic bytes, every field read once and
+written once per bounce and frequency, 2 F K B_c with F = 82 fields, over
+the two bounce kernels' combined time per frequency-bounce).  `traffic` is
+HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by
+the same counter's reading for a copy kernel of known size.
 
   python bench.py --summarize-pmc FETCH_DB WRITE_DB
   python bench.py --summarize-trace TRACE_DB
@@ -143,7 +145,7 @@ def cpu_baseline(coords, sb, freqs, budget_s=20.0):
     sample: a few bounces of as many points as threads, extrapolated linearly in
     bounces to the full nmax+1."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    model, host_cpus, threads = host_cpu_info()
     oc = O.Coords(coords.X, coords.Y, coords.kX, coords.kY)
     os_ = O.Setup(sb.distance, sb.eps, sb.r)
     # calibrate: one bounce of one point
@@ -151,14 +153,16 @@ def cpu_baseline(coords, sb, freqs, budget_s=20.0):
     O.dancer_sweep(os_, oc, 0, 0, freqs[:1], nthreads=1, diskR=DISKR)
     one = time.perf_counter() - t
     nb = max(1, min(NMAX + 1, int(budget_s / max(one, 1e-6))))
-    sample_f = freqs[:threads]
+    sample_f = np.resize(freqs, threads)  # one point per thread
     t = time.perf_counter()
     O.dancer_sweep(os_, oc, 0, nb - 1, sample_f, nthreads=threads, diskR=DISKR)
     dt = time.perf_counter() - t
     per_point = dt / len(sample_f) * threads * (NMAX + 1) / nb  # seconds per point per thread
     return {"value": threads / per_point, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"{len(sample_f)} frequency points x {nb} of {NMAX + 1} bounces on {threads} threads "
-                      f"(FP64 C restatement of the reference dancer), extrapolated linearly in bounces"}
+            "cpu_model": model, "host_logical_cpus": host_cpus,
+            "sample": f"{len(sample_f)} frequency points x {nb} of {NMAX + 1} bounces on {threads} threads, the "
+                      f"process's CPU share (FP64 C restatement of the reference dancer), extrapolated linearly in "
+                      f"bounces"}
 
 
 def cheerleader_cpu_baseline(coords, sb, freqs, nmax, budget_s=15.0):
@@ -307,9 +311,57 @@ def transformer_main(args):
 N5, NF5, NCONF5 = 256, 100, 4096
 
 
-def sweep_geometries(coords, nconf, first=0):
-    from BoostFractor import workloads as W
-    return W.sweep_geometries(coords, nconf, first=first)
+def host_cpu_info():
+    """CPU model, logical CPUs of the host, and the CPUs this process may run on
+    (its share of the box; a one-GPU box gives a process 16 of its cores)"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        share = min(share, int(omp))
+    return model, os.cpu_count() or 1, max(1, share)
+
+
+def sweep_cpu_baseline(coords, geoms, set_of, sets, freqs, budget_s=20.0):
+    """FP64 CPU restatement of the reference transformer (the oracle) on a bounded
+    sample: per mode set, the propagation matrix of one air and one disk region
+    (the per-point cost is the sum over the geometry's regions; the chain is
+    negligible), every matrix computed `threads` times concurrently on the
+    process's CPU share; the whole sweep's CPU time then follows from the region
+    counts of all its geometries."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    model, host_cpus, threads = host_cpu_info()
+    oc = O.Coords(coords.X, coords.Y, coords.kX, coords.kY)
+    lam = 299792458.0 / float(freqs[len(freqs) // 2])
+    per = {}
+    for k, md in enumerate(sets):
+        pat, kt = md.patterns_flat()[:, 0], md.mode_kt.reshape(-1)
+        for kind, eps, dz in (("air", complex(1.0, -0.0), 8e-3), ("disk", complex(24.0, -0.0), 1e-3)):
+            def one(_):
+                return O.propagation_matrix(dz, DISKR, eps, 0.0, 0.0, None, lam, oc, pat, kt)
+            t = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                list(ex.map(one, range(threads)))
+            per[(k, kind)] = (time.perf_counter() - t) / threads  # seconds per matrix with every thread busy
+    total = 0.0
+    for g, k in zip(geoms, set_of):
+        nd = (len(g.distance) - 2) // 2
+        total += len(freqs) * ((nd + 1) * per[(int(k), "air")] + nd * per[(int(k), "disk")])
+    npts = len(geoms) * len(freqs)
+    return {"value": npts / total, "unit": "points/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_logical_cpus": host_cpus,
+            "sample": f"propagation matrices of one air and one disk region per mode set (n = 9, 15, 25) at 256^2, "
+                      f"each computed by {threads} threads at once on the process's {threads}-CPU share, scaled by "
+                      f"the region counts of all {len(geoms)} geometries x {len(freqs)} frequencies (FP64 C "
+                      f"restatement of the reference transformer; the chain is negligible)"}
 
 
 def _eps_copies(geoms):
@@ -322,72 +374,96 @@ def _eps_copies(geoms):
     return out
 
 
+SWEEP_WORKLOAD = ("disk-position sweep (transformer): {nconf} seeded boosters with 10-80 disks (spacings 8 mm "
+                  "+- U(0.5 mm), eps 24, 1 mm, flat), 9/15/25 Bessel modes, 256x256 grid over 1 m, 100 freqs "
+                  "19-21 GHz")
+
+
 def sweep_main(args):
+    """The disk-position sweep: 4096 seeded geometries x 100 frequencies.  The
+    timed steps split the geometries into `steps` disjoint contiguous slices, so
+    `steps` steps cover the whole sweep once; warmup steps run the first slices.
+    Inputs resident: the grid, mode sets, their axion sources and the
+    frequencies are uploaded once (TransformerBatchPlan); each step hands its
+    slice's geometries over and computes every boost on the device (pattern
+    transforms, group Grams, all propagation matrices, inverses, chains)."""
     import BoostFractor as B
     from BoostFractor import workloads as W
     coords = W.grid_coords(N5)
     nconf = args.nconf
-    geoms, set_of = sweep_geometries(coords, nconf)
+    geoms, set_of = W.sweep_geometries(coords, nconf)
     sets = [B.SeedModes(coords, ThreeDim=True, Mmax=M, Lmax=L, diskR=DISKR) for M, L in W.SWEEP_MODESETS]
     freqs = np.linspace(19e9, 21e9, NF5)
     ctx = B.context()
     ndisks = [(len(g.distance) - 2) // 2 for g in geoms]
-    print(f"[bench] sweep: {nconf} geometries (disks {min(ndisks)}..{max(ndisks)}), {N5}x{N5}, {NF5} freqs",
-          file=sys.stderr, flush=True)
+    K = max(1, args.steps)
+    slices = [sl for sl in np.array_split(np.arange(nconf), K)]
+    print(f"[bench] sweep: {nconf} geometries (disks {min(ndisks)}..{max(ndisks)}), {N5}x{N5}, {NF5} freqs, "
+          f"{K} steps of ~{nconf // K} geometries", file=sys.stderr, flush=True)
     emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
-    # inputs resident: grid, mode sets, their sources and the frequencies are uploaded once;
-    # each step hands the candidate geometries over and computes every boost on the device
     plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
-    for _ in range(args.warmup):
-        plan.run(_eps_copies(geoms), mode_set_of=set_of)
+    work = [(_eps_copies([geoms[i] for i in sl]), set_of[sl]) for sl in slices]
+
+    def step(k):
+        gs, so = work[k % K]
+        plan.run(_eps_copies(gs), mode_set_of=so)
+
+    for w in range(args.warmup):
+        step(w)
     ctx.synchronize()
+    print(f"[bench]   {args.warmup} warmup step(s) done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.run(_eps_copies(geoms), mode_set_of=set_of)
+    boosts = []
+    for k in range(K):
+        step(k)
+        boosts += plan.fetch() if args.keep_boosts else []
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    ctx.timing(True)  # one more, instrumented step for the kernel means
-    plan.run(_eps_copies(geoms), mode_set_of=set_of)
+    print(f"[bench]   {K} timed steps done: {dt:.3f} s", file=sys.stderr, flush=True)
+    pts = nconf * NF5 / dt
+    ctx.timing(True)  # one more, instrumented step (slice 0) for the kernel means
+    step(0)
     ctx.synchronize()
     kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
     ctx.timing(False)
-    boost = plan.fetch()
+    b0 = plan.fetch()
     plan.close()
-    # the host-buffer API on the same sweep (uploads the mode sets every call), for reference
-    t1 = time.perf_counter()
-    B.transformer_batch(_eps_copies(geoms), coords, sets, f=freqs, mode_set_of=set_of, diskR=DISKR, emits=emits)
-    ctx.synchronize()
-    host_pts = nconf * NF5 / (time.perf_counter() - t1)
-    pts = nconf * NF5 * args.steps / dt
     nmodes = [sets[s_].n for s_ in set_of]
     alg = sum((len(g.distance) - 1) * 8.0 * m * m * N5 * N5 for g, m in zip(geoms, nmodes)) * NF5
-    roof = mm_roofline(kt_, "zgemm", TRAFFIC_PROFILE_SW, alg * args.steps / dt)
-    roof["kernel_ms_from"] = "one instrumented step after the timed ones"
+    roof = mm_roofline(kt_, "zgemm", TRAFFIC_PROFILE_SW, alg / dt)
+    roof["kernel_ms_from"] = "one instrumented step (slice 0) after the timed ones"
+    bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)
+    roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)
+    b2 = np.concatenate([np.sum(np.abs(b) ** 2, axis=-1) for b in b0])
     out = {
-        "metric": "disk-position sweep boost-factor points/s on 1 MI355X (transformer, FP64)",
+        "metric": "boost-factor freq points/s on 1 MI355X: mixed-prec batched vs plain FP64",
         "value": round(pts, 2),
         "unit": "points/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 1),
+        "ms_per_step": round(dt / K * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64 geometries; Bessel modes)",
-        "config": {"workload": f"disk-position sweep, first {nconf} of {NCONF5} seeded boosters with 10-80 disks, "
-                               f"{N5}x{N5} grid over 1 m, 9/15/25 Bessel modes, {NF5} freqs 19-21 GHz",
-                   "geometries": nconf, "grid": N5, "frequencies": NF5},
+        "data": "synthetic (seeded SplitMix64 geometries; Bessel modes from SeedModes)",
+        "config": {"workload": SWEEP_WORKLOAD.format(nconf=nconf), "geometries": nconf, "grid": N5,
+                   "frequencies": NF5, "points": nconf * NF5,
+                   "steps_cover": f"{K} disjoint slices of the {nconf} geometries, one per timed step"},
+        "fp64": {"value": round(pts, 2), "unit": "points/s", "mixed_over_fp64": 1.0,
+                 "note": "the sweep has no reduced-precision path: the Parseval products run on FP64 MFMA (a "
+                         "complex64 product measured beta^2 errors up to 7.9e-2 against FP64, outside the 1e-4 "
+                         "bar), so the batched path timed here is the FP64 one and the ratio is 1.0"},
         "roofline": roof,
-        "note": f"inputs resident (TransformerBatchPlan); the host-buffer API (transformer_batch) on the same "
-                f"sweep: {host_pts:.1f} points/s",
         "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
         # useful flops (8 n^2 per complex multiply-add of the product) over each class's HIP-event time
         "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
-        "beta2": {"min": float(min(np.sum(np.abs(b) ** 2, axis=-1).min() for b in boost)),
-                  "max": float(max(np.sum(np.abs(b) ** 2, axis=-1).max() for b in boost))},
+        "beta2_slice0": {"min": float(b2.min()), "max": float(b2.max())},
     }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = sweep_cpu_baseline(coords, geoms, set_of, sets, freqs)
     print(json.dumps(out), flush=True)
 
 
@@ -496,8 +572,8 @@ def main():
     global N, NFREQ, NMAX  # the dancer512 workload resizes the dancer sweep
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--freq-batch", type=int, default=0)
@@ -506,13 +582,16 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"))
     ap.add_argument("--summarize-trace", metavar="TRACE_DB")
     ap.add_argument("--workload", choices=("dancer", "dancer512", "cheerleader", "transformer", "sweep", "model1d"),
-                    default="dancer",
-                    help="dancer: the headline sweep (default); dancer512: the same booster at 512x512, 100 "
-                         "frequencies, 3000 bounces (run it with --steps 1 --warmup 0: ~1 min per sweep); "
-                         "cheerleader: the Gauss-Seidel solver on the 20-disk booster; transformer: the "
-                         "mode-matching sweep; sweep: the many-geometry disk-position sweep; model1d: the 1D "
+                    default="sweep",
+                    help="sweep (default): the disk-position sweep, 4096 geometries x 100 frequencies, the steps "
+                         "splitting the geometries into disjoint slices; dancer: the 20-disk booster at 256x256, "
+                         "50 frequencies, 2000 bounces; dancer512: the same booster at 512x512, 100 frequencies, "
+                         "3000 bounces (run it with --steps 1 --warmup 0: ~1 min per sweep); cheerleader: the "
+                         "Gauss-Seidel solver on the 20-disk booster; transformer: the mode-matching sweep of the "
+                         "tilted, rough booster at 512x512 with 45 Laguerre-Gauss modes; model1d: the 1D "
                          "analytical model")
-    ap.add_argument("--nconf", type=int, default=32, help="geometries of the sweep workload")
+    ap.add_argument("--nconf", type=int, default=NCONF5, help="geometries of the sweep workload (default: all 4096)")
+    ap.add_argument("--keep-boosts", action="store_true", help="fetch every step's boosts (sweep workload)")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:
         if args.summarize_trace:
@@ -580,17 +659,20 @@ def main():
                                           "p90": float(np.percentile(err, 90))},
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt64.items()}}
 
-    # roofline of the dominant kernel of the mixed path
-    names = {"row_pass": "k_dancer_rows (row IFFT, tilt/surface, boundary update, mask, row FFT)",
-             "col_pass": "k_cols (column FFT, exp(-i kz dz)/K, column IFFT)"}
-    dom = max(kt32, key=lambda k: kt32[k][0])
-    ms_tot, launches = kt32[dom]
-    avg_ms = ms_tot / launches
-    nb_per_launch = NFREQ * (nmax + 1) / launches  # frequency-bounces per launch (one instrumented run)
+    # roofline of the mixed path's bounce: a frequency batch's bounce is two launches (column pass, fused
+    # row pass) that together read and write every field once; the bounce's algorithmic bytes are divided
+    # by the two launches' combined mean time (each pass alone moves the same fields, so neither gets the
+    # whole bounce's bytes on its own time)
+    col_ms, launches = kt32["col_pass"]
+    row_ms, row_launches = kt32["row_pass"]
+    avg_ms = (col_ms + row_ms) / launches  # one column + one row launch per frequency batch and bounce
+    nb_per_launch = NFREQ * (nmax + 1) / launches  # frequency-bounces per launch pair (one instrumented run)
     alg_bytes = nb_per_launch * 2 * F * K * 8  # every field read once and written once, complex64
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    # the committed counter summary covers the 256 x 256 workload only
-    traffic = traffic_from_profile("k_cols_reg", fb32) if (dom == "col_pass" and N == 256) else None
+    # HBM bytes of both passes per launch pair, from the committed counter summary (256 x 256 only)
+    tc = traffic_from_profile("k_cols_reg", fb32) if N == 256 else None
+    tr = traffic_from_profile("k_dancer_rows_reg4", fb32) if N == 256 else None
+    traffic = (tc + tr) if (tc is not None and tr is not None) else None
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=10)
     stream_gbs = 2 * (1 << 30) / (bw_ms * 1e-3) / 1e9
 
@@ -615,7 +697,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic if traffic is None else int(traffic),
-                     "kernel": names[dom], "avg_launch_ms": round(avg_ms, 4),
+                     "kernel": "dancer bounce: k_cols_reg (column FFT, exp(-i kz dz)/K, column IFFT) + "
+                               "k_dancer_rows_reg (row IFFT, tilt/surface, boundary update, mask, row FFT)",
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "col_pass_ms": round(col_ms / launches, 4), "row_pass_ms": round(row_ms / row_launches, 4),
                      "alg_bytes_per_launch": int(alg_bytes),
                      # the same algorithmic bytes over the whole step (both bounce kernels, the gaps between
                      # launches and the per-batch setup): the rate a user of the sweep sees

@@ -60,9 +60,11 @@ def grid_coords(N, span=1.0):
 
 
 def booster(coords, ndisk=20, eps_disk=24.0, thickness=1e-3, spacing=8e-3, jitter=0.3e-3, seed=1000,
-            spacings=None):
+            spacings=None, flat=False):
     """Mirror + ndisk dielectric disks: eps = [NaN, (1, eps_disk) x ndisk, 1],
-    distance = [0, s1, t, s2, t, ..., s_d, t, 0], s_i = spacing + U(-jitter, jitter)."""
+    distance = [0, s1, t, s2, t, ..., s_d, t, 0], s_i = spacing + U(-jitter, jitter).
+    flat=True: no surface stack at all (relative_surfaces of shape (0, ny, nx),
+    which the engine reads as flat), instead of (nreg, ny, nx) zeros."""
     if spacings is None:
         rng = SplitMix64(seed)
         spacings = [spacing + (2.0 * rng.uniform() - 1.0) * jitter for _ in range(ndisk)]
@@ -73,7 +75,12 @@ def booster(coords, ndisk=20, eps_disk=24.0, thickness=1e-3, spacing=8e-3, jitte
         eps += [1.0, float(eps_disk)]
     distance.append(0.0)
     eps.append(1.0)
-    return SeedSetupBoundaries(coords, diskno=ndisk, distance=distance, epsilon=eps)
+    if not flat:
+        return SeedSetupBoundaries(coords, diskno=ndisk, distance=distance, epsilon=eps)
+    sb = SeedSetupBoundaries(coords, diskno=ndisk, distance=distance, epsilon=eps,
+                             relative_surfaces=np.zeros((len(distance), 0, 0), dtype=np.complex128))
+    sb.relative_surfaces = np.zeros((0, coords.ny, coords.nx), dtype=np.complex128)
+    return sb
 
 
 def _genlaguerre(p, a, x):
@@ -164,9 +171,7 @@ def sweep_geometries(coords, nconf, seed=5000, first=0):
         sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
         if c < first:
             continue
-        g = booster(coords, ndisk=nd, spacings=list(sp))
-        g.relative_surfaces = np.zeros((0, coords.ny, coords.nx), dtype=np.complex128)  # flat disks
-        geoms.append(g)
+        geoms.append(booster(coords, ndisk=nd, spacings=list(sp), flat=True))
         set_of.append(ms)
     return geoms, np.array(set_of, dtype=np.int32)
 
