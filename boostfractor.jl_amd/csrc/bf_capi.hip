@@ -134,6 +134,7 @@ int bf_ctx_create(int32_t device, bf_ctx** out) {
         if (device < 0 || device >= n) throw bf::Error(BF_ERR_ARG, "device index out of range");
         BF_HIP(hipSetDevice(device));
         BF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        bf::noalloc_init(device);  // the allocation-free entry point's scratch address on this device
     });
     if (rc) {
         delete c;
@@ -700,6 +701,29 @@ int bf_transformer_trace_back(bf_ctx* ctx, const double* reflected, const double
     return guarded(ctx, [&] {
         bf::trace_back_dev(ctx, reflected, input, setup, coords, modes, f, prop_kind, prop_mats, diskR, includes_axion,
                            emit, fields);
+    });
+}
+
+int bf_transformer_chain_noalloc(bf_ctx* ctx, int32_t nreg, const double* eps, int32_t n, int32_t nfreq,
+                                 const double* prop_mats, const double* emit, const double* reflect, double* boost,
+                                 double* refl) {
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    BF_REQUIRE(ctx, eps && prop_mats && emit && boost, BF_ERR_ARG, "eps, prop_mats, emit and boost must not be NULL");
+    BF_REQUIRE(ctx, !reflect == !refl, BF_ERR_ARG, "reflect and refl must be given together");
+    BF_REQUIRE(ctx, nreg >= 2, BF_ERR_DIM,
+               "the arrays in your SetupBoundaries objects don't fit together! (too few regions)");
+    BF_REQUIRE(ctx, n >= 1 && nfreq >= 1, BF_ERR_ARG, "n and nfreq must be >= 1");
+    BF_REQUIRE(ctx, n <= 48, BF_ERR_UNSUPPORTED, "the allocation-free chain holds n <= 48 modes");
+    BF_REQUIRE(ctx, nreg <= BF_NOALLOC_MAX_REGIONS, BF_ERR_UNSUPPORTED,
+               "the allocation-free chain holds at most BF_NOALLOC_MAX_REGIONS (512) regions");
+    BF_REQUIRE(ctx, bf::noalloc_bytes(nreg, n, nfreq) <= (size_t)BF_NOALLOC_SCRATCH_BYTES, BF_ERR_UNSUPPORTED,
+               "the call needs " + std::to_string(bf::noalloc_bytes(nreg, n, nfreq)) +
+                   " bytes of working memory, above the allocation-free chain's BF_NOALLOC_SCRATCH_BYTES (32 MiB): "
+                   "split the frequencies over several calls");
+    return guarded(ctx, [&] {
+        bf::transformer_chain_noalloc(ctx, nreg, eps, n, nfreq, reinterpret_cast<const zd*>(prop_mats),
+                                      reinterpret_cast<const zd*>(emit), reinterpret_cast<const zd*>(reflect),
+                                      reinterpret_cast<zd*>(boost), reinterpret_cast<zd*>(refl));
     });
 }
 

@@ -291,6 +291,13 @@ void spacing_itp_bounds(const bf_spacing_itp* it, double* lo, double* hi);
 void spacing_itp_matrices(bf_spacing_itp* it, int ncand, const double* shifts, double* out);
 void spacing_itp_transformer(bf_spacing_itp* it, int ncand, const double* shifts, const double* eps,
                              const double* emit, const double* reflect, double* boost, double* refl);
+constexpr int BF_NOALLOC_MAX_DEVICES = 64;
+// device address of the allocation-free entry point's scratch array on `device`
+// (taken once per device, with the device current); bytes a call needs
+void noalloc_init(int device);
+size_t noalloc_bytes(int nreg, int n, int nfreq);
+void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps, int n, int nfreq, const zd* P, const zd* e0,
+                               const zd* reflect, zd* boost, zd* refl);
 void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, const bf_setup* st,
                     const bf_coords* co, const bf_modes* md, double f, int kind, const double* prop_mats,
                     double diskR, int includes_axion, const double* emit, double* fields);

@@ -25,6 +25,7 @@
 #include <rocblas/rocblas.h>
 
 #include <memory>
+#include <mutex>
 
 #include <algorithm>
 #include <cmath>
@@ -1623,23 +1624,26 @@ std::vector<zd> eps_fixed(const bf_setup* st) {
     return v;
 }
 
+// step q of the chain over regions nreg-1 .. 1 (0-based; eps with NaN already 1e30)
+static ChainStep chain_step(const zd* eps, int nreg, int q) {
+    const int sidx = nreg - 1 - q;                      // s in the reference loop
+    const int ir = nreg - sidx, ir1 = nreg - sidx - 1;  // idx_reg(s)-1, idx_reg(s+1)-1
+    const zd n1 = csqrt_j(eps[ir1]), n0 = csqrt_j(eps[ir]);
+    const zd one = cmk<double>(1.0, 0.0);
+    const zd S = rmul(0.5, csub(cdiv_h(one, cmul(n1, n1)), cdiv_h(one, cmul(n0, n0))));
+    // get_boundary_matrix(n_left = n0, n_right = n1): G = 1/(2 nr) [(nr+nl) I, (nr-nl) I; ...]
+    const zd sc = cdiv_h(one, rmul(2.0, n1));
+    ChainStep c;
+    c.S = S;
+    c.a = cmul(sc, cadd(n1, n0));
+    c.b = cmul(sc, csub(n1, n0));
+    c.reg = ir;
+    c.pad = 0;
+    return c;
+}
+
 void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainStep>& steps) {
-    for (int q = 0; q < nreg - 1; q++) {
-        const int sidx = nreg - 1 - q;                      // s in the reference loop
-        const int ir = nreg - sidx, ir1 = nreg - sidx - 1;  // idx_reg(s)-1, idx_reg(s+1)-1
-        const zd n1 = csqrt_j(eps[ir1]), n0 = csqrt_j(eps[ir]);
-        const zd one = cmk<double>(1.0, 0.0);
-        const zd S = rmul(0.5, csub(cdiv_h(one, cmul(n1, n1)), cdiv_h(one, cmul(n0, n0))));
-        // get_boundary_matrix(n_left = n0, n_right = n1): G = 1/(2 nr) [(nr+nl) I, (nr-nl) I; ...]
-        const zd sc = cdiv_h(one, rmul(2.0, n1));
-        ChainStep c;
-        c.S = S;
-        c.a = cmul(sc, cadd(n1, n0));
-        c.b = cmul(sc, csub(n1, n0));
-        c.reg = ir;
-        c.pad = 0;
-        steps.push_back(c);
-    }
+    for (int q = 0; q < nreg - 1; q++) steps.push_back(chain_step(eps.data(), nreg, q));
 }
 
 void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
@@ -1679,6 +1683,89 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         }
         BF_CHECK_LAUNCH();
     }
+    BF_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free chain (bf_transformer_chain_noalloc)
+// ---------------------------------------------------------------------------
+// The entry point's working memory (inverses, selection list, chain jobs and
+// steps) is one fixed array per GPU, held from library load to process end.
+// Host code never passes `scratch` itself (its name is a host-side stand-in):
+// the array's device address on each GPU comes from hipGetSymbolAddress at
+// context creation (noalloc_init, device current) and is kept per device.
+static __device__ __align__(256) unsigned char scratch[BF_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[BF_NOALLOC_MAX_DEVICES] = {};
+
+void noalloc_init(int device) {
+    if (device < 0 || device >= BF_NOALLOC_MAX_DEVICES) throw Error(BF_ERR_UNSUPPORTED, "device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    BF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(BF_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = p;
+}
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bytes of scratch a call needs: inverses, selection list, jobs, steps
+size_t noalloc_bytes(int nreg, int n, int nfreq) {
+    const size_t slots = (size_t)nfreq * nreg, nsel = (size_t)nfreq * (nreg - 1);
+    return align256(sizeof(zd) * (size_t)n * n * slots) + align256(sizeof(int) * nsel) +
+           align256(sizeof(ChainJob) * (size_t)nfreq) + align256(sizeof(ChainStep) * (size_t)(nreg - 1));
+}
+
+// one chain per frequency over regions nreg-1 .. 1; sel lists the matrices to invert
+__global__ void k_noalloc_jobs(ChainJob* jobs, int* sel, int nfreq, int nreg) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfreq) return;
+    ChainJob j;
+    j.step0 = 0;
+    j.pbase = (int64_t)f * nreg;
+    j.out = f;
+    j.e0off = 0;
+    j.nsteps = nreg - 1;
+    j.pad = 0;
+    jobs[f] = j;
+    for (int r = 1; r < nreg; r++) sel[(size_t)f * (nreg - 1) + (r - 1)] = f * nreg + r;
+}
+
+void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps_h, int n, int nfreq, const zd* P,
+                               const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    hipStream_t s = ctx->stream;
+    unsigned char* base = ctx->device >= 0 && ctx->device < BF_NOALLOC_MAX_DEVICES
+                              ? static_cast<unsigned char*>(g_scratch_ptr[ctx->device])
+                              : nullptr;
+    if (!base) throw Error(BF_ERR_HIP, "the scratch array of this device has no address (context not initialised)");
+    if (n > INV_WAVE_MAX || n > CHAIN_LDS_MAX) throw Error(BF_ERR_UNSUPPORTED, "n above 48");
+    if (noalloc_bytes(nreg, n, nfreq) > (size_t)BF_NOALLOC_SCRATCH_BYTES)
+        throw Error(BF_ERR_UNSUPPORTED, "call exceeds the scratch array");
+    const size_t nn = (size_t)n * n, slots = (size_t)nfreq * nreg, nsel = (size_t)nfreq * (nreg - 1);
+    zd* Pinv = reinterpret_cast<zd*>(base);
+    int* sel = reinterpret_cast<int*>(base + align256(sizeof(zd) * nn * slots));
+    ChainJob* jobs = reinterpret_cast<ChainJob*>(reinterpret_cast<unsigned char*>(sel) + align256(sizeof(int) * nsel));
+    ChainStep* steps =
+        reinterpret_cast<ChainStep*>(reinterpret_cast<unsigned char*>(jobs) + align256(sizeof(ChainJob) * nfreq));
+    // steps from the permittivities (bdry.eps[isnan] .= 1e30, src/boost_fractor_transformer.jl:289), staged
+    // from this stack frame; the stream is synchronised before it returns
+    zd eps[BF_NOALLOC_MAX_REGIONS];
+    ChainStep hsteps[BF_NOALLOC_MAX_REGIONS - 1];
+    for (int r = 0; r < nreg; r++) {
+        eps[r] = cmk<double>(eps_h[2 * r], eps_h[2 * r + 1]);
+        if (std::isnan(eps[r].re) || std::isnan(eps[r].im)) eps[r] = cmk<double>(1e30, 0.0);
+    }
+    for (int q = 0; q < nreg - 1; q++) hsteps[q] = chain_step(eps, nreg, q);
+    BF_HIP(hipMemcpyAsync(steps, hsteps, sizeof(ChainStep) * (nreg - 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg);
+    BF_CHECK_LAUNCH();
+    launch_inverse(ctx, P, Pinv, n, sel, nsel);
+    const size_t smem = 4 * sizeof(zd) * nn;
+    allow_lds((const void*)k_chain_lds, smem);
+    hipLaunchKernelGGL(k_chain_lds, dim3((unsigned)nfreq), dim3(256), smem, s, jobs, steps, P, Pinv, n, e0, reflect,
+                       boost, refl);
+    BF_CHECK_LAUNCH();
     BF_HIP(hipStreamSynchronize(s));
 }
 

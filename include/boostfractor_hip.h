@@ -341,6 +341,33 @@ int bf_transformer_trace_back(bf_ctx *ctx, const double *reflected, const double
                               const double *prop_mats, double diskR, int32_t includes_axion, const double *emit,
                               double *fields);
 
+/* ---- allocation-free chain --------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops such as an
+ * optimiser's candidate evaluations, harnesses that forbid hipMalloc): the same
+ * computation as bf_transformer with precomputed propagation matrices
+ * (transformer(...; propagation_matrices=P), src/boost_fractor_transformer.jl:
+ * 285-359, run by the same kernels), with every input and output already in
+ * device memory.  It calls no hipMalloc/hipFree: its working memory (the
+ * inverses, selection list and chain descriptors) is a fixed array of
+ * BF_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the process
+ * ends.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (several contexts or threads) race; calls on one context are serialised.
+ *   eps: HOST complex [nreg] (NaN taken as 1e30, as the reference's mutation);
+ *   prop_mats: DEVICE complex [nfreq][nreg][n][n] column-major (region 0 unused);
+ *   emit: DEVICE complex [n]; reflect: DEVICE complex [n] or NULL;
+ *   boost: DEVICE complex [nfreq][n]; refl: DEVICE complex [nfreq][n] (with reflect).
+ * Limits: 1 <= n <= 48, 2 <= nreg <= BF_NOALLOC_MAX_REGIONS, and the call's
+ * working memory nfreq * nreg * n^2 * 16 bytes (plus descriptors) within
+ * BF_NOALLOC_SCRATCH_BYTES; larger calls return BF_ERR_UNSUPPORTED before any
+ * device work.  Returns after the results are written (the context's stream is
+ * synchronised). */
+#define BF_NOALLOC_SCRATCH_BYTES (32 << 20)
+#define BF_NOALLOC_MAX_REGIONS 512
+int bf_transformer_chain_noalloc(bf_ctx *ctx, int32_t nreg, const double *eps, int32_t n, int32_t nfreq,
+                                 const double *prop_mats, const double *emit, const double *reflect, double *boost,
+                                 double *refl);
+
 #ifdef __cplusplus
 }
 #endif
