@@ -149,7 +149,6 @@ int bf_ctx_destroy(bf_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     bf::clear_timers(ctx);
-    bf::release_blas(ctx);
     ctx->twiddles.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -201,7 +200,6 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
             tmp = *ip;
             return &tmp;
         }
-    if (!strcmp(name, "parseval_bytes")) return &o.parseval_bytes;
     if (!strcmp(name, "batch_bytes")) return &o.batch_bytes;
     return nullptr;
 }

@@ -92,7 +92,7 @@ struct EngineOptions {
     int col_tx = 0, rows = 0; // dancer launch shapes (0: tuned/default)
     int parseval_groups = 1;  // flat air regions by k-bin groups
     int disk_expansion = 1;   // tilted/rough disks by a frequency series of moment Grams
-    int64_t parseval_bytes = 0, batch_bytes = 0;  // chunk budgets (0: defaults)
+    int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: default 2 GiB)
 };
 // Options of the context whose call runs on this thread (defaults outside a call).
 const EngineOptions& opts();
@@ -116,7 +116,6 @@ struct bf_ctx {
     std::map<std::pair<int, int>, bf::DevBuf> twiddles;
     bool timing = false;
     std::map<std::string, bf_kernel_timer> timers;
-    void* blas = nullptr;  // rocBLAS handle on `stream`, created on first use
     // grow-only device buffers reused by consecutive calls on this context
     // (work arrays of the mode-matching batches), released with the context
     std::map<int, bf::DevBuf> scratch;
@@ -130,7 +129,7 @@ inline DevBuf& scratch_buf(bf_ctx* ctx, int slot, size_t bytes) {
     return b;
 }
 enum ScratchSlot : int {
-    SB_W = 1, SB_CM, SB_PW, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
+    SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
     SB_SLOTS, SB_FLAG, SB_K0, SB_DZ
 };
 }  // namespace bf
@@ -257,7 +256,6 @@ void propagation_matrices_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* 
 void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const bf_modes* md, const double* freqs,
                      int nfreq, int kind, const double* prop_mats, double diskR, const double* emit,
                      const double* reflect, int precision, double* boost, double* refl);
-void release_blas(bf_ctx* ctx);
 void disk_system_1d_dev(bf_ctx* ctx, const double* freqs, int nfreq, int ncand, int num_disk, const double* spacings,
                         double thickness, const double* eps_disk, const double* eps_gap, int mirror,
                         const double* surfaceloss, double* gamma, double* axion);

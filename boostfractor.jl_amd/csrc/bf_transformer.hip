@@ -22,8 +22,6 @@
 // [a P, b inv(P); b P, a inv(P)].
 #include "bf_modematch.h"
 
-#include <rocblas/rocblas.h>
-
 #include <memory>
 #include <mutex>
 
@@ -852,15 +850,139 @@ __global__ void k_group_gram(const zd* __restrict__ A, const zd* __restrict__ B,
     }
 }
 
-// W[m][g] = exp(-1im*kz*dz) * scale with kz = sqrt(k0^2 - kx^2 - ky^2) at the group's
-// representative bin (the same phase as the per-bin tables of the propagator)
-__global__ void k_parseval_w(zd* W, const PwDesc* __restrict__ d, const double* __restrict__ gkx,
-                             const double* __restrict__ gky, int G, double scale) {
-    const int m = blockIdx.y;
-    const PwDesc dd = d[m];
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
-        const zd ph = kz_phase(dd.k0sq, gkx[g], gky[g], dd.dz);
-        W[(size_t)m * G + g] = cmk<double>(ph.re * scale, ph.im * scale);
+// Flat air regions by k-bin groups on the FP64 matrix cores:
+//     P[m] = sum_g W[m][g] H_g,   W[m][g] = exp(-1im*kz*dz_m) * scale,
+// kz = sqrt(k0_m^2 - kx_g^2 - ky_g^2) at the group's representative bin (the
+// phase of the propagator's per-bin tables, kz_phase).  A block computes a
+// PG_BM x PG_BN tile of [problems m] x [matrix entries e = i + n j]: per chunk of
+// PG_KC groups it forms its rows' weights in registers (never stored in HBM),
+// loads the chunk of H (H[g][e], shared by every problem of the mode set), stages
+// both in LDS and runs v_mfma_f64_16x16x4f64 with two accumulators per complex
+// 16 x 16 tile (re += Wre Hre + (-Wim) Him, im += Wre Him + Wim Hre).  Each wave
+// owns 32 problems x 64 entries (2 x 4 tiles).  Blocks of one column slab are
+// consecutive in the grid, so the blocks in flight read the same H columns.
+// Results go straight to each problem's matrix (column-major n x n).
+constexpr int PG_WM = 2, PG_WN = 2;                    // waves along problems, along entries
+constexpr int PG_BM = 32 * PG_WM, PG_BN = 64 * PG_WN;  // block tile
+constexpr int PG_KC = 16;                              // groups per chunk
+constexpr int PG_NT = 64 * PG_WM * PG_WN;
+constexpr int PG_LDA = PG_BM + 2, PG_LDB = PG_BN + 2;  // padded LDS rows (doubles)
+constexpr int PG_WPT = PG_BM * PG_KC / PG_NT;          // weights formed per thread and chunk
+constexpr int PG_HPT = PG_BN * PG_KC / PG_NT;          // H elements loaded per thread and chunk
+static_assert(PG_NT % PG_BM == 0 && PG_NT % PG_BN == 0, "Parseval GEMM thread mapping");
+
+__global__ void __launch_bounds__(PG_NT, 2) k_parseval_gemm(const zd* __restrict__ H, int nn, int NG,
+                                                            const double* __restrict__ gkx,
+                                                            const double* __restrict__ gky,
+                                                            const PwDesc* __restrict__ pd, int M, double scale,
+                                                            zd* const* __restrict__ dst, int nmb) {
+    __shared__ double sWr[PG_KC][PG_LDA], sWi[PG_KC][PG_LDA];
+    __shared__ double sHr[PG_KC][PG_LDB], sHi[PG_KC][PG_LDB];
+    const int mb = blockIdx.x % nmb, cb = blockIdx.x / nmb;
+    const int m0 = mb * PG_BM, c0 = cb * PG_BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w % PG_WM, wn = w / PG_WM;
+    // weights: thread t forms rows m0 + t % PG_BM, chunk columns t / PG_BM + (PG_NT / PG_BM) j
+    const int wrow = t % PG_BM, wk0 = t / PG_BM;
+    const bool wvalid = m0 + wrow < M;
+    PwDesc dd;
+    if (wvalid) dd = pd[m0 + wrow];
+    // H: thread t loads entries c0 + t % PG_BN, chunk rows t / PG_BN + (PG_NT / PG_BN) j
+    const int he = t % PG_BN, hk0 = t / PG_BN;
+    const bool hvalid = c0 + he < nn;
+    zd wreg[PG_WPT], hreg[PG_HPT];
+    auto fetch = [&](int g0) {
+#pragma unroll
+        for (int j = 0; j < PG_HPT; j++) {
+            const int g = g0 + hk0 + (PG_NT / PG_BN) * j;
+            hreg[j] = (hvalid && g < NG) ? H[(size_t)g * nn + c0 + he] : cmk<double>(0.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < PG_WPT; j++) {
+            const int g = g0 + wk0 + (PG_NT / PG_BM) * j;
+            zd v = cmk<double>(0.0, 0.0);
+            if (wvalid && g < NG) {
+                const zd ph = kz_phase(dd.k0sq, gkx[g], gky[g], dd.dz);
+                v = cmk<double>(ph.re * scale, ph.im * scale);
+            }
+            wreg[j] = v;
+        }
+    };
+    // tiles of this wave inside the problem range and the entry range (uniform per wave)
+    const int r0 = m0 + wm * 32, e0 = c0 + wn * 64;
+    const int nrt = r0 >= M ? 0 : (r0 + 16 >= M ? 1 : 2);
+    int nct = 0;
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) nct += (e0 + 16 * ct < nn) ? 1 : 0;
+    d4 are[2][4], aim[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) are[rt][ct] = aim[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+    const int nchunks = (NG + PG_KC - 1) / PG_KC;
+    fetch(0);
+    for (int c = 0; c < nchunks; c++) {
+        __syncthreads();  // the previous chunk's operands have been read
+#pragma unroll
+        for (int j = 0; j < PG_WPT; j++) {
+            const int k = wk0 + (PG_NT / PG_BM) * j;
+            sWr[k][wrow] = wreg[j].re;
+            sWi[k][wrow] = wreg[j].im;
+        }
+#pragma unroll
+        for (int j = 0; j < PG_HPT; j++) {
+            const int k = hk0 + (PG_NT / PG_BN) * j;
+            sHr[k][he] = hreg[j].re;
+            sHi[k][he] = hreg[j].im;
+        }
+        __syncthreads();
+        if (c + 1 < nchunks) fetch((c + 1) * PG_KC);  // next chunk's loads and weights overlap this chunk's MFMAs
+        if (nrt == 0 || nct == 0) continue;
+#pragma unroll
+        for (int ks = 0; ks < PG_KC; ks += 4) {
+            // A operand 16 x 4: lane -> (row lane % 16, group ks + lane / 16); B operand 4 x 16: (group, entry)
+            const int kk = ks + (lane >> 4);
+            double ar[2], ai[2], br[4], bi[4];
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                ar[rt] = sWr[kk][wm * 32 + rt * 16 + (lane & 15)];
+                ai[rt] = sWi[kk][wm * 32 + rt * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                br[ct] = sHr[kk][wn * 64 + ct * 16 + (lane & 15)];
+                bi[ct] = sHi[kk][wn * 64 + ct * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++) {
+                if (rt >= nrt) break;
+                const double nai = -ai[rt];
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++) {
+                    if (ct >= nct) break;
+                    are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], br[ct], are[rt][ct], 0, 0, 0);
+                    aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], bi[ct], aim[rt][ct], 0, 0, 0);
+                    are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[ct], are[rt][ct], 0, 0, 0);
+                    aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], br[ct], aim[rt][ct], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D layout (v_mfma_f64_16x16x4f64): register v of lane holds D[4 v + lane / 16][lane % 16]
+#pragma unroll
+    for (int rt = 0; rt < 2; rt++) {
+        if (rt >= nrt) break;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int m = r0 + rt * 16 + 4 * v + (lane >> 4);
+            if (m >= M) continue;
+            zd* out = dst[m];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const int e = e0 + ct * 16 + (lane & 15);
+                if (ct < nct && e < nn) out[e] = cmk<double>(are[rt][ct][v], aim[rt][ct][v]);
+            }
+        }
     }
 }
 
@@ -940,14 +1062,6 @@ __global__ void k_combine_moments(zd* P, const zd* __restrict__ M, const CombJob
     }
 }
 
-// *dsts[m] = Cm[m] (n x n each)
-__global__ void k_scatter_mats(zd* const* __restrict__ dsts, const zd* __restrict__ Cm, int nn) {
-    const int m = blockIdx.y;
-    zd* dst = dsts[m];
-    const zd* src = Cm + (size_t)m * nn;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += gridDim.x * blockDim.x) dst[e] = src[e];
-}
-
 __global__ void k_copy_z(zd* dst, const zd* __restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -985,24 +1099,6 @@ static void launch_inverse(bf_ctx* ctx, const zd* P, zd* Pinv, int n, const int*
         hipLaunchKernelGGL(k_batched_inverse, dim3((unsigned)count), dim3(256), 0, s, P, Pinv, n, work.as<zd>(), dsel);
         BF_CHECK_LAUNCH();
         BF_HIP(hipStreamSynchronize(s));  // `work` is released on return
-    }
-}
-
-static rocblas_handle blas_of(bf_ctx* ctx) {
-    if (!ctx->blas) {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(BF_ERR_HIP, "rocblas_create_handle failed");
-        ctx->blas = h;
-        if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success)
-            throw Error(BF_ERR_HIP, "rocblas_set_stream failed");
-    }
-    return static_cast<rocblas_handle>(ctx->blas);
-}
-
-void release_blas(bf_ctx* ctx) {
-    if (ctx->blas) {
-        (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
-        ctx->blas = nullptr;
     }
 }
 
@@ -1471,37 +1567,21 @@ void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, Parse
 void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q) {
     if (q.pd.empty()) return;
     hipStream_t s = ctx->stream;
-    const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, total = q.pd.size();
-    // problems per GEMM: W and the GEMM output within 4 GB (option parseval_bytes overrides), grid.y <= 65535
-    const size_t budget = opts().parseval_bytes > 0 ? (size_t)opts().parseval_bytes : (size_t)4 << 30;
-    const size_t mc_ = std::max<size_t>(1, std::min<size_t>({total, (size_t)65535, budget / ((NG + nn) * sizeof(zd))}));
-    DevBuf& Wt = scratch_buf(ctx, SB_W, sizeof(zd) * mc_ * NG);
-    DevBuf& Cm = scratch_buf(ctx, SB_CM, sizeof(zd) * mc_ * nn);
-    DevBuf& dpw = scratch_buf(ctx, SB_PW, sizeof(PwDesc) * mc_);
-    DevBuf& ddst = scratch_buf(ctx, SB_DST, sizeof(zd*) * mc_);
-    for (size_t m0 = 0; m0 < total; m0 += mc_) {
-        const size_t M = std::min(mc_, total - m0);
-        upload(dpw.p, q.pd.data() + m0, sizeof(PwDesc) * M, s);
-        upload(ddst.p, q.dst.data() + m0, sizeof(zd*) * M, s);
-        hipLaunchKernelGGL(k_parseval_w, dim3(g1(NG, 64), (unsigned)M), dim3(256), 0, s, Wt.as<zd>(),
-                           dpw.as<PwDesc>(), mc.gkx.as<double>(), mc.gky.as<double>(), (int)NG, scale);
+    const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, M = q.pd.size();
+    if (M > (size_t)INT32_MAX) throw Error(BF_ERR_UNSUPPORTED, "too many Parseval problems in one flush");
+    DevBuf& dpw = scratch_buf(ctx, SB_PW, sizeof(PwDesc) * M);
+    DevBuf& ddst = scratch_buf(ctx, SB_DST, sizeof(zd*) * M);
+    upload(dpw.p, q.pd.data(), sizeof(PwDesc) * M, s);
+    upload(ddst.p, q.dst.data(), sizeof(zd*) * M, s);
+    const int nmb = (int)((M + PG_BM - 1) / PG_BM), ncb = (int)((nn + PG_BN - 1) / PG_BN);
+    {
+        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
+        hipLaunchKernelGGL(k_parseval_gemm, dim3((unsigned)((size_t)nmb * ncb)), dim3(PG_NT), 0, s, mc.H.as<zd>(),
+                           (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), dpw.as<PwDesc>(), (int)M, scale,
+                           ddst.as<zd*>(), nmb);
         BF_CHECK_LAUNCH();
-        {
-            TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
-            const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-            if (rocblas_zgemm(blas_of(ctx), rocblas_operation_none, rocblas_operation_none, (rocblas_int)nn,
-                              (rocblas_int)M, (rocblas_int)NG, &one,
-                              reinterpret_cast<const rocblas_double_complex*>(mc.H.as<zd>()), (rocblas_int)nn,
-                              reinterpret_cast<const rocblas_double_complex*>(Wt.as<zd>()), (rocblas_int)NG, &zero,
-                              reinterpret_cast<rocblas_double_complex*>(Cm.as<zd>()),
-                              (rocblas_int)nn) != rocblas_status_success)
-                throw Error(BF_ERR_HIP, "rocblas_zgemm failed");
-        }
-        hipLaunchKernelGGL(k_scatter_mats, dim3(g1(nn, 64), (unsigned)M), dim3(256), 0, s, ddst.as<zd*>(),
-                           Cm.as<zd>(), (int)nn);
-        BF_CHECK_LAUNCH();
-        BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next chunk's uploads
     }
+    BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads
     q.pd.clear();
     q.dst.clear();
 }

@@ -112,7 +112,7 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * transforms, wavefront order), "graph" (dancer plans replay HIP graphs),
  * "parseval_groups", "disk_expansion" (transformer work reductions).  Shapes
  * and budgets (0 = default): "col_tx", "rows" (dancer launch shapes),
- * "parseval_bytes", "batch_bytes" (chunk budgets).  Unknown names and negative
+ * "batch_bytes" (matrix budget of a sweep chunk).  Unknown names and negative
  * values return BF_ERR_ARG.  No option changes a result beyond rounding. */
 int bf_ctx_set_option(bf_ctx *ctx, const char *name, int64_t value);
 int bf_ctx_get_option(bf_ctx *ctx, const char *name, int64_t *value);
