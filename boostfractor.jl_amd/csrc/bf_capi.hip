@@ -191,7 +191,7 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"chwave", &bf::EngineOptions::chwave}, {"graph", &bf::EngineOptions::graph},
         {"col_tx", &bf::EngineOptions::col_tx}, {"rows", &bf::EngineOptions::rows},
         {"parseval_groups", &bf::EngineOptions::parseval_groups},
-        {"disk_expansion", &bf::EngineOptions::disk_expansion},
+        {"disk_expansion", &bf::EngineOptions::disk_expansion}, {"pg_wn", &bf::EngineOptions::pg_wn},
     };
     ip = nullptr;
     for (const Int& e : ints)

@@ -105,13 +105,21 @@ __host__ __device__ inline zd cexp_j(zd z) {
 
 // Angular-spectrum phase exp(-1im*kz*dz) with kz = sqrt(k0^2 - Kx^2 - Ky^2),
 // op order of src/boost_fractor.jl:171-174.
-__host__ __device__ inline zd kz_phase(zd k0sq, double Kx, double Ky, double dz) {
+// kz = sqrt(k0^2 - Kx^2 - Ky^2)
+__host__ __device__ inline zd kz_of(zd k0sq, double Kx, double Ky) {
 #pragma clang fp contract(off)
     zd arg = cmk<double>(k0sq.re - Kx * Kx, k0sq.im);
     arg = cmk<double>(arg.re - Ky * Ky, arg.im);
-    zd kz = csqrt_j(arg);
+    return csqrt_j(arg);
+}
+// exp(-1im*kz*dz)
+__host__ __device__ inline zd kz_phase_of(zd kz, double dz) {
+#pragma clang fp contract(off)
     zd e = neg_i(kz);
     return cexp_j(cmk<double>(dz * e.re, dz * e.im));
+}
+__host__ __device__ inline zd kz_phase(zd k0sq, double Kx, double Ky, double dz) {
+    return kz_phase_of(kz_of(k0sq, Kx, Ky), dz);
 }
 
 // Aperture predicate of the propagators, abs(x^2 + y^2) < diskR^2

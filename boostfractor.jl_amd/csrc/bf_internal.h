@@ -92,6 +92,7 @@ struct EngineOptions {
     int col_tx = 0, rows = 0; // dancer launch shapes (0: tuned/default)
     int parseval_groups = 1;  // flat air regions by k-bin groups
     int disk_expansion = 1;   // tilted/rough disks by a frequency series of moment Grams
+    int pg_wn = 0;            // Parseval product: waves along the matrix entries, 2 or 4 (0: by n)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: default 2 GiB)
 };
 // Options of the context whose call runs on this thread (defaults outside a call).
@@ -130,7 +131,7 @@ inline DevBuf& scratch_buf(bf_ctx* ctx, int slot, size_t bytes) {
 }
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
-    SB_SLOTS, SB_FLAG, SB_K0, SB_DZ
+    SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0
 };
 }  // namespace bf
 

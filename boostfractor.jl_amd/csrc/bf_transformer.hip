@@ -854,112 +854,144 @@ __global__ void k_group_gram(const zd* __restrict__ A, const zd* __restrict__ B,
 //     P[m] = sum_g W[m][g] H_g,   W[m][g] = exp(-1im*kz*dz_m) * scale,
 // kz = sqrt(k0_m^2 - kx_g^2 - ky_g^2) at the group's representative bin (the
 // phase of the propagator's per-bin tables, kz_phase).  A block computes a
-// PG_BM x PG_BN tile of [problems m] x [matrix entries e = i + n j]: per chunk of
-// PG_KC groups it forms its rows' weights in registers (never stored in HBM),
-// loads the chunk of H (H[g][e], shared by every problem of the mode set), stages
-// both in LDS and runs v_mfma_f64_16x16x4f64 with two accumulators per complex
-// 16 x 16 tile (re += Wre Hre + (-Wim) Him, im += Wre Him + Wim Hre).  Each wave
-// owns 32 problems x 64 entries (2 x 4 tiles).  Blocks of one column slab are
+// BM x BN tile of [problems m] x [matrix entries e = i + n j] and forms its
+// rows' weights in registers (never stored in HBM).  Consecutive problems are
+// mostly one geometry's air gaps at one frequency, so a block's rows share few
+// k0^2 values: the host lists them per block (slots), kz is formed once per
+// (slot, group) two chunks ahead into a three-deep LDS table, and each weight
+// then costs only exp(-1im*kz*dz) (the same operations as kz_phase: the
+// weights are identical).  Per chunk of KC groups the block loads the chunk
+// of H (H[g][e], shared by every problem of the mode set), stages weights and
+// H in LDS, double-buffered (one barrier per chunk); the MFMAs of a chunk run
+// between the next chunk's H loads (issued first) and its weights (formed
+// after).  v_mfma_f64_16x16x4f64 with two accumulators per complex 16 x 16
+// tile (re += Wre Hre + (-Wim) Him, im += Wre Him + Wim Hre); each wave owns
+// 32 problems x 64 entries (2 x 4 tiles) on zero-padded operands (the block
+// waits for its fullest wave anyway).  Blocks of one column slab are
 // consecutive in the grid, so the blocks in flight read the same H columns.
 // Results go straight to each problem's matrix (column-major n x n).
-constexpr int PG_WM = 2, PG_WN = 2;                    // waves along problems, along entries
-constexpr int PG_BM = 32 * PG_WM, PG_BN = 64 * PG_WN;  // block tile
-constexpr int PG_KC = 16;                              // groups per chunk
-constexpr int PG_NT = 64 * PG_WM * PG_WN;
-constexpr int PG_LDA = PG_BM + 2, PG_LDB = PG_BN + 2;  // padded LDS rows (doubles)
-constexpr int PG_WPT = PG_BM * PG_KC / PG_NT;          // weights formed per thread and chunk
-constexpr int PG_HPT = PG_BN * PG_KC / PG_NT;          // H elements loaded per thread and chunk
-static_assert(PG_NT % PG_BM == 0 && PG_NT % PG_BN == 0, "Parseval GEMM thread mapping");
+template <int WN>
+struct PgShape {
+    static constexpr int WM = 2;                     // waves along problems
+    static constexpr int BM = 32 * WM, BN = 64 * WN;  // block tile
+    static constexpr int KC = 8;                     // groups per chunk
+    static constexpr int NT = 64 * WM * WN;
+    static constexpr int LDA = BM + 2, LDB = BN + 2;  // padded LDS rows (doubles)
+    static constexpr int WPT = BM * KC / NT;          // weights formed per thread and chunk
+    static constexpr int HPT = BN * KC / NT;          // H elements loaded per thread and chunk
+    static constexpr int KZPT = (BM * KC + NT - 1) / NT;  // (slot, group) pairs per thread at most
+    static_assert(NT % BM == 0 && NT % BN == 0 && KC % 4 == 0 && WPT >= 1, "Parseval GEMM thread mapping");
+};
+constexpr int PG_BM = 64;  // rows per block (all shapes)
 
-__global__ void __launch_bounds__(PG_NT, 2) k_parseval_gemm(const zd* __restrict__ H, int nn, int NG,
-                                                            const double* __restrict__ gkx,
-                                                            const double* __restrict__ gky,
-                                                            const PwDesc* __restrict__ pd, int M, double scale,
-                                                            zd* const* __restrict__ dst, int nmb) {
-    __shared__ double sWr[PG_KC][PG_LDA], sWi[PG_KC][PG_LDA];
-    __shared__ double sHr[PG_KC][PG_LDB], sHi[PG_KC][PG_LDB];
+template <int WN>
+__global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
+    const zd* __restrict__ H, int nn, int NG, const double* __restrict__ gkx, const double* __restrict__ gky,
+    const PwDesc* __restrict__ pd, const int* __restrict__ row_slot, const int* __restrict__ blk_nk,
+    const zd* __restrict__ blk_k0sq, int M, double scale, zd* const* __restrict__ dst, int nmb) {
+    using S = PgShape<WN>;
+    constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NT = S::NT;
+    __shared__ double swr[2][KC][S::LDA], swi[2][KC][S::LDA];
+    __shared__ double shr[2][KC][S::LDB], shi[2][KC][S::LDB];
+    __shared__ zd skz[3][BM][KC];
     const int mb = blockIdx.x % nmb, cb = blockIdx.x / nmb;
-    const int m0 = mb * PG_BM, c0 = cb * PG_BN;
+    const int m0 = mb * BM, c0 = cb * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = w % PG_WM, wn = w / PG_WM;
-    // weights: thread t forms rows m0 + t % PG_BM, chunk columns t / PG_BM + (PG_NT / PG_BM) j
-    const int wrow = t % PG_BM, wk0 = t / PG_BM;
+    const int wm = w % S::WM, wn = w / S::WM;
+    const int nk = blk_nk[mb];
+    const zd* k0s = blk_k0sq + (size_t)mb * BM;
+    // weights: thread t forms row m0 + t % BM, chunk columns t / BM + (NT / BM) j
+    const int wrow = t % BM, wk0 = t / BM;
     const bool wvalid = m0 + wrow < M;
-    PwDesc dd;
-    if (wvalid) dd = pd[m0 + wrow];
-    // H: thread t loads entries c0 + t % PG_BN, chunk rows t / PG_BN + (PG_NT / PG_BN) j
-    const int he = t % PG_BN, hk0 = t / PG_BN;
+    double dz = 0.0;
+    int slot = 0;
+    if (wvalid) {
+        dz = pd[m0 + wrow].dz;
+        slot = row_slot[m0 + wrow];
+    }
+    // H: thread t loads entries c0 + t % BN, chunk rows t / BN + (NT / BN) j
+    const int he = t % BN, hk0 = t / BN;
     const bool hvalid = c0 + he < nn;
-    zd wreg[PG_WPT], hreg[PG_HPT];
-    auto fetch = [&](int g0) {
+    zd wreg[S::WPT], hreg[S::HPT];
+    auto form_kz = [&](int c) {  // kz of chunk c into skz[c % 3]
 #pragma unroll
-        for (int j = 0; j < PG_HPT; j++) {
-            const int g = g0 + hk0 + (PG_NT / PG_BN) * j;
+        for (int i = 0; i < S::KZPT; i++) {
+            const int idx = t + NT * i, s = idx / KC, k = idx % KC, g = c * KC + k;
+            if (s < nk && g < NG) skz[c % 3][s][k] = kz_of(k0s[s], gkx[g], gky[g]);
+        }
+    };
+    auto load_h = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < S::HPT; j++) {
+            const int g = c * KC + hk0 + (NT / BN) * j;
             hreg[j] = (hvalid && g < NG) ? H[(size_t)g * nn + c0 + he] : cmk<double>(0.0, 0.0);
         }
+    };
+    auto form_w = [&](int c) {  // reads skz[c % 3]
 #pragma unroll
-        for (int j = 0; j < PG_WPT; j++) {
-            const int g = g0 + wk0 + (PG_NT / PG_BM) * j;
+        for (int j = 0; j < S::WPT; j++) {
+            const int k = wk0 + (NT / BM) * j, g = c * KC + k;
             zd v = cmk<double>(0.0, 0.0);
             if (wvalid && g < NG) {
-                const zd ph = kz_phase(dd.k0sq, gkx[g], gky[g], dd.dz);
+                const zd ph = kz_phase_of(skz[c % 3][slot][k], dz);
                 v = cmk<double>(ph.re * scale, ph.im * scale);
             }
             wreg[j] = v;
         }
     };
-    // tiles of this wave inside the problem range and the entry range (uniform per wave)
-    const int r0 = m0 + wm * 32, e0 = c0 + wn * 64;
-    const int nrt = r0 >= M ? 0 : (r0 + 16 >= M ? 1 : 2);
-    int nct = 0;
+    auto stage = [&](int c) {
+        const int b = c & 1;
 #pragma unroll
-    for (int ct = 0; ct < 4; ct++) nct += (e0 + 16 * ct < nn) ? 1 : 0;
+        for (int j = 0; j < S::WPT; j++) {
+            const int k = wk0 + (NT / BM) * j;
+            swr[b][k][wrow] = wreg[j].re;
+            swi[b][k][wrow] = wreg[j].im;
+        }
+#pragma unroll
+        for (int j = 0; j < S::HPT; j++) {
+            const int k = hk0 + (NT / BN) * j;
+            shr[b][k][he] = hreg[j].re;
+            shi[b][k][he] = hreg[j].im;
+        }
+    };
     d4 are[2][4], aim[2][4];
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) are[rt][ct] = aim[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
-    const int nchunks = (NG + PG_KC - 1) / PG_KC;
-    fetch(0);
+    const int nchunks = (NG + KC - 1) / KC;
+    form_kz(0);
+    form_kz(1);
+    load_h(0);
+    __syncthreads();
+    form_w(0);
+    stage(0);
+    __syncthreads();
     for (int c = 0; c < nchunks; c++) {
-        __syncthreads();  // the previous chunk's operands have been read
+        const int b = c & 1;
+        const bool more = c + 1 < nchunks;
+        if (more) load_h(c + 1);
+        form_kz(c + 2);  // skz[(c + 2) % 3] was last read in chunk c - 1's weights, before the previous barrier
 #pragma unroll
-        for (int j = 0; j < PG_WPT; j++) {
-            const int k = wk0 + (PG_NT / PG_BM) * j;
-            sWr[k][wrow] = wreg[j].re;
-            sWi[k][wrow] = wreg[j].im;
-        }
-#pragma unroll
-        for (int j = 0; j < PG_HPT; j++) {
-            const int k = hk0 + (PG_NT / PG_BN) * j;
-            sHr[k][he] = hreg[j].re;
-            sHi[k][he] = hreg[j].im;
-        }
-        __syncthreads();
-        if (c + 1 < nchunks) fetch((c + 1) * PG_KC);  // next chunk's loads and weights overlap this chunk's MFMAs
-        if (nrt == 0 || nct == 0) continue;
-#pragma unroll
-        for (int ks = 0; ks < PG_KC; ks += 4) {
+        for (int ks = 0; ks < KC; ks += 4) {
             // A operand 16 x 4: lane -> (row lane % 16, group ks + lane / 16); B operand 4 x 16: (group, entry)
             const int kk = ks + (lane >> 4);
             double ar[2], ai[2], br[4], bi[4];
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
-                ar[rt] = sWr[kk][wm * 32 + rt * 16 + (lane & 15)];
-                ai[rt] = sWi[kk][wm * 32 + rt * 16 + (lane & 15)];
+                ar[rt] = swr[b][kk][wm * 32 + rt * 16 + (lane & 15)];
+                ai[rt] = swi[b][kk][wm * 32 + rt * 16 + (lane & 15)];
             }
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
-                br[ct] = sHr[kk][wn * 64 + ct * 16 + (lane & 15)];
-                bi[ct] = sHi[kk][wn * 64 + ct * 16 + (lane & 15)];
+                br[ct] = shr[b][kk][wn * 64 + ct * 16 + (lane & 15)];
+                bi[ct] = shi[b][kk][wn * 64 + ct * 16 + (lane & 15)];
             }
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
-                if (rt >= nrt) break;
                 const double nai = -ai[rt];
 #pragma unroll
                 for (int ct = 0; ct < 4; ct++) {
-                    if (ct >= nct) break;
                     are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], br[ct], are[rt][ct], 0, 0, 0);
                     aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], bi[ct], aim[rt][ct], 0, 0, 0);
                     are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[ct], are[rt][ct], 0, 0, 0);
@@ -967,11 +999,16 @@ __global__ void __launch_bounds__(PG_NT, 2) k_parseval_gemm(const zd* __restrict
                 }
             }
         }
+        if (more) {
+            form_w(c + 1);  // skz[(c + 1) % 3] was formed in chunk c - 1 (or the prologue)
+            stage(c + 1);   // stage buffer (c + 1) & 1 was read in chunk c - 1, before the previous barrier
+        }
+        __syncthreads();
     }
     // D layout (v_mfma_f64_16x16x4f64): register v of lane holds D[4 v + lane / 16][lane % 16]
+    const int r0 = m0 + wm * 32, e0 = c0 + wn * 64;
 #pragma unroll
     for (int rt = 0; rt < 2; rt++) {
-        if (rt >= nrt) break;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int m = r0 + rt * 16 + 4 * v + (lane >> 4);
@@ -980,7 +1017,7 @@ __global__ void __launch_bounds__(PG_NT, 2) k_parseval_gemm(const zd* __restrict
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
                 const int e = e0 + ct * 16 + (lane & 15);
-                if (ct < nct && e < nn) out[e] = cmk<double>(are[rt][ct][v], aim[rt][ct][v]);
+                if (e < nn) out[e] = cmk<double>(are[rt][ct][v], aim[rt][ct][v]);
             }
         }
     }
@@ -1568,17 +1605,51 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
     if (q.pd.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, M = q.pd.size();
-    if (M > (size_t)INT32_MAX) throw Error(BF_ERR_UNSUPPORTED, "too many Parseval problems in one flush");
+    if (M > (size_t)INT32_MAX / 2) throw Error(BF_ERR_UNSUPPORTED, "too many Parseval problems in one flush");
+    const int nmb = (int)((M + PG_BM - 1) / PG_BM);
+    // the distinct k0^2 of each block of PG_BM problems (bitwise) and every problem's slot among them
+    std::vector<int> slot(M), nk(nmb);
+    std::vector<zd> k0s((size_t)nmb * PG_BM, cmk<double>(0.0, 0.0));
+    for (int b = 0; b < nmb; b++) {
+        int cnt = 0;
+        zd* list = k0s.data() + (size_t)b * PG_BM;
+        for (size_t m = (size_t)b * PG_BM; m < std::min(M, (size_t)(b + 1) * PG_BM); m++) {
+            const zd v = q.pd[m].k0sq;
+            int k = 0;
+            while (k < cnt && !(std::memcmp(&list[k], &v, sizeof(zd)) == 0)) k++;
+            if (k == cnt) list[cnt++] = v;
+            slot[m] = k;
+        }
+        nk[b] = cnt;
+    }
     DevBuf& dpw = scratch_buf(ctx, SB_PW, sizeof(PwDesc) * M);
     DevBuf& ddst = scratch_buf(ctx, SB_DST, sizeof(zd*) * M);
+    DevBuf& dslot = scratch_buf(ctx, SB_PGSLOT, sizeof(int) * M);
+    DevBuf& dnk = scratch_buf(ctx, SB_PGNK, sizeof(int) * nmb);
+    DevBuf& dk0 = scratch_buf(ctx, SB_PGK0, sizeof(zd) * k0s.size());
     upload(dpw.p, q.pd.data(), sizeof(PwDesc) * M, s);
     upload(ddst.p, q.dst.data(), sizeof(zd*) * M, s);
-    const int nmb = (int)((M + PG_BM - 1) / PG_BM), ncb = (int)((nn + PG_BN - 1) / PG_BN);
+    upload(dslot.p, slot.data(), sizeof(int) * M, s);
+    upload(dnk.p, nk.data(), sizeof(int) * nmb, s);
+    upload(dk0.p, k0s.data(), sizeof(zd) * k0s.size(), s);
+    // waves along the entries: 4 (256 entries per block) unless the matrix fills 128 better
+    int wn = opts().pg_wn;
+    if (wn != 2 && wn != 4) wn = ((nn + 127) / 128 * 128 <= (nn + 255) / 256 * 256) ? 2 : 4;
     {
         TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
-        hipLaunchKernelGGL(k_parseval_gemm, dim3((unsigned)((size_t)nmb * ncb)), dim3(PG_NT), 0, s, mc.H.as<zd>(),
-                           (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), dpw.as<PwDesc>(), (int)M, scale,
-                           ddst.as<zd*>(), nmb);
+        if (wn == 2) {
+            const int ncb = (int)((nn + PgShape<2>::BN - 1) / PgShape<2>::BN);
+            hipLaunchKernelGGL(k_parseval_gemm<2>, dim3((unsigned)((size_t)nmb * ncb)), dim3(PgShape<2>::NT), 0, s,
+                               mc.H.as<zd>(), (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(),
+                               dpw.as<PwDesc>(), dslot.as<int>(), dnk.as<int>(), dk0.as<zd>(), (int)M, scale,
+                               ddst.as<zd*>(), nmb);
+        } else {
+            const int ncb = (int)((nn + PgShape<4>::BN - 1) / PgShape<4>::BN);
+            hipLaunchKernelGGL(k_parseval_gemm<4>, dim3((unsigned)((size_t)nmb * ncb)), dim3(PgShape<4>::NT), 0, s,
+                               mc.H.as<zd>(), (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(),
+                               dpw.as<PwDesc>(), dslot.as<int>(), dnk.as<int>(), dk0.as<zd>(), (int)M, scale,
+                               ddst.as<zd*>(), nmb);
+        }
         BF_CHECK_LAUNCH();
     }
     BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads
