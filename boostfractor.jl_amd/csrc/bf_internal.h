@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -123,11 +125,91 @@ struct bf_ctx {
 };
 
 namespace bf {
-// scratch slot `slot` of the context with at least `bytes` bytes (contents undefined)
+// scratch slot `slot` of the context with at least `bytes` bytes (contents undefined); a buffer that
+// grows is freed only after the work queued on the context's stream (which may still use it) is done
 inline DevBuf& scratch_buf(bf_ctx* ctx, int slot, size_t bytes) {
     DevBuf& b = ctx->scratch[slot];
-    if (b.bytes < bytes) b.alloc(bytes);
+    if (b.bytes < bytes) {
+        if (b.p) BF_HIP(hipStreamSynchronize(ctx->stream));
+        b.alloc(bytes);
+    }
     return b;
+}
+
+// Host-to-device uploads through pinned staging memory for a sequence of
+// chunks whose host preparation overlaps the device work of the previous one:
+// put() copies into the current slot's pinned arena and queues an asynchronous
+// copy; end_chunk() records an event; begin_chunk() switches to the other slot
+// and waits, if needed, until the chunk that last used it has completed.
+struct Stager {
+    struct Block {
+        void* p = nullptr;
+        size_t bytes = 0, used = 0;
+    };
+    struct Slot {
+        std::vector<Block> blocks;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Slot slot[2];
+    int cur = 0;
+    Stager() = default;
+    Stager(const Stager&) = delete;
+    Stager& operator=(const Stager&) = delete;
+    ~Stager() {
+        for (Slot& sl : slot) {
+            if (sl.pending) (void)hipEventSynchronize(sl.done);
+            for (Block& b : sl.blocks) (void)hipHostFree(b.p);
+            if (sl.done) (void)hipEventDestroy(sl.done);
+        }
+    }
+    void begin_chunk() {
+        cur ^= 1;
+        Slot& sl = slot[cur];
+        if (sl.pending) BF_HIP(hipEventSynchronize(sl.done));
+        sl.pending = false;
+        if (sl.blocks.size() > 1) {  // merge into one block of the total size
+            size_t total = 0;
+            for (Block& b : sl.blocks) {
+                total += b.bytes;
+                BF_HIP(hipHostFree(b.p));
+            }
+            sl.blocks.clear();
+            Block nb;
+            BF_HIP(hipHostMalloc(&nb.p, total));
+            nb.bytes = total;
+            sl.blocks.push_back(nb);
+        }
+        for (Block& b : sl.blocks) b.used = 0;
+    }
+    void put(hipStream_t s, void* dst, const void* src, size_t bytes) {
+        if (!bytes) return;
+        Slot& sl = slot[cur];
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (sl.blocks.empty() || sl.blocks.back().used + need > sl.blocks.back().bytes) {
+            Block nb;
+            nb.bytes = std::max(need, (size_t)4 << 20);
+            BF_HIP(hipHostMalloc(&nb.p, nb.bytes));
+            sl.blocks.push_back(nb);
+        }
+        Block& b = sl.blocks.back();
+        void* h = static_cast<char*>(b.p) + b.used;
+        b.used += need;
+        std::memcpy(h, src, bytes);
+        BF_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+    }
+    void end_chunk(hipStream_t s) {
+        Slot& sl = slot[cur];
+        if (!sl.done) BF_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        BF_HIP(hipEventRecord(sl.done, s));
+        sl.pending = true;
+    }
+};
+// upload through the stager when there is one (no wait), else directly
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+inline void stage_upload(Stager* st, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (st) st->put(s, dst, src, bytes);
+    else upload(dst, src, bytes, s);
 }
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,

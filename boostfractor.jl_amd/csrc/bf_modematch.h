@@ -66,9 +66,10 @@ struct ParsevalQueue {
 };
 
 // P = sum_g W_g H_g for every queued problem (GEMMs in chunks bounded by memory); empties the queue
-void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q);
+// (with a stager: uploads staged, no wait at the end)
+void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st = nullptr);
 // the queued disk regions of `q` (diag(exp(-i kz dz)) from the left) in one launch
-void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q);
+void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st = nullptr);
 
 // one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
 struct ChainStep {
@@ -108,7 +109,7 @@ void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainS
 // inv(P) of the matrices `sel`, then one chain per job; boost (and refl) [njobs][n]
 void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
                         const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
-                        const zd* reflect, zd* boost, zd* refl);
+                        const zd* reflect, zd* boost, zd* refl, Stager* st = nullptr);
 
 }  // namespace bf
 
@@ -145,6 +146,7 @@ struct bf_transformer_batch_plan {
     bf::DevBuf dboost;  // complex, the last run's results at the caller's offsets
     size_t nboost = 0;  // extent of the last run's results (complex elements)
     std::vector<std::pair<size_t, size_t>> blocks;  // the last run's (offset, length) per geometry
+    bf::Stager stager;  // pinned staging of the chunks' descriptor uploads
 };
 
 // cubic B-spline coefficients of a propagation-matrix grid over the spacing axis

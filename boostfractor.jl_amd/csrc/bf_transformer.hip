@@ -1579,7 +1579,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     if (!queue || Mom.p || ktabs.p || rtabs.p || scratch.p || prop.p || scr2.p) BF_HIP(hipStreamSynchronize(s));
 }
 
-void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q) {
+void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st) {
     if (q.dslot.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t m = q.dslot.size();
@@ -1587,21 +1587,21 @@ void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, Parse
     DevBuf& dflag = scratch_buf(ctx, SB_FLAG, sizeof(int) * m);
     DevBuf& dk0 = scratch_buf(ctx, SB_K0, sizeof(zd) * m);
     DevBuf& ddz = scratch_buf(ctx, SB_DZ, sizeof(double) * m);
-    upload(dslots.p, q.dslot.data(), sizeof(int) * m, s);
-    upload(dflag.p, q.dfrom.data(), sizeof(int) * m, s);
-    upload(dk0.p, q.dk0sq.data(), sizeof(zd) * m, s);
-    upload(ddz.p, q.ddz.data(), sizeof(double) * m, s);
+    stage_upload(st, dslots.p, q.dslot.data(), sizeof(int) * m, s);
+    stage_upload(st, dflag.p, q.dfrom.data(), sizeof(int) * m, s);
+    stage_upload(st, dk0.p, q.dk0sq.data(), sizeof(zd) * m, s);
+    stage_upload(st, ddz.p, q.ddz.data(), sizeof(double) * m, s);
     hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)m), dim3(256), 0, s, q.diag_base, dslots.as<int>(), dflag.as<int>(),
                        mc.G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), md.n);
     BF_CHECK_LAUNCH();
-    BF_HIP(hipStreamSynchronize(s));  // the queue's host arrays are cleared next
+    if (!st) BF_HIP(hipStreamSynchronize(s));  // the queue's host arrays are cleared next
     q.dslot.clear();
     q.dfrom.clear();
     q.dk0sq.clear();
     q.ddz.clear();
 }
 
-void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q) {
+void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st) {
     if (q.pd.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, M = q.pd.size();
@@ -1627,11 +1627,11 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
     DevBuf& dslot = scratch_buf(ctx, SB_PGSLOT, sizeof(int) * M);
     DevBuf& dnk = scratch_buf(ctx, SB_PGNK, sizeof(int) * nmb);
     DevBuf& dk0 = scratch_buf(ctx, SB_PGK0, sizeof(zd) * k0s.size());
-    upload(dpw.p, q.pd.data(), sizeof(PwDesc) * M, s);
-    upload(ddst.p, q.dst.data(), sizeof(zd*) * M, s);
-    upload(dslot.p, slot.data(), sizeof(int) * M, s);
-    upload(dnk.p, nk.data(), sizeof(int) * nmb, s);
-    upload(dk0.p, k0s.data(), sizeof(zd) * k0s.size(), s);
+    stage_upload(st, dpw.p, q.pd.data(), sizeof(PwDesc) * M, s);
+    stage_upload(st, ddst.p, q.dst.data(), sizeof(zd*) * M, s);
+    stage_upload(st, dslot.p, slot.data(), sizeof(int) * M, s);
+    stage_upload(st, dnk.p, nk.data(), sizeof(int) * nmb, s);
+    stage_upload(st, dk0.p, k0s.data(), sizeof(zd) * k0s.size(), s);
     // waves along the entries: 4 (256 entries per block) unless the matrix fills 128 better
     int wn = opts().pg_wn;
     if (wn != 2 && wn != 4) wn = ((nn + 127) / 128 * 128 <= (nn + 255) / 256 * 256) ? 2 : 4;
@@ -1652,7 +1652,7 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
         }
         BF_CHECK_LAUNCH();
     }
-    BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads
+    if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads
     q.pd.clear();
     q.dst.clear();
 }
@@ -1799,12 +1799,12 @@ void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainS
 
 void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
                         const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
-                        const zd* reflect, zd* boost, zd* refl) {
+                        const zd* reflect, zd* boost, zd* refl, Stager* st) {
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n;
     const int N2 = 2 * n;
     DevBuf& dsel = scratch_buf(ctx, SB_SEL, sizeof(int) * sel.size());
-    upload(dsel.p, sel.data(), sizeof(int) * sel.size(), s);
+    stage_upload(st, dsel.p, sel.data(), sizeof(int) * sel.size(), s);
     {
         TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
         launch_inverse(ctx, P, Pinv, n, dsel.as<int>(), sel.size());
@@ -1815,8 +1815,8 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     DevBuf& dsteps = scratch_buf(ctx, SB_STEPS, sizeof(ChainStep) * steps.size());
     DevBuf& Tb = scratch_buf(ctx, SB_TB, lds ? 0 : sizeof(zd) * 2 * (size_t)N2 * N2 * nj);
     DevBuf& cw = scratch_buf(ctx, SB_CW, lds ? 0 : sizeof(zd) * (nn + 3 * (size_t)n) * nj);
-    upload(djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
-    upload(dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
+    stage_upload(st, djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
+    stage_upload(st, dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
     double flops = 0.0;
     for (const ChainJob& j : jobs) flops += (lds ? 16.0 : 64.0) * (double)n * n * n * j.nsteps;
     {
@@ -1834,7 +1834,7 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         }
         BF_CHECK_LAUNCH();
     }
-    BF_HIP(hipStreamSynchronize(s));
+    if (!st) BF_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------------------------
@@ -2155,18 +2155,21 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
                 }
                 base += (size_t)nreg * nfreq;
             }
-            parseval_flush(ctx, cache, n, 1.0 / (double)g.K(), pq);
-            disk_diag_flush(ctx, cache, md, pq);
+            // the chunk's uploads go through pinned staging, its kernels queue behind the previous chunk's
+            // (every device buffer is used in stream order), and the host goes on to prepare the next chunk
+            pl->stager.begin_chunk();
+            parseval_flush(ctx, cache, n, 1.0 / (double)g.K(), pq, &pl->stager);
+            disk_diag_flush(ctx, cache, md, pq, &pl->stager);
             inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, pl->de0[set].as<zd>(), nullptr,
-                               dboost.as<zd>(), nullptr);
+                               dboost.as<zd>(), nullptr, &pl->stager);
             for (size_t q = c0; q < c1; q++)
                 BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
                                       sizeof(zd) * n * nfreq, hipMemcpyDeviceToDevice, s));
-            // the chunk's host-side job lists and the scratch buffers are reused by the next chunk
-            BF_HIP(hipStreamSynchronize(s));
+            pl->stager.end_chunk(s);
             c0 = c1;
         }
     }
+    BF_HIP(hipStreamSynchronize(s));
 }
 
 // copies the last run's geometry blocks to the same offsets of `boost` (the
