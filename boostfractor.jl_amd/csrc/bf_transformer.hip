@@ -2101,25 +2101,30 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
     pl->nboost = total;
     // bytes of P + Pinv per chunk (option batch_bytes overrides, e.g. to exercise chunking on small cases)
     const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : (size_t)2 << 30;
+    bool first_chunk = true;
     for (int set = 0; set < (int)pl->md.size(); set++) {
         std::vector<int> confs;
         for (int c = 0; c < nconf; c++)
             if (set_of[c] == set) confs.push_back(c);
         if (confs.empty()) continue;
         const ModesDev& md = pl->md[set];
+        // the mode set's pattern transforms, k-bin Gram tables and flat-disk Gram depend only on the plan's
+        // modes and grid: formed by the first run that needs them and kept (no wait between mode sets)
         ModeCache& cache = pl->mc[set];
-        cache.reset_data();  // pattern FFTs and Gram tables are rebuilt every run
         const int n = md.n;
         const size_t nn = (size_t)n * n;
         for (size_t c0 = 0; c0 < confs.size();) {
-            // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry)
+            // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry); the
+            // run's first chunk holds a few geometries only, so the device starts while the host prepares more
+            const size_t chunk_budget = first_chunk ? budget / 16 : budget;
             size_t c1 = c0, mats = 0;
             while (c1 < confs.size()) {
                 const size_t m = (size_t)setups[confs[c1]].nreg * nfreq;
-                if (c1 > c0 && (mats + m) * nn * sizeof(zd) * 2 > budget) break;
+                if (c1 > c0 && (mats + m) * nn * sizeof(zd) * 2 > chunk_budget) break;
                 mats += m;
                 c1++;
             }
+            first_chunk = false;
             DevBuf& P = scratch_buf(ctx, SB_P, sizeof(zd) * nn * mats);
             DevBuf& Pinv = scratch_buf(ctx, SB_PINV, sizeof(zd) * nn * mats);
             DevBuf& dboost = scratch_buf(ctx, SB_BOOST, sizeof(zd) * n * nfreq * (c1 - c0));
