@@ -54,8 +54,8 @@ NDISK, N, NFREQ, NMAX, DISKR = 20, 256, 50, 2000, 0.15
 # 45 Laguerre-Gauss modes on 512^2, 200 frequencies
 N3, NF3 = 512, 200
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cols_reg_dancer_mixed_256x256_20disk.md")
-TRAFFIC_PROFILE_TR = os.path.join(ROOT, "profiles", "zgemm_parseval_transformer_512x512_45modes.md")
-TRAFFIC_PROFILE_SW = os.path.join(ROOT, "profiles", "zgemm_parseval_sweep_256x256_bessel.md")
+TRAFFIC_PROFILE_TR = os.path.join(ROOT, "profiles", "k_parseval_gemm_transformer_512x512_45modes.md")
+TRAFFIC_PROFILE_SW = os.path.join(ROOT, "profiles", "k_parseval_gemm_sweep_256x256_bessel.md")
 
 
 def pmc_per_kernel(db_path, counter):
@@ -213,8 +213,8 @@ def transformer_cpu_baseline(coords, sb, pat, kt):
 
 
 MM_KERNELS = {
-    "pgemm": "rocBLAS zgemm (Cijk_Ailk_Bljk_ZB, v_mfma_f64_16x16x4f64): flat air regions, "
-             "P = sum over k-bin groups of exp(-i kz dz) H_g",
+    "pgemm": "k_parseval_gemm (v_mfma_f64_16x16x4f64, weights exp(-i kz dz)/K formed in registers): flat air "
+             "regions, P = sum over k-bin groups of W_g H_g",
     "wgram": "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)",
 }
 
@@ -270,7 +270,7 @@ def transformer_main(args):
     host_dt = time.perf_counter() - t1
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)  # also the byte-counter calibration of profiler runs
     alg = (len(sb0.distance) - 1) * 8.0 * n * n * N3 * N3  # survey: sum over regions 2..Nreg of 8 n^2 K
-    roof = mm_roofline(kt_, "zgemm", TRAFFIC_PROFILE_TR, alg * NF3 * args.steps / dt)
+    roof = mm_roofline(kt_, "k_parseval_gemm", TRAFFIC_PROFILE_TR, alg * NF3 * args.steps / dt)
     roof["kernel_ms_from"] = "one instrumented step after the timed ones"
     roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)
     beta2 = np.sum(np.abs(boost) ** 2, axis=-1)
@@ -430,7 +430,7 @@ def sweep_main(args):
     plan.close()
     nmodes = [sets[s_].n for s_ in set_of]
     alg = sum((len(g.distance) - 1) * 8.0 * m * m * N5 * N5 for g, m in zip(geoms, nmodes)) * NF5
-    roof = mm_roofline(kt_, "zgemm", TRAFFIC_PROFILE_SW, alg / dt)
+    roof = mm_roofline(kt_, "k_parseval_gemm", TRAFFIC_PROFILE_SW, alg / dt)
     roof["kernel_ms_from"] = "one instrumented step (slice 0) after the timed ones"
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)
     roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)

@@ -1946,17 +1946,17 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
     std::vector<int> sel;
     std::vector<ChainJob> jobs;
     std::vector<ChainStep> steps;
+    append_chain_steps(eps, nreg, steps);  // the same region sequence at every frequency
     for (int f = 0; f < nfreq; f++) {
         for (int r = 1; r < nreg; r++) sel.push_back(f * nreg + r);
         ChainJob j;
-        j.step0 = (int64_t)steps.size();
+        j.step0 = 0;
         j.nsteps = nreg - 1;
         j.pbase = (int64_t)f * nreg;
         j.out = f;
         j.e0off = 0;
         j.pad = 0;
         jobs.push_back(j);
-        append_chain_steps(eps, nreg, steps);
     }
     DevBuf de0(sizeof(zd) * n), dref, dboost(sizeof(zd) * n * nfreq), drefl;
     upload(de0.p, emit, sizeof(zd) * n, s);
@@ -2146,17 +2146,19 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
                 propagation_matrices_core(ctx, g, md, pl->kind, pl->diskR, nreg, st->distance, eps.data(),
                                           st->tilt_x, st->tilt_y, surf.as<zd>(), flat, freqs, nfreq, regs,
                                           P.as<zd>() + base * nn, &cache, &pq);
+                // the geometry's chain steps do not depend on the frequency: one list for all its jobs
+                const int64_t step0 = (int64_t)steps.size();
+                append_chain_steps(eps, nreg, steps);
                 for (int f = 0; f < nfreq; f++) {
                     for (int r = 1; r < nreg; r++) sel.push_back((int)(base + (size_t)f * nreg + r));
                     ChainJob j;
-                    j.step0 = (int64_t)steps.size();
+                    j.step0 = step0;
                     j.nsteps = nreg - 1;
                     j.pbase = (int64_t)(base + (size_t)f * nreg);
                     j.out = (int64_t)((q - c0) * nfreq + f);
                     j.e0off = 0;
                     j.pad = 0;
                     jobs.push_back(j);
-                    append_chain_steps(eps, nreg, steps);
                 }
                 base += (size_t)nreg * nfreq;
             }
