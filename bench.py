@@ -54,8 +54,8 @@ NDISK, N, NFREQ, NMAX, DISKR = 20, 256, 50, 2000, 0.15
 # 45 Laguerre-Gauss modes on 512^2, 200 frequencies
 N3, NF3 = 512, 200
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_cols_reg_dancer_mixed_256x256_20disk.md")
-TRAFFIC_PROFILE_TR = os.path.join(ROOT, "profiles", "k_parseval_gemm_transformer_512x512_45modes.md")
-TRAFFIC_PROFILE_SW = os.path.join(ROOT, "profiles", "k_parseval_gemm_sweep_256x256_bessel.md")
+TRAFFIC_PROFILE_TR = os.path.join(ROOT, "profiles", "transformer_512x512_45modes.md")
+TRAFFIC_PROFILE_SW = os.path.join(ROOT, "profiles", "transformer_sweep_256x256_bessel.md")
 
 
 def pmc_per_kernel(db_path, counter):
@@ -212,24 +212,33 @@ def transformer_cpu_baseline(coords, sb, pat, kt):
                       f"{n_disk} disk regions per point (FP64 C restatement of the reference transformer)"}
 
 
+MM_CLASSES = ("wgram", "pgemm", "pseries", "inverse", "chain")
 MM_KERNELS = {
-    "pgemm": "k_parseval_gemm (v_mfma_f64_16x16x4f64, weights exp(-i kz dz)/K formed in registers): flat air "
-             "regions, P = sum over k-bin groups of W_g H_g",
-    "wgram": "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)",
+    "pgemm": ("k_parseval_gemm", "k_parseval_gemm (v_mfma_f64_16x16x4f64, weights exp(-i kz dz)/K formed in "
+              "registers or read from the spacing series' table): flat air regions, P = sum over k-bin groups of "
+              "W_g H_g"),
+    "wgram": ("k_wgram", "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)"),
+    "inverse": ("k_batched_inverse_wave", "k_batched_inverse_wave (FP64 LU with pivoting and triangular solves, one "
+                "wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
+    "chain": ("k_chain_lds", "k_chain_lds (bottom block row of the transfer-matrix chain in LDS, 16 n^3 flop per "
+              "region; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
+    "pseries": ("k_series_eval", "k_series_coef / k_series_eval (spacing-series weights and Horner sums)"),
 }
 
 
-def mm_roofline(kt_, traffic_key, profile, alg_flops_per_s):
-    """roofline of the dominant mode-matching kernel: executed flops per launch over
-    its HIP-event time; the survey's algorithmic figure (8 n^2 K per region and
-    point, which the work reductions undercut) is reported beside it, not as `achieved`"""
-    name = max(("pgemm", "wgram"), key=lambda k: kt_[k][0])
+def mm_roofline(kt_, profile, alg_flops_per_s):
+    """roofline of the dominant mode-matching kernel class (by HIP-event time):
+    executed flops per launch over its mean launch time, against the FP64 peak;
+    the survey's algorithmic figure (8 n^2 K per region and point, which the work
+    reductions undercut) is reported beside it, not as `achieved`"""
+    name = max(MM_CLASSES, key=lambda k: kt_[k][0])
     ms, launches, flops = kt_[name]
     achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
-    t = traffic_from_profile(traffic_key, 1, profile) if name == "pgemm" else None
+    key, desc = MM_KERNELS[name]
+    t = traffic_from_profile(key, 1, profile)
     return {"bound": "mfma", "achieved": round(achieved, 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4), "traffic": None if t is None else int(t),
-            "kernel": MM_KERNELS[name], "avg_launch_ms": round(ms / max(launches, 1), 3),
+            "kernel": desc, "kernel_class": name, "avg_launch_ms": round(ms / max(launches, 1), 3),
             "executed_flops_per_launch": flops / max(launches, 1),
             "survey_algorithmic_TFs": round(alg_flops_per_s / 1e12, 1)}
 
@@ -246,6 +255,8 @@ def transformer_main(args):
     ctx = B.context()
     print(f"[bench] transformer: {NDISK}-disk rough booster, {N3}x{N3}, {n} LG modes, {NF3} freqs",
           file=sys.stderr, flush=True)
+    if not args.no_autotune:
+        ctx.set_option("autotune", 1)  # the product's column blocks are timed in the (untimed) warmup
     # inputs resident: the plan uploads setup and modes once; each step recomputes the sweep
     plan = B.TransformerPlan(sb0, coords, modes, freqs, diskR=DISKR)
     for _ in range(args.warmup):
@@ -259,7 +270,7 @@ def transformer_main(args):
     ctx.timing(True)  # one more, instrumented step for the kernel means
     plan.run()
     ctx.synchronize()
-    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
+    kt_ = {k: ctx.kernel_time(k) for k in MM_CLASSES}
     ctx.timing(False)
     boost = plan.fetch()
     plan.close()
@@ -270,7 +281,7 @@ def transformer_main(args):
     host_dt = time.perf_counter() - t1
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)  # also the byte-counter calibration of profiler runs
     alg = (len(sb0.distance) - 1) * 8.0 * n * n * N3 * N3  # survey: sum over regions 2..Nreg of 8 n^2 K
-    roof = mm_roofline(kt_, "k_parseval_gemm", TRAFFIC_PROFILE_TR, alg * NF3 * args.steps / dt)
+    roof = mm_roofline(kt_, TRAFFIC_PROFILE_TR, alg * NF3 * args.steps / dt)
     roof["kernel_ms_from"] = "one instrumented step after the timed ones"
     roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)
     beta2 = np.sum(np.abs(boost) ** 2, axis=-1)
@@ -298,6 +309,7 @@ def transformer_main(args):
         "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
         "work_reductions": "air regions by k-bin groups (8 n^2 G per region, G = distinct (kx^2, ky^2) pairs); "
                            "tilted/rough disks by a frequency series of moment Grams",
+        "tuning": dict(zip(("chosen", "ms"), ctx.tuning_report())),
         "beta2": {"min": float(beta2.min()), "max": float(beta2.max())},
     }
     if not args.no_cpu_baseline:
@@ -401,6 +413,8 @@ def sweep_main(args):
     print(f"[bench] sweep: {nconf} geometries (disks {min(ndisks)}..{max(ndisks)}), {N5}x{N5}, {NF5} freqs, "
           f"{K} steps of ~{nconf // K} geometries", file=sys.stderr, flush=True)
     emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
+    if not args.no_autotune:
+        ctx.set_option("autotune", 1)  # shapes timed during the (untimed) warmup, reported as `tuning`
     plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
     work = [(_eps_copies([geoms[i] for i in sl]), set_of[sl]) for sl in slices]
 
@@ -424,13 +438,13 @@ def sweep_main(args):
     ctx.timing(True)  # one more, instrumented step (slice 0) for the kernel means
     step(0)
     ctx.synchronize()
-    kt_ = {k: ctx.kernel_time(k) for k in ("wgram", "pgemm", "inverse", "chain")}
+    kt_ = {k: ctx.kernel_time(k) for k in MM_CLASSES}
     ctx.timing(False)
     b0 = plan.fetch()
     plan.close()
     nmodes = [sets[s_].n for s_ in set_of]
     alg = sum((len(g.distance) - 1) * 8.0 * m * m * N5 * N5 for g, m in zip(geoms, nmodes)) * NF5
-    roof = mm_roofline(kt_, "k_parseval_gemm", TRAFFIC_PROFILE_SW, alg / dt)
+    roof = mm_roofline(kt_, TRAFFIC_PROFILE_SW, alg / dt)
     roof["kernel_ms_from"] = "one instrumented step (slice 0) after the timed ones"
     bw_ms = ctx.stream_copy_probe(1 << 30, reps=3)
     roof["measured_stream_copy_GBs"] = round(2 * (1 << 30) / (bw_ms * 1e-3) / 1e9, 1)
@@ -452,15 +466,20 @@ def sweep_main(args):
                    "frequencies": NF5, "points": nconf * NF5,
                    "steps_cover": f"{K} disjoint slices of the {nconf} geometries, one per timed step"},
         "fp64": {"value": round(pts, 2), "unit": "points/s", "mixed_over_fp64": 1.0,
-                 "note": "the sweep has no reduced-precision path: the Parseval products run on FP64 MFMA (a "
-                         "complex64 product measured beta^2 errors up to 7.9e-2 against FP64, outside the 1e-4 "
-                         "bar), so the batched path timed here is the FP64 one and the ratio is 1.0"},
+                 "note": "the sweep has no reduced-precision path: every product, inverse and chain is FP64 (a "
+                         "complex64 Parseval product measured beta^2 errors up to 7.9e-2 against FP64, outside the "
+                         "1e-4 bar), so the batched path timed here is the FP64 one and the ratio is 1.0"},
         "roofline": roof,
         "kernel_ms": {k: [round(v[0], 2), v[1]] for k, v in kt_.items()},
         # useful flops (8 n^2 per complex multiply-add of the product) over each class's HIP-event time
         "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
         "beta2_slice0": {"min": float(b2.min()), "max": float(b2.max())},
     }
+    tuned, tune_ms = ctx.tuning_report()
+    out["tuning"] = {"enabled": not args.no_autotune, "ms": round(tune_ms, 1), "chosen": tuned,
+                     "note": "Parseval product column-block width (2: 128, 4: 256 matrix entries per block) per "
+                             "matrix size, timed on the first flush of at least 256 blocks in the warmup; not in "
+                             "the timed steps"}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = sweep_cpu_baseline(coords, geoms, set_of, sets, freqs)
@@ -575,6 +594,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true", help="use the launch-shape heuristics only")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="dancer plans launch every bounce instead of replaying HIP graphs (profiler runs)")
+    ap.add_argument("--fp32-steps", type=int, default=0,
+                    help="dancer workloads: also time the FP32-arithmetic path (compared with FP64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--freq-batch", type=int, default=0)
     ap.add_argument("--nmax", type=int, default=NMAX,
@@ -631,6 +655,10 @@ def main():
 
     nmax = args.nmax
     print(f"[bench] workload: {NDISK}-disk booster, {N}x{N}, {NFREQ} freqs, nmax={nmax}", file=sys.stderr, flush=True)
+    if not args.no_autotune:
+        ctx.set_option("autotune", 1)  # plans time their launch shapes when created (untimed), see `tuning`
+    if args.no_graph:
+        ctx.set_option("graph", 0)
     plan32 = B.DancerPlan(0, nmax, sb, coords, freqs, diskR=DISKR, precision="mixed", freq_batch=args.freq_batch)
     info32 = plan32.info()
     fb32 = info32["freq_batch"]
@@ -658,6 +686,21 @@ def main():
                 "beta2_rel_err_vs_fp64": {"max": float(np.max(err)), "median": float(np.median(err)),
                                           "p90": float(np.percentile(err, 90))},
                 "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt64.items()}}
+        if args.fp32_steps > 0:  # FP32 storage and FP32 arithmetic, timed and compared with FP64 beside
+            planf = B.DancerPlan(0, nmax, sb, coords, freqs, diskR=DISKR, precision="fp32",
+                                 freq_batch=args.freq_batch)
+            dtf, _ = timed_steps(planf, ctx, args.fp32_steps, 1, timing=False)
+            _, pf = planf.fetch(fields=False)
+            fbf = planf.freq_batch
+            planf.close()
+            print(f"[bench] fp32: {dtf / args.fp32_steps:.3f} s/step", file=sys.stderr, flush=True)
+            errf = np.abs(pf * dx * dx / (math.pi * DISKR ** 2) / b64 - 1.0)
+            ptsf = NFREQ * args.fp32_steps / dtf
+            fp64["fp32_arithmetic"] = {
+                "value": round(ptsf, 4), "unit": "points/s", "freq_batch": fbf, "over_fp64": round(ptsf / pts64, 3),
+                "beta2_rel_err_vs_fp64": {"max": float(np.max(errf)), "median": float(np.median(errf)),
+                                          "p90": float(np.percentile(errf, 90))},
+                "note": "complex64 fields with FP32 FFT, phase and update arithmetic (phase argument in FP64)"}
 
     # roofline of the mixed path's bounce: a frequency batch's bounce is two launches (column pass, fused
     # row pass) that together read and write every field once; the bounce's algorithmic bytes are divided
@@ -713,6 +756,10 @@ def main():
                                     "not folded into roofline.achieved"},
         "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in kt32.items()},
     }
+    tuned, tune_ms = ctx.tuning_report()
+    out["tuning"] = {"enabled": not args.no_autotune, "ms": round(tune_ms, 1), "chosen": tuned,
+                     "note": "dancer plans: frequency batch, column-pass tile, row-pass rows (0: kernel default), "
+                             "timed on short runs of the plan's inputs at plan creation; not in the timed steps"}
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(coords, sb, freqs)

@@ -86,6 +86,8 @@ _SIGS = {
     "bf_ctx_kernel_time": (ctypes.c_int, [_P, ctypes.c_char_p, _P, _P, _P]),
     "bf_ctx_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int64]),
     "bf_ctx_get_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "bf_ctx_set_tuning_file": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "bf_ctx_tuning_report": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
     "bf_dancer_plan_destroy": (ctypes.c_int, [_P]),
     "bf_stream_copy_probe": (ctypes.c_int, [_P, ctypes.c_int64, _I32, ctypes.POINTER(_D)]),
     "bf_seed_modes": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, _I32, _D, _P, _P]),
@@ -214,6 +216,21 @@ class Context:
         v = ctypes.c_int64()
         check(lib().bf_ctx_get_option(self.handle, name.encode(), ctypes.byref(v)), self.handle)
         return v.value
+
+    def set_tuning_file(self, path):
+        """load tuned launch shapes from `path` and append new ones to it"""
+        check(lib().bf_ctx_set_tuning_file(self.handle, None if path is None else os.fsencode(path)), self.handle)
+
+    def tuning_report(self):
+        """({shape key: [values]}, milliseconds spent tuning) of this context"""
+        buf = ctypes.create_string_buffer(1 << 16)
+        ms = ctypes.c_double()
+        check(lib().bf_ctx_tuning_report(self.handle, buf, len(buf), ctypes.byref(ms)), self.handle)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            k, _, v = line.partition("\t")
+            out[k] = [int(x) for x in v.split()]
+        return out, ms.value
 
     @contextlib.contextmanager
     def options(self, **kw):

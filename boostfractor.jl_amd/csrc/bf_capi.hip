@@ -37,6 +37,19 @@ OptionScope::OptionScope(const EngineOptions* o) : prev(t_opt) { t_opt = o; }
 OptionScope::~OptionScope() { t_opt = prev; }
 }  // namespace bf
 
+namespace bf {
+// one line per entry: the key, a tab, the values separated by spaces
+void save_tuned(bf_ctx* ctx, const std::string& key) {
+    if (ctx->tune_file.empty()) return;
+    FILE* f = std::fopen(ctx->tune_file.c_str(), "a");
+    if (!f) return;  // a read-only location only loses the cache
+    std::fprintf(f, "%s\t", key.c_str());
+    for (int64_t v : ctx->tuned[key]) std::fprintf(f, " %lld", (long long)v);
+    std::fprintf(f, "\n");
+    std::fclose(f);
+}
+}  // namespace bf
+
 template <typename F>
 static int guarded(bf_ctx* ctx, F&& body) {
     try {
@@ -192,6 +205,7 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"col_tx", &bf::EngineOptions::col_tx}, {"rows", &bf::EngineOptions::rows},
         {"parseval_groups", &bf::EngineOptions::parseval_groups},
         {"disk_expansion", &bf::EngineOptions::disk_expansion}, {"pg_wn", &bf::EngineOptions::pg_wn},
+        {"autotune", &bf::EngineOptions::autotune}, {"parseval_series", &bf::EngineOptions::parseval_series},
     };
     ip = nullptr;
     for (const Int& e : ints)
@@ -223,6 +237,48 @@ int bf_ctx_get_option(bf_ctx* ctx, const char* name, int64_t* value) {
     int64_t* slot = option_slot(ctx->opt, name, tmp, ip);
     BF_REQUIRE(ctx, slot != nullptr, BF_ERR_ARG, std::string("unknown option: ") + name);
     *value = *slot;
+    return BF_OK;
+}
+
+int bf_ctx_set_tuning_file(bf_ctx* ctx, const char* path) {
+    BF_REQUIRE(ctx, ctx, BF_ERR_ARG, "ctx is NULL");
+    ctx->tune_file = path ? path : "";
+    if (!path) return BF_OK;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return BF_OK;  // a new cache
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        char* tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        std::vector<int64_t> v;
+        char* p = tab + 1;
+        char* end = nullptr;
+        for (long long x = std::strtoll(p, &end, 10); end != p; x = std::strtoll(p, &end, 10)) {
+            v.push_back(x);
+            p = end;
+        }
+        if (!v.empty()) ctx->tuned[line] = v;  // a later line for a key replaces an earlier one
+    }
+    std::fclose(f);
+    return BF_OK;
+}
+
+int bf_ctx_tuning_report(bf_ctx* ctx, char* buf, int64_t len, double* tune_ms) {
+    BF_REQUIRE(ctx, ctx, BF_ERR_ARG, "ctx is NULL");
+    std::string r;
+    for (const auto& e : ctx->tuned) {
+        r += e.first + "\t";
+        for (int64_t v : e.second) r += " " + std::to_string(v);
+        r += "\n";
+    }
+    if (tune_ms) *tune_ms = ctx->tune_ms;
+    if (buf && len > 0) {
+        const size_t n = std::min(r.size(), (size_t)len - 1);
+        std::memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    BF_REQUIRE(ctx, !buf || (int64_t)r.size() < len, BF_ERR_ARG, "report buffer too small");
     return BF_OK;
 }
 

@@ -20,6 +20,7 @@
 #include "bf_fftreg.h"
 
 #include <algorithm>
+#include <chrono>
 #include <array>
 #include <type_traits>
 #include <cmath>
@@ -1905,9 +1906,9 @@ struct DancerPlan : DancerPlanBase {
     }
 };
 
-DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax,
-                                 const double* fr, int nfr, int kind, double diskR, int precision,
-                                 const DancerOptions& o) {
+static DancerPlanBase* make_dancer_plan_fixed(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin,
+                                              int nmax, const double* fr, int nfr, int kind, double diskR,
+                                              int precision, const DancerOptions& o) {
     if (precision == BF_PREC_MIXED) {
         auto* p = new DancerPlan<float>();
         try { p->init(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, o); } catch (...) { delete p; throw; }
@@ -1921,6 +1922,103 @@ DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coord
     auto* p = new DancerPlan<double>();
     try { p->init(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, o); } catch (...) { delete p; throw; }
     return p;
+}
+
+// Autotuner of the dancer's launch shapes (a measured plan, where the reference's
+// FFTW.fft! calls, src/boost_fractor.jl:163-164, plan by estimate): short runs of the plan's own inputs
+// (nmax cut to TUNE_NMAX, the frequencies cut to a few batches) time candidate
+// frequency batches around the cache-size heuristic, then column-pass tiles,
+// then row-pass rows per block (coordinate descent), per shape key.
+static std::string dancer_tune_key(const bf_coords* co, const bf_setup* st, int precision, int kind, double amin,
+                                   int nfr, const DancerOptions& o) {
+    return "dancer p" + std::to_string(precision) + " " + std::to_string(co->nx) + "x" + std::to_string(co->ny) +
+           " reg" + std::to_string(st->nreg) + " k" + std::to_string(kind) + " a" + std::to_string(amin != 0.0) +
+           " f" + std::to_string(std::min(nfr, 64)) + " s" + std::to_string(o.returnsum) +
+           std::to_string(o.immediatesum) + " e" + std::to_string(o.emit != nullptr) +
+           std::to_string(o.reflect != nullptr);
+}
+
+static void tune_dancer(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax,
+                        const double* fr, int nfr, int kind, double diskR, int precision, const DancerOptions& o,
+                        const std::string& key) {
+    constexpr int TUNE_NMAX = 20;
+    const auto t_start = std::chrono::steady_clock::now();
+    hipStream_t s = ctx->stream;
+    DancerOptions base = o;
+    base.graph = 0;
+    base.returnsum = 1;
+    base.immediatesum = 1;
+    const int nm = std::min(nmax, TUNE_NMAX);
+    // the heuristic batch, from a plan that picks it itself
+    int fb0;
+    {
+        std::unique_ptr<DancerPlanBase> p(make_dancer_plan_fixed(ctx, st, co, amin, nm, fr, nfr, kind, diskR,
+                                                                 precision, base));
+        fb0 = p->freq_batch;
+    }
+    const int fbmax = std::min(nfr, 64);
+    const int nprobe = std::min(nfr, 2 * std::min(fbmax, 2 * fb0));  // every candidate runs the same points
+    auto time_of = [&](const DancerOptions& t) {
+        std::unique_ptr<DancerPlanBase> p(make_dancer_plan_fixed(ctx, st, co, amin, nm, fr, nprobe, kind, diskR,
+                                                                 precision, t));
+        p->run();  // first run: lazy tables
+        BF_HIP(hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
+        p->run();
+        BF_HIP(hipStreamSynchronize(s));
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    DancerOptions best = base;
+    best.freq_batch = fb0;
+    double tbest = time_of(best);
+    for (int fb : {std::max(1, fb0 * 2 / 3), fb0 * 3 / 2, fb0 * 2}) {
+        if (fb < 1 || fb > fbmax || fb == best.freq_batch) continue;
+        DancerOptions t = best;
+        t.freq_batch = fb;
+        const double tt = time_of(t);
+        if (tt < tbest) { tbest = tt; best = t; }
+    }
+    // column tile (columns per column-pass block; the kernel's default is 128 bytes of a line) and rows
+    // per row-pass block (default 128 threads): halved and doubled, in the kernels' valid ranges
+    const int esize = precision == BF_PREC_FP64 ? 16 : 8;
+    const int tx0 = 128 / esize, TT = co->nx >= 16 ? co->nx / 16 : 1;
+    for (int tx : {tx0 / 2, tx0 * 2}) {
+        if (tx < 1 || tx > 512 / std::max(1, co->ny / 16)) continue;
+        DancerOptions t = best;
+        t.col_tx = tx;
+        const double tt = time_of(t);
+        if (tt < tbest) { tbest = tt; best = t; }
+    }
+    const int rows0 = std::max(1, 128 / TT);
+    for (int r : {rows0 / 2, rows0 * 2}) {
+        if (r < 1 || r > 256 / TT) continue;
+        DancerOptions t = best;
+        t.rows = r;
+        const double tt = time_of(t);
+        if (tt < tbest) { tbest = tt; best = t; }
+    }
+    ctx->tuned[key] = {best.freq_batch, best.col_tx, best.rows};
+    ctx->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    save_tuned(ctx, key);
+}
+
+DancerPlanBase* make_dancer_plan(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, double amin, int nmax,
+                                 const double* fr, int nfr, int kind, double diskR, int precision,
+                                 const DancerOptions& o) {
+    if (opts().autotune && o.freq_batch == 0 && o.col_tx == 0 && o.rows == 0 && nmax >= 1) {
+        const std::string key = dancer_tune_key(co, st, precision, kind, amin, nfr, o);
+        if (!ctx->tuned.count(key)) tune_dancer(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, precision, o, key);
+        const std::vector<int64_t>& v = ctx->tuned[key];
+        if (v.size() == 3) {
+            DancerOptions t = o;
+            const int nb = (int)((nfr + v[0] - 1) / v[0]);  // the tuned batch, spread evenly over the batches
+            t.freq_batch = (nfr + nb - 1) / nb;
+            t.col_tx = (int)v[1];
+            t.rows = (int)v[2];
+            return make_dancer_plan_fixed(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, precision, t);
+        }
+    }
+    return make_dancer_plan_fixed(ctx, st, co, amin, nmax, fr, nfr, kind, diskR, precision, o);
 }
 
 // ---------------------------------------------------------------------------

@@ -95,6 +95,8 @@ struct EngineOptions {
     int parseval_groups = 1;  // flat air regions by k-bin groups
     int disk_expansion = 1;   // tilted/rough disks by a frequency series of moment Grams
     int pg_wn = 0;            // Parseval product: waves along the matrix entries, 2 or 4 (0: by n)
+    int parseval_series = 1;  // air gaps of one k0 with nearby spacings by a Taylor series in the spacing
+    int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: default 2 GiB)
 };
 // Options of the context whose call runs on this thread (defaults outside a call).
@@ -122,9 +124,17 @@ struct bf_ctx {
     // grow-only device buffers reused by consecutive calls on this context
     // (work arrays of the mode-matching batches), released with the context
     std::map<int, bf::DevBuf> scratch;
+    // launch shapes picked by the autotuner, by shape key (also written to
+    // tune_file when the caller names one), and the time spent tuning
+    std::map<std::string, std::vector<int64_t>> tuned;
+    std::string tune_file;
+    double tune_ms = 0.0;
 };
 
 namespace bf {
+// a tuned entry was found: append it to the context's tuning file, if any
+void save_tuned(bf_ctx* ctx, const std::string& key);
+
 // scratch slot `slot` of the context with at least `bytes` bytes (contents undefined); a buffer that
 // grows is freed only after the work queued on the context's stream (which may still use it) is done
 inline DevBuf& scratch_buf(bf_ctx* ctx, int slot, size_t bytes) {
@@ -213,7 +223,8 @@ inline void stage_upload(Stager* st, void* dst, const void* src, size_t bytes, h
 }
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
-    SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0
+    SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0,
+    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART
 };
 }  // namespace bf
 

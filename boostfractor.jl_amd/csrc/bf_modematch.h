@@ -4,6 +4,8 @@
 
 #include "bf_internal.h"
 
+#include <array>
+#include <map>
 #include <vector>
 
 namespace bf {
@@ -31,6 +33,8 @@ struct ModesDev {
 struct ModeCache {
     DevBuf G, Fhat, FhatM;
     bool haveG = false, haveF = false;
+    DevBuf Ginv;  // inv(G), formed by the first flat-disk flush that needs it
+    bool haveGinv = false, ginv_ok = false;
     // Parseval regions by k-bin group: bins with equal (kx^2, ky^2) (and, on a
     // square grid with kX == kY, the swapped pair) share one weight, so
     // P = sum_g w_g H_g with H_g = sum_{k in g} conj(Fhat_i(k)) FhatM_j(k).
@@ -40,9 +44,28 @@ struct ModeCache {
     bool haveIdx = false;
     int ngroups = 0;
     DevBuf gkx, gky;    // representative kX, kY of each group (double)
+    double gk2max = 0.0;  // max over groups of kX^2 + kY^2
     DevBuf goff, gbin;  // group g holds bins gbin[goff[g] .. goff[g+1])
     // forget everything computed from the patterns (keeps the grouping)
-    void reset_data() { haveG = haveF = haveH = false; }
+    void reset_data() {
+        haveG = haveF = haveH = haveGinv = false;
+        reset_series();
+    }
+    // coefficient matrices of the spacing series (parseval_flush) formed so far in this run, by
+    // (k0^2 bits, spacing bin); bin -1 holds dz = 0 for every k0
+    struct SeriesEntry {
+        const zd* d;  // q coefficient matrices, nn apart
+        int q;
+        double c, h;
+    };
+    std::map<std::array<uint64_t, 3>, SeriesEntry> series;
+    std::vector<DevBuf> series_blocks;
+    double series_w = 0.0;  // spacing bin width, fixed by the first flush of the run
+    void reset_series() {
+        series.clear();
+        series_blocks.clear();
+        series_w = 0.0;
+    }
 };
 
 // Parseval weight of one (region, frequency): exp(-1im*kz*dz), kz from k0^2
@@ -67,9 +90,13 @@ struct ParsevalQueue {
 
 // P = sum_g W_g H_g for every queued problem (GEMMs in chunks bounded by memory); empties the queue
 // (with a stager: uploads staged, no wait at the end)
-void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st = nullptr);
+void parseval_flush(bf_ctx* ctx, ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st = nullptr);
 // the queued disk regions of `q` (diag(exp(-i kz dz)) from the left) in one launch
-void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st = nullptr);
+// inv(G) formed if needed; true if it may stand in for the LU of the flat disks' matrices
+bool ginv_usable(bf_ctx* ctx, ModeCache& mc, const ModesDev& md);
+// with pinv_base, disks taken from the Gram matrix also get their inverses (from inv(G)) at the same slots
+void disk_diag_flush(bf_ctx* ctx, ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st = nullptr,
+                     zd* pinv_base = nullptr);
 
 // one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
 struct ChainStep {

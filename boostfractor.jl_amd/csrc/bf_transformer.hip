@@ -22,6 +22,8 @@
 // [a P, b inv(P); b P, a inv(P)].
 #include "bf_modematch.h"
 
+#include <chrono>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -547,7 +549,7 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
     extern __shared__ unsigned char smem_raw[];
     zd* M = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
     zd* X = M + n * n;                        // X[i * n + j]: row i of solution column j
-    __shared__ int piv[INV_WAVE_MAX];
+    __shared__ int piv[INV_WAVE_MAX], sig[INV_WAVE_MAX];
     const int lane = threadIdx.x;
     const int b = sel ? sel[blockIdx.x] : blockIdx.x;
     const int nn = n * n;
@@ -591,25 +593,42 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
         }
         __syncthreads();
     }
-    for (int j = lane; j < n; j += 64) {
-        for (int i = 0; i < n; i++) X[i * n + j] = cmk<double>(i == j ? 1.0 : 0.0, 0.0);
-        for (int k = 0; k < n; k++)
-            if (piv[k] != k) {
-                const zd t = X[k * n + j];
-                X[k * n + j] = X[piv[k] * n + j];
-                X[piv[k] * n + j] = t;
-            }
-        for (int i = 0; i < n; i++)
-            for (int k = 0; k < i; k++) X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
-        for (int i = n - 1; i >= 0; i--) {
-            zd x = X[i * n + j];
-            for (int k = i + 1; k < n; k++) x = csub(x, cmul(M[i + n * k], X[k * n + j]));
-            const zd d = M[i + n * i];
-            const double den = d.re * d.re + d.im * d.im;
-            X[i * n + j] = cmk<double>((x.re * d.re + x.im * d.im) / den, (x.im * d.re - x.re * d.im) / den);
+    // X = inv(P): the identity with the pivots' row swaps (the permutation sigma), then the unit-lower
+    // and the upper triangular solves, each as rank-1 updates of the rows still to solve, spread over
+    // all lanes (X[i][j] takes its forward terms in increasing k, as a row-by-row solve would)
+    if (lane == 0) {
+        for (int i = 0; i < n; i++) sig[i] = i;
+        for (int k = 0; k < n; k++) {
+            const int t = sig[k];
+            sig[k] = sig[piv[k]];
+            sig[piv[k]] = t;
         }
     }
     __syncthreads();
+    for (int e = lane; e < nn; e += 64) X[e] = cmk<double>(sig[e / n] == e % n ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    for (int k = 0; k + 1 < n; k++) {
+        const int rows = n - k - 1;
+        for (int e = lane; e < rows * n; e += 64) {
+            const int i = k + 1 + e / n, j = e % n;
+            X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
+        }
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        const zd d = M[k + n * k];
+        const double den = d.re * d.re + d.im * d.im;
+        for (int j = lane; j < n; j += 64) {
+            const zd x = X[k * n + j];
+            X[k * n + j] = cmk<double>((x.re * d.re + x.im * d.im) / den, (x.im * d.re - x.re * d.im) / den);
+        }
+        __syncthreads();
+        for (int e = lane; e < k * n; e += 64) {
+            const int i = e / n, j = e % n;
+            X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
+        }
+        __syncthreads();
+    }
     zd* dst = Pinv + (size_t)b * nn;
     for (int e = lane; e < nn; e += 64) dst[e] = X[(e % n) * n + e / n];
 }
@@ -695,94 +714,117 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
 // of T feeds the axion sum and the solves, and it updates by itself:
 //     T21' = (a T21 + b T22) P,    T22' = (b T21 + a T22) inv(P)
 // (16 n^3 flop per region instead of 64 n^3 for the whole 2n x 2n product).
-// Every value is formed with the same operations in the same order as in
-// k_chain.  LDS: T21, T22, P, inv(P) (4 n^2 complex).
+// Every output sums over k in the same order as k_chain.  Thread t owns row
+// t % n and the columns t / n + c floor(256 / n) of both products (all 256
+// threads busy for n = 16, 25, 32, ...), forms U and V for its row on the fly,
+// and loads the next region's P and inv(P) into registers while it computes,
+// so each region costs two barriers and no exposed HBM latency.  LDS: T21,
+// T22, P, inv(P) (4 n^2 complex).  NM: the largest n of the instantiation
+// (its register arrays).
 constexpr int CHAIN_LDS_MAX = 48;
-constexpr int CHAIN_NJ = CHAIN_LDS_MAX / 4;  // output columns per wave (4 waves)
 
+template <int NM>
+struct ChainShape {
+    static constexpr int COLS = (NM + 256 / NM - 1) / (256 / NM);  // columns per thread for n <= NM
+    static constexpr bool PF = NM <= 32;  // next region's matrices through registers (else loaded after the barrier)
+    static constexpr int PPT = PF ? (2 * NM * NM + 255) / 256 : 1;  // P and inv(P) entries per thread
+};
+
+template <int NM>
 __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ jobs,
                                                  const ChainStep* __restrict__ steps, const zd* __restrict__ P,
                                                  const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
                                                  const zd* __restrict__ reflect, zd* boost, zd* refl) {
+    using CS = ChainShape<NM>;
     extern __shared__ unsigned char smem_raw[];
     const int nn = n * n;
     zd* T21 = reinterpret_cast<zd*>(smem_raw);  // column-major n x n: [i + n k]
     zd* T22 = T21 + nn;
-    zd* Pl = T22 + nn;
+    zd* Pl = T22 + nn;  // P and inv(P) of the current region, contiguous
     zd* Pil = Pl + nn;
     __shared__ zd ab[CHAIN_LDS_MAX], e0[CHAIN_LDS_MAX], tmp[2 * CHAIN_LDS_MAX];
     __shared__ double sval[256];
     __shared__ int sidx[256];
     __shared__ int piv[CHAIN_LDS_MAX];
     const ChainJob job = jobs[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
+    const int ng = 256 / n, row = tid % n, cg = tid / n;
+    const bool act = cg < ng;
     for (int e = tid; e < nn; e += 256) {
         T21[e] = cmk<double>(0.0, 0.0);
         T22[e] = cmk<double>((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
     }
-    for (int i = tid; i < n; i += 256) {
-        ab[i] = cmk<double>(0.0, 0.0);
-        e0[i] = e0_all[job.e0off + i];
-    }
+    for (int i = tid; i < n; i += 256) e0[i] = e0_all[job.e0off + i];
+    auto region = [&](int q, int r) -> const zd* {  // entry r of [P; inv(P)] of step q (r < 2 nn)
+        const int reg = steps[job.step0 + q].reg;
+        return r < nn ? P + (size_t)(job.pbase + reg) * nn + r : Pinv + (size_t)(job.pbase + reg) * nn + (r - nn);
+    };
+    if (job.nsteps > 0)
+        for (int r = tid; r < 2 * nn; r += 256) Pl[r] = *region(0, r);
+    zd abr = cmk<double>(0.0, 0.0);  // axion_beam[tid] for tid < n
     __syncthreads();
     for (int q = 0; q < job.nsteps; q++) {
         const ChainStep st = steps[job.step0 + q];
+        const bool more = q + 1 < job.nsteps;
+        zd pn[CS::PPT];
+        if constexpr (CS::PF) {
+#pragma unroll
+            for (int i = 0; i < CS::PPT; i++) {
+                const int r = tid + 256 * i;
+                if (more && r < 2 * nn) pn[i] = *region(q + 1, r);
+            }
+        }
         // axion_beam += S * (T21*e0 + T22*e0)
-        for (int i = tid; i < n; i += 256) {
+        if (tid < n) {
             zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
             for (int k = 0; k < n; k++) {
-                s21 = cadd(s21, cmul(T21[i + n * k], e0[k]));
-                s22 = cadd(s22, cmul(T22[i + n * k], e0[k]));
+                s21 = cadd(s21, cmul(T21[tid + n * k], e0[k]));
+                s22 = cadd(s22, cmul(T22[tid + n * k], e0[k]));
             }
-            ab[i] = cadd(ab[i], cmul(st.S, cadd(s21, s22)));
+            abr = cadd(abr, cmul(st.S, cadd(s21, s22)));
         }
-        const zd* Pm = P + (size_t)(job.pbase + st.reg) * nn;
-        const zd* Pi = Pinv + (size_t)(job.pbase + st.reg) * nn;
-        for (int e = tid; e < nn; e += 256) {
-            Pl[e] = Pm[e];
-            Pil[e] = Pi[e];
-        }
-        __syncthreads();
-        // U = a T21 + b T22 (into T21), V = b T21 + a T22 (into T22)
-        for (int e = tid; e < nn; e += 256) {
-            const zd t21 = T21[e], t22 = T22[e];
-            T21[e] = cadd(cmul(st.a, t21), cmul(st.b, t22));
-            T22[e] = cadd(cmul(st.b, t21), cmul(st.a, t22));
-        }
-        __syncthreads();
-        // T21' = U P, T22' = V inv(P): lane = row, wave w takes columns w, w + 4, ...
-        zd o21[CHAIN_NJ], o22[CHAIN_NJ];
+        // T21' = U P, T22' = V inv(P), U = a T21 + b T22, V = b T21 + a T22
+        zd o21[CS::COLS], o22[CS::COLS];
 #pragma unroll
-        for (int t = 0; t < CHAIN_NJ; t++) {
-            o21[t] = cmk<double>(0.0, 0.0);
-            o22[t] = cmk<double>(0.0, 0.0);
-        }
-        if (lane < n) {
+        for (int c = 0; c < CS::COLS; c++) o21[c] = o22[c] = cmk<double>(0.0, 0.0);
+        if (act) {
             for (int k = 0; k < n; k++) {
-                const zd u = T21[lane + n * k], v = T22[lane + n * k];
+                const zd t21 = T21[row + n * k], t22 = T22[row + n * k];
+                const zd u = cadd(cmul(st.a, t21), cmul(st.b, t22));
+                const zd v = cadd(cmul(st.b, t21), cmul(st.a, t22));
 #pragma unroll
-                for (int t = 0; t < CHAIN_NJ; t++) {
-                    const int j = w + 4 * t;
+                for (int c = 0; c < CS::COLS; c++) {
+                    const int j = cg + ng * c;
                     if (j < n) {
-                        o21[t] = cadd(o21[t], cmul(u, Pl[k + n * j]));
-                        o22[t] = cadd(o22[t], cmul(v, Pil[k + n * j]));
+                        o21[c] = cadd(o21[c], cmul(u, Pl[k + n * j]));
+                        o22[c] = cadd(o22[c], cmul(v, Pil[k + n * j]));
                     }
                 }
             }
         }
         __syncthreads();
-        if (lane < n) {
+        if (act) {
 #pragma unroll
-            for (int t = 0; t < CHAIN_NJ; t++) {
-                const int j = w + 4 * t;
+            for (int c = 0; c < CS::COLS; c++) {
+                const int j = cg + ng * c;
                 if (j < n) {
-                    T21[lane + n * j] = o21[t];
-                    T22[lane + n * j] = o22[t];
+                    T21[row + n * j] = o21[c];
+                    T22[row + n * j] = o22[c];
                 }
             }
         }
+        if constexpr (CS::PF) {
+#pragma unroll
+            for (int i = 0; i < CS::PPT; i++) {
+                const int r = tid + 256 * i;
+                if (more && r < 2 * nn) Pl[r] = pn[i];
+            }
+        } else if (more) {
+            for (int r = tid; r < 2 * nn; r += 256) Pl[r] = *region(q + 1, r);
+        }
         __syncthreads();
     }
+    if (tid < n) ab[tid] = abr;
     // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
     if (reflect)
         for (int i = tid; i < n; i += 256) {
@@ -803,21 +845,50 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
     }
 }
 
+// the LDS chain for n <= CHAIN_LDS_MAX: one block per job
+static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
+                             const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    const size_t smem = 4 * sizeof(zd) * (size_t)n * n;
+    auto go = [&](const void* fn, auto kern) {
+        allow_lds(fn, smem);
+        hipLaunchKernelGGL(kern, dim3((unsigned)nj), dim3(256), smem, s, jobs, steps, P, Pinv, n, e0, reflect, boost,
+                           refl);
+    };
+    if (n <= 16)
+        go((const void*)k_chain_lds<16>, k_chain_lds<16>);
+    else if (n <= 32)
+        go((const void*)k_chain_lds<32>, k_chain_lds<32>);
+    else
+        go((const void*)k_chain_lds<CHAIN_LDS_MAX>, k_chain_lds<CHAIN_LDS_MAX>);
+    BF_CHECK_LAUNCH();
+}
+
 // disk-region diagonal propagation: P[f][reg] = diag(exp(-1im*kz_m*dz)) * X with
 // X = the shared Gram matrix G (flat untilted disk, from_gram[q] != 0) or P[f][reg] itself
+// With Pinv and Ginv = inv(G) given, a disk taken from G also gets its inverse,
+// inv(diag(ph) G) = inv(G) diag(1 / ph): Pinv[slot] = Ginv with column j divided by ph_j.
 __global__ void k_disk_diag(zd* P, const int* __restrict__ slots, const int* __restrict__ from_gram,
                             const zd* __restrict__ G, const zd* __restrict__ k0sq, const double* dz,
-                            const zd* __restrict__ kt, int n) {
+                            const zd* __restrict__ kt, int n, zd* Pinv, const zd* __restrict__ Ginv) {
+    __shared__ zd sph[BF_MAX_MODES];
     const int q = blockIdx.x;
     const zd k2 = k0sq[q];
-    zd* Pm = P + (size_t)slots[q] * n * n;
-    const zd* X = from_gram[q] ? G : Pm;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e % n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
-        const zd ph = cexp_j(rmul(dz[q], neg_i(kz)));
-        Pm[e] = cmul(ph, X[e]);
+        sph[i] = cexp_j(rmul(dz[q], neg_i(kz)));
     }
+    __syncthreads();
+    const size_t off = (size_t)slots[q] * n * n;
+    zd* Pm = P + off;
+    const bool fg = from_gram[q] != 0;
+    const zd* X = fg ? G : Pm;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Pm[e] = cmul(sph[e % n], X[e]);
+    if (fg && Pinv)
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const zd a = Ginv[e], d = sph[e / n];
+            const double den = d.re * d.re + d.im * d.im;
+            Pinv[off + e] = cmk<double>((a.re * d.re + a.im * d.im) / den, (a.im * d.re - a.re * d.im) / den);
+        }
 }
 
 // P[slot] = s * G  (1D propagator in an air region: prop multiplies by a scalar phase)
@@ -870,6 +941,9 @@ __global__ void k_group_gram(const zd* __restrict__ A, const zd* __restrict__ B,
 // waits for its fullest wave anyway).  Blocks of one column slab are
 // consecutive in the grid, so the blocks in flight read the same H columns.
 // Results go straight to each problem's matrix (column-major n x n).
+// TAB = true: the weights are read from a table wtab[g][m] (M rows) instead
+// (the coefficient rows of the spacing series below); kz and the per-block
+// k0^2 lists are then unused.
 template <int WN>
 struct PgShape {
     static constexpr int WM = 2;                     // waves along problems
@@ -884,28 +958,31 @@ struct PgShape {
 };
 constexpr int PG_BM = 64;  // rows per block (all shapes)
 
-template <int WN>
+template <int WN, bool TAB>
 __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
     const zd* __restrict__ H, int nn, int NG, const double* __restrict__ gkx, const double* __restrict__ gky,
     const PwDesc* __restrict__ pd, const int* __restrict__ row_slot, const int* __restrict__ blk_nk,
-    const zd* __restrict__ blk_k0sq, int M, double scale, zd* const* __restrict__ dst, int nmb) {
+    const zd* __restrict__ blk_k0sq, int M, double scale, zd* const* __restrict__ dst, int nmb,
+    const zd* __restrict__ wtab, int ncb, int cps, zd* __restrict__ part) {
     using S = PgShape<WN>;
     constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NT = S::NT;
     __shared__ double swr[2][KC][S::LDA], swi[2][KC][S::LDA];
     __shared__ double shr[2][KC][S::LDB], shi[2][KC][S::LDB];
-    __shared__ zd skz[3][BM][KC];
-    const int mb = blockIdx.x % nmb, cb = blockIdx.x / nmb;
+    __shared__ zd skz[TAB ? 1 : 3][BM][KC];
+    // block -> (row block, column block, group split); split ks sums the chunks [ks cps, (ks + 1) cps)
+    const int ks_ = blockIdx.x / (nmb * ncb), rem = blockIdx.x % (nmb * ncb);
+    const int mb = rem % nmb, cb = rem / nmb;
     const int m0 = mb * BM, c0 = cb * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w % S::WM, wn = w / S::WM;
-    const int nk = blk_nk[mb];
-    const zd* k0s = blk_k0sq + (size_t)mb * BM;
+    const int nk = TAB ? 0 : blk_nk[mb];
+    const zd* k0s = TAB ? nullptr : blk_k0sq + (size_t)mb * BM;
     // weights: thread t forms row m0 + t % BM, chunk columns t / BM + (NT / BM) j
     const int wrow = t % BM, wk0 = t / BM;
     const bool wvalid = m0 + wrow < M;
     double dz = 0.0;
     int slot = 0;
-    if (wvalid) {
+    if (!TAB && wvalid) {
         dz = pd[m0 + wrow].dz;
         slot = row_slot[m0 + wrow];
     }
@@ -914,10 +991,12 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
     const bool hvalid = c0 + he < nn;
     zd wreg[S::WPT], hreg[S::HPT];
     auto form_kz = [&](int c) {  // kz of chunk c into skz[c % 3]
+        if constexpr (!TAB) {
 #pragma unroll
-        for (int i = 0; i < S::KZPT; i++) {
-            const int idx = t + NT * i, s = idx / KC, k = idx % KC, g = c * KC + k;
-            if (s < nk && g < NG) skz[c % 3][s][k] = kz_of(k0s[s], gkx[g], gky[g]);
+            for (int i = 0; i < S::KZPT; i++) {
+                const int idx = t + NT * i, s = idx / KC, k = idx % KC, g = c * KC + k;
+                if (s < nk && g < NG) skz[c % 3][s][k] = kz_of(k0s[s], gkx[g], gky[g]);
+            }
         }
     };
     auto load_h = [&](int c) {
@@ -932,7 +1011,9 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
         for (int j = 0; j < S::WPT; j++) {
             const int k = wk0 + (NT / BM) * j, g = c * KC + k;
             zd v = cmk<double>(0.0, 0.0);
-            if (wvalid && g < NG) {
+            if constexpr (TAB) {
+                if (wvalid && g < NG) v = wtab[(size_t)g * M + m0 + wrow];
+            } else if (wvalid && g < NG) {
                 const zd ph = kz_phase_of(skz[c % 3][slot][k], dz);
                 v = cmk<double>(ph.re * scale, ph.im * scale);
             }
@@ -959,17 +1040,17 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) are[rt][ct] = aim[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
-    const int nchunks = (NG + KC - 1) / KC;
-    form_kz(0);
-    form_kz(1);
-    load_h(0);
+    const int cbeg = ks_ * cps, cend = min((NG + KC - 1) / KC, cbeg + cps);
+    form_kz(cbeg);
+    form_kz(cbeg + 1);
+    load_h(cbeg);
     __syncthreads();
-    form_w(0);
-    stage(0);
+    form_w(cbeg);
+    stage(cbeg);
     __syncthreads();
-    for (int c = 0; c < nchunks; c++) {
+    for (int c = cbeg; c < cend; c++) {
         const int b = c & 1;
-        const bool more = c + 1 < nchunks;
+        const bool more = c + 1 < cend;
         if (more) load_h(c + 1);
         form_kz(c + 2);  // skz[(c + 2) % 3] was last read in chunk c - 1's weights, before the previous barrier
 #pragma unroll
@@ -1013,7 +1094,7 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
         for (int v = 0; v < 4; v++) {
             const int m = r0 + rt * 16 + 4 * v + (lane >> 4);
             if (m >= M) continue;
-            zd* out = dst[m];
+            zd* out = part ? part + ((size_t)ks_ * M + m) * nn : dst[m];
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
                 const int e = e0 + ct * 16 + (lane & 15);
@@ -1021,6 +1102,16 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
             }
         }
     }
+}
+
+// dst[m][e] = sum over the splits s (in order) of part[s][m][e]
+__global__ void __launch_bounds__(256) k_pgemm_reduce(const zd* __restrict__ part, int nsplit, int M, int nn,
+                                                      zd* const* __restrict__ dst) {
+    const int e = blockIdx.x * 256 + threadIdx.x, m = blockIdx.y;
+    if (e >= nn) return;
+    zd acc = part[(size_t)m * nn + e];
+    for (int s = 1; s < nsplit; s++) acc = cadd(acc, part[((size_t)s * M + m) * nn + e]);
+    dst[m][e] = acc;
 }
 
 // Tilted/rough disk regions over a band of frequencies.  The weight of region r
@@ -1178,12 +1269,14 @@ static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md
         std::sort(keyed.begin(), keyed.end());
         std::vector<int> off, bins(K);
         std::vector<double> gx, gy;
+        double k2max = 0.0;
         for (size_t q = 0; q < K; q++) {
             if (q == 0 || keyed[q].first != keyed[q - 1].first) {
                 off.push_back((int)q);
                 const int bin = keyed[q].second;  // lowest bin index of the group
                 gx.push_back(g.hkX[((bin % nx) + nx / 2) % nx]);
                 gy.push_back(g.hkY[((bin / nx) + ny / 2) % ny]);
+                k2max = std::max(k2max, gx.back() * gx.back() + gy.back() * gy.back());
             }
             bins[q] = keyed[q].second;
         }
@@ -1199,6 +1292,7 @@ static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md
         upload(mc.gky.p, gy.data(), sizeof(double) * G, s);
         BF_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
         mc.ngroups = G;
+        mc.gk2max = k2max;
         mc.haveIdx = true;
     }
     const int G = mc.ngroups;
@@ -1532,7 +1626,8 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             upload(dk0.p, k0sq.data(), sizeof(zd) * slots.size(), s);
             upload(ddz.p, dzs.data(), sizeof(double) * slots.size(), s);
             hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)slots.size()), dim3(256), 0, s, P, dslots.as<int>(),
-                               dflag.as<int>(), G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n);
+                               dflag.as<int>(), G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n,
+                               nullptr, nullptr);
             BF_CHECK_LAUNCH();
             BF_HIP(hipStreamSynchronize(s));
         }
@@ -1579,10 +1674,41 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     if (!queue || Mom.p || ktabs.p || rtabs.p || scratch.p || prop.p || scr2.p) BF_HIP(hipStreamSynchronize(s));
 }
 
-void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st) {
+// inv(G) of the mode set's Gram matrix, formed on first use; usable for the flat disks' inverses when
+// G is well conditioned (1-norm condition number below 100: the products inv(G) diag(1/ph) then carry
+// the LU's accuracy; non-orthogonal user patterns with a worse G keep the LU of every disk matrix)
+bool ginv_usable(bf_ctx* ctx, ModeCache& mc, const ModesDev& md) {
+    if (!mc.haveG || md.n > BF_MAX_MODES) return false;
+    if (!mc.haveGinv) {
+        const int n = md.n;
+        const size_t nn = (size_t)n * n;
+        mc.Ginv.alloc(sizeof(zd) * nn);
+        launch_inverse(ctx, mc.G.as<zd>(), mc.Ginv.as<zd>(), n, nullptr, 1);
+        std::vector<zd> g(nn), gi(nn);
+        download(g.data(), mc.G.p, sizeof(zd) * nn, ctx->stream);
+        download(gi.data(), mc.Ginv.p, sizeof(zd) * nn, ctx->stream);
+        BF_HIP(hipStreamSynchronize(ctx->stream));
+        auto norm1 = [&](const std::vector<zd>& a) {
+            double mx = 0.0;
+            for (int j = 0; j < n; j++) {
+                double c = 0.0;
+                for (int i = 0; i < n; i++) c += std::hypot(a[(size_t)j * n + i].re, a[(size_t)j * n + i].im);
+                mx = std::max(mx, c);
+            }
+            return mx;
+        };
+        const double cond = norm1(g) * norm1(gi);
+        mc.ginv_ok = std::isfinite(cond) && cond < 100.0;
+        mc.haveGinv = true;
+    }
+    return mc.ginv_ok;
+}
+
+void disk_diag_flush(bf_ctx* ctx, ModeCache& mc, const ModesDev& md, ParsevalQueue& q, Stager* st, zd* pinv_base) {
     if (q.dslot.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t m = q.dslot.size();
+    const zd* ginv = pinv_base && ginv_usable(ctx, mc, md) ? mc.Ginv.as<zd>() : nullptr;
     DevBuf& dslots = scratch_buf(ctx, SB_SLOTS, sizeof(int) * m);
     DevBuf& dflag = scratch_buf(ctx, SB_FLAG, sizeof(int) * m);
     DevBuf& dk0 = scratch_buf(ctx, SB_K0, sizeof(zd) * m);
@@ -1592,7 +1718,8 @@ void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, Parse
     stage_upload(st, dk0.p, q.dk0sq.data(), sizeof(zd) * m, s);
     stage_upload(st, ddz.p, q.ddz.data(), sizeof(double) * m, s);
     hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)m), dim3(256), 0, s, q.diag_base, dslots.as<int>(), dflag.as<int>(),
-                       mc.G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), md.n);
+                       mc.G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), md.n, ginv ? pinv_base : nullptr,
+                       ginv);
     BF_CHECK_LAUNCH();
     if (!st) BF_HIP(hipStreamSynchronize(s));  // the queue's host arrays are cleared next
     q.dslot.clear();
@@ -1601,7 +1728,343 @@ void disk_diag_flush(bf_ctx* ctx, const ModeCache& mc, const ModesDev& md, Parse
     q.ddz.clear();
 }
 
-void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st) {
+// Splits of the group sum: blocks (row blocks x column blocks x splits) fill the
+// launch's last round of resident blocks to at least 90% where a few splits can
+// (the machine holds 256 CUs x 2 blocks of the 128-entry shape; the 256-entry
+// shape, half the blocks at one per CU, fills the same rounds); partial sums go
+// to a workspace and k_pgemm_reduce adds them in split order.  The split does
+// not depend on the shape, so both shapes sum every entry in the same order.
+// Returns the split count and the chunks per split.
+static std::pair<int, int> parseval_splits(size_t nmb, size_t nn, size_t NG) {
+    const int nch = (int)((NG + PgShape<2>::KC - 1) / PgShape<2>::KC);
+    const size_t ncb = (nn + PgShape<2>::BN - 1) / PgShape<2>::BN;
+    const double slots = 512.0;
+    int best = 1;
+    double best_eff = 0.0;
+    for (int sp = 1; sp <= 8 && sp <= nch / 16; sp++) {
+        const double rounds = (double)(nmb * ncb * sp) / slots;
+        const double eff = rounds / std::ceil(rounds);
+        if (eff > best_eff + 1e-9) { best = sp; best_eff = eff; }
+        if (eff >= 0.9) break;
+    }
+    const int cps = (nch + best - 1) / best;
+    return {(nch + cps - 1) / cps, cps};
+}
+
+template <bool TAB>
+static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeCache& mc, size_t nn, size_t NG,
+                                 const PwDesc* pd, const int* slot, const int* nk, const zd* k0, size_t M,
+                                 double scale, zd* const* dst, int nmb, const zd* wtab) {
+    const size_t BN = wn == 2 ? PgShape<2>::BN : PgShape<4>::BN;
+    const int ncb = (int)((nn + BN - 1) / BN);
+    const auto sp = parseval_splits((size_t)nmb, nn, NG);
+    zd* part = nullptr;
+    if (sp.first > 1) part = scratch_buf(ctx, SB_PGPART, sizeof(zd) * (size_t)sp.first * M * nn).as<zd>();
+    const dim3 grid((unsigned)((size_t)nmb * ncb * sp.first));
+    if (wn == 2)
+        hipLaunchKernelGGL((k_parseval_gemm<2, TAB>), grid, dim3(PgShape<2>::NT), 0, s, mc.H.as<zd>(), (int)nn,
+                           (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst,
+                           nmb, wtab, ncb, sp.second, part);
+    else
+        hipLaunchKernelGGL((k_parseval_gemm<4, TAB>), grid, dim3(PgShape<4>::NT), 0, s, mc.H.as<zd>(), (int)nn,
+                           (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst,
+                           nmb, wtab, ncb, sp.second, part);
+    BF_CHECK_LAUNCH();
+    if (part) {
+        hipLaunchKernelGGL(k_pgemm_reduce, dim3((unsigned)((nn + 255) / 256), (unsigned)M), dim3(256), 0, s, part,
+                           sp.first, (int)M, (int)nn, dst);
+        BF_CHECK_LAUNCH();
+    }
+}
+
+// Column-block width of the Parseval product for n x n matrices: the tuned one
+// if the autotuner has timed this size, else 2 (128 entries per block) unless
+// 256 fill the matrix better
+static int parseval_wn(bf_ctx* ctx, size_t nn, size_t NG) {
+    int wn = opts().pg_wn;
+    const std::string key = "pgemm nn" + std::to_string(nn) + " G" + std::to_string(NG);
+    if (wn != 2 && wn != 4 && ctx->tuned.count(key)) wn = (int)ctx->tuned[key][0];
+    if (wn != 2 && wn != 4) wn = ((nn + 127) / 128 * 128 <= (nn + 255) / 256 * 256) ? 2 : 4;
+    return wn;
+}
+
+// ---------------------------------------------------------------------------
+// Air gaps by a series in the spacing.  For one k0 the weight of group g,
+// dz -> exp(-1im*kz_g*dz), is an entire function of the spacing.  Around the
+// centre c of a spacing bin [c - h, c + h], with t = (dz - c)/h and
+// beta_g = -1im*kz_g*h,
+//     exp(-1im*kz_g*dz) = exp(-1im*kz_g*c) * sum_p (beta_g t)^p / p!,
+// so P(dz) = sum_p t^p D_p with D_p = sum_g exp(-1im*kz_g*c) beta_g^p / p! * scale * H_g:
+// Q coefficient matrices per (k0^2, bin), formed by the same product on a
+// table of weights, stand in for one product per gap, and each gap's matrix is
+// a Horner sum of Q terms.  The bins have a fixed width per run (2/kmax for up
+// to twice the first flush's k0, so |beta_g| <= 1 for every group,
+// |kz|^2 <= |k0^2| + max(kx^2 + ky^2)), so every flush of a run reuses the
+// coefficients of the bins it shares with earlier flushes; Q is the first
+// count whose remainder bound e^|beta| |beta|^Q / Q! is below 1e-17 (the
+// weights are at most 1 in modulus for k0^2 in the lower half plane and
+// dz >= 0).  dz = 0 is its own bin (one term, the same for every k0: the
+// weight is exactly the scale).  The matrices agree with the direct product to
+// rounding (a few ulp per weight instead of one).
+// ---------------------------------------------------------------------------
+struct SeriesKey {
+    zd k0sq;
+    double c, h;  // centre and half width of the bin
+    int q, row0;  // terms, first coefficient row in the table being formed
+    const zd* d;  // coefficient matrices (q of them, nn apart)
+};
+struct SeriesProb {
+    zd* dst;
+    double t;  // (dz - c)/h
+    int key, pad;
+};
+struct SeriesSlice {
+    int key, first, count, pad;
+};
+constexpr int SERIES_QMAX = 24;
+constexpr int SERIES_SLICE = 64;
+
+// weight table of the coefficient rows, wt[g][row0 + p] = exp(-1im*kz*c) * beta^p / p! * scale
+__global__ void __launch_bounds__(256) k_series_coef(const SeriesKey* __restrict__ keys,
+                                                     const double* __restrict__ gkx, const double* __restrict__ gky,
+                                                     int NG, int nrows, double scale, zd* __restrict__ wt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NG) return;
+    const SeriesKey k = keys[blockIdx.y];
+    const zd kz = kz_of(k.k0sq, gkx[g], gky[g]);
+    const zd e = kz_phase_of(kz, k.c);
+    const zd beta = cmk<double>(kz.im * k.h, -kz.re * k.h);
+    zd b = cmk<double>(e.re * scale, e.im * scale);
+    zd* out = wt + (size_t)g * nrows + k.row0;
+    out[0] = b;
+    for (int p = 1; p < k.q; p++) {
+        b = cmul(b, beta);
+        b = cmk<double>(b.re / p, b.im / p);
+        out[p] = b;
+    }
+}
+
+// each gap's matrix from its bin's coefficient matrices, Horner in t;
+// block: 128 entries of the slice's problems (all of one key)
+__global__ void __launch_bounds__(128) k_series_eval(const SeriesSlice* __restrict__ sl,
+                                                     const SeriesProb* __restrict__ pr,
+                                                     const SeriesKey* __restrict__ keys, int nn) {
+    const int e = blockIdx.x * 128 + threadIdx.x;
+    if (e >= nn) return;
+    const SeriesSlice S = sl[blockIdx.y];
+    const SeriesKey k = keys[S.key];
+    zd d[SERIES_QMAX];
+#pragma unroll
+    for (int p = 0; p < SERIES_QMAX; p++) d[p] = p < k.q ? k.d[(size_t)p * nn + e] : cmk<double>(0.0, 0.0);
+    for (int i = 0; i < S.count; i++) {
+        const SeriesProb P = pr[S.first + i];
+        zd acc = cmk<double>(0.0, 0.0);  // terms beyond q are zero and leave acc zero
+#pragma unroll
+        for (int p = SERIES_QMAX - 1; p >= 0; p--)
+            acc = cmk<double>(d[p].re + P.t * acc.re, d[p].im + P.t * acc.im);
+        P.dst[e] = acc;
+    }
+}
+
+// Serves the problems of q whose (k0^2, spacing bin) has coefficients from an
+// earlier flush of the run, or enough gaps in this flush to pay for them (at
+// least q + 4), and leaves the others in q.
+static void parseval_series(bf_ctx* ctx, ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st) {
+    const size_t M = q.pd.size(), nn = (size_t)n * n, NG = (size_t)mc.ngroups;
+    hipStream_t s = ctx->stream;
+    using Key = std::array<uint64_t, 3>;
+    auto bits = [](double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, 8);
+        return b;
+    };
+    if (mc.series_w <= 0.0) {
+        double k2 = 0.0;
+        for (size_t m = 0; m < M; m++) k2 = std::max(k2, std::hypot(q.pd[m].k0sq.re, q.pd[m].k0sq.im));
+        const double kb = std::sqrt(4.0 * k2 + mc.gk2max);
+        if (!(kb > 0.0) || !std::isfinite(kb)) return;
+        mc.series_w = 2.0 / kb;
+    }
+    const double w = mc.series_w;
+    constexpr uint64_t ZERO_BIN = ~(uint64_t)0;
+    // every problem's key as a dense index: 0 for dz = 0, else 1 + k0 slot * nb + (bin - bmin), where
+    // consecutive problems mostly share their k0 (a geometry's gaps at one frequency); -1: direct
+    std::vector<int> slot(M, -1);
+    std::vector<int64_t> bin(M, 0);
+    std::map<std::pair<uint64_t, uint64_t>, int> k0map;
+    std::vector<zd> k0v;
+    int last = -1;
+    uint64_t lre = 0, lim = 0;
+    int64_t bmin = INT64_MAX, bmax = INT64_MIN;
+    bool any_zero = false;
+    for (size_t m = 0; m < M; m++) {
+        const zd k0 = q.pd[m].k0sq;
+        const double dz = q.pd[m].dz;
+        if (!(k0.im <= 0.0) || !std::isfinite(k0.re)) continue;
+        if (dz == 0.0) {
+            slot[m] = -2;
+            any_zero = true;
+            continue;
+        }
+        if (!(dz > 0.0) || !std::isfinite(dz) || dz / w > 1e15) continue;
+        const uint64_t re = bits(k0.re), im = bits(k0.im);
+        if (last < 0 || re != lre || im != lim) {
+            auto it = k0map.find({re, im});
+            if (it == k0map.end()) {
+                it = k0map.emplace(std::make_pair(re, im), (int)k0v.size()).first;
+                k0v.push_back(k0);
+            }
+            last = it->second;
+            lre = re;
+            lim = im;
+        }
+        slot[m] = last;
+        bin[m] = (int64_t)std::floor(dz / w);
+        bmin = std::min(bmin, bin[m]);
+        bmax = std::max(bmax, bin[m]);
+    }
+    const int64_t nb = bmax >= bmin ? bmax - bmin + 1 : 0;
+    const bool dense_ok = nb > 0 && nb <= 4096 && (int64_t)k0v.size() * nb <= ((int64_t)1 << 22);
+    const size_t nidx = 1 + (dense_ok ? k0v.size() * (size_t)nb : 0);
+    auto idx_of = [&](size_t m) -> int64_t {
+        if (slot[m] == -2) return 0;
+        if (slot[m] < 0 || !dense_ok) return -1;
+        return 1 + (int64_t)slot[m] * nb + (bin[m] - bmin);
+    };
+    std::vector<int> cnt(nidx, 0);
+    for (size_t m = 0; m < M; m++) {
+        const int64_t i = idx_of(m);
+        if (i >= 0) cnt[(size_t)i]++;
+    }
+    auto key_at = [&](size_t i, zd& k0) -> Key {
+        if (i == 0) {
+            k0 = k0v.empty() ? cmk<double>(0.0, -0.0) : k0v[0];
+            for (size_t m = 0; m < M; m++)
+                if (slot[m] == -2) {
+                    k0 = q.pd[m].k0sq;
+                    break;
+                }
+            return {0, 0, ZERO_BIN};
+        }
+        const size_t sl = (i - 1) / (size_t)nb;
+        k0 = k0v[sl];
+        return {bits(k0.re), bits(k0.im), (uint64_t)(bmin + (int64_t)((i - 1) % (size_t)nb))};
+    };
+    (void)any_zero;
+    // new keys worth forming
+    std::vector<SeriesKey> fresh;
+    std::vector<Key> fresh_key;
+    int nrows = 0;
+    for (size_t i = 0; i < nidx; i++) {
+        if (!cnt[i]) continue;
+        zd k0;
+        const Key key = key_at(i, k0);
+        if (mc.series.count(key)) continue;
+        const bool zero = i == 0;
+        const double h = zero ? 0.0 : 0.5 * w, c = zero ? 0.0 : ((double)(int64_t)key[2] + 0.5) * w;
+        const double beta = std::sqrt(std::hypot(k0.re, k0.im) + mc.gk2max) * h;
+        int Q = 1;
+        if (h > 0.0) {
+            double term = beta;  // beta^Q / Q!
+            while (std::exp(beta) * term >= 1e-17 && Q < SERIES_QMAX) {
+                Q++;
+                term *= beta / Q;
+            }
+            if (std::exp(beta) * term >= 1e-17) continue;  // k0 beyond the bins' allowance: direct
+        }
+        if (cnt[i] < Q + 4) continue;  // the direct product is cheaper
+        fresh.push_back(SeriesKey{k0, c, h, Q, nrows, nullptr});
+        fresh_key.push_back(key);
+        nrows += Q;
+    }
+    if (!fresh.empty()) {
+        DevBuf blk(sizeof(zd) * nn * (size_t)nrows);
+        std::vector<zd*> rowdst((size_t)nrows);
+        for (int r = 0; r < nrows; r++) rowdst[(size_t)r] = blk.as<zd>() + (size_t)r * nn;
+        DevBuf& dkey = scratch_buf(ctx, SB_SKEY, sizeof(SeriesKey) * fresh.size());
+        DevBuf& drow = scratch_buf(ctx, SB_SROWDST, sizeof(zd*) * (size_t)nrows);
+        DevBuf& dw = scratch_buf(ctx, SB_SW, sizeof(zd) * NG * (size_t)nrows);
+        stage_upload(st, dkey.p, fresh.data(), sizeof(SeriesKey) * fresh.size(), s);
+        stage_upload(st, drow.p, rowdst.data(), sizeof(zd*) * rowdst.size(), s);
+        {
+            TimedLaunch tl(ctx, "pseries", 0.0);
+            hipLaunchKernelGGL(k_series_coef, dim3((unsigned)((NG + 255) / 256), (unsigned)fresh.size()), dim3(256),
+                               0, s, dkey.as<SeriesKey>(), mc.gkx.as<double>(), mc.gky.as<double>(), (int)NG, nrows,
+                               scale, dw.as<zd>());
+            BF_CHECK_LAUNCH();
+        }
+        {
+            TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)nrows);
+            const int nmb = (nrows + PG_BM - 1) / PG_BM;
+            launch_parseval_gemm<true>(ctx, parseval_wn(ctx, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr,
+                                       nullptr, (size_t)nrows, scale, drow.as<zd*>(), nmb, dw.as<zd>());
+        }
+        for (size_t k = 0; k < fresh.size(); k++)
+            mc.series[fresh_key[k]] = ModeCache::SeriesEntry{blk.as<zd>() + (size_t)fresh[k].row0 * nn, fresh[k].q,
+                                                             fresh[k].c, fresh[k].h};
+        mc.series_blocks.push_back(std::move(blk));
+        if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
+    }
+    // problems with coefficients: ordered by key (counting sort), slices of SERIES_SLICE; the rest stays in q
+    std::vector<int> key_of_idx(nidx, -1);
+    std::vector<SeriesKey> keys;
+    for (size_t i = 0; i < nidx; i++) {
+        if (!cnt[i]) continue;
+        zd k0;
+        const Key key = key_at(i, k0);
+        auto c = mc.series.find(key);
+        if (c == mc.series.end()) continue;
+        key_of_idx[i] = (int)keys.size();
+        keys.push_back(SeriesKey{k0, c->second.c, c->second.h, c->second.q, 0, c->second.d});
+    }
+    std::vector<int> pkey(M, -1);
+    for (size_t m = 0; m < M; m++) {
+        const int64_t i = idx_of(m);
+        if (i >= 0) pkey[m] = key_of_idx[(size_t)i];
+    }
+    if (keys.empty()) return;
+    std::vector<int> kfirst(keys.size() + 1, 0);
+    for (size_t m = 0; m < M; m++)
+        if (pkey[m] >= 0) kfirst[(size_t)pkey[m] + 1]++;
+    for (size_t k = 0; k < keys.size(); k++) kfirst[k + 1] += kfirst[k];
+    std::vector<SeriesProb> probs((size_t)kfirst.back());
+    std::vector<int> fill(kfirst.begin(), kfirst.end() - 1);
+    size_t keep = 0;
+    for (size_t m = 0; m < M; m++) {
+        const int k = pkey[m];
+        if (k < 0) {
+            q.pd[keep] = q.pd[m];
+            q.dst[keep] = q.dst[m];
+            keep++;
+            continue;
+        }
+        const SeriesKey& K = keys[(size_t)k];
+        probs[(size_t)fill[k]++] = SeriesProb{q.dst[m], K.h > 0.0 ? (q.pd[m].dz - K.c) / K.h : 0.0, k, 0};
+    }
+    q.pd.resize(keep);
+    q.dst.resize(keep);
+    std::vector<SeriesSlice> slices;
+    for (size_t k = 0; k < keys.size(); k++)
+        for (int f = kfirst[k]; f < kfirst[k + 1]; f += SERIES_SLICE)
+            slices.push_back(SeriesSlice{(int)k, f, std::min(SERIES_SLICE, kfirst[k + 1] - f), 0});
+    DevBuf& dkey = scratch_buf(ctx, SB_SD, sizeof(SeriesKey) * keys.size());
+    DevBuf& dprob = scratch_buf(ctx, SB_SPROB, sizeof(SeriesProb) * probs.size());
+    DevBuf& dsl = scratch_buf(ctx, SB_SSLICE, sizeof(SeriesSlice) * slices.size());
+    stage_upload(st, dkey.p, keys.data(), sizeof(SeriesKey) * keys.size(), s);
+    stage_upload(st, dprob.p, probs.data(), sizeof(SeriesProb) * probs.size(), s);
+    stage_upload(st, dsl.p, slices.data(), sizeof(SeriesSlice) * slices.size(), s);
+    {
+        TimedLaunch tl(ctx, "pseries", 8.0 * (double)nn * (double)probs.size() * SERIES_QMAX);
+        hipLaunchKernelGGL(k_series_eval, dim3((unsigned)((nn + 127) / 128), (unsigned)slices.size()), dim3(128), 0,
+                           s, dsl.as<SeriesSlice>(), dprob.as<SeriesProb>(), dkey.as<SeriesKey>(), (int)nn);
+        BF_CHECK_LAUNCH();
+    }
+    if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
+}
+
+void parseval_flush(bf_ctx* ctx, ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st) {
+    if (q.pd.empty()) return;
+    if (opts().parseval_series) parseval_series(ctx, mc, n, scale, q, st);
     if (q.pd.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups, M = q.pd.size();
@@ -1632,25 +2095,41 @@ void parseval_flush(bf_ctx* ctx, const ModeCache& mc, int n, double scale, Parse
     stage_upload(st, dslot.p, slot.data(), sizeof(int) * M, s);
     stage_upload(st, dnk.p, nk.data(), sizeof(int) * nmb, s);
     stage_upload(st, dk0.p, k0s.data(), sizeof(zd) * k0s.size(), s);
-    // waves along the entries: 4 (256 entries per block) unless the matrix fills 128 better
+    auto launch = [&](int w) {
+        launch_parseval_gemm<false>(ctx, w, s, mc, nn, NG, dpw.as<PwDesc>(), dslot.as<int>(), dnk.as<int>(),
+                                    dk0.as<zd>(), M, scale, ddst.as<zd*>(), nmb, nullptr);
+    };
+    // waves along the entries; with the autotuner, both shapes are timed on the first flush of a matrix
+    // size with at least 256 blocks of 128 entries (they compute the same sums in the same order, so the second launch rewrites identical
+    // values)
     int wn = opts().pg_wn;
-    if (wn != 2 && wn != 4) wn = ((nn + 127) / 128 * 128 <= (nn + 255) / 256 * 256) ? 2 : 4;
+    const std::string key = "pgemm nn" + std::to_string(nn) + " G" + std::to_string(NG);
+    if (wn != 2 && wn != 4 && !ctx->tuned.count(key) && opts().autotune &&
+        (size_t)nmb * ((nn + PgShape<2>::BN - 1) / PgShape<2>::BN) >= 256) {
+        const auto t0 = std::chrono::steady_clock::now();
+        double best = 1e300;
+        for (int cand : {2, 4}) {
+            hipEvent_t e0, e1;
+            BF_HIP(hipEventCreate(&e0));
+            BF_HIP(hipEventCreate(&e1));
+            BF_HIP(hipEventRecord(e0, s));
+            launch(cand);
+            BF_HIP(hipEventRecord(e1, s));
+            BF_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            BF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (ms < best) { best = ms; wn = cand; }
+        }
+        ctx->tuned[key] = {wn};
+        ctx->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        save_tuned(ctx, key);
+    }
+    wn = parseval_wn(ctx, nn, NG);
     {
         TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
-        if (wn == 2) {
-            const int ncb = (int)((nn + PgShape<2>::BN - 1) / PgShape<2>::BN);
-            hipLaunchKernelGGL(k_parseval_gemm<2>, dim3((unsigned)((size_t)nmb * ncb)), dim3(PgShape<2>::NT), 0, s,
-                               mc.H.as<zd>(), (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(),
-                               dpw.as<PwDesc>(), dslot.as<int>(), dnk.as<int>(), dk0.as<zd>(), (int)M, scale,
-                               ddst.as<zd*>(), nmb);
-        } else {
-            const int ncb = (int)((nn + PgShape<4>::BN - 1) / PgShape<4>::BN);
-            hipLaunchKernelGGL(k_parseval_gemm<4>, dim3((unsigned)((size_t)nmb * ncb)), dim3(PgShape<4>::NT), 0, s,
-                               mc.H.as<zd>(), (int)nn, (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(),
-                               dpw.as<PwDesc>(), dslot.as<int>(), dnk.as<int>(), dk0.as<zd>(), (int)M, scale,
-                               ddst.as<zd*>(), nmb);
-        }
-        BF_CHECK_LAUNCH();
+        launch(wn);
     }
     if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads
     q.pd.clear();
@@ -1823,10 +2302,8 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         // per region: the bottom block row (16 n^3 flop) or the whole 2n x 2n product (64 n^3 flop)
         TimedLaunch tl(ctx, "chain", flops);
         if (lds) {
-            const size_t smem = 4 * sizeof(zd) * nn;
-            allow_lds((const void*)k_chain_lds, smem);
-            hipLaunchKernelGGL(k_chain_lds, dim3((unsigned)nj), dim3(256), smem, s, djobs.as<ChainJob>(),
-                               dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, boost, refl);
+            launch_chain_lds(s, nj, djobs.as<ChainJob>(), dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, boost,
+                             refl);
         } else {
             hipLaunchKernelGGL(k_chain, dim3((unsigned)nj), dim3(256), 0, s, djobs.as<ChainJob>(),
                                dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, Tb.as<zd>(), cw.as<zd>(), boost,
@@ -1912,11 +2389,7 @@ void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps_h, int n
     hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg);
     BF_CHECK_LAUNCH();
     launch_inverse(ctx, P, Pinv, n, sel, nsel);
-    const size_t smem = 4 * sizeof(zd) * nn;
-    allow_lds((const void*)k_chain_lds, smem);
-    hipLaunchKernelGGL(k_chain_lds, dim3((unsigned)nfreq), dim3(256), smem, s, jobs, steps, P, Pinv, n, e0, reflect,
-                       boost, refl);
-    BF_CHECK_LAUNCH();
+    launch_chain_lds(s, (size_t)nfreq, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
     BF_HIP(hipStreamSynchronize(s));
 }
 
@@ -2089,6 +2562,9 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
     const Grid& g = pl->g;
     const int nfreq = (int)pl->freqs.size();
     const double* freqs = pl->freqs.data();
+    // the spacing-series coefficients depend on the geometries' spacings: formed anew in every run
+    BF_HIP(hipStreamSynchronize(s));
+    for (ModeCache& c : pl->mc) c.reset_series();
     // results: geometry c's [nfreq][n] block at complex element boost_off[c]
     size_t total = 0;
     pl->blocks.clear();
@@ -2166,7 +2642,18 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
             // (every device buffer is used in stream order), and the host goes on to prepare the next chunk
             pl->stager.begin_chunk();
             parseval_flush(ctx, cache, n, 1.0 / (double)g.K(), pq, &pl->stager);
-            disk_diag_flush(ctx, cache, md, pq, &pl->stager);
+            // flat disks taken from the Gram matrix get their inverses from inv(G) with the disk kernel
+            const bool use_ginv = ginv_usable(ctx, cache, md);
+            if (use_ginv) {
+                std::vector<char> from_g(mats, 0);
+                for (size_t i = 0; i < pq.dslot.size(); i++)
+                    if (pq.dfrom[i]) from_g[(size_t)pq.dslot[i]] = 1;
+                size_t keep = 0;
+                for (size_t i = 0; i < sel.size(); i++)
+                    if (!from_g[(size_t)sel[i]]) sel[keep++] = sel[i];
+                sel.resize(keep);
+            }
+            disk_diag_flush(ctx, cache, md, pq, &pl->stager, use_ginv ? Pinv.as<zd>() : nullptr);
             inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, pl->de0[set].as<zd>(), nullptr,
                                dboost.as<zd>(), nullptr, &pl->stager);
             for (size_t q = c0; q < c1; q++)

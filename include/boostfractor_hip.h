@@ -110,12 +110,26 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * k-space tables for complex64 fields), "band" (skip the rows the aperture
  * zeroes), "chfuse", "chxf", "chwave" (cheerleader pair fusion, stored row
  * transforms, wavefront order), "graph" (dancer plans replay HIP graphs),
- * "parseval_groups", "disk_expansion" (transformer work reductions).  Shapes
- * and budgets (0 = default): "col_tx", "rows" (dancer launch shapes),
- * "batch_bytes" (matrix budget of a sweep chunk).  Unknown names and negative
+ * "parseval_groups", "disk_expansion" (transformer work reductions),
+ * "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
+ * (dancer launch shapes), "pg_wn" (2 or 4: Parseval product column blocks of
+ * 128 or 256 matrix entries), "batch_bytes" (matrix budget of a sweep chunk).  Unknown names and negative
  * values return BF_ERR_ARG.  No option changes a result beyond rounding. */
 int bf_ctx_set_option(bf_ctx *ctx, const char *name, int64_t value);
 int bf_ctx_get_option(bf_ctx *ctx, const char *name, int64_t *value);
+/* Autotuner (option "autotune" = 1).  The reference's FFTs
+ * (src/boost_fractor.jl:163-164, FFTW.fft!) are planned by FFTW's estimate on
+ * every call; this is the engine's counterpart of a measured plan: on the first use of a
+ * shape key the engine times candidate launch shapes on the call's own inputs
+ * and keeps the fastest in the context: dancer plans (frequency batch,
+ * column-pass tile, row-pass rows; short runs with nmax cut to 20) and the
+ * Parseval product of the mode-matching sweep (column-block width).  Explicit
+ * options (col_tx, rows, pg_wn, a plan's freq_batch) take precedence.
+ * bf_ctx_set_tuning_file loads entries from a text file (key, tab, values) and
+ * appends every new winner to it; bf_ctx_tuning_report writes the context's
+ * entries as such lines into buf (len bytes) and the milliseconds spent tuning. */
+int bf_ctx_set_tuning_file(bf_ctx *ctx, const char *path);
+int bf_ctx_tuning_report(bf_ctx *ctx, char *buf, int64_t len, double *tune_ms);
 
 /* ---- propagators --------------------------------------------------------
  * replaces prop(E0, dz, diskR, eps, tilt_x, tilt_y, surface, lambda, coords)

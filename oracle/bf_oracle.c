@@ -21,6 +21,31 @@
  *   - Julia Complex{Float64} sqrt / exp / * semantics, including signed zeros
  *     (trap: real eps is stored as re - 0.0im, src/boost_fractor.jl:98,140).
  *
+ * What pins each function (tests/test_oracle_goldens.py, tests/test_oracle_pins.py;
+ * goldens transcribed from the reference's test/runtests.jl and notebooks):
+ *   bfo_wavelength             closed form c/f
+ *   bfo_seed_reflectivities    goldens runtests.jl:211-217 (r from eps), Fresnel form in the pins
+ *   bfo_propagate              propagator / NoTilts: goldens :211, :215; 1D: golden :214 and its
+ *                              closed-form phase; tilt and surface: closed form on a non-square
+ *                              grid; MomentumSpace: identities against numpy's FFT and NoTilts
+ *   bfo_dance_intro(_ex)       goldens (every dancer golden starts here); B field and velocity:
+ *                              in-place write-back and InexactError cases
+ *   bfo_dancer(_sweep)         goldens :214, :215; reflect / emit linearity, history and reverse
+ *                              sum, amin stop against brute-force 1D bounce matrices, the
+ *                              effective-reflection recursion, convergence to the cheerleader
+ *   bfo_cheerleader            goldens :211, :212; reflect / left output against the
+ *                              effective-reflection recursion, energy conservation
+ *   bfo_besselj_zero           tabulated zeros (Abramowitz & Stegun 9.5)
+ *   bfo_seed_modes             goldens :168, Beam Shapes M7 / M4
+ *   bfo_field2modes            goldens :168, Beam Shapes; velocity phase closed form
+ *   bfo_modes2field            golden (modes2field of the :168 coefficients)
+ *   bfo_propagation_matrix     golden :171; through the transformer goldens
+ *   bfo_transformer(_sweep)    golden :217; refl against the effective-reflection recursion
+ *   bfo_transformer_trace_back golden :221 (Deps = 9 geometry)
+ *   bfo_gauss_profile          golden runtests.jl:47
+ *   bfo_unnormalized_match     closed forms (norm, conjugate symmetry, sum)
+ * No function is left unpinned.
+ *
  * Layout conventions (identical to the C-ABI in include/boostfractor_hip.h):
  * complex = interleaved (re, im) doubles; a field is [ny][nx] with x fastest,
  * i.e. the Julia column-major E[x, y]; stacked fields put the field index

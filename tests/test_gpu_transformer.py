@@ -473,6 +473,37 @@ def test_parseval_groups_match_per_bin_weights(engine_options):
             assert rel(per_bin[q][r], want[r]) < TOL and rel(got[q][r], want[r]) < TOL, (q, r)
 
 
+def test_parseval_series_matches_direct(engine_options):
+    """air gaps of one frequency whose spacings lie in one cluster take the Taylor
+    series in the spacing (a few coefficient matrices per cluster, a Horner sum
+    per gap): a 64-disk booster (64 gaps of 8 mm +- 0.3 mm per frequency and the
+    last region at dz = 0, which stays direct) at 64 x 64 against the direct
+    product (option parseval_series=0) to 1e-13 per matrix, and against the
+    oracle on some gaps at 1e-10"""
+    coords = W.grid_coords(64)
+    sb = W.booster(coords, ndisk=64, seed=1710)
+    modes = _lg_like_modes(coords, 3, 1)
+    freqs = [19.8e9, 20.2e9]
+    ctx = B.context()
+    ctx.timing(True)
+    got = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    series_launches = ctx.kernel_time("pseries")[1]
+    ctx.timing(False)
+    assert series_launches >= 2  # coefficient and evaluation launches ran
+    engine_options(parseval_series=0)
+    direct = B.calc_propagation_matrices(sb, coords, modes, f=freqs, diskR=DISKR)
+    nreg = len(sb.distance)
+    lam = [B.wavelength(f) for f in freqs]
+    pat, kt = modes.patterns_flat()[:, 0], modes.mode_kt.reshape(-1)
+    for q in range(len(freqs)):
+        for r in range(1, nreg):
+            assert rel(got[q][r], direct[q][r]) < 1e-13, (q, r, rel(got[q][r], direct[q][r]))
+        for r in (1, 3, 63, nreg - 3, nreg - 1):
+            want = O.propagation_matrix(sb.distance[r], DISKR, sb.eps[r], 0.0, 0.0, None, lam[q], ocoords(coords),
+                                        pat, kt, O.PROP_PROPAGATOR)
+            assert rel(got[q][r], want) < TOL, (q, r, rel(got[q][r], want))
+
+
 def test_transformer_batch_plan_matches_single_geometry_256():
     """The sweep benchmark's shape (256 x 256, Bessel modes n = 9 and 25, boosters
     of 12 and 24 disks with spacings 8 mm +- 0.5 mm): the resident batch plan

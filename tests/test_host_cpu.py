@@ -141,6 +141,8 @@ def test_cabi_validation_without_device():
     assert lib.bf_ctx_set_option(None, b"prune", 0) == L.BF_ERR_ARG
     v = ctypes.c_int64()
     assert lib.bf_ctx_get_option(None, b"prune", ctypes.byref(v)) == L.BF_ERR_ARG
+    assert lib.bf_ctx_set_tuning_file(None, b"/tmp/x") == L.BF_ERR_ARG
+    assert lib.bf_ctx_tuning_report(None, None, 0, None) == L.BF_ERR_ARG
     # gauss_profile / unnormalized_match argument checks
     g = np.zeros((8, 8), dtype=complex)
     assert lib.bf_gauss_profile(None, ctypes.byref(c), 1e-9, -0.1, 10e9, L.cptr(g)) == L.BF_ERR_ARG

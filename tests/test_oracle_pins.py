@@ -17,7 +17,9 @@ code, chosen so that a plausible slip fails it:
   against a brute-force 1D bounce written as matrix products;
 * dancer (Jacobi) against cheerleader (Gauss-Seidel) at convergence on a
   tilted, rough 3D booster (docs/src/3d_algorithms.md:67): the dancer's
-  output is the cheerleader's behind the aperture mask.
+  output is the cheerleader's behind the aperture mask;
+* the axion-velocity phase of axion_induced_modes (transformer.jl:127-132):
+  a pattern carrying the same phase projects to sqrt(#disk pixels) exactly.
 """
 import math
 
@@ -356,6 +358,31 @@ def test_amin_3d_stops_on_a_history_prefix():
         ks.append(k)
     assert ks == sorted(ks) and len(set(ks)) == len(ks)
     assert np.all(O.dancer(os_, oc, 1.0000001, 80, 20e9, diskR=DISKR) == 0)
+
+
+# ---- axion velocity phase ----------------------------------------------------
+@pytest.mark.parametrize("v", [0.05, -0.08])
+def test_field2modes_velocity_phase(v):
+    """axion_induced_modes with velocity_x (transformer.jl:127-132): the source is
+    B exp(i Ma v x) on the disk, normalised to power 1.  A pattern equal to that
+    phase on the disk projects to sqrt(#disk pixels) exactly; the conjugate
+    phase (a sign slip) or the phase along y (an axis slip) do not.  The
+    velocities are large (a phase of several radians across the disk) so that
+    the slips are visible."""
+    oc = _grid(48, 40, span=0.4)
+    f = 20e9
+    Ma = 2 * math.pi * f / C0
+    XX, YY = np.meshgrid(oc.X, oc.Y)
+    mask = (np.sqrt(XX ** 2 + YY ** 2) <= DISKR).astype(float)
+    pats = np.stack([mask * np.exp(1j * Ma * v * XX), mask * np.exp(-1j * Ma * v * XX),
+                     mask * np.exp(1j * Ma * v * YY)])
+    c = O.field2modes(oc, pats, None, v, DISKR, f)
+    npix = mask.sum()
+    assert abs(c[0] - math.sqrt(npix)) < 1e-12 * math.sqrt(npix)
+    assert abs(c[1]) < 0.5 * math.sqrt(npix) and abs(c[2]) < 0.5 * math.sqrt(npix)
+    # v = 0: the plain disk
+    c0 = O.field2modes(oc, mask[None].astype(complex), None, 0.0, DISKR, f)
+    assert abs(c0[0] - math.sqrt(npix)) < 1e-12 * math.sqrt(npix)
 
 
 # ---- dancer against cheerleader ----------------------------------------------
