@@ -415,6 +415,8 @@ def sweep_main(args):
     emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
     if not args.no_autotune:
         ctx.set_option("autotune", 1)  # shapes timed during the (untimed) warmup, reported as `tuning`
+    if args.batch_gib > 0:
+        ctx.set_option("batch_bytes", int(args.batch_gib * (1 << 30)))
     plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
     work = [(_eps_copies([geoms[i] for i in sl]), set_of[sl]) for sl in slices]
 
@@ -439,6 +441,8 @@ def sweep_main(args):
     step(0)
     ctx.synchronize()
     kt_ = {k: ctx.kernel_time(k) for k in MM_CLASSES}
+    host_prep, host_wait = ctx.kernel_time("host_prep"), ctx.kernel_time("host_wait")
+    host_geo = ctx.kernel_time("host_geometries")
     ctx.timing(False)
     b0 = plan.fetch()
     plan.close()
@@ -474,6 +478,10 @@ def sweep_main(args):
         # useful flops (8 n^2 per complex multiply-add of the product) over each class's HIP-event time
         "kernel_tflops": {k: round(v[2] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0 for k, v in kt_.items()},
         "beta2_slice0": {"min": float(b2.min()), "max": float(b2.max())},
+        "host": {"prep_ms": round(host_prep[0], 2), "of_which_geometries_ms": round(host_geo[0], 2),
+                 "wait_ms": round(host_wait[0], 2), "chunks": host_prep[1],
+                 "note": "host work of the instrumented step (region lists, descriptors, launches) and its waits "
+                         "for a free staging slot; the prep of a chunk overlaps the previous chunk's kernels"},
     }
     tuned, tune_ms = ctx.tuning_report()
     out["tuning"] = {"enabled": not args.no_autotune, "ms": round(tune_ms, 1), "chosen": tuned,
@@ -615,6 +623,8 @@ def main():
                          "tilted, rough booster at 512x512 with 45 Laguerre-Gauss modes; model1d: the 1D "
                          "analytical model")
     ap.add_argument("--nconf", type=int, default=NCONF5, help="geometries of the sweep workload (default: all 4096)")
+    ap.add_argument("--batch-gib", type=float, default=0.0,
+                    help="sweep: matrix budget of a chunk in GiB (engine option batch_bytes; 0: the default)")
     ap.add_argument("--keep-boosts", action="store_true", help="fetch every step's boosts (sweep workload)")
     args = ap.parse_args()
     if args.summarize_pmc or args.summarize_trace:

@@ -45,6 +45,7 @@ struct ModeCache {
     int ngroups = 0;
     DevBuf gkx, gky;    // representative kX, kY of each group (double)
     double gk2max = 0.0;  // max over groups of kX^2 + kY^2
+    int gmax = 0;         // most bins in one group
     DevBuf goff, gbin;  // group g holds bins gbin[goff[g] .. goff[g+1])
     // forget everything computed from the patterns (keeps the grouping)
     void reset_data() {

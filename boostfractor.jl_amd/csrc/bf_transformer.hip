@@ -459,6 +459,27 @@ __global__ void k_wgram_reduce(const GemmProb* __restrict__ probs, const zd* __r
 // In-place LU with partial pivoting of an n x n column-major matrix M, then
 // solve M X = RHS for nrhs right-hand sides (RHS may be the identity when
 // `identity` is set).  Result X written to X (n x nrhs, column-major).
+// Visits e = idx, idx + stride, ... < rows * cols as (r, c) with e = r * cols + c, stepping the pair
+// instead of dividing every index.
+template <typename F>
+__device__ __forceinline__ void each2d(int idx, int stride, int rows, int cols, F&& f) {
+    int r = idx / cols, c = idx - r * cols;
+    const int dr = stride / cols, dc = stride - dr * cols;
+    while (r < rows) {
+        f(r, c);
+        c += dc;
+        r += dr;
+        if (c >= cols) {
+            c -= cols;
+            r++;
+        }
+    }
+}
+template <typename F>
+__device__ __forceinline__ void wave_each2d(int lane, int rows, int cols, F&& f) {
+    each2d(lane, 64, rows, cols, f);
+}
+
 __device__ void lu_factor_block(zd* M, int n, int* piv, double* sval, int* sidx) {
     for (int k = 0; k < n; k++) {
         // pivot: first index of max |re|+|im| in column k, rows k..n-1
@@ -472,10 +493,11 @@ __device__ void lu_factor_block(zd* M, int n, int* piv, double* sval, int* sidx)
         sval[threadIdx.x] = best;
         sidx[threadIdx.x] = bi;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0) {  // only the first n - k threads hold candidates
             double b = -1.0;
             int p = k;
-            for (int t = 0; t < (int)blockDim.x; t++)
+            const int nt = min((int)blockDim.x, n - k);
+            for (int t = 0; t < nt; t++)
                 if (sval[t] > b || (sval[t] == b && sidx[t] < p)) { b = sval[t]; p = sidx[t]; }
             piv[k] = p;
         }
@@ -497,10 +519,11 @@ __device__ void lu_factor_block(zd* M, int n, int* piv, double* sval, int* sidx)
         }
         __syncthreads();
         const int m = n - k - 1;
-        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-            const int i = k + 1 + e % m, j = k + 1 + e / m;
-            M[i + (size_t)n * j] = csub(M[i + (size_t)n * j], cmul(M[i + (size_t)n * k], M[k + (size_t)n * j]));
-        }
+        if (m > 0)
+            each2d(threadIdx.x, blockDim.x, m, m, [&](int jj, int ii) {
+                const int i = k + 1 + ii, j = k + 1 + jj;
+                M[i + (size_t)n * j] = csub(M[i + (size_t)n * j], cmul(M[i + (size_t)n * k], M[k + (size_t)n * j]));
+            });
         __syncthreads();
     }
 }
@@ -544,14 +567,22 @@ __global__ void k_batched_inverse(const zd* __restrict__ P, zd* Pinv, int n, zd*
 // lu_solve_col in the same order, so the results are identical.
 constexpr int INV_WAVE_MAX = 48;
 
+// Matrix index of block (x, y): sel[x], or, with the (base, stride) list `bs`, bs[x].base + y bs[x].stride
+// (a region's matrix at frequency y), or x.
+struct InvSel {
+    int64_t base;
+    int64_t stride;
+};
 __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restrict__ P, zd* Pinv, int n,
-                                                           const int* __restrict__ sel) {
+                                                           const int* __restrict__ sel,
+                                                           const InvSel* __restrict__ bs) {
     extern __shared__ unsigned char smem_raw[];
     zd* M = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
     zd* X = M + n * n;                        // X[i * n + j]: row i of solution column j
     __shared__ int piv[INV_WAVE_MAX], sig[INV_WAVE_MAX];
     const int lane = threadIdx.x;
-    const int b = sel ? sel[blockIdx.x] : blockIdx.x;
+    const int64_t b = bs ? bs[blockIdx.x].base + (int64_t)blockIdx.y * bs[blockIdx.x].stride
+                         : (sel ? sel[blockIdx.x] : blockIdx.x);
     const int nn = n * n;
     const zd* src = P + (size_t)b * nn;
     for (int e = lane; e < nn; e += 64) M[e] = src[e];
@@ -587,10 +618,11 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
         }
         __syncthreads();
         const int m = n - k - 1;
-        for (int e = lane; e < m * m; e += 64) {
-            const int i = k + 1 + e % m, j = k + 1 + e / m;
-            M[i + n * j] = csub(M[i + n * j], cmul(M[i + n * k], M[k + n * j]));
-        }
+        if (m > 0)
+            wave_each2d(lane, m, m, [&](int jj, int ii) {
+                const int i = k + 1 + ii, j = k + 1 + jj;
+                M[i + n * j] = csub(M[i + n * j], cmul(M[i + n * k], M[k + n * j]));
+            });
         __syncthreads();
     }
     // X = inv(P): the identity with the pivots' row swaps (the permutation sigma), then the unit-lower
@@ -605,14 +637,13 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
         }
     }
     __syncthreads();
-    for (int e = lane; e < nn; e += 64) X[e] = cmk<double>(sig[e / n] == e % n ? 1.0 : 0.0, 0.0);
+    wave_each2d(lane, n, n, [&](int i, int j) { X[i * n + j] = cmk<double>(sig[i] == j ? 1.0 : 0.0, 0.0); });
     __syncthreads();
     for (int k = 0; k + 1 < n; k++) {
-        const int rows = n - k - 1;
-        for (int e = lane; e < rows * n; e += 64) {
-            const int i = k + 1 + e / n, j = e % n;
+        wave_each2d(lane, n - k - 1, n, [&](int ii, int j) {
+            const int i = k + 1 + ii;
             X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
-        }
+        });
         __syncthreads();
     }
     for (int k = n - 1; k >= 0; k--) {
@@ -623,14 +654,14 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
             X[k * n + j] = cmk<double>((x.re * d.re + x.im * d.im) / den, (x.im * d.re - x.re * d.im) / den);
         }
         __syncthreads();
-        for (int e = lane; e < k * n; e += 64) {
-            const int i = e / n, j = e % n;
-            X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
-        }
+        if (k > 0)
+            wave_each2d(lane, k, n, [&](int i, int j) {
+                X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
+            });
         __syncthreads();
     }
     zd* dst = Pinv + (size_t)b * nn;
-    for (int e = lane; e < nn; e += 64) dst[e] = X[(e % n) * n + e / n];
+    wave_each2d(lane, n, n, [&](int j, int i) { dst[j * n + i] = X[i * n + j]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -716,9 +747,11 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
 // (16 n^3 flop per region instead of 64 n^3 for the whole 2n x 2n product).
 // Every output sums over k in the same order as k_chain.  Thread t owns row
 // t % n and the columns t / n + c floor(256 / n) of both products (all 256
-// threads busy for n = 16, 25, 32, ...), forms U and V for its row on the fly,
-// and loads the next region's P and inv(P) into registers while it computes,
-// so each region costs two barriers and no exposed HBM latency.  LDS: T21,
+// threads busy for n = 16, 25, 32, ...), forms U and V for its row on the fly
+// (threads t < n also the row's axion sum from the same operands), and loads
+// the next region's P and inv(P) into registers while it computes, so each
+// region costs two barriers and no exposed HBM latency.  (Loading each k's
+// operands one k ahead as well measured slower: 230 instead of 140 VGPRs.)  LDS: T21,
 // T22, P, inv(P) (4 n^2 complex).  NM: the largest n of the instantiation
 // (its register arrays).
 constexpr int CHAIN_LDS_MAX = 48;
@@ -768,28 +801,40 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
         const bool more = q + 1 < job.nsteps;
         zd pn[CS::PPT];
         if constexpr (CS::PF) {
+            // every element assigned on every path, so the array stays in registers
+            const zd* nP = P;
+            const zd* nPi = Pinv;
+            if (more) {
+                const int reg = steps[job.step0 + q + 1].reg;
+                nP = P + (size_t)(job.pbase + reg) * nn;
+                nPi = Pinv + (size_t)(job.pbase + reg) * nn;
+            }
 #pragma unroll
             for (int i = 0; i < CS::PPT; i++) {
                 const int r = tid + 256 * i;
-                if (more && r < 2 * nn) pn[i] = *region(q + 1, r);
+                zd v = cmk<double>(0.0, 0.0);
+                if (more && r < nn)
+                    v = nP[r];
+                else if (more && r < 2 * nn)
+                    v = nPi[r - nn];
+                pn[i] = v;
             }
         }
-        // axion_beam += S * (T21*e0 + T22*e0)
-        if (tid < n) {
-            zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
-            for (int k = 0; k < n; k++) {
-                s21 = cadd(s21, cmul(T21[tid + n * k], e0[k]));
-                s22 = cadd(s22, cmul(T22[tid + n * k], e0[k]));
-            }
-            abr = cadd(abr, cmul(st.S, cadd(s21, s22)));
-        }
-        // T21' = U P, T22' = V inv(P), U = a T21 + b T22, V = b T21 + a T22
+        // T21' = U P, T22' = V inv(P), U = a T21 + b T22, V = b T21 + a T22; the row's first column group
+        // (thread tid = row < n) also forms axion_beam += S * (T21*e0 + T22*e0) from the same operands
         zd o21[CS::COLS], o22[CS::COLS];
 #pragma unroll
         for (int c = 0; c < CS::COLS; c++) o21[c] = o22[c] = cmk<double>(0.0, 0.0);
         if (act) {
+            const bool axion = cg == 0;
+            zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
             for (int k = 0; k < n; k++) {
                 const zd t21 = T21[row + n * k], t22 = T22[row + n * k];
+                if (axion) {
+                    const zd e = e0[k];
+                    s21 = cadd(s21, cmul(t21, e));
+                    s22 = cadd(s22, cmul(t22, e));
+                }
                 const zd u = cadd(cmul(st.a, t21), cmul(st.b, t22));
                 const zd v = cadd(cmul(st.b, t21), cmul(st.a, t22));
 #pragma unroll
@@ -801,6 +846,7 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
                     }
                 }
             }
+            if (axion) abr = cadd(abr, cmul(st.S, cadd(s21, s22)));
         }
         __syncthreads();
         if (act) {
@@ -882,13 +928,16 @@ __global__ void k_disk_diag(zd* P, const int* __restrict__ slots, const int* __r
     zd* Pm = P + off;
     const bool fg = from_gram[q] != 0;
     const zd* X = fg ? G : Pm;
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Pm[e] = cmul(sph[e % n], X[e]);
-    if (fg && Pinv)
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-            const zd a = Ginv[e], d = sph[e / n];
+    const bool inv = fg && Pinv;
+    each2d(threadIdx.x, blockDim.x, n, n, [&](int j, int i) {
+        const int e = i + n * j;
+        Pm[e] = cmul(sph[i], X[e]);
+        if (inv) {
+            const zd a = Ginv[e], d = sph[j];
             const double den = d.re * d.re + d.im * d.im;
             Pinv[off + e] = cmk<double>((a.re * d.re + a.im * d.im) / den, (a.im * d.re - a.re * d.im) / den);
         }
+    });
 }
 
 // P[slot] = s * G  (1D propagator in an air region: prop multiplies by a scalar phase)
@@ -919,6 +968,38 @@ __global__ void k_group_gram(const zd* __restrict__ A, const zd* __restrict__ B,
         }
         H[(size_t)g * nn + e] = cmk<double>(re, im);
     }
+}
+
+// The same with the group's pattern values staged in LDS first (each read once
+// instead of once per matrix entry); the sums run in the same order.  Dynamic
+// LDS: 2 n (bins x C) complex.
+__global__ void __launch_bounds__(256) k_group_gram_lds(const zd* __restrict__ A, const zd* __restrict__ B, size_t K,
+                                                        int C, int n, const int* __restrict__ off,
+                                                        const int* __restrict__ bins, zd* H) {
+    extern __shared__ unsigned char smem_raw[];
+    const int g = blockIdx.x;
+    const int nn = n * n;
+    const int b0 = off[g], b1 = off[g + 1], np = (b1 - b0) * C;
+    zd* sa = reinterpret_cast<zd*>(smem_raw);  // [i][pair], pair = (q - b0) * C + c
+    zd* sb = sa + (size_t)n * np;
+    const size_t Ktot = (size_t)C * K;
+    each2d(threadIdx.x, blockDim.x, n, np, [&](int i, int p) {
+        const size_t k = (size_t)bins[b0 + p / C] + (size_t)(p % C) * K;
+        sa[i * np + p] = A[(size_t)i * Ktot + k];
+        sb[i * np + p] = B[(size_t)i * Ktot + k];
+    });
+    __syncthreads();
+    each2d(threadIdx.x, blockDim.x, n, n, [&](int j, int i) {
+        double re = 0.0, im = 0.0;
+        const zd* a_ = sa + i * np;
+        const zd* b_ = sb + j * np;
+        for (int p = 0; p < np; p++) {
+            const zd a = a_[p], b = b_[p];
+            re += a.re * b.re + a.im * b.im;
+            im += a.re * b.im - a.im * b.re;
+        }
+        H[(size_t)g * nn + i + n * j] = cmk<double>(re, im);
+    });
 }
 
 // Flat air regions by k-bin groups on the FP64 matrix cores:
@@ -1220,7 +1301,8 @@ static void launch_inverse(bf_ctx* ctx, const zd* P, zd* Pinv, int n, const int*
     if (n <= INV_WAVE_MAX) {
         const size_t smem = 2 * sizeof(zd) * (size_t)n * n;
         allow_lds((const void*)k_batched_inverse_wave, smem);
-        hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)count), dim3(64), smem, s, P, Pinv, n, dsel);
+        hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)count), dim3(64), smem, s, P, Pinv, n, dsel,
+                           (const InvSel*)nullptr);
         BF_CHECK_LAUNCH();
     } else {
         DevBuf work(sizeof(zd) * (size_t)n * n * count);
@@ -1293,12 +1375,21 @@ static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md
         BF_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
         mc.ngroups = G;
         mc.gk2max = k2max;
+        mc.gmax = 0;
+        for (int q = 0; q < G; q++) mc.gmax = std::max(mc.gmax, off[q + 1] - off[q]);
         mc.haveIdx = true;
     }
     const int G = mc.ngroups;
     if (mc.H.bytes < sizeof(zd) * nn * G) mc.H.alloc(sizeof(zd) * nn * G);
-    hipLaunchKernelGGL(k_group_gram, dim3(G), dim3(256), 0, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(), K, md.C, n,
-                       mc.goff.as<int>(), mc.gbin.as<int>(), mc.H.as<zd>());
+    const size_t smem = 2 * sizeof(zd) * (size_t)n * mc.gmax * md.C;
+    if (smem <= 64 * 1024) {
+        allow_lds((const void*)k_group_gram_lds, smem);
+        hipLaunchKernelGGL(k_group_gram_lds, dim3(G), dim3(256), smem, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(), K,
+                           md.C, n, mc.goff.as<int>(), mc.gbin.as<int>(), mc.H.as<zd>());
+    } else {
+        hipLaunchKernelGGL(k_group_gram, dim3(G), dim3(256), 0, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(), K, md.C, n,
+                           mc.goff.as<int>(), mc.gbin.as<int>(), mc.H.as<zd>());
+    }
     BF_CHECK_LAUNCH();
     mc.haveH = true;
     return true;
@@ -1398,6 +1489,37 @@ struct Gram {
 // All propagation matrices P[f][reg] for the regions in `regs` (0-based), at frequencies fr[0..nf).
 // eps: the permittivities to use (the transformer passes NaN-fixed values).
 
+// The mode set's frequency-independent tables, formed on first use and kept in mc: the Gram matrix G
+// (flat disks, 1D air), the pattern transforms Fhat (masked and unmasked) and, for Parseval regions, the
+// k-bin groups and their Grams H.  Returns whether the grouped Parseval product is available.
+bool mode_tables(bf_ctx* ctx, const Grid& g, const ModesDev& md, ModeCache& mc, bool need_gram, bool need_parseval,
+                 double diskR) {
+    hipStream_t s = ctx->stream;
+    const int n = md.n;
+    const size_t K = g.K(), Ktot = md.Ktot, nn = (size_t)n * n;
+    if (need_gram && !mc.haveG) {
+        Gram gram{ctx, n, Ktot, K};
+        if (mc.G.bytes < sizeof(zd) * nn) mc.G.alloc(sizeof(zd) * nn);
+        GemmProb p{md.pat.as<zd>(), md.pat.as<zd>(), nullptr, 0, 0};
+        gram.run({p}, mc.G.as<zd>());
+        mc.haveG = true;
+    }
+    if (need_parseval && !mc.haveF) {
+        if (mc.Fhat.bytes < sizeof(zd) * n * Ktot) mc.Fhat.alloc(sizeof(zd) * n * Ktot);
+        if (mc.FhatM.bytes < sizeof(zd) * n * Ktot) mc.FhatM.alloc(sizeof(zd) * n * Ktot);
+        hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, mc.Fhat.as<zd>(), md.pat.as<zd>(),
+                           n * Ktot);
+        hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, mc.FhatM.as<zd>(), md.pat.as<zd>(),
+                           n * Ktot);
+        BF_CHECK_LAUNCH();
+        fft_fields(ctx, g, mc.Fhat.as<zd>(), n * md.C, false, diskR);
+        fft_fields(ctx, g, mc.FhatM.as<zd>(), n * md.C, true, diskR);
+        mc.haveF = true;
+    }
+    if (!need_parseval || !opts().parseval_groups) return false;
+    return mc.haveH || build_parseval_groups(ctx, g, md, mc);
+}
+
 void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, int kind, double diskR, int nreg,
                                const double* distance, const zd* eps, const double* tx, const double* ty,
                                const zd* surfaces_dev, const std::vector<bool>& flat,
@@ -1422,28 +1544,10 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     if (!onedim.empty()) need_gram = true;
     ModeCache local;
     ModeCache& mc = cache ? *cache : local;
-    DevBuf& G = mc.G;
     DevBuf& Fhat = mc.Fhat;
     DevBuf& FhatM = mc.FhatM;
-    if (need_gram && !mc.haveG) {
-        if (G.bytes < sizeof(zd) * nn) G.alloc(sizeof(zd) * nn);
-        GemmProb p{md.pat.as<zd>(), md.pat.as<zd>(), nullptr, 0, 0};
-        gram.run({p}, G.as<zd>());
-        mc.haveG = true;
-    }
-    if (!parseval.empty() && !mc.haveF) {
-        if (Fhat.bytes < sizeof(zd) * n * Ktot) Fhat.alloc(sizeof(zd) * n * Ktot);
-        if (FhatM.bytes < sizeof(zd) * n * Ktot) FhatM.alloc(sizeof(zd) * n * Ktot);
-        hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, Fhat.as<zd>(), md.pat.as<zd>(), n * Ktot);
-        hipLaunchKernelGGL(k_copy_z, dim3(g1(n * Ktot, 4096)), dim3(256), 0, s, FhatM.as<zd>(), md.pat.as<zd>(), n * Ktot);
-        BF_CHECK_LAUNCH();
-        fft_fields(ctx, g, Fhat.as<zd>(), n * md.C, false, diskR);
-        fft_fields(ctx, g, FhatM.as<zd>(), n * md.C, true, diskR);
-        mc.haveF = true;
-    }
     // Parseval regions through k-bin groups (option parseval_groups=0: per-bin weights)
-    bool grouped = false;
-    if (!parseval.empty() && opts().parseval_groups) grouped = mc.haveH || build_parseval_groups(ctx, g, md, mc);
+    const bool grouped = mode_tables(ctx, g, md, mc, need_gram, !parseval.empty(), diskR);
     // frequency-dependent pieces, in frequency batches bounded by table memory
     // tilted/rough disk regions: moments of the frequency expansion (option disk_expansion=0: per-frequency tables)
     std::vector<int> exp_q0(nreg, -1), exp_np(nreg, 0);
@@ -1525,17 +1629,30 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
     DevBuf& ddz = ctx->scratch[SB_DZ];
     ParsevalQueue local_q;
     ParsevalQueue& pq = queue ? *queue : local_q;
+    // k0 of a permittivity at the batch's frequencies, formed once per distinct permittivity (bitwise)
+    std::vector<std::pair<zd, std::vector<zd>>> k0_memo;
+    auto k0_at = [&](const zd& e, int f0, int nb) -> const std::vector<zd>& {
+        for (auto& m : k0_memo)
+            if (std::memcmp(&m.first, &e, sizeof(zd)) == 0 && (int)m.second.size() == nb) return m.second;
+        std::vector<zd> v((size_t)nb);
+        for (int f = 0; f < nb; f++) v[(size_t)f] = k0_of(e, 299792458.0 / fr[f0 + f]);
+        k0_memo.emplace_back(e, std::move(v));
+        return k0_memo.back().second;
+    };
     for (int f0 = 0; f0 < nf; f0 += fb) {
         const int nb = std::min(fb, nf - f0);
+        k0_memo.clear();
         // Parseval air regions: w = exp(-i kz dz)/K in unshifted bins
         std::vector<GemmProb> probs;
         if (grouped) {
             // queued: P[slot] = sum_g W[m][g] H_g, one GEMM for all of them
+            pq.pd.reserve(pq.pd.size() + (size_t)nb * parseval.size());
+            pq.dst.reserve(pq.dst.size() + (size_t)nb * parseval.size());
             for (int f = 0; f < nb; f++)
                 for (int r : parseval) {
                     zd e = eps[r];
                     if (kind == BF_PROP_PROPAGATOR && e.im == 0.0) e.im = -0.0;
-                    const zd k0 = k0_of(e, 299792458.0 / fr[f0 + f]);
+                    const zd k0 = k0_at(e, f0, nb)[(size_t)f];
                     pq.pd.push_back(PwDesc{cmul(k0, k0), distance[r], 0, 0});
                     pq.dst.push_back(P + ((size_t)f * nreg + r) * nn);
                 }
@@ -1569,7 +1686,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             std::vector<std::pair<int, int>> rd_slot;
             for (int f = 0; f < nb; f++)
                 for (int r : disks) {
-                    const zd k0 = k0_of(eps[r], 299792458.0 / fr[f0 + f]);
+                    const zd k0 = k0_at(eps[r], f0, nb)[(size_t)f];
                     const int slot = (int)((size_t)f * nreg + r);
                     slots.push_back(slot);
                     k0sq.push_back(cmul(k0, k0));
@@ -1626,7 +1743,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             upload(dk0.p, k0sq.data(), sizeof(zd) * slots.size(), s);
             upload(ddz.p, dzs.data(), sizeof(double) * slots.size(), s);
             hipLaunchKernelGGL(k_disk_diag, dim3((unsigned)slots.size()), dim3(256), 0, s, P, dslots.as<int>(),
-                               dflag.as<int>(), G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n,
+                               dflag.as<int>(), mc.G.as<zd>(), dk0.as<zd>(), ddz.as<double>(), md.kt.as<zd>(), n,
                                nullptr, nullptr);
             BF_CHECK_LAUNCH();
             BF_HIP(hipStreamSynchronize(s));
@@ -1647,7 +1764,7 @@ void propagation_matrices_core(bf_ctx* ctx, const Grid& g, const ModesDev& md, i
             upload(dslots.p, sl.data(), sizeof(int) * sl.size(), s);
             upload(dk0.p, ph.data(), sizeof(zd) * sl.size(), s);
             hipLaunchKernelGGL(k_scaled_copy, dim3((unsigned)sl.size()), dim3(256), 0, s, P, dslots.as<int>(),
-                               dk0.as<zd>(), G.as<zd>(), (int)nn);
+                               dk0.as<zd>(), mc.G.as<zd>(), (int)nn);
             BF_CHECK_LAUNCH();
             BF_HIP(hipStreamSynchronize(s));
         }
@@ -1866,13 +1983,72 @@ __global__ void __launch_bounds__(128) k_series_eval(const SeriesSlice* __restri
     }
 }
 
+using SeriesKeyId = std::array<uint64_t, 3>;  // k0^2 bits (re, im) and spacing bin; {0, 0, ~0}: dz = 0
+constexpr uint64_t SERIES_ZERO_BIN = ~(uint64_t)0;
+
+// The series of key `key` (k0^2 = k0, bin of width mc.series_w): centre, half width and terms; false if its
+// k0 is beyond the bins' allowance or if `cnt` gaps do not pay for the terms (at least q + 4 gaps)
+static bool series_key_for(const ModeCache& mc, const SeriesKeyId& key, zd k0, int cnt, SeriesKey& out) {
+    const double w = mc.series_w;
+    const bool zero = key[2] == SERIES_ZERO_BIN;
+    const double h = zero ? 0.0 : 0.5 * w, c = zero ? 0.0 : ((double)(int64_t)key[2] + 0.5) * w;
+    const double beta = std::sqrt(std::hypot(k0.re, k0.im) + mc.gk2max) * h;
+    int Q = 1;
+    if (h > 0.0) {
+        double term = beta;  // beta^Q / Q!
+        while (std::exp(beta) * term >= 1e-17 && Q < SERIES_QMAX) {
+            Q++;
+            term *= beta / Q;
+        }
+        if (std::exp(beta) * term >= 1e-17) return false;
+    }
+    if (cnt < Q + 4) return false;
+    out = SeriesKey{k0, c, h, Q, 0, nullptr};
+    return true;
+}
+
+// The coefficient matrices of the keys `fresh` (rows row0 .. row0 + q - 1 of nrows): weight table, one
+// product on it, entries added to mc.series for the rest of the run
+static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const std::vector<SeriesKey>& fresh,
+                        const std::vector<SeriesKeyId>& fresh_key, int nrows, Stager* st) {
+    if (fresh.empty()) return;
+    hipStream_t s = ctx->stream;
+    const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups;
+    DevBuf blk(sizeof(zd) * nn * (size_t)nrows);
+    std::vector<zd*> rowdst((size_t)nrows);
+    for (int r = 0; r < nrows; r++) rowdst[(size_t)r] = blk.as<zd>() + (size_t)r * nn;
+    DevBuf& dkey = scratch_buf(ctx, SB_SKEY, sizeof(SeriesKey) * fresh.size());
+    DevBuf& drow = scratch_buf(ctx, SB_SROWDST, sizeof(zd*) * (size_t)nrows);
+    DevBuf& dw = scratch_buf(ctx, SB_SW, sizeof(zd) * NG * (size_t)nrows);
+    stage_upload(st, dkey.p, fresh.data(), sizeof(SeriesKey) * fresh.size(), s);
+    stage_upload(st, drow.p, rowdst.data(), sizeof(zd*) * rowdst.size(), s);
+    {
+        TimedLaunch tl(ctx, "pseries", 0.0);
+        hipLaunchKernelGGL(k_series_coef, dim3((unsigned)((NG + 255) / 256), (unsigned)fresh.size()), dim3(256), 0, s,
+                           dkey.as<SeriesKey>(), mc.gkx.as<double>(), mc.gky.as<double>(), (int)NG, nrows, scale,
+                           dw.as<zd>());
+        BF_CHECK_LAUNCH();
+    }
+    {
+        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)nrows);
+        const int nmb = (nrows + PG_BM - 1) / PG_BM;
+        launch_parseval_gemm<true>(ctx, parseval_wn(ctx, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr, nullptr,
+                                   (size_t)nrows, scale, drow.as<zd*>(), nmb, dw.as<zd>());
+    }
+    for (size_t k = 0; k < fresh.size(); k++)
+        mc.series[fresh_key[k]] =
+            ModeCache::SeriesEntry{blk.as<zd>() + (size_t)fresh[k].row0 * nn, fresh[k].q, fresh[k].c, fresh[k].h};
+    mc.series_blocks.push_back(std::move(blk));
+    if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
+}
+
 // Serves the problems of q whose (k0^2, spacing bin) has coefficients from an
 // earlier flush of the run, or enough gaps in this flush to pay for them (at
 // least q + 4), and leaves the others in q.
 static void parseval_series(bf_ctx* ctx, ModeCache& mc, int n, double scale, ParsevalQueue& q, Stager* st) {
-    const size_t M = q.pd.size(), nn = (size_t)n * n, NG = (size_t)mc.ngroups;
+    const size_t M = q.pd.size(), nn = (size_t)n * n;
     hipStream_t s = ctx->stream;
-    using Key = std::array<uint64_t, 3>;
+    using Key = SeriesKeyId;
     auto bits = [](double v) {
         uint64_t b;
         std::memcpy(&b, &v, 8);
@@ -1886,7 +2062,6 @@ static void parseval_series(bf_ctx* ctx, ModeCache& mc, int n, double scale, Par
         mc.series_w = 2.0 / kb;
     }
     const double w = mc.series_w;
-    constexpr uint64_t ZERO_BIN = ~(uint64_t)0;
     // every problem's key as a dense index: 0 for dz = 0, else 1 + k0 slot * nb + (bin - bmin), where
     // consecutive problems mostly share their k0 (a geometry's gaps at one frequency); -1: direct
     std::vector<int> slot(M, -1);
@@ -1944,7 +2119,7 @@ static void parseval_series(bf_ctx* ctx, ModeCache& mc, int n, double scale, Par
                     k0 = q.pd[m].k0sq;
                     break;
                 }
-            return {0, 0, ZERO_BIN};
+            return {0, 0, SERIES_ZERO_BIN};
         }
         const size_t sl = (i - 1) / (size_t)nb;
         k0 = k0v[sl];
@@ -1953,58 +2128,21 @@ static void parseval_series(bf_ctx* ctx, ModeCache& mc, int n, double scale, Par
     (void)any_zero;
     // new keys worth forming
     std::vector<SeriesKey> fresh;
-    std::vector<Key> fresh_key;
+    std::vector<SeriesKeyId> fresh_key;
     int nrows = 0;
     for (size_t i = 0; i < nidx; i++) {
         if (!cnt[i]) continue;
         zd k0;
         const Key key = key_at(i, k0);
         if (mc.series.count(key)) continue;
-        const bool zero = i == 0;
-        const double h = zero ? 0.0 : 0.5 * w, c = zero ? 0.0 : ((double)(int64_t)key[2] + 0.5) * w;
-        const double beta = std::sqrt(std::hypot(k0.re, k0.im) + mc.gk2max) * h;
-        int Q = 1;
-        if (h > 0.0) {
-            double term = beta;  // beta^Q / Q!
-            while (std::exp(beta) * term >= 1e-17 && Q < SERIES_QMAX) {
-                Q++;
-                term *= beta / Q;
-            }
-            if (std::exp(beta) * term >= 1e-17) continue;  // k0 beyond the bins' allowance: direct
-        }
-        if (cnt[i] < Q + 4) continue;  // the direct product is cheaper
-        fresh.push_back(SeriesKey{k0, c, h, Q, nrows, nullptr});
+        SeriesKey sk;
+        if (!series_key_for(mc, key, k0, cnt[i], sk)) continue;
+        sk.row0 = nrows;
+        fresh.push_back(sk);
         fresh_key.push_back(key);
-        nrows += Q;
+        nrows += sk.q;
     }
-    if (!fresh.empty()) {
-        DevBuf blk(sizeof(zd) * nn * (size_t)nrows);
-        std::vector<zd*> rowdst((size_t)nrows);
-        for (int r = 0; r < nrows; r++) rowdst[(size_t)r] = blk.as<zd>() + (size_t)r * nn;
-        DevBuf& dkey = scratch_buf(ctx, SB_SKEY, sizeof(SeriesKey) * fresh.size());
-        DevBuf& drow = scratch_buf(ctx, SB_SROWDST, sizeof(zd*) * (size_t)nrows);
-        DevBuf& dw = scratch_buf(ctx, SB_SW, sizeof(zd) * NG * (size_t)nrows);
-        stage_upload(st, dkey.p, fresh.data(), sizeof(SeriesKey) * fresh.size(), s);
-        stage_upload(st, drow.p, rowdst.data(), sizeof(zd*) * rowdst.size(), s);
-        {
-            TimedLaunch tl(ctx, "pseries", 0.0);
-            hipLaunchKernelGGL(k_series_coef, dim3((unsigned)((NG + 255) / 256), (unsigned)fresh.size()), dim3(256),
-                               0, s, dkey.as<SeriesKey>(), mc.gkx.as<double>(), mc.gky.as<double>(), (int)NG, nrows,
-                               scale, dw.as<zd>());
-            BF_CHECK_LAUNCH();
-        }
-        {
-            TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)nrows);
-            const int nmb = (nrows + PG_BM - 1) / PG_BM;
-            launch_parseval_gemm<true>(ctx, parseval_wn(ctx, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr,
-                                       nullptr, (size_t)nrows, scale, drow.as<zd*>(), nmb, dw.as<zd>());
-        }
-        for (size_t k = 0; k < fresh.size(); k++)
-            mc.series[fresh_key[k]] = ModeCache::SeriesEntry{blk.as<zd>() + (size_t)fresh[k].row0 * nn, fresh[k].q,
-                                                             fresh[k].c, fresh[k].h};
-        mc.series_blocks.push_back(std::move(blk));
-        if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
-    }
+    series_form(ctx, mc, n, scale, fresh, fresh_key, nrows, st);
     // problems with coefficients: ordered by key (counting sort), slices of SERIES_SLICE; the rest stays in q
     std::vector<int> key_of_idx(nidx, -1);
     std::vector<SeriesKey> keys;
@@ -2284,7 +2422,7 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     const int N2 = 2 * n;
     DevBuf& dsel = scratch_buf(ctx, SB_SEL, sizeof(int) * sel.size());
     stage_upload(st, dsel.p, sel.data(), sizeof(int) * sel.size(), s);
-    {
+    if (!sel.empty()) {
         TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)sel.size());
         launch_inverse(ctx, P, Pinv, n, dsel.as<int>(), sel.size());
     }
@@ -2555,6 +2693,294 @@ bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_c
     return pl;
 }
 
+// ---------------------------------------------------------------------------
+// Flat chunks of the batch plan.  When every geometry of a chunk has flat,
+// untilted regions (no surface map, zero tilts), every region's matrix at
+// every frequency follows from frequency-independent descriptors: an air gap's
+// from the spacing series of its bin (coefficients per frequency), a disk's as
+// diag(exp(-1im*kz*dz)) G.  The kernels expand the descriptors over the
+// frequencies (grid y), so the host's work per chunk grows with the chunk's
+// regions, not with regions x frequencies.  The matrices, inverses and chains
+// are the ones the general path forms (the same kernels' arithmetic).
+// ---------------------------------------------------------------------------
+struct FlatRegion {
+    int64_t base;  // matrix index at frequency 0; frequency f adds f * stride
+    int stride;    // the geometry's region count
+    int slot;      // permittivity index (k0^2 table row)
+    double dz;
+};
+struct FlatSlice {  // series evaluation: entries [first, first + count) of one bin, key `key`, frequency f
+    int key, first, count, f;
+};
+
+__global__ void __launch_bounds__(128) k_series_eval_flat(const FlatSlice* __restrict__ sl,
+                                                          const FlatRegion* __restrict__ ent,
+                                                          const SeriesKey* __restrict__ keys, zd* P, int nn) {
+    const int e = blockIdx.x * 128 + threadIdx.x;
+    if (e >= nn) return;
+    const FlatSlice S = sl[blockIdx.y];
+    const SeriesKey k = keys[S.key];
+    zd d[SERIES_QMAX];
+#pragma unroll
+    for (int p = 0; p < SERIES_QMAX; p++) d[p] = p < k.q ? k.d[(size_t)p * nn + e] : cmk<double>(0.0, 0.0);
+    for (int i = 0; i < S.count; i++) {
+        const FlatRegion R = ent[S.first + i];
+        const double t = k.h > 0.0 ? (R.dz - k.c) / k.h : 0.0;
+        zd acc = cmk<double>(0.0, 0.0);
+#pragma unroll
+        for (int p = SERIES_QMAX - 1; p >= 0; p--) acc = cmk<double>(d[p].re + t * acc.re, d[p].im + t * acc.im);
+        P[(size_t)(R.base + (int64_t)S.f * R.stride) * nn + e] = acc;
+    }
+}
+
+// flat disks, block (entry, frequency): P = diag(exp(-1im*kz*dz)) G and, with Ginv, inv(P) = Ginv diag(1/ph)
+__global__ void __launch_bounds__(256) k_disk_flat(const FlatRegion* __restrict__ ent, const zd* __restrict__ k0sq,
+                                                   int nf, const zd* __restrict__ G, const zd* __restrict__ Ginv,
+                                                   const zd* __restrict__ kt, int n, zd* P, zd* Pinv) {
+    __shared__ zd sph[BF_MAX_MODES];
+    const FlatRegion R = ent[blockIdx.x];
+    const int f = blockIdx.y;
+    const zd k2 = k0sq[(size_t)R.slot * nf + f];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
+        sph[i] = cexp_j(rmul(R.dz, neg_i(kz)));
+    }
+    __syncthreads();
+    const size_t off = (size_t)(R.base + (int64_t)f * R.stride) * n * n;
+    each2d(threadIdx.x, blockDim.x, n, n, [&](int j, int i) {
+        const int e = i + n * j;
+        P[off + e] = cmul(sph[i], G[e]);
+        if (Ginv) {
+            const zd a = Ginv[e], d = sph[j];
+            const double den = d.re * d.re + d.im * d.im;
+            Pinv[off + e] = cmk<double>((a.re * d.re + a.im * d.im) / den, (a.im * d.re - a.re * d.im) / den);
+        }
+    });
+}
+
+// true if the flat chunk path serves geometries confs[c0, c1)
+static bool flat_chunk_ok(const bf_transformer_batch_plan* pl, const ModesDev& md, const bf_setup* setups,
+                          const std::vector<int>& confs, size_t c0, size_t c1) {
+    if (pl->kind != BF_PROP_PROPAGATOR && pl->kind != BF_PROP_NOTILTS) return false;
+    if (!opts().parseval_groups || !opts().parseval_series) return false;
+    if (md.n > INV_WAVE_MAX || md.n > CHAIN_LDS_MAX) return false;
+    for (size_t q = c0; q < c1; q++) {
+        const bf_setup* st = &setups[confs[q]];
+        if (st->surfaces) return false;
+        for (int r = 0; r < st->nreg; r++)
+            if (st->tilt_x[r] != 0.0 || st->tilt_y[r] != 0.0 || !std::isfinite(st->distance[r])) return false;
+    }
+    return true;
+}
+
+// The host side of a flat chunk (flat_prepare) and its launches (flat_launch).
+struct FlatWork {
+    std::vector<FlatRegion> ent, disks, inv;  // gaps by group; disks; regions whose inverses take the LU
+    std::vector<zd> dk0;                      // disks' k0^2 [permittivity][frequency]
+    std::vector<SeriesKey> fresh, keys;
+    std::vector<SeriesKeyId> fresh_key, key_ids;
+    std::vector<FlatSlice> slices;
+    std::vector<ChainStep> steps;
+    std::vector<ChainJob> jobs;
+    ParsevalQueue direct;  // gaps whose (frequency, bin) the series does not serve
+    int nrows = 0;
+    bool ginv = false;
+};
+
+// Host preparation of geometries confs[c0, c1) (matrices at P, chunk-relative indices).  False if the
+// mode set's tables do not allow the flat path (then nothing was queued).
+static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, ModeCache& mc, const bf_setup* setups,
+                         const std::vector<int>& confs, size_t c0, size_t c1, zd* P, FlatWork& fw) {
+    const ModesDev& md = pl->md[set];
+    const int n = md.n, nf = (int)pl->freqs.size();
+    const size_t nn = (size_t)n * n;
+    // regions: air gaps (Parseval) and disks, permittivities by bits
+    std::vector<FlatRegion> gaps;
+    std::vector<zd> gap_eps, disk_eps;
+    auto eps_index = [](std::vector<zd>& list, const zd& e) {
+        for (size_t i = 0; i < list.size(); i++)
+            if (std::memcmp(&list[i], &e, sizeof(zd)) == 0) return (int)i;
+        list.push_back(e);
+        return (int)list.size() - 1;
+    };
+    int64_t base = 0;
+    for (size_t q = c0; q < c1; q++) {
+        const bf_setup* sq = &setups[confs[q]];
+        const int nreg = sq->nreg;
+        const std::vector<zd> eps = eps_fixed(sq);
+        for (int r = 1; r < nreg; r++) {
+            zd e = eps[r];
+            if (e.re == 1.0) {
+                if (pl->kind == BF_PROP_PROPAGATOR && e.im == 0.0) e.im = -0.0;
+                gaps.push_back(FlatRegion{base + r, nreg, eps_index(gap_eps, e), sq->distance[r]});
+            } else {
+                fw.disks.push_back(FlatRegion{base + r, nreg, eps_index(disk_eps, e), sq->distance[r]});
+            }
+        }
+        const int64_t step0 = (int64_t)fw.steps.size();
+        append_chain_steps(eps, nreg, fw.steps);
+        for (int f = 0; f < nf; f++) {
+            ChainJob j;
+            j.step0 = step0;
+            j.nsteps = nreg - 1;
+            j.pbase = base + (int64_t)f * nreg;
+            j.out = (int64_t)((q - c0) * nf + f);
+            j.e0off = 0;
+            j.pad = 0;
+            fw.jobs.push_back(j);
+        }
+        base += (int64_t)nreg * nf;
+    }
+    const bool grouped = mode_tables(ctx, pl->g, md, mc, !fw.disks.empty(), !gaps.empty(), pl->diskR);
+    if (!grouped && !gaps.empty()) return false;
+    // k0^2 per permittivity and frequency
+    auto k0_table = [&](const std::vector<zd>& list) {
+        std::vector<zd> t(list.size() * (size_t)nf);
+        for (size_t v = 0; v < list.size(); v++)
+            for (int f = 0; f < nf; f++) {
+                const zd k0 = k0_of(list[v], 299792458.0 / pl->freqs[(size_t)f]);
+                t[v * nf + f] = cmul(k0, k0);
+            }
+        return t;
+    };
+    const std::vector<zd> gk0 = k0_table(gap_eps);
+    fw.dk0 = k0_table(disk_eps);
+    fw.ginv = !fw.disks.empty() && ginv_usable(ctx, mc, md);
+    if (!fw.ginv) fw.inv = fw.disks;
+    // the gaps by (permittivity, bin), dz = 0 apart; their keys per frequency: formed earlier in the run,
+    // formed now, or (too few gaps) the direct product
+    if (mc.series_w <= 0.0 && !gk0.empty()) {
+        double k2 = 0.0;
+        for (const zd& v : gk0) k2 = std::max(k2, std::hypot(v.re, v.im));
+        mc.series_w = 2.0 / std::sqrt(4.0 * k2 + mc.gk2max);
+    }
+    const double w = mc.series_w;
+    std::map<std::pair<int, int64_t>, std::vector<int>> groups;  // (permittivity, bin; zero: INT64_MIN) -> gaps
+    for (size_t i = 0; i < gaps.size(); i++) {
+        const double dz = gaps[i].dz;
+        const int64_t b =
+            dz == 0.0 ? INT64_MIN : (dz > 0.0 && dz / w < 1e15 ? (int64_t)std::floor(dz / w) : INT64_MAX);
+        groups[{gaps[i].slot, b}].push_back((int)i);
+    }
+    auto bits = [](double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, 8);
+        return b;
+    };
+    std::map<SeriesKeyId, int> key_index;
+    for (const auto& gkv : groups) {
+        const int v = gkv.first.first;
+        const int64_t b = gkv.first.second;
+        const int first = (int)fw.ent.size(), cnt = (int)gkv.second.size();
+        for (int i : gkv.second) fw.ent.push_back(gaps[(size_t)i]);
+        fw.inv.insert(fw.inv.end(), fw.ent.begin() + first, fw.ent.end());
+        for (int f = 0; f < nf; f++) {
+            const zd k0 = gk0[(size_t)v * nf + f];
+            int ki = -1;
+            if (b != INT64_MAX && k0.im <= 0.0 && std::isfinite(k0.re)) {
+                const SeriesKeyId id = b == INT64_MIN ? SeriesKeyId{0, 0, SERIES_ZERO_BIN}
+                                                      : SeriesKeyId{bits(k0.re), bits(k0.im), (uint64_t)b};
+                auto it = key_index.find(id);
+                if (it != key_index.end()) {
+                    ki = it->second;
+                } else {
+                    auto c = mc.series.find(id);
+                    SeriesKey sk;
+                    bool ok = false;
+                    if (c != mc.series.end()) {
+                        sk = SeriesKey{k0, c->second.c, c->second.h, c->second.q, 0, c->second.d};
+                        ok = true;
+                    } else if (series_key_for(mc, id, k0, b == INT64_MIN ? cnt * nf : cnt, sk)) {
+                        sk.row0 = fw.nrows;
+                        fw.nrows += sk.q;
+                        fw.fresh.push_back(sk);
+                        fw.fresh_key.push_back(id);
+                        ok = true;
+                    }
+                    if (ok) {
+                        ki = (int)fw.keys.size();
+                        key_index[id] = ki;
+                        fw.keys.push_back(sk);
+                        fw.key_ids.push_back(id);
+                    }
+                }
+            }
+            if (ki >= 0) {
+                for (int o = 0; o < cnt; o += SERIES_SLICE)
+                    fw.slices.push_back(FlatSlice{ki, first + o, std::min(SERIES_SLICE, cnt - o), f});
+            } else {
+                for (int i = first; i < first + cnt; i++) {
+                    const FlatRegion& R = fw.ent[(size_t)i];
+                    fw.direct.pd.push_back(PwDesc{k0, R.dz, 0, 0});
+                    fw.direct.dst.push_back(P + (size_t)(R.base + (int64_t)f * R.stride) * nn);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// The launches of a prepared flat chunk (uploads through the plan's stager)
+static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, ModeCache& mc, FlatWork& fw, zd* P,
+                        zd* Pinv, zd* dboost) {
+    hipStream_t s = ctx->stream;
+    const ModesDev& md = pl->md[set];
+    const int n = md.n, nf = (int)pl->freqs.size();
+    const size_t nn = (size_t)n * n;
+    const double scale = 1.0 / (double)pl->g.K();
+    Stager* st = &pl->stager;
+    // series coefficients of the new keys, then every key's device pointer
+    series_form(ctx, mc, n, scale, fw.fresh, fw.fresh_key, fw.nrows, st);
+    for (size_t k = 0; k < fw.keys.size(); k++)
+        if (!fw.keys[k].d) fw.keys[k].d = mc.series[fw.key_ids[k]].d;
+    if (!fw.slices.empty()) {
+        DevBuf& dent = scratch_buf(ctx, SB_FENT, sizeof(FlatRegion) * fw.ent.size());
+        DevBuf& dkey = scratch_buf(ctx, SB_SD, sizeof(SeriesKey) * fw.keys.size());
+        DevBuf& dsl = scratch_buf(ctx, SB_SSLICE, sizeof(FlatSlice) * fw.slices.size());
+        stage_upload(st, dent.p, fw.ent.data(), sizeof(FlatRegion) * fw.ent.size(), s);
+        stage_upload(st, dkey.p, fw.keys.data(), sizeof(SeriesKey) * fw.keys.size(), s);
+        stage_upload(st, dsl.p, fw.slices.data(), sizeof(FlatSlice) * fw.slices.size(), s);
+        TimedLaunch tl(ctx, "pseries", 0.0);
+        for (size_t s0 = 0; s0 < fw.slices.size(); s0 += 65535) {
+            const size_t ns = std::min<size_t>(65535, fw.slices.size() - s0);
+            hipLaunchKernelGGL(k_series_eval_flat, dim3((unsigned)((nn + 127) / 128), (unsigned)ns), dim3(128), 0, s,
+                               dsl.as<FlatSlice>() + s0, dent.as<FlatRegion>(), dkey.as<SeriesKey>(), P, (int)nn);
+            BF_CHECK_LAUNCH();
+        }
+    }
+    parseval_flush(ctx, mc, n, scale, fw.direct, st);
+    if (!fw.disks.empty()) {
+        DevBuf& dd = scratch_buf(ctx, SB_FDISK, sizeof(FlatRegion) * fw.disks.size());
+        DevBuf& dk = scratch_buf(ctx, SB_FK0, sizeof(zd) * fw.dk0.size());
+        stage_upload(st, dd.p, fw.disks.data(), sizeof(FlatRegion) * fw.disks.size(), s);
+        stage_upload(st, dk.p, fw.dk0.data(), sizeof(zd) * fw.dk0.size(), s);
+        for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
+            const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
+            hipLaunchKernelGGL(k_disk_flat, dim3((unsigned)nd, (unsigned)nf), dim3(256), 0, s,
+                               dd.as<FlatRegion>() + d0, dk.as<zd>(), nf, mc.G.as<zd>(),
+                               fw.ginv ? mc.Ginv.as<zd>() : nullptr, md.kt.as<zd>(), n, P, Pinv);
+            BF_CHECK_LAUNCH();
+        }
+    }
+    if (!fw.inv.empty()) {
+        std::vector<InvSel> bs(fw.inv.size());
+        for (size_t i = 0; i < fw.inv.size(); i++) bs[i] = InvSel{fw.inv[i].base, fw.inv[i].stride};
+        DevBuf& di = scratch_buf(ctx, SB_FINV, sizeof(InvSel) * bs.size());
+        stage_upload(st, di.p, bs.data(), sizeof(InvSel) * bs.size(), s);
+        TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)bs.size() * nf);
+        const size_t smem = 2 * sizeof(zd) * nn;
+        allow_lds((const void*)k_batched_inverse_wave, smem);
+        for (size_t i0 = 0; i0 < bs.size(); i0 += 65535) {
+            const size_t ni = std::min<size_t>(65535, bs.size() - i0);
+            hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)ni, (unsigned)nf), dim3(64), smem, s, P, Pinv,
+                               n, (const int*)nullptr, di.as<InvSel>() + i0);
+            BF_CHECK_LAUNCH();
+        }
+    }
+    inverse_and_chains(ctx, n, P, Pinv, std::vector<int>(), fw.jobs, fw.steps, pl->de0[set].as<zd>(), nullptr, dboost,
+                       nullptr, st);
+}
+
 void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,
                                 const int32_t* set_of, const int64_t* boost_off) {
     bf_ctx* ctx = pl->ctx;
@@ -2578,6 +3004,14 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
     // bytes of P + Pinv per chunk (option batch_bytes overrides, e.g. to exercise chunking on small cases)
     const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : (size_t)2 << 30;
     bool first_chunk = true;
+    // with timing on: the host's own work per chunk (lists, descriptors, launches) and its waits for a
+    // staging slot, as pseudo kernel classes "host_prep" and "host_wait"
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    double host_ms = 0.0, wait_ms = 0.0, geo_ms = 0.0;
+    int64_t nchunks = 0;
     for (int set = 0; set < (int)pl->md.size(); set++) {
         std::vector<int> confs;
         for (int c = 0; c < nconf; c++)
@@ -2590,6 +3024,7 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
         const int n = md.n;
         const size_t nn = (size_t)n * n;
         for (size_t c0 = 0; c0 < confs.size();) {
+            const auto h0 = clk::now();
             // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry); the
             // run's first chunk holds a few geometries only, so the device starts while the host prepares more
             const size_t chunk_budget = first_chunk ? budget / 16 : budget;
@@ -2604,6 +3039,27 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
             DevBuf& P = scratch_buf(ctx, SB_P, sizeof(zd) * nn * mats);
             DevBuf& Pinv = scratch_buf(ctx, SB_PINV, sizeof(zd) * nn * mats);
             DevBuf& dboost = scratch_buf(ctx, SB_BOOST, sizeof(zd) * n * nfreq * (c1 - c0));
+            // chunks of flat, untilted geometries: frequency-independent descriptors expanded on the device
+            FlatWork fw;
+            if (flat_chunk_ok(pl, md, setups, confs, c0, c1) &&
+                flat_prepare(ctx, pl, set, cache, setups, confs, c0, c1, P.as<zd>(), fw)) {
+                const auto h1 = clk::now();
+                pl->stager.begin_chunk();
+                const auto h2 = clk::now();
+                flat_launch(ctx, pl, set, cache, fw, P.as<zd>(), Pinv.as<zd>(), dboost.as<zd>());
+                for (size_t q = c0; q < c1; q++)
+                    BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]],
+                                          dboost.as<zd>() + (q - c0) * nfreq * n, sizeof(zd) * n * nfreq,
+                                          hipMemcpyDeviceToDevice, s));
+                pl->stager.end_chunk(s);
+                const auto h3 = clk::now();
+                host_ms += ms_since(h0, h1) + ms_since(h2, h3);
+                geo_ms += ms_since(h0, h1);
+                wait_ms += ms_since(h1, h2);
+                nchunks++;
+                c0 = c1;
+                continue;
+            }
             std::vector<int> sel;
             std::vector<ChainJob> jobs;
             std::vector<ChainStep> steps;
@@ -2640,7 +3096,9 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
             }
             // the chunk's uploads go through pinned staging, its kernels queue behind the previous chunk's
             // (every device buffer is used in stream order), and the host goes on to prepare the next chunk
+            const auto h1 = clk::now();
             pl->stager.begin_chunk();
+            const auto h2 = clk::now();
             parseval_flush(ctx, cache, n, 1.0 / (double)g.K(), pq, &pl->stager);
             // flat disks taken from the Gram matrix get their inverses from inv(G) with the disk kernel
             const bool use_ginv = ginv_usable(ctx, cache, md);
@@ -2660,8 +3118,24 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
                 BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
                                       sizeof(zd) * n * nfreq, hipMemcpyDeviceToDevice, s));
             pl->stager.end_chunk(s);
+            const auto h3 = clk::now();
+            host_ms += ms_since(h0, h1) + ms_since(h2, h3);
+            geo_ms += ms_since(h0, h1);
+            wait_ms += ms_since(h1, h2);
+            nchunks++;
             c0 = c1;
         }
+    }
+    if (ctx->timing) {
+        bf_kernel_timer& hp = ctx->timers["host_prep"];
+        hp.ms += host_ms;
+        hp.launches += nchunks;
+        bf_kernel_timer& hw = ctx->timers["host_wait"];
+        hw.ms += wait_ms;
+        hw.launches += nchunks;
+        bf_kernel_timer& hg = ctx->timers["host_geometries"];  // the part of host_prep spent per geometry
+        hg.ms += geo_ms;
+        hg.launches += nchunks;
     }
     BF_HIP(hipStreamSynchronize(s));
 }
