@@ -573,12 +573,18 @@ struct InvSel {
     int64_t base;
     int64_t stride;
 };
+// Gauss-Jordan inversion in place with partial (row) pivoting, one wave per
+// matrix held in LDS (n^2 complex): for each k the pivot row (first row of
+// largest |re| + |im| in column k, as izamax) is swapped in, scaled by the
+// reciprocal of the pivot, and column k is eliminated from every other row;
+// the row swaps are undone on the columns when the result is written.  n
+// steps of three barriers each, every lane busy in the eliminations.
 __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restrict__ P, zd* Pinv, int n,
                                                            const int* __restrict__ sel,
                                                            const InvSel* __restrict__ bs) {
     extern __shared__ unsigned char smem_raw[];
     zd* M = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
-    zd* X = M + n * n;                        // X[i * n + j]: row i of solution column j
+    __shared__ zd colk[INV_WAVE_MAX];
     __shared__ int piv[INV_WAVE_MAX], sig[INV_WAVE_MAX];
     const int lane = threadIdx.x;
     const int64_t b = bs ? bs[blockIdx.x].base + (int64_t)blockIdx.y * bs[blockIdx.x].stride
@@ -610,58 +616,34 @@ __global__ void __launch_bounds__(64) k_batched_inverse_wave(const zd* __restric
             }
         if (lane == 0) piv[k] = p;
         __syncthreads();
+        // r = 1 / pivot; row k scaled by r (its pivot entry becomes r); column k kept for the elimination
         const zd d = M[k + n * k];
-        for (int i = k + 1 + lane; i < n; i += 64) {
-            const zd v = M[i + n * k];
-            const double den = d.re * d.re + d.im * d.im;
-            M[i + n * k] = cmk<double>((v.re * d.re + v.im * d.im) / den, (v.im * d.re - v.re * d.im) / den);
-        }
+        const double den = d.re * d.re + d.im * d.im;
+        const zd r = cmk<double>(d.re / den, -d.im / den);
+        for (int i = lane; i < n; i += 64)
+            if (i != k) colk[i] = M[i + n * k];
+        for (int j = lane; j < n; j += 64) M[k + n * j] = j == k ? r : cmul(M[k + n * j], r);
         __syncthreads();
-        const int m = n - k - 1;
-        if (m > 0)
-            wave_each2d(lane, m, m, [&](int jj, int ii) {
-                const int i = k + 1 + ii, j = k + 1 + jj;
-                M[i + n * j] = csub(M[i + n * j], cmul(M[i + n * k], M[k + n * j]));
-            });
+        // row i != k: M[i][j] -= M_old[i][k] M[k][j] (j != k), M[i][k] = -M_old[i][k] r
+        wave_each2d(lane, n, n - 1, [&](int j, int ii) {
+            const int i = ii < k ? ii : ii + 1;
+            const zd f = colk[i];
+            M[i + n * j] = j == k ? csub(cmk<double>(0.0, 0.0), cmul(f, r)) : csub(M[i + n * j], cmul(f, M[k + n * j]));
+        });
         __syncthreads();
     }
-    // X = inv(P): the identity with the pivots' row swaps (the permutation sigma), then the unit-lower
-    // and the upper triangular solves, each as rank-1 updates of the rows still to solve, spread over
-    // all lanes (X[i][j] takes its forward terms in increasing k, as a row-by-row solve would)
+    // undo the row swaps on the columns, in reverse order: result column j is column sig[j]
     if (lane == 0) {
-        for (int i = 0; i < n; i++) sig[i] = i;
-        for (int k = 0; k < n; k++) {
+        for (int j = 0; j < n; j++) sig[j] = j;
+        for (int k = n - 1; k >= 0; k--) {
             const int t = sig[k];
             sig[k] = sig[piv[k]];
             sig[piv[k]] = t;
         }
     }
     __syncthreads();
-    wave_each2d(lane, n, n, [&](int i, int j) { X[i * n + j] = cmk<double>(sig[i] == j ? 1.0 : 0.0, 0.0); });
-    __syncthreads();
-    for (int k = 0; k + 1 < n; k++) {
-        wave_each2d(lane, n - k - 1, n, [&](int ii, int j) {
-            const int i = k + 1 + ii;
-            X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
-        });
-        __syncthreads();
-    }
-    for (int k = n - 1; k >= 0; k--) {
-        const zd d = M[k + n * k];
-        const double den = d.re * d.re + d.im * d.im;
-        for (int j = lane; j < n; j += 64) {
-            const zd x = X[k * n + j];
-            X[k * n + j] = cmk<double>((x.re * d.re + x.im * d.im) / den, (x.im * d.re - x.re * d.im) / den);
-        }
-        __syncthreads();
-        if (k > 0)
-            wave_each2d(lane, k, n, [&](int i, int j) {
-                X[i * n + j] = csub(X[i * n + j], cmul(M[i + n * k], X[k * n + j]));
-            });
-        __syncthreads();
-    }
     zd* dst = Pinv + (size_t)b * nn;
-    wave_each2d(lane, n, n, [&](int j, int i) { dst[j * n + i] = X[i * n + j]; });
+    wave_each2d(lane, n, n, [&](int j, int i) { dst[i + n * j] = M[i + n * sig[j]]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -1299,7 +1281,7 @@ static void launch_inverse(bf_ctx* ctx, const zd* P, zd* Pinv, int n, const int*
     hipStream_t s = ctx->stream;
     if (count == 0) return;
     if (n <= INV_WAVE_MAX) {
-        const size_t smem = 2 * sizeof(zd) * (size_t)n * n;
+        const size_t smem = sizeof(zd) * (size_t)n * n;
         allow_lds((const void*)k_batched_inverse_wave, smem);
         hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)count), dim3(64), smem, s, P, Pinv, n, dsel,
                            (const InvSel*)nullptr);
@@ -2968,7 +2950,7 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         DevBuf& di = scratch_buf(ctx, SB_FINV, sizeof(InvSel) * bs.size());
         stage_upload(st, di.p, bs.data(), sizeof(InvSel) * bs.size(), s);
         TimedLaunch tl(ctx, "inverse", (8.0 / 3.0 + 8.0) * (double)n * n * n * (double)bs.size() * nf);
-        const size_t smem = 2 * sizeof(zd) * nn;
+        const size_t smem = sizeof(zd) * nn;
         allow_lds((const void*)k_batched_inverse_wave, smem);
         for (size_t i0 = 0; i0 < bs.size(); i0 += 65535) {
             const size_t ni = std::min<size_t>(65535, bs.size() - i0);
