@@ -38,6 +38,11 @@ template <typename T>
 __host__ __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
     return cmk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
+// acc + a b as four fused multiply-adds (real part: acc.re + a.re b.re, then - a.im b.im)
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmac(cplx<T> acc, cplx<T> a, cplx<T> b) {
+    return cmk<T>(fma(-a.im, b.im, fma(a.re, b.re, acc.re)), fma(a.im, b.re, fma(a.re, b.im, acc.im)));
+}
 // a * conj(b)
 template <typename T>
 __host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {

@@ -727,7 +727,7 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
 // of T feeds the axion sum and the solves, and it updates by itself:
 //     T21' = (a T21 + b T22) P,    T22' = (b T21 + a T22) inv(P)
 // (16 n^3 flop per region instead of 64 n^3 for the whole 2n x 2n product).
-// Every output sums over k in the same order as k_chain.  Thread t owns row
+// Every output sums over k in increasing order (fused multiply-adds).  Thread t owns row
 // t % n and the columns t / n + c floor(256 / n) of both products (all 256
 // threads busy for n = 16, 25, 32, ...), forms U and V for its row on the fly
 // (threads t < n also the row's axion sum from the same operands), and loads
@@ -810,21 +810,26 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
         if (act) {
             const bool axion = cg == 0;
             zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
+            // columns beyond n read column n - 1 (their sums are never stored): no divergent branches;
+            // columns past ceil(n / ng) (none of them valid) are skipped block-uniformly
+            int jc[CS::COLS];
+            const int ncol = (n + ng - 1) / ng;
+#pragma unroll
+            for (int c = 0; c < CS::COLS; c++) jc[c] = min(cg + ng * c, n - 1) * n;
             for (int k = 0; k < n; k++) {
                 const zd t21 = T21[row + n * k], t22 = T22[row + n * k];
                 if (axion) {
                     const zd e = e0[k];
-                    s21 = cadd(s21, cmul(t21, e));
-                    s22 = cadd(s22, cmul(t22, e));
+                    s21 = cmac(s21, t21, e);
+                    s22 = cmac(s22, t22, e);
                 }
                 const zd u = cadd(cmul(st.a, t21), cmul(st.b, t22));
                 const zd v = cadd(cmul(st.b, t21), cmul(st.a, t22));
 #pragma unroll
                 for (int c = 0; c < CS::COLS; c++) {
-                    const int j = cg + ng * c;
-                    if (j < n) {
-                        o21[c] = cadd(o21[c], cmul(u, Pl[k + n * j]));
-                        o22[c] = cadd(o22[c], cmul(v, Pil[k + n * j]));
+                    if (c < ncol) {
+                        o21[c] = cmac(o21[c], u, Pl[k + jc[c]]);
+                        o22[c] = cmac(o22[c], v, Pil[k + jc[c]]);
                     }
                 }
             }
