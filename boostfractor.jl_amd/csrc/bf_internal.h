@@ -78,6 +78,10 @@ struct bf_kernel_timer {
 };
 
 namespace bf {
+// default bytes of P + inv(P) per chunk of a batched sweep: large chunks mean fewer, fuller launches
+// (measured on the 4096-geometry sweep: 2 GiB 418k, 4 GiB 450k, 12 GiB 475k, 24 GiB 481k points/s)
+constexpr size_t BF_DEFAULT_BATCH_BYTES = (size_t)16 << 30;
+
 // Switches of the engine's work reductions, launch shapes and chunk budgets.
 // Each context holds its own set (bf_ctx_set_option); the defaults are the
 // product configuration, the other values exist for A/B checks and tuning.
@@ -97,7 +101,7 @@ struct EngineOptions {
     int pg_wn = 0;            // Parseval product: waves along the matrix entries, 2 or 4 (0: by n)
     int parseval_series = 1;  // air gaps of one k0 with nearby spacings by a Taylor series in the spacing
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
-    int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: default 2 GiB)
+    int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)
 };
 // Options of the context whose call runs on this thread (defaults outside a call).
 const EngineOptions& opts();

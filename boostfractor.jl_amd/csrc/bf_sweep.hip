@@ -291,7 +291,7 @@ static void spline_eval(bf_spacing_itp* it, const std::vector<SplinePoint>& pts,
 
 // candidates per chunk so that P and inv(P) stay within `budget` bytes
 static int cand_chunk(const bf_spacing_itp* it, int ncand) {
-    const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : (size_t)2 << 30;
+    const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : BF_DEFAULT_BATCH_BYTES;
     const size_t per = (size_t)it->nf * it->nreg * it->n * it->n * sizeof(zd) * 2;
     return (int)std::max<size_t>(1, std::min<size_t>((size_t)ncand, budget / per));
 }

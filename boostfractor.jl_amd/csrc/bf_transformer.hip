@@ -725,72 +725,114 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
 
 // The chain in LDS for n <= CHAIN_LDS_MAX.  Only the bottom block row [T21 T22]
 // of T feeds the axion sum and the solves, and it updates by itself:
-//     T21' = (a T21 + b T22) P,    T22' = (b T21 + a T22) inv(P)
+//     T21' = U P,  T22' = V inv(P),  U = a T21 + b T22,  V = b T21 + a T22
 // (16 n^3 flop per region instead of 64 n^3 for the whole 2n x 2n product).
-// Every output sums over k in increasing order (fused multiply-adds).  Thread t owns row
-// t % n and the columns t / n + c floor(256 / n) of both products (all 256
-// threads busy for n = 16, 25, 32, ...), forms U and V for its row on the fly
-// (threads t < n also the row's axion sum from the same operands), and loads
-// the next region's P and inv(P) into registers while it computes, so each
-// region costs two barriers and no exposed HBM latency.  (Loading each k's
-// operands one k ahead as well measured slower: 230 instead of 140 VGPRs.)  LDS: T21,
-// T22, P, inv(P) (4 n^2 complex).  NM: the largest n of the instantiation
-// (its register arrays).
+// LDS holds U and V of the coming step, each entry formed once by the thread
+// that produced it from the previous step's products (T21 and T22 themselves
+// after the last step).  Since a + b = 1 (a = (n1 + n0) / (2 n1),
+// b = (n1 - n0) / (2 n1)), U + V = T21 + T22 and the axion sum
+// S (T21 + T22) e0 is S (U e0 + V e0): e0 sits in LDS as column n of both P
+// and inv(P), and whenever the thread grid has a spare column
+// (floor(256 / n) ceil(n / floor(256 / n)) > n: n = 9, 15, 25, ...) the
+// products form it on the way; otherwise the row's first column group adds it.
+// Every output sums over k in increasing order (fused multiply-adds).  Thread
+// t owns row t % n and the columns t / n + c floor(256 / n) of both products
+// (all 256 threads busy for n = 16, 25, 32, ...), and loads the next region's
+// P and inv(P) into registers while it computes, so each region costs two
+// barriers and no exposed HBM latency.  (Loading each k's operands one k ahead
+// as well measured slower: 230 instead of 140 VGPRs.)  LDS: U, V (n^2 complex
+// each), P, inv(P) (n (n + 1) complex each, column n = e0).  NM: the largest n
+// of the instantiation (its register arrays).
 constexpr int CHAIN_LDS_MAX = 48;
 
-template <int NM>
+// column groups and columns per thread of k_chain_lds for n modes
+__host__ __device__ constexpr int chain_groups(int n) { return 256 / n; }
+__host__ __device__ constexpr int chain_cols(int n) { return (n + chain_groups(n) - 1) / chain_groups(n); }
+// the largest n <= NM with chain_cols(n) == NC
+__host__ __device__ constexpr int chain_nmax(int NM, int NC) {
+    int m = 0;
+    for (int n = 1; n <= NM; n++)
+        if (chain_cols(n) == NC) m = n;
+    return m;
+}
+
+template <int NM, int NC>
 struct ChainShape {
-    static constexpr int COLS = (NM + 256 / NM - 1) / (256 / NM);  // columns per thread for n <= NM
     static constexpr bool PF = NM <= 32;  // next region's matrices through registers (else loaded after the barrier)
-    static constexpr int PPT = PF ? (2 * NM * NM + 255) / 256 : 1;  // P and inv(P) entries per thread
+    static constexpr int NX = chain_nmax(NM, NC);
+    static constexpr int PPT = PF ? (2 * NX * NX + 255) / 256 : 1;  // P and inv(P) entries per thread
 };
 
-template <int NM>
+// columns of P and inv(P) in the LDS of k_chain_lds: n, e0, and padding up to the thread grid's
+// floor(256 / n) ceil(n / floor(256 / n)) columns (read, never used)
+__host__ __device__ constexpr int chain_width(int n) {
+    return n + 1 > chain_groups(n) * chain_cols(n) ? n + 1 : chain_groups(n) * chain_cols(n);
+}
+// dynamic LDS of k_chain_lds: U, V, P, inv(P), then the solves' vectors and pivot scratch
+__host__ __device__ inline size_t chain_lds_bytes(int n) {
+    return sizeof(zd) * ((size_t)2 * n * n + (size_t)2 * n * chain_width(n) + 3 * n) +
+           (sizeof(double) + sizeof(int)) * 256 + sizeof(int) * n;
+}
+
+// NF > 0: the kernel for exactly n = NF modes (all LDS offsets immediate, the k loop unrolled)
+template <int NM, int NC, int NF>
 __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ jobs,
                                                  const ChainStep* __restrict__ steps, const zd* __restrict__ P,
-                                                 const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
+                                                 const zd* __restrict__ Pinv, int n_arg, const zd* __restrict__ e0_all,
                                                  const zd* __restrict__ reflect, zd* boost, zd* refl) {
-    using CS = ChainShape<NM>;
+    using CS = ChainShape<NM, NC>;
     extern __shared__ unsigned char smem_raw[];
+    const int n = NF > 0 ? NF : n_arg;
     const int nn = n * n;
-    zd* T21 = reinterpret_cast<zd*>(smem_raw);  // column-major n x n: [i + n k]
-    zd* T22 = T21 + nn;
-    zd* Pl = T22 + nn;  // P and inv(P) of the current region, contiguous
-    zd* Pil = Pl + nn;
-    __shared__ zd ab[CHAIN_LDS_MAX], e0[CHAIN_LDS_MAX], tmp[2 * CHAIN_LDS_MAX];
-    __shared__ double sval[256];
-    __shared__ int sidx[256];
-    __shared__ int piv[CHAIN_LDS_MAX];
+    zd* U = reinterpret_cast<zd*>(smem_raw);  // column-major n x n: [i + n k]; T21 after the last step
+    zd* V = U + nn;                           // T22 after the last step
+    const int nw = n * chain_width(n);
+    zd* Pl = V + nn;                          // P of the current region, n x chain_width(n), column n = e0
+    zd* Pil = Pl + nw;                        // inv(P), likewise
+    zd* ab = Pil + nw;                        // axion_beam
+    zd* tmp = ab + n;                         // right-hand sides of the solves
+    double* sval = reinterpret_cast<double*>(tmp + 2 * n);
+    int* sidx = reinterpret_cast<int*>(sval + 256);
+    int* piv = sidx + 256;
     const ChainJob job = jobs[blockIdx.x];
     const int tid = threadIdx.x;
-    const int ng = 256 / n, row = tid % n, cg = tid / n;
+    const int ng = 256 / n, row = tid % n, cg = tid / n;  // NC = ceil(n / ng)
     const bool act = cg < ng;
-    for (int e = tid; e < nn; e += 256) {
-        T21[e] = cmk<double>(0.0, 0.0);
-        T22[e] = cmk<double>((e % n) == (e / n) ? 1.0 : 0.0, 0.0);
+    const bool spare = ng * NC > n;                 // block-uniform
+    const int own = spare ? n - ng * (NC - 1) : 0;  // column group of the row's axion sum
+    {
+        // T21 = 0, T22 = I: U = b I, V = a I for the first step
+        const zd z = cmk<double>(0.0, 0.0);
+        zd du = z, dv = cmk<double>(1.0, 0.0);
+        if (job.nsteps > 0) {
+            du = steps[job.step0].b;
+            dv = steps[job.step0].a;
+        }
+        for (int e = tid; e < nn; e += 256) {
+            const bool d = (e % n) == (e / n);
+            U[e] = d ? du : z;
+            V[e] = d ? dv : z;
+        }
     }
-    for (int i = tid; i < n; i += 256) e0[i] = e0_all[job.e0off + i];
+    for (int i = tid; i < n; i += 256) Pl[nn + i] = Pil[nn + i] = e0_all[job.e0off + i];
     auto region = [&](int q, int r) -> const zd* {  // entry r of [P; inv(P)] of step q (r < 2 nn)
         const int reg = steps[job.step0 + q].reg;
         return r < nn ? P + (size_t)(job.pbase + reg) * nn + r : Pinv + (size_t)(job.pbase + reg) * nn + (r - nn);
     };
+    auto lds_at = [&](int r) -> int { return r < nn ? r : r + nw - nn; };  // entry r of [P; inv(P)] in Pl
     if (job.nsteps > 0)
-        for (int r = tid; r < 2 * nn; r += 256) Pl[r] = *region(0, r);
-    zd abr = cmk<double>(0.0, 0.0);  // axion_beam[tid] for tid < n
+        for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = *region(0, r);
+    zd abr = cmk<double>(0.0, 0.0);  // axion_beam[row] in the thread of column group `own`
     __syncthreads();
     for (int q = 0; q < job.nsteps; q++) {
         const ChainStep st = steps[job.step0 + q];
         const bool more = q + 1 < job.nsteps;
+        const ChainStep sn = more ? steps[job.step0 + q + 1] : st;
         zd pn[CS::PPT];
         if constexpr (CS::PF) {
             // every element assigned on every path, so the array stays in registers
-            const zd* nP = P;
-            const zd* nPi = Pinv;
-            if (more) {
-                const int reg = steps[job.step0 + q + 1].reg;
-                nP = P + (size_t)(job.pbase + reg) * nn;
-                nPi = Pinv + (size_t)(job.pbase + reg) * nn;
-            }
+            const zd* nP = P + (size_t)(job.pbase + sn.reg) * nn;
+            const zd* nPi = Pinv + (size_t)(job.pbase + sn.reg) * nn;
 #pragma unroll
             for (int i = 0; i < CS::PPT; i++) {
                 const int r = tid + 256 * i;
@@ -802,47 +844,63 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
                 pn[i] = v;
             }
         }
-        // T21' = U P, T22' = V inv(P), U = a T21 + b T22, V = b T21 + a T22; the row's first column group
-        // (thread tid = row < n) also forms axion_beam += S * (T21*e0 + T22*e0) from the same operands
-        zd o21[CS::COLS], o22[CS::COLS];
+        zd o21[NC], o22[NC];
 #pragma unroll
-        for (int c = 0; c < CS::COLS; c++) o21[c] = o22[c] = cmk<double>(0.0, 0.0);
+        for (int c = 0; c < NC; c++) o21[c] = o22[c] = cmk<double>(0.0, 0.0);
         if (act) {
-            const bool axion = cg == 0;
-            zd s21 = cmk<double>(0.0, 0.0), s22 = cmk<double>(0.0, 0.0);
-            // columns beyond n read column n - 1 (their sums are never stored): no divergent branches;
-            // columns past ceil(n / ng) (none of them valid) are skipped block-uniformly
-            int jc[CS::COLS];
-            const int ncol = (n + ng - 1) / ng;
+            // columns beyond n read column n (their sums are never stored): no divergent branches
+            // thread (row, cg) reads U[row, k], V[row, k] and column cg + ng c of P and inv(P): the
+            // columns beyond n (padding) give sums that are never stored
+            const zd* Ur = U + row;
+            const zd* Vr = V + row;
+            const zd* Pc = Pl + cg * n;
+            const int cs = ng * n;  // LDS distance between a thread's columns
+            auto kstep = [&](int k) {
+                const zd u = Ur[n * k], v = Vr[n * k];
+                zd p[NC], pi[NC];
 #pragma unroll
-            for (int c = 0; c < CS::COLS; c++) jc[c] = min(cg + ng * c, n - 1) * n;
-            for (int k = 0; k < n; k++) {
-                const zd t21 = T21[row + n * k], t22 = T22[row + n * k];
-                if (axion) {
-                    const zd e = e0[k];
-                    s21 = cmac(s21, t21, e);
-                    s22 = cmac(s22, t22, e);
+                for (int c = 0; c < NC; c++) {
+                    p[c] = Pc[k + cs * c];
+                    pi[c] = Pc[nw + k + cs * c];
                 }
-                const zd u = cadd(cmul(st.a, t21), cmul(st.b, t22));
-                const zd v = cadd(cmul(st.b, t21), cmul(st.a, t22));
 #pragma unroll
-                for (int c = 0; c < CS::COLS; c++) {
-                    if (c < ncol) {
-                        o21[c] = cmac(o21[c], u, Pl[k + jc[c]]);
-                        o22[c] = cmac(o22[c], v, Pil[k + jc[c]]);
-                    }
+                for (int c = 0; c < NC; c++) {
+                    o21[c] = cmac(o21[c], u, p[c]);
+                    o22[c] = cmac(o22[c], v, pi[c]);
                 }
+            };
+            if constexpr (NF > 0) {
+                constexpr int UK = NF % 5 == 0 ? 5 : (NF % 3 == 0 ? 3 : 1);
+#pragma unroll 1
+                for (int k0 = 0; k0 < NF; k0 += UK) {
+#pragma unroll
+                    for (int kk = 0; kk < UK; kk++) kstep(k0 + kk);
+                }
+            } else {
+#pragma unroll 2
+                for (int k = 0; k < n; k++) kstep(k);
             }
-            if (axion) abr = cadd(abr, cmul(st.S, cadd(s21, s22)));
+            if (spare) {
+                if (cg == own) abr = cadd(abr, cmul(st.S, cadd(o21[NC - 1], o22[NC - 1])));
+            } else if (cg == 0) {
+                zd s = cmk<double>(0.0, 0.0);
+                for (int k = 0; k < n; k++) s = cmac(s, cadd(U[row + n * k], V[row + n * k]), Pl[nn + k]);
+                abr = cadd(abr, cmul(st.S, s));
+            }
         }
         __syncthreads();
         if (act) {
 #pragma unroll
-            for (int c = 0; c < CS::COLS; c++) {
+            for (int c = 0; c < NC; c++) {
                 const int j = cg + ng * c;
                 if (j < n) {
-                    T21[row + n * j] = o21[c];
-                    T22[row + n * j] = o22[c];
+                    zd t21 = o21[c], t22 = o22[c];
+                    if (more) {
+                        t21 = cadd(cmul(sn.a, o21[c]), cmul(sn.b, o22[c]));
+                        t22 = cadd(cmul(sn.b, o21[c]), cmul(sn.a, o22[c]));
+                    }
+                    U[row + n * j] = t21;
+                    V[row + n * j] = t22;
                 }
             }
         }
@@ -850,14 +908,16 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
 #pragma unroll
             for (int i = 0; i < CS::PPT; i++) {
                 const int r = tid + 256 * i;
-                if (more && r < 2 * nn) Pl[r] = pn[i];
+                if (more && r < 2 * nn) Pl[lds_at(r)] = pn[i];
             }
         } else if (more) {
-            for (int r = tid; r < 2 * nn; r += 256) Pl[r] = *region(q + 1, r);
+            for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = *region(q + 1, r);
         }
         __syncthreads();
     }
-    if (tid < n) ab[tid] = abr;
+    if (act && cg == own) ab[row] = abr;
+    zd* T21 = U;
+    zd* T22 = V;
     // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
     if (reflect)
         for (int i = tid; i < n; i += 256) {
@@ -881,18 +941,52 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
 // the LDS chain for n <= CHAIN_LDS_MAX: one block per job
 static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
                              const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
-    const size_t smem = 4 * sizeof(zd) * (size_t)n * n;
+    const size_t smem = chain_lds_bytes(n);
     auto go = [&](const void* fn, auto kern) {
         allow_lds(fn, smem);
         hipLaunchKernelGGL(kern, dim3((unsigned)nj), dim3(256), smem, s, jobs, steps, P, Pinv, n, e0, reflect, boost,
                            refl);
     };
-    if (n <= 16)
-        go((const void*)k_chain_lds<16>, k_chain_lds<16>);
-    else if (n <= 32)
-        go((const void*)k_chain_lds<32>, k_chain_lds<32>);
-    else
-        go((const void*)k_chain_lds<CHAIN_LDS_MAX>, k_chain_lds<CHAIN_LDS_MAX>);
+#define BF_CHAIN_CASE(NM, NC)                                         \
+    case NC:                                                          \
+        go((const void*)k_chain_lds<NM, NC, 0>, k_chain_lds<NM, NC, 0>); \
+        break
+    const int nc = chain_cols(n);
+    switch (n) {  // the Bessel sets of the disk-position sweep (9, 15, 25) and 45 Laguerre-Gauss modes
+        case 9:
+            go((const void*)k_chain_lds<16, 1, 9>, k_chain_lds<16, 1, 9>);
+            break;
+        case 15:
+            go((const void*)k_chain_lds<16, 1, 15>, k_chain_lds<16, 1, 15>);
+            break;
+        case 25:
+            go((const void*)k_chain_lds<32, 3, 25>, k_chain_lds<32, 3, 25>);
+            break;
+        case 45:
+            go((const void*)k_chain_lds<CHAIN_LDS_MAX, 9, 45>, k_chain_lds<CHAIN_LDS_MAX, 9, 45>);
+            break;
+        default:
+            if (n <= 16) {
+                go((const void*)k_chain_lds<16, 1, 0>, k_chain_lds<16, 1, 0>);  // floor(256 / n) >= n: one column
+            } else if (n <= 32) {
+                switch (nc) {
+                    BF_CHAIN_CASE(32, 2);
+                    BF_CHAIN_CASE(32, 3);
+                    BF_CHAIN_CASE(32, 4);
+                    default: throw Error(BF_ERR_UNSUPPORTED, "chain column count");
+                }
+            } else {
+                switch (nc) {  // n = 33..48: 5, 6, 7, 9 or 10 columns
+                    BF_CHAIN_CASE(CHAIN_LDS_MAX, 5);
+                    BF_CHAIN_CASE(CHAIN_LDS_MAX, 6);
+                    BF_CHAIN_CASE(CHAIN_LDS_MAX, 7);
+                    BF_CHAIN_CASE(CHAIN_LDS_MAX, 9);
+                    BF_CHAIN_CASE(CHAIN_LDS_MAX, 10);
+                    default: throw Error(BF_ERR_UNSUPPORTED, "chain column count");
+                }
+            }
+    }
+#undef BF_CHAIN_CASE
     BF_CHECK_LAUNCH();
 }
 
@@ -2989,7 +3083,7 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
     if (pl->dboost.bytes < sizeof(zd) * total) pl->dboost.alloc(sizeof(zd) * total);
     pl->nboost = total;
     // bytes of P + Pinv per chunk (option batch_bytes overrides, e.g. to exercise chunking on small cases)
-    const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : (size_t)2 << 30;
+    const size_t budget = opts().batch_bytes > 0 ? (size_t)opts().batch_bytes : BF_DEFAULT_BATCH_BYTES;
     bool first_chunk = true;
     // with timing on: the host's own work per chunk (lists, descriptors, launches) and its waits for a
     // staging slot, as pseudo kernel classes "host_prep" and "host_wait"

@@ -113,7 +113,8 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * "parseval_groups", "disk_expansion" (transformer work reductions),
  * "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
  * (dancer launch shapes), "pg_wn" (2 or 4: Parseval product column blocks of
- * 128 or 256 matrix entries), "batch_bytes" (matrix budget of a sweep chunk).  Unknown names and negative
+ * 128 or 256 matrix entries), "batch_bytes" (bytes of P and inv(P) per
+ * chunk of a batched sweep; default 16 GiB).  Unknown names and negative
  * values return BF_ERR_ARG.  No option changes a result beyond rounding. */
 int bf_ctx_set_option(bf_ctx *ctx, const char *name, int64_t value);
 int bf_ctx_get_option(bf_ctx *ctx, const char *name, int64_t *value);

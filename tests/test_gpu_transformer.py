@@ -558,3 +558,38 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
     for c in range(len(geoms)):
         assert rel(out["flat"][c], out["general"][c]) < 1e-10, (c, rel(out["flat"][c], out["general"][c]))
         assert rel(out["flat_chunked"][c], out["general"][c]) < 1e-10, c
+
+
+# one n per LDS chain kernel shape: the fixed-n kernels (9, 15, 25, 45) and the generic ones with 1, 2, 3, 4,
+# 5, 6, 7, 9 and 10 columns per thread; 16, 32, 35, 42 and 45 have no spare column of the thread grid
+# (the axion sum takes its own loop), the others form it as column n of the products
+CHAIN_NS = [1, 4, 9, 15, 16, 20, 24, 25, 30, 32, 33, 35, 36, 40, 42, 44, 45, 48]
+
+
+@pytest.mark.parametrize("n", CHAIN_NS)
+def test_chain_every_shape_matches_oracle(n):
+    """boost and reflectivity from given propagation matrices (random, well conditioned) for every mode
+    count class of the chain kernels, against the oracle's chain"""
+    coords, sb = _flat_booster(N=16, ndisk=3)
+    modes = _lg_like_modes(coords, n, 0)
+    assert modes.n == n
+    rng = np.random.default_rng(500 + n)
+    nreg = len(sb.distance)
+    mats = [np.eye(n, dtype=complex)]
+    for _ in range(1, nreg):
+        ph = np.exp(2j * np.pi * rng.uniform(size=n))
+        X = np.eye(n) + 0.2 / np.sqrt(n) * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        mats.append(ph[:, None] * X)
+    emit = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    refl_in = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    boost, refl = B.transformer(sb.copy(), coords, modes, f=20e9, diskR=DISKR, emit=emit, reflect=refl_in,
+                                propagation_matrices=mats)
+    eps = sb.eps.copy()
+    eps[np.isnan(eps.real)] = 1e30
+    os_ = O.Setup(sb.distance, eps, None, sb.relative_tilt_x, sb.relative_tilt_y, None)
+    want_b, want_r = O.transformer(os_, ocoords(coords), modes.patterns_flat()[:, 0], modes.mode_kt.reshape(-1),
+                                   20e9, O.PROP_PROPAGATOR, prop_mats=mats, diskR=DISKR, emit=emit, reflect=refl_in)
+    sens = chain_sensitivity(mats, eps, emit)
+    tol = max(TOL, 10 * sens)
+    assert tol < 1e-7
+    assert rel(boost, want_b) < tol and rel(refl, want_r) < tol, (rel(boost, want_b), rel(refl, want_r), sens)
