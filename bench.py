@@ -212,17 +212,20 @@ def transformer_cpu_baseline(coords, sb, pat, kt):
                       f"{n_disk} disk regions per point (FP64 C restatement of the reference transformer)"}
 
 
-MM_CLASSES = ("wgram", "pgemm", "pseries", "inverse", "chain")
+MM_CLASSES = ("wgram", "pgemm", "pseries", "sinverse", "inverse", "chain")
 MM_KERNELS = {
     "pgemm": ("k_parseval_gemm", "k_parseval_gemm (v_mfma_f64_16x16x4f64, weights exp(-i kz dz)/K formed in "
               "registers or read from the spacing series' table): flat air regions, P = sum over k-bin groups of "
               "W_g H_g"),
     "wgram": ("k_wgram", "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)"),
-    "inverse": ("k_batched_inverse_wave", "k_batched_inverse_wave (FP64 LU with pivoting and triangular solves, one "
-                "wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
+    "inverse": ("k_batched_inverse_wave", "k_batched_inverse_wave (FP64 Gauss-Jordan with partial pivoting in place in "
+                "LDS, one wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
+    "sinverse": ("k_series_inverse", "k_series_inverse (coefficients of the gaps' inverse series, E_0 = inv(D_0), "
+                 "E_p = -E_0 sum_j D_j E_(p-j), one block per key)"),
     "chain": ("k_chain_lds", "k_chain_lds (bottom block row of the transfer-matrix chain in LDS, 16 n^3 flop per "
               "region; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
-    "pseries": ("k_series_eval", "k_series_coef / k_series_eval (spacing-series weights and Horner sums)"),
+    "pseries": ("k_series_eval", "k_series_coef / k_series_eval (spacing-series weights and Horner sums of the "
+                "gaps' matrices and inverses)"),
 }
 
 

@@ -310,6 +310,53 @@ class Marshalled:
             surf = self.hold(as_c128(s))
         return Setup(len(dist), cptr(dist), cptr(r), cptr(eps), cptr(tx), cptr(ty), cptr(surf))
 
+    def setups(self, bdrys, ny, nx):
+        """ctypes array of Setup for many geometries: each field of all of them in one buffer, the structs
+        filled at once in a numpy array laid out like Setup (the same structs as setup() per geometry)"""
+        n = len(bdrys)
+        arr = (Setup * max(n, 1))()
+        if n == 0:
+            return arr
+        dists = [np.asarray(b.distance, dtype=np.float64).reshape(-1) for b in bdrys]
+        lens = np.array([len(d) for d in dists], dtype=np.int64)
+        rs = [np.asarray(b.r, dtype=np.complex128).reshape(-1) for b in bdrys]
+        rlens = np.array([len(r) for r in rs], dtype=np.int64)
+        dist = self.hold(np.concatenate(dists))
+        eps = self.hold(np.concatenate([np.asarray(b.eps, dtype=np.complex128).reshape(-1) for b in bdrys]))
+        if len(eps) != len(dist):
+            raise DimensionMismatch("eps and distance differ in length")
+        r = self.hold(np.concatenate(rs))
+        def per_region(v, l):  # a tilt array of the geometry's region count, or a value broadcast to it
+            a = np.asarray(v, dtype=np.float64)
+            return a if a.shape == (l,) else np.broadcast_to(a, (l,))
+
+        tx = self.hold(np.concatenate([per_region(b.relative_tilt_x, l) for b, l in zip(bdrys, lens)]))
+        ty = self.hold(np.concatenate([per_region(b.relative_tilt_y, l) for b, l in zip(bdrys, lens)]))
+        off = np.concatenate(([0], np.cumsum(lens)[:-1]))
+        roff = np.concatenate(([0], np.cumsum(rlens)[:-1]))
+        f = Setup
+        rec = np.frombuffer(arr, dtype=np.dtype({
+            "names": ["nreg", "distance", "r", "eps", "tilt_x", "tilt_y", "surfaces"],
+            "formats": ["<i4", "<u8", "<u8", "<u8", "<u8", "<u8", "<u8"],
+            "offsets": [f.nreg.offset, f.distance.offset, f.r.offset, f.eps.offset, f.tilt_x.offset,
+                        f.tilt_y.offset, f.surfaces.offset],
+            "itemsize": ctypes.sizeof(Setup)}), count=n)
+        rec["nreg"] = lens
+        rec["distance"] = dist.ctypes.data + 8 * off
+        rec["r"] = (r.ctypes.data + 16 * roff) if len(r) else 0
+        rec["eps"] = eps.ctypes.data + 16 * off
+        rec["tilt_x"] = tx.ctypes.data + 8 * off
+        rec["tilt_y"] = ty.ctypes.data + 8 * off
+        for i, b in enumerate(bdrys):
+            s = np.asarray(b.relative_surfaces)
+            if s.size and any(np.asarray(s[q]).view(np.float64 if s.dtype == np.complex128 else s.dtype).any()
+                              for q in range(s.shape[0])):
+                if s.shape != (int(lens[i]), ny, nx):
+                    raise DimensionMismatch(
+                        f"relative_surfaces has shape {s.shape}, expected {(int(lens[i]), ny, nx)}")
+                rec["surfaces"][i] = self.hold(as_c128(s)).ctypes.data
+        return arr
+
     def modes(self, m):
         pat = self.hold(as_c128(m.patterns_flat()))
         kt = self.hold(as_c128(np.asarray(m.mode_kt).reshape(-1)))

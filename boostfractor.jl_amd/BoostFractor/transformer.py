@@ -196,8 +196,9 @@ def calc_propagation_matrices(bdry, coords, modes, f=10.0e9, prop=propagator, di
 
 
 def _nan_to_1e30(bdry):
-    bad = np.isnan(bdry.eps.real) | np.isnan(bdry.eps.imag)
-    bdry.eps[bad] = 1e30  # the reference mutates its argument (src/boost_fractor_transformer.jl:289)
+    bad = np.isnan(bdry.eps)  # complex: either part NaN
+    if bad.any():
+        bdry.eps[bad] = 1e30  # the reference mutates its argument (src/boost_fractor_transformer.jl:289)
 
 
 def transformer(bdry, coords, modes, f=10.0e9, velocity_x=0, prop=propagator, propagation_matrices=None, diskR=0.15,
@@ -358,7 +359,7 @@ class TransformerBatchPlan:
         m = L.Marshalled()
         for b in bdrys:
             _nan_to_1e30(b)
-        setups = (L.Setup * max(nconf, 1))(*[m.setup(b, self.coords.ny, self.coords.nx) for b in bdrys])
+        setups = m.setups(bdrys, self.coords.ny, self.coords.nx)
         L.check(L.lib().bf_transformer_batch_plan_run(self.handle, nconf, setups, L.cptr(mode_set_of), L.cptr(offs)),
                 self.ctx.handle)
         self._layout = (offs, sizes)
@@ -405,7 +406,7 @@ def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=
     mds, emp = _batch_sources(m, coords, modesets, emits, diskR, float(freqs[0]))
     for b in bdrys:
         _nan_to_1e30(b)
-    setups = (L.Setup * max(nconf, 1))(*[m.setup(b, coords.ny, coords.nx) for b in bdrys])
+    setups = m.setups(bdrys, coords.ny, coords.nx)
     fr = m.hold(L.as_f64(freqs))
     nf = len(fr)
     out = np.zeros(int(sizes.sum()), dtype=np.complex128)

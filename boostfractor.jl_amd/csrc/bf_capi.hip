@@ -206,6 +206,9 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"parseval_groups", &bf::EngineOptions::parseval_groups},
         {"disk_expansion", &bf::EngineOptions::disk_expansion}, {"pg_wn", &bf::EngineOptions::pg_wn},
         {"autotune", &bf::EngineOptions::autotune}, {"parseval_series", &bf::EngineOptions::parseval_series},
+        {"series_inverse", &bf::EngineOptions::series_inverse},
+        {"chain_tile", &bf::EngineOptions::chain_tile},
+        {"chain_mfma", &bf::EngineOptions::chain_mfma},
     };
     ip = nullptr;
     for (const Int& e : ints)

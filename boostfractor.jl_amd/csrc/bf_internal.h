@@ -100,6 +100,10 @@ struct EngineOptions {
     int disk_expansion = 1;   // tilted/rough disks by a frequency series of moment Grams
     int pg_wn = 0;            // Parseval product: waves along the matrix entries, 2 or 4 (0: by n)
     int parseval_series = 1;  // air gaps of one k0 with nearby spacings by a Taylor series in the spacing
+    int series_inverse = 1;   // their inverses by the inverse's series (flat sweep chunks) instead of the LU
+    int chain_tile = 1;       // register-tiled chain kernels for n = 9, 15, 25, 45 (0: the LDS chain kernel)
+    int chain_mfma = 1;       // chain on the matrix cores: 1 where it pays, 2 / 3 for every n <= 32 (four / three
+                              // MFMAs per complex product), 0 never
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)
 };
@@ -228,7 +232,7 @@ inline void stage_upload(Stager* st, void* dst, const void* src, size_t bytes, h
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
     SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0,
-    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV
+    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV, SB_SINVJ, SB_SINVSEL, SB_FISLICE
 };
 }  // namespace bf
 

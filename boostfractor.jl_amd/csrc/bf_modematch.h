@@ -58,13 +58,34 @@ struct ModeCache {
         const zd* d;  // q coefficient matrices, nn apart
         int q;
         double c, h;
+        const zd* e = nullptr;  // q coefficient matrices of the inverse (flat path), nn apart, or none
     };
     std::map<std::array<uint64_t, 3>, SeriesEntry> series;
-    std::vector<DevBuf> series_blocks;
+    // device memory of the coefficients: blocks kept across runs and handed out again from the start by
+    // each run (kernels of consecutive runs are ordered on the stream), so a run allocates nothing once
+    // the blocks suffice
+    struct Arena {
+        std::vector<DevBuf> blocks;
+        size_t cur = 0, used = 0;
+        void reset() { cur = used = 0; }
+        void* take(size_t bytes) {
+            bytes = (bytes + 255) & ~(size_t)255;
+            for (; cur < blocks.size(); cur++, used = 0)
+                if (used + bytes <= blocks[cur].bytes) {
+                    void* p = static_cast<unsigned char*>(blocks[cur].p) + used;
+                    used += bytes;
+                    return p;
+                }
+            blocks.emplace_back(std::max(bytes, (size_t)64 << 20));
+            cur = blocks.size() - 1;
+            used = bytes;
+            return blocks[cur].p;
+        }
+    } series_mem;
     double series_w = 0.0;  // spacing bin width, fixed by the first flush of the run
     void reset_series() {
         series.clear();
-        series_blocks.clear();
+        series_mem.reset();
         series_w = 0.0;
     }
 };
@@ -175,6 +196,8 @@ struct bf_transformer_batch_plan {
     size_t nboost = 0;  // extent of the last run's results (complex elements)
     std::vector<std::pair<size_t, size_t>> blocks;  // the last run's (offset, length) per geometry
     bf::Stager stager;  // pinned staging of the chunks' descriptor uploads
+    bf::DevBuf dbad;    // int: a key's inverse series did not converge in this run
+    int64_t reruns = 0; // runs redone with the LU inverses for that reason
 };
 
 // cubic B-spline coefficients of a propagation-matrix grid over the spacing axis

@@ -774,15 +774,13 @@ __host__ __device__ inline size_t chain_lds_bytes(int n) {
            (sizeof(double) + sizeof(int)) * 256 + sizeof(int) * n;
 }
 
-// NF > 0: the kernel for exactly n = NF modes (all LDS offsets immediate, the k loop unrolled)
-template <int NM, int NC, int NF>
+template <int NM, int NC>
 __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ jobs,
                                                  const ChainStep* __restrict__ steps, const zd* __restrict__ P,
-                                                 const zd* __restrict__ Pinv, int n_arg, const zd* __restrict__ e0_all,
+                                                 const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
                                                  const zd* __restrict__ reflect, zd* boost, zd* refl) {
     using CS = ChainShape<NM, NC>;
     extern __shared__ unsigned char smem_raw[];
-    const int n = NF > 0 ? NF : n_arg;
     const int nn = n * n;
     zd* U = reinterpret_cast<zd*>(smem_raw);  // column-major n x n: [i + n k]; T21 after the last step
     zd* V = U + nn;                           // T22 after the last step
@@ -810,8 +808,8 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
         }
         for (int e = tid; e < nn; e += 256) {
             const bool d = (e % n) == (e / n);
-            U[e] = d ? du : z;
-            V[e] = d ? dv : z;
+            U[e] = cmk<double>(d ? du.re : 0.0, d ? du.im : 0.0);
+            V[e] = cmk<double>(d ? dv.re : 0.0, d ? dv.im : 0.0);
         }
     }
     for (int i = tid; i < n; i += 256) Pl[nn + i] = Pil[nn + i] = e0_all[job.e0off + i];
@@ -869,17 +867,8 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
                     o22[c] = cmac(o22[c], v, pi[c]);
                 }
             };
-            if constexpr (NF > 0) {
-                constexpr int UK = NF % 5 == 0 ? 5 : (NF % 3 == 0 ? 3 : 1);
-#pragma unroll 1
-                for (int k0 = 0; k0 < NF; k0 += UK) {
-#pragma unroll
-                    for (int kk = 0; kk < UK; kk++) kstep(k0 + kk);
-                }
-            } else {
 #pragma unroll 2
-                for (int k = 0; k < n; k++) kstep(k);
-            }
+            for (int k = 0; k < n; k++) kstep(k);
             if (spare) {
                 if (cg == own) abr = cadd(abr, cmul(st.S, cadd(o21[NC - 1], o22[NC - 1])));
             } else if (cg == 0) {
@@ -938,6 +927,427 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
     }
 }
 
+// The chain for the mode counts of the production sweeps (9, 15, 25, 45), register tiled.  k_chain_lds
+// is bound by LDS issue: per k a thread reads U and V of its row and C entries each of P and inv(P) for
+// 2 C complex multiply-adds.  Here thread t of NT owns rows rg + NRG r (r < R) and columns cg + NCG c
+// (c < C), rg = t % NRG, cg = t / NRG, of both products, and per k reads R entries each of U and V and
+// C each of P and inv(P) for 2 R C multiply-adds.  The columns run to NF (column NF of P and inv(P) is
+// e0: the axion sum); rows and columns past that are padding whose sums are never stored.  Layout,
+// steps and the order of every sum are those of k_chain_lds.
+template <int NF, int R, int C, int NT>
+struct ChainTile {
+    static constexpr int NRG = (NF + R - 1) / R;      // row groups
+    static constexpr int NCG = (NF + 1 + C - 1) / C;  // column groups over columns 0 .. NF
+    static constexpr int W = NCG * C;                 // LDS columns of P and inv(P)
+    static constexpr int ACT = NRG * NCG;             // threads with a tile
+    static constexpr int OWN_CG = NF % NCG, OWN_C = NF / NCG;  // tile holding column NF
+    static constexpr int PPT = (2 * NF * NF + NT - 1) / NT;    // next region's entries per thread
+    static constexpr int UK = NF % 5 == 0 ? 5 : (NF % 3 == 0 ? 3 : 1);  // k unroll
+    static_assert(ACT <= NT, "tile grid exceeds the block");
+    static constexpr size_t lds_bytes() {
+        return sizeof(zd) * ((size_t)2 * NF * NF + (size_t)2 * NF * W + 3 * NF) + (sizeof(double) + sizeof(int)) * NT +
+               sizeof(int) * NF;
+    }
+};
+
+template <int NF, int R, int C, int NT>
+__global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ jobs,
+                                                   const ChainStep* __restrict__ steps, const zd* __restrict__ P,
+                                                   const zd* __restrict__ Pinv, const zd* __restrict__ e0_all,
+                                                   const zd* __restrict__ reflect, zd* boost, zd* refl) {
+    using T = ChainTile<NF, R, C, NT>;
+    constexpr int n = NF, nn = NF * NF, nw = NF * T::W;
+    extern __shared__ unsigned char smem_raw[];
+    zd* U = reinterpret_cast<zd*>(smem_raw);  // column-major n x n; T21 after the last step
+    zd* V = U + nn;                           // T22 after the last step
+    zd* Pl = V + nn;                          // P of the current region, n x W, column n = e0
+    zd* Pil = Pl + nw;                        // inv(P), likewise
+    zd* ab = Pil + nw;
+    zd* tmp = ab + n;
+    double* sval = reinterpret_cast<double*>(tmp + 2 * n);
+    int* sidx = reinterpret_cast<int*>(sval + NT);
+    int* piv = sidx + NT;
+    const ChainJob job = jobs[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int rg = tid % T::NRG, cg = tid / T::NRG;
+    const bool act = tid < T::ACT;
+    {
+        const zd z = cmk<double>(0.0, 0.0);
+        zd du = z, dv = cmk<double>(1.0, 0.0);
+        if (job.nsteps > 0) {
+            du = steps[job.step0].b;
+            dv = steps[job.step0].a;
+        }
+        for (int e = tid; e < nn; e += NT) {
+            const bool d = (e % n) == (e / n);
+            U[e] = cmk<double>(d ? du.re : 0.0, d ? du.im : 0.0);
+            V[e] = cmk<double>(d ? dv.re : 0.0, d ? dv.im : 0.0);
+        }
+    }
+    for (int i = tid; i < n; i += NT) Pl[nn + i] = Pil[nn + i] = e0_all[job.e0off + i];
+    auto lds_at = [&](int r) -> int { return r < nn ? r : r + nw - nn; };
+    if (job.nsteps > 0) {
+        const int reg = steps[job.step0].reg;
+        for (int r = tid; r < 2 * nn; r += NT)
+            Pl[lds_at(r)] = r < nn ? P[(size_t)(job.pbase + reg) * nn + r] : Pinv[(size_t)(job.pbase + reg) * nn + r - nn];
+    }
+    zd abr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) abr[r] = cmk<double>(0.0, 0.0);
+    __syncthreads();
+    for (int q = 0; q < job.nsteps; q++) {
+        const ChainStep st = steps[job.step0 + q];
+        const bool more = q + 1 < job.nsteps;
+        const ChainStep sn = more ? steps[job.step0 + q + 1] : st;
+        zd pn[T::PPT];
+        {
+            const zd* nP = P + (size_t)(job.pbase + sn.reg) * nn;
+            const zd* nPi = Pinv + (size_t)(job.pbase + sn.reg) * nn;
+#pragma unroll
+            for (int i = 0; i < T::PPT; i++) {
+                const int r = tid + NT * i;
+                zd v = cmk<double>(0.0, 0.0);
+                if (more && r < nn)
+                    v = nP[r];
+                else if (more && r < 2 * nn)
+                    v = nPi[r - nn];
+                pn[i] = v;
+            }
+        }
+        zd o21[R][C], o22[R][C];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int c = 0; c < C; c++) o21[r][c] = o22[r][c] = cmk<double>(0.0, 0.0);
+        if (act) {
+            const zd* Ur = U + rg;
+            const zd* Vr = V + rg;
+            const zd* Pc = Pl + cg * n;
+#pragma unroll 1
+            for (int k0 = 0; k0 < n; k0 += T::UK) {
+#pragma unroll
+                for (int kk = 0; kk < T::UK; kk++) {
+                    const int k = k0 + kk;
+                    zd u[R], v[R], p[C], pi[C];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        u[r] = Ur[T::NRG * r + n * k];
+                        v[r] = Vr[T::NRG * r + n * k];
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        p[c] = Pc[k + T::NCG * n * c];
+                        pi[c] = Pc[nw + k + T::NCG * n * c];
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; r++)
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            o21[r][c] = cmac(o21[r][c], u[r], p[c]);
+                            o22[r][c] = cmac(o22[r][c], v[r], pi[c]);
+                        }
+                }
+            }
+            if (cg == T::OWN_CG) {
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    abr[r] = cadd(abr[r], cmul(st.S, cadd(o21[r][T::OWN_C], o22[r][T::OWN_C])));
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int i = rg + T::NRG * r, j = cg + T::NCG * c;
+                    if (i < n && j < n) {
+                        zd t21 = o21[r][c], t22 = o22[r][c];
+                        if (more) {
+                            t21 = cadd(cmul(sn.a, o21[r][c]), cmul(sn.b, o22[r][c]));
+                            t22 = cadd(cmul(sn.b, o21[r][c]), cmul(sn.a, o22[r][c]));
+                        }
+                        U[i + n * j] = t21;
+                        V[i + n * j] = t22;
+                    }
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < T::PPT; i++) {
+            const int r = tid + NT * i;
+            if (more && r < 2 * nn) Pl[lds_at(r)] = pn[i];
+        }
+        __syncthreads();
+    }
+    if (act && cg == T::OWN_CG) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = rg + T::NRG * r;
+            if (i < n) ab[i] = abr[r];
+        }
+    }
+    zd* T21 = U;
+    zd* T22 = V;
+    // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
+    if (reflect)
+        for (int i = tid; i < n; i += NT) {
+            zd s = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++) s = cadd(s, cmul(T21[i + n * k], reflect[k]));
+            tmp[n + i] = s;
+        }
+    __syncthreads();
+    lu_factor_block(T22, n, piv, sval, sidx);
+    if (tid == 0) {
+        for (int i = 0; i < n; i++) tmp[i] = ab[i];
+        lu_solve_col(T22, n, piv, tmp);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        if (reflect) {
+            lu_solve_col(T22, n, piv, tmp + n);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+        }
+    }
+}
+
+template <int NF, int R, int C, int NT>
+static void launch_chain_tile(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
+                              const zd* Pinv, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    const size_t smem = ChainTile<NF, R, C, NT>::lds_bytes();
+    allow_lds((const void*)k_chain_tile<NF, R, C, NT>, smem);
+    hipLaunchKernelGGL((k_chain_tile<NF, R, C, NT>), dim3((unsigned)nj), dim3(NT), smem, s, jobs, steps, P, Pinv, e0,
+                       reflect, boost, refl);
+}
+
+// The chain on the matrix cores for n <= 32 (v_mfma_f64_16x16x4f64).  The operands are zero-padded to
+// NP x NP (NP = 16 or 32); wave w of the (NP / 16)^2 owns output tile (w % (NP / 16), w / (NP / 16)) of
+// both products and forms it with four MFMAs per complex 4-k step (re += Ar Br + (-Ai) Bi,
+// im += Ar Bi + Ai Br), so per k step a lane reads one complex entry each of U, V, P and inv(P) for
+// 8 x 256 multiply-adds of the wave (the VALU kernels read 2 (R + C) entries for 2 R C).  U and V are
+// column-major, P and inv(P) transposed (both with leading dimension NP + 1: every operand read is 16
+// consecutive entries per k, conflict-free); column n of P and inv(P) is e0 when n < NP (the axion sum
+// comes out of the products), else the first wave adds it by a dot product.  The padding stays zero: U
+// and V are written for rows and columns below n only, P's rows and columns beyond n never.  Each MFMA
+// sums its four k in an order of its own, so the results agree with the VALU kernels to rounding.
+template <int NP>
+struct ChainMfma {
+    static constexpr int TW = NP / 16;        // tiles per dimension
+    static constexpr int NT = 64 * TW * TW;   // one wave per tile
+    static constexpr int LD = NP + 1;         // LDS leading dimension (entries)
+    static constexpr int PPT = NP * NP / NT;  // matrix entries per thread (n^2 <= NP^2)
+    static size_t lds_bytes(int n) {
+        return sizeof(zd) * ((size_t)4 * NP * LD + 3 * (size_t)n) + (sizeof(double) + sizeof(int)) * NT +
+               sizeof(int) * (size_t)n;
+    }
+};
+
+template <int NP, bool GAUSS>
+__global__ void __launch_bounds__(ChainMfma<NP>::NT) k_chain_mfma(const ChainJob* __restrict__ jobs,
+                                                                  const ChainStep* __restrict__ steps,
+                                                                  const zd* __restrict__ P, const zd* __restrict__ Pinv,
+                                                                  int n, const zd* __restrict__ e0_all,
+                                                                  const zd* __restrict__ reflect, zd* boost, zd* refl) {
+    using T = ChainMfma<NP>;
+    constexpr int LD = T::LD, NT = T::NT, TW = T::TW;
+    constexpr int SZ = NP * LD;
+    extern __shared__ unsigned char smem_raw[];
+    zd* U = reinterpret_cast<zd*>(smem_raw);  // U[i + LD k]
+    zd* V = U + SZ;
+    zd* Pt = V + SZ;  // Pt[j + LD k] = P[k][j]
+    zd* Pit = Pt + SZ;
+    zd* ab = Pit + SZ;
+    zd* tmp = ab + n;
+    double* sval = reinterpret_cast<double*>(tmp + 2 * n);
+    int* sidx = reinterpret_cast<int*>(sval + NT);
+    int* piv = sidx + NT;
+    const ChainJob job = jobs[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tr = wave % TW, tc = wave / TW;  // output tile
+    const int nn = n * n;
+    const bool spare = n < NP;
+    {
+        const zd z = cmk<double>(0.0, 0.0);
+        zd du = z, dv = cmk<double>(1.0, 0.0);
+        if (job.nsteps > 0) {
+            du = steps[job.step0].b;
+            dv = steps[job.step0].a;
+        }
+        for (int e = tid; e < SZ; e += NT) {
+            const int i = e % LD, k = e / LD;
+            const bool d = i == k && i < n;
+            U[e] = cmk<double>(d ? du.re : 0.0, d ? du.im : 0.0);
+            V[e] = cmk<double>(d ? dv.re : 0.0, d ? dv.im : 0.0);
+            Pt[e] = z;
+            Pit[e] = z;
+        }
+    }
+    __syncthreads();
+    if (spare)
+        for (int k = tid; k < n; k += NT) Pt[n + LD * k] = Pit[n + LD * k] = e0_all[job.e0off + k];
+    // the thread's entries of a region's matrices: global index g (column-major n x n), LDS index (transposed)
+    int gsrc[T::PPT], ldst[T::PPT];
+#pragma unroll
+    for (int m = 0; m < T::PPT; m++) {
+        const int g = tid + NT * m;
+        gsrc[m] = g < nn ? g : -1;
+        ldst[m] = g < nn ? (g / n) + LD * (g % n) : 0;
+    }
+    if (job.nsteps > 0) {
+        const size_t b = (size_t)(job.pbase + steps[job.step0].reg) * nn;
+#pragma unroll
+        for (int m = 0; m < T::PPT; m++)
+            if (gsrc[m] >= 0) {
+                Pt[ldst[m]] = P[b + gsrc[m]];
+                Pit[ldst[m]] = Pinv[b + gsrc[m]];
+            }
+    }
+    zd abr[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) abr[v] = cmk<double>(0.0, 0.0);
+    const bool own = spare && tc == n / 16 && (lane & 15) == n % 16;  // lanes holding column n
+    const int ar = tr * 16 + (lane & 15), bc = tc * 16 + (lane & 15), kq = lane >> 4;
+    const int kp = (n + 3) & ~3;  // k steps cover k < n
+    // matrices of region q arrive two regions ahead: region q + 1 sits in `cur` while region q runs, and
+    // the loads of region q + 2 go to `nxt` (the register sets alternate)
+    auto fetch = [&](int q, zd* dp, zd* dpi) {
+        const bool ok = q < job.nsteps;
+        const size_t b = ok ? (size_t)(job.pbase + steps[job.step0 + q].reg) * nn : 0;
+#pragma unroll
+        for (int m = 0; m < T::PPT; m++) {
+            dp[m] = dpi[m] = cmk<double>(0.0, 0.0);
+            if (ok && gsrc[m] >= 0) {
+                dp[m] = P[b + gsrc[m]];
+                dpi[m] = Pinv[b + gsrc[m]];
+            }
+        }
+    };
+    auto region = [&](int q, const zd* cur, const zd* curi, zd* nxt, zd* nxti) {
+        const ChainStep st = steps[job.step0 + q];
+        const bool more = q + 1 < job.nsteps;
+        const ChainStep sn = more ? steps[job.step0 + q + 1] : st;
+        fetch(q + 2, nxt, nxti);
+        d4 c21r = {0.0, 0.0, 0.0, 0.0}, c21i = c21r, c22r = c21r, c22i = c21r;
+        d4 g21 = c21r, g22 = c21r;  // GAUSS: sums of (Ar + Ai)(Br + Bi); c..r, c..i then hold Ar Br, Ai Bi
+#pragma unroll
+        for (int ks = 0; ks < NP; ks += 4) {
+            if (ks >= kp) break;  // k >= n are zero padding
+            const int k = ks + kq;
+            const zd u = U[ar + LD * k], v = V[ar + LD * k];
+            const zd p = Pt[bc + LD * k], pi = Pit[bc + LD * k];
+            if constexpr (GAUSS) {
+                c21r = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, p.re, c21r, 0, 0, 0);
+                c21i = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, p.im, c21i, 0, 0, 0);
+                g21 = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re + u.im, p.re + p.im, g21, 0, 0, 0);
+                c22r = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, pi.re, c22r, 0, 0, 0);
+                c22i = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, pi.im, c22i, 0, 0, 0);
+                g22 = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re + v.im, pi.re + pi.im, g22, 0, 0, 0);
+            } else {
+                c21r = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, p.re, c21r, 0, 0, 0);
+                c21i = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, p.im, c21i, 0, 0, 0);
+                c22r = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, pi.re, c22r, 0, 0, 0);
+                c22i = __builtin_amdgcn_mfma_f64_16x16x4f64(v.re, pi.im, c22i, 0, 0, 0);
+                c21r = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.im, p.im, c21r, 0, 0, 0);
+                c21i = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, p.re, c21i, 0, 0, 0);
+                c22r = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, pi.im, c22r, 0, 0, 0);
+                c22i = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, pi.re, c22i, 0, 0, 0);
+            }
+        }
+        if constexpr (GAUSS) {  // re = Ar Br - Ai Bi, im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double a1 = c21r[v], a2 = c21i[v], b1 = c22r[v], b2 = c22i[v];
+                c21r[v] = a1 - a2;
+                c21i[v] = g21[v] - a1 - a2;
+                c22r[v] = b1 - b2;
+                c22i[v] = g22[v] - b1 - b2;
+            }
+        }
+        // D layout: register v of lane holds D[4 v + lane / 16][lane % 16] of the tile
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                abr[v] = cadd(abr[v], cmul(st.S, cmk<double>(c21r[v] + c22r[v], c21i[v] + c22i[v])));
+        }
+        if (!spare && wave == 0 && lane < n) {
+            zd sacc = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++)
+                sacc = cmac(sacc, cadd(U[lane + LD * k], V[lane + LD * k]), e0_all[job.e0off + k]);
+            abr[0] = cadd(abr[0], cmul(st.S, sacc));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = tr * 16 + 4 * v + (lane >> 4), j = bc;
+            if (i < n && j < n) {
+                const zd o21 = cmk<double>(c21r[v], c21i[v]), o22 = cmk<double>(c22r[v], c22i[v]);
+                zd t21 = o21, t22 = o22;
+                if (more) {
+                    t21 = cadd(cmul(sn.a, o21), cmul(sn.b, o22));
+                    t22 = cadd(cmul(sn.b, o21), cmul(sn.a, o22));
+                }
+                U[i + LD * j] = t21;
+                V[i + LD * j] = t22;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < T::PPT; m++)
+            if (more && gsrc[m] >= 0) {
+                Pt[ldst[m]] = cur[m];
+                Pit[ldst[m]] = curi[m];
+            }
+        __syncthreads();
+    };
+    zd pa[T::PPT], pia[T::PPT], pb[T::PPT], pib[T::PPT];
+    fetch(1, pa, pia);
+    __syncthreads();
+    for (int q = 0; q < job.nsteps; q += 2) {
+        region(q, pa, pia, pb, pib);
+        if (q + 1 < job.nsteps) region(q + 1, pb, pib, pa, pia);
+    }
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = tr * 16 + 4 * v + (lane >> 4);
+            if (i < n) ab[i] = abr[v];
+        }
+    }
+    if (!spare && wave == 0 && lane < n) ab[lane] = abr[0];
+    // T21, T22 compacted to n x n (column-major) over the P tables for the solves
+    zd* T21 = Pt;
+    zd* T22 = Pit;
+    for (int e = tid; e < nn; e += NT) {
+        const int i = e % n, j = e / n;
+        T21[e] = U[i + LD * j];
+        T22[e] = V[i + LD * j];
+    }
+    __syncthreads();
+    // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
+    if (reflect)
+        for (int i = tid; i < n; i += NT) {
+            zd s = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++) s = cadd(s, cmul(T21[i + n * k], reflect[k]));
+            tmp[n + i] = s;
+        }
+    __syncthreads();
+    lu_factor_block(T22, n, piv, sval, sidx);
+    if (tid == 0) {
+        for (int i = 0; i < n; i++) tmp[i] = ab[i];
+        lu_solve_col(T22, n, piv, tmp);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        if (reflect) {
+            lu_solve_col(T22, n, piv, tmp + n);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+        }
+    }
+}
+
+template <int NP, bool GAUSS>
+static void launch_chain_mfma(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
+                              const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    const size_t smem = ChainMfma<NP>::lds_bytes(n);
+    allow_lds((const void*)k_chain_mfma<NP, GAUSS>, smem);
+    hipLaunchKernelGGL((k_chain_mfma<NP, GAUSS>), dim3((unsigned)nj), dim3(ChainMfma<NP>::NT), smem, s, jobs, steps,
+                       P, Pinv, n, e0, reflect, boost, refl);
+}
+
 // the LDS chain for n <= CHAIN_LDS_MAX: one block per job
 static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
                              const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
@@ -949,25 +1359,41 @@ static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, con
     };
 #define BF_CHAIN_CASE(NM, NC)                                         \
     case NC:                                                          \
-        go((const void*)k_chain_lds<NM, NC, 0>, k_chain_lds<NM, NC, 0>); \
+        go((const void*)k_chain_lds<NM, NC>, k_chain_lds<NM, NC>);       \
         break
     const int nc = chain_cols(n);
-    switch (n) {  // the Bessel sets of the disk-position sweep (9, 15, 25) and 45 Laguerre-Gauss modes
+    // matrix cores where the padding to 16 or 32 wastes little (measured on the sweep's Bessel sets: n = 15
+    // and 25 run faster on the MFMA kernel, n = 9 on the register-tiled one); option 2 / 3 force the
+    // four- / three-MFMA form for every n <= 32
+    const int mf = opts().chain_mfma;
+    const bool mfma_pays = (n > 12 && n <= 16) || (n > 24 && n <= 32);
+    if (n <= 32 && (mf >= 2 || (mf == 1 && mfma_pays))) {
+        const bool gauss = mf != 2;
+        if (n <= 16)
+            gauss ? launch_chain_mfma<16, true>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl)
+                  : launch_chain_mfma<16, false>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
+        else
+            gauss ? launch_chain_mfma<32, true>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl)
+                  : launch_chain_mfma<32, false>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
+        BF_CHECK_LAUNCH();
+        return;
+    }
+    switch (opts().chain_tile ? n : 0) {  // the Bessel sets of the disk-position sweep (9, 15, 25), 45 LG modes
         case 9:
-            go((const void*)k_chain_lds<16, 1, 9>, k_chain_lds<16, 1, 9>);
+            launch_chain_tile<9, 1, 2, 64>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
             break;
         case 15:
-            go((const void*)k_chain_lds<16, 1, 15>, k_chain_lds<16, 1, 15>);
+            launch_chain_tile<15, 2, 2, 64>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
             break;
         case 25:
-            go((const void*)k_chain_lds<32, 3, 25>, k_chain_lds<32, 3, 25>);
+            launch_chain_tile<25, 2, 3, 128>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
             break;
         case 45:
-            go((const void*)k_chain_lds<CHAIN_LDS_MAX, 9, 45>, k_chain_lds<CHAIN_LDS_MAX, 9, 45>);
+            launch_chain_tile<45, 3, 3, 256>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
             break;
         default:
             if (n <= 16) {
-                go((const void*)k_chain_lds<16, 1, 0>, k_chain_lds<16, 1, 0>);  // floor(256 / n) >= n: one column
+                go((const void*)k_chain_lds<16, 1>, k_chain_lds<16, 1>);  // floor(256 / n) >= n: one column
             } else if (n <= 32) {
                 switch (nc) {
                     BF_CHAIN_CASE(32, 2);
@@ -2010,6 +2436,7 @@ struct SeriesKey {
     double c, h;  // centre and half width of the bin
     int q, row0;  // terms, first coefficient row in the table being formed
     const zd* d;  // coefficient matrices (q of them, nn apart)
+    const zd* e;  // coefficient matrices of the inverse (q of them), or null
 };
 struct SeriesProb {
     zd* dst;
@@ -2064,6 +2491,101 @@ __global__ void __launch_bounds__(128) k_series_eval(const SeriesSlice* __restri
     }
 }
 
+// The inverse of a gap's matrix by the same kind of series.  With P(c + t h) = sum_p t^p D_p, the
+// coefficients of t^p in P(c + t h) inv(P(c + t h)) = I give inv(P(c + t h)) = sum_p t^p E_p with
+//     E_0 = inv(D_0),    E_p = -E_0 sum_{j=1..p} D_j E_{p-j},
+// q terms like P's own series.  P is entire in the spacing and invertible on and near the real axis,
+// so the E_p fall off like the D_p (|beta|^p / p!); the block checks that the last two are below
+// SERIES_INV_TAIL of E_0 (largest |re| + |im|) and otherwise raises *bad, so that the caller takes the LU
+// for the run instead.  One block per key: E_0 (from the wave inverse) in LDS, D_j and E_{p-j}
+// staged through LDS for each (p, j); every entry sums over j, then k, in increasing order.
+constexpr double SERIES_INV_TAIL = 1e-14;
+struct SeriesInvJob {
+    const zd* d;  // D_0 .. D_{q-1}
+    zd* e;        // E_0 (already inverted) .. E_{q-1}
+    int q, pad;
+};
+constexpr int SERIES_INV_EPT = (INV_WAVE_MAX * INV_WAVE_MAX + 255) / 256;  // entries per thread
+
+__global__ void __launch_bounds__(256) k_series_inverse(const SeriesInvJob* __restrict__ jobs, int n, int* bad) {
+    extern __shared__ unsigned char smem_raw[];
+    const int nn = n * n;
+    zd* E0 = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
+    zd* A = E0 + nn;                           // D_j, then the sum S
+    zd* Bm = A + nn;                           // E_{p-j}
+    __shared__ double red[256];
+    const SeriesInvJob J = jobs[blockIdx.x];
+    const int tid = threadIdx.x;
+    double m0 = 0.0, mt = 0.0;  // largest |re| + |im| of E_0 and of the last two terms
+    for (int e = tid; e < nn; e += 256) {
+        const zd v = J.e[e];
+        E0[e] = v;
+        m0 = fmax(m0, fabs(v.re) + fabs(v.im));
+    }
+    for (int p = 1; p < J.q; p++) {
+        zd acc[SERIES_INV_EPT];
+#pragma unroll
+        for (int r = 0; r < SERIES_INV_EPT; r++) acc[r] = cmk<double>(0.0, 0.0);
+        for (int j = 1; j <= p; j++) {
+            __syncthreads();
+            const zd* Dj = J.d + (size_t)j * nn;
+            const zd* Ep = J.e + (size_t)(p - j) * nn;
+            for (int e = tid; e < nn; e += 256) {
+                A[e] = Dj[e];
+                Bm[e] = Ep[e];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < SERIES_INV_EPT; r++) {
+                const int e = tid + 256 * r;
+                if (e < nn) {
+                    const int i = e % n, l = e / n;
+                    zd a = acc[r];
+                    for (int k = 0; k < n; k++) a = cmac(a, A[i + n * k], Bm[k + n * l]);
+                    acc[r] = a;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SERIES_INV_EPT; r++) {
+            const int e = tid + 256 * r;
+            if (e < nn) A[e] = acc[r];
+        }
+        __syncthreads();
+        zd* Eo = J.e + (size_t)p * nn;
+#pragma unroll
+        for (int r = 0; r < SERIES_INV_EPT; r++) {
+            const int e = tid + 256 * r;
+            if (e < nn) {
+                const int i = e % n, l = e / n;
+                zd a = cmk<double>(0.0, 0.0);
+                for (int k = 0; k < n; k++) a = cmac(a, E0[i + n * k], A[k + n * l]);
+                a = cmk<double>(-a.re, -a.im);
+                Eo[e] = a;
+                if (p >= J.q - 2) mt = fmax(mt, fabs(a.re) + fabs(a.im));
+            }
+        }
+        // E_p is read back by other threads for the next terms: the stores above are visible to the
+        // block after the barrier at the top of the next j loop
+    }
+    red[tid] = m0;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    const double e0max = red[0];
+    __syncthreads();
+    red[tid] = mt;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    if (tid == 0 && !(red[0] <= SERIES_INV_TAIL * e0max)) *bad = 1;
+}
+
 using SeriesKeyId = std::array<uint64_t, 3>;  // k0^2 bits (re, im) and spacing bin; {0, 0, ~0}: dz = 0
 constexpr uint64_t SERIES_ZERO_BIN = ~(uint64_t)0;
 
@@ -2089,15 +2611,16 @@ static bool series_key_for(const ModeCache& mc, const SeriesKeyId& key, zd k0, i
 }
 
 // The coefficient matrices of the keys `fresh` (rows row0 .. row0 + q - 1 of nrows): weight table, one
-// product on it, entries added to mc.series for the rest of the run
+// product on it, entries added to mc.series for the rest of the run.  With `bad` (a device flag), also
+// the inverse's coefficients (k_series_inverse, which raises the flag if they do not converge).
 static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const std::vector<SeriesKey>& fresh,
-                        const std::vector<SeriesKeyId>& fresh_key, int nrows, Stager* st) {
+                        const std::vector<SeriesKeyId>& fresh_key, int nrows, Stager* st, int* bad = nullptr) {
     if (fresh.empty()) return;
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n, NG = (size_t)mc.ngroups;
-    DevBuf blk(sizeof(zd) * nn * (size_t)nrows);
+    zd* blk = static_cast<zd*>(mc.series_mem.take(sizeof(zd) * nn * (size_t)nrows));
     std::vector<zd*> rowdst((size_t)nrows);
-    for (int r = 0; r < nrows; r++) rowdst[(size_t)r] = blk.as<zd>() + (size_t)r * nn;
+    for (int r = 0; r < nrows; r++) rowdst[(size_t)r] = blk + (size_t)r * nn;
     DevBuf& dkey = scratch_buf(ctx, SB_SKEY, sizeof(SeriesKey) * fresh.size());
     DevBuf& drow = scratch_buf(ctx, SB_SROWDST, sizeof(zd*) * (size_t)nrows);
     DevBuf& dw = scratch_buf(ctx, SB_SW, sizeof(zd) * NG * (size_t)nrows);
@@ -2116,10 +2639,37 @@ static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const s
         launch_parseval_gemm<true>(ctx, parseval_wn(ctx, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr, nullptr,
                                    (size_t)nrows, scale, drow.as<zd*>(), nmb, dw.as<zd>());
     }
+    zd* inv = nullptr;
+    std::vector<int> rows0(fresh.size());
+    std::vector<SeriesInvJob> ij(fresh.size());
+    if (bad) {
+        // E_0 = inv(D_0) by the wave inverse, then the further terms per key
+        inv = static_cast<zd*>(mc.series_mem.take(sizeof(zd) * nn * (size_t)nrows));
+        int qmax = 1;
+        for (size_t k = 0; k < fresh.size(); k++) {
+            rows0[k] = fresh[k].row0;
+            ij[k] = SeriesInvJob{blk + (size_t)fresh[k].row0 * nn, inv + (size_t)fresh[k].row0 * nn, fresh[k].q, 0};
+            qmax = std::max(qmax, fresh[k].q);
+        }
+        DevBuf& dsel = scratch_buf(ctx, SB_SINVSEL, sizeof(int) * rows0.size());
+        DevBuf& dj = scratch_buf(ctx, SB_SINVJ, sizeof(SeriesInvJob) * ij.size());
+        stage_upload(st, dsel.p, rows0.data(), sizeof(int) * rows0.size(), s);
+        stage_upload(st, dj.p, ij.data(), sizeof(SeriesInvJob) * ij.size(), s);
+        TimedLaunch tl(ctx, "sinverse", 8.0 * (double)nn * n * (double)fresh.size() * (1.0 + qmax * (qmax + 1) / 2));
+        const size_t smem1 = sizeof(zd) * nn;
+        allow_lds((const void*)k_batched_inverse_wave, smem1);
+        hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)fresh.size()), dim3(64), smem1, s, blk, inv, n,
+                           dsel.as<int>(), (const InvSel*)nullptr);
+        BF_CHECK_LAUNCH();
+        const size_t smem3 = 3 * sizeof(zd) * nn;
+        allow_lds((const void*)k_series_inverse, smem3);
+        hipLaunchKernelGGL(k_series_inverse, dim3((unsigned)fresh.size()), dim3(256), smem3, s,
+                           dj.as<SeriesInvJob>(), n, bad);
+        BF_CHECK_LAUNCH();
+    }
     for (size_t k = 0; k < fresh.size(); k++)
-        mc.series[fresh_key[k]] =
-            ModeCache::SeriesEntry{blk.as<zd>() + (size_t)fresh[k].row0 * nn, fresh[k].q, fresh[k].c, fresh[k].h};
-    mc.series_blocks.push_back(std::move(blk));
+        mc.series[fresh_key[k]] = ModeCache::SeriesEntry{blk + (size_t)fresh[k].row0 * nn, fresh[k].q, fresh[k].c,
+                                                         fresh[k].h, bad ? inv + (size_t)fresh[k].row0 * nn : nullptr};
     if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays go out of scope
 }
 
@@ -2794,6 +3344,8 @@ struct FlatSlice {  // series evaluation: entries [first, first + count) of one 
     int key, first, count, f;
 };
 
+// INV: the inverses from the inverse's coefficients (key.e) into P = inv(P)'s array
+template <bool INV>
 __global__ void __launch_bounds__(128) k_series_eval_flat(const FlatSlice* __restrict__ sl,
                                                           const FlatRegion* __restrict__ ent,
                                                           const SeriesKey* __restrict__ keys, zd* P, int nn) {
@@ -2801,9 +3353,10 @@ __global__ void __launch_bounds__(128) k_series_eval_flat(const FlatSlice* __res
     if (e >= nn) return;
     const FlatSlice S = sl[blockIdx.y];
     const SeriesKey k = keys[S.key];
+    const zd* coef = INV ? k.e : k.d;
     zd d[SERIES_QMAX];
 #pragma unroll
-    for (int p = 0; p < SERIES_QMAX; p++) d[p] = p < k.q ? k.d[(size_t)p * nn + e] : cmk<double>(0.0, 0.0);
+    for (int p = 0; p < SERIES_QMAX; p++) d[p] = p < k.q ? coef[(size_t)p * nn + e] : cmk<double>(0.0, 0.0);
     for (int i = 0; i < S.count; i++) {
         const FlatRegion R = ent[S.first + i];
         const double t = k.h > 0.0 ? (R.dz - k.c) / k.h : 0.0;
@@ -2860,12 +3413,13 @@ struct FlatWork {
     std::vector<zd> dk0;                      // disks' k0^2 [permittivity][frequency]
     std::vector<SeriesKey> fresh, keys;
     std::vector<SeriesKeyId> fresh_key, key_ids;
-    std::vector<FlatSlice> slices;
+    std::vector<FlatSlice> slices, islices;  // series evaluations of the matrices and of the inverses
     std::vector<ChainStep> steps;
     std::vector<ChainJob> jobs;
     ParsevalQueue direct;  // gaps whose (frequency, bin) the series does not serve
     int nrows = 0;
     bool ginv = false;
+    int* bad = nullptr;  // device flag of the inverse series (null: every gap's inverse by the LU)
 };
 
 // Host preparation of geometries confs[c0, c1) (matrices at P, chunk-relative indices).  False if the
@@ -2949,12 +3503,13 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
         return b;
     };
     std::map<SeriesKeyId, int> key_index;
+    std::vector<int> kf((size_t)nf);
     for (const auto& gkv : groups) {
         const int v = gkv.first.first;
         const int64_t b = gkv.first.second;
         const int first = (int)fw.ent.size(), cnt = (int)gkv.second.size();
         for (int i : gkv.second) fw.ent.push_back(gaps[(size_t)i]);
-        fw.inv.insert(fw.inv.end(), fw.ent.begin() + first, fw.ent.end());
+        bool inv_all = fw.bad != nullptr;  // every frequency's inverses from the series
         for (int f = 0; f < nf; f++) {
             const zd k0 = gk0[(size_t)v * nf + f];
             int ki = -1;
@@ -2969,7 +3524,7 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
                     SeriesKey sk;
                     bool ok = false;
                     if (c != mc.series.end()) {
-                        sk = SeriesKey{k0, c->second.c, c->second.h, c->second.q, 0, c->second.d};
+                        sk = SeriesKey{k0, c->second.c, c->second.h, c->second.q, 0, c->second.d, c->second.e};
                         ok = true;
                     } else if (series_key_for(mc, id, k0, b == INT64_MIN ? cnt * nf : cnt, sk)) {
                         sk.row0 = fw.nrows;
@@ -2986,6 +3541,10 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
                     }
                 }
             }
+            kf[(size_t)f] = ki;
+            // keys formed in this chunk get their inverse coefficients; earlier ones have them if they
+            // were formed by a flat chunk of this run
+            if (ki < 0 || (fw.keys[(size_t)ki].d && !fw.keys[(size_t)ki].e)) inv_all = false;
             if (ki >= 0) {
                 for (int o = 0; o < cnt; o += SERIES_SLICE)
                     fw.slices.push_back(FlatSlice{ki, first + o, std::min(SERIES_SLICE, cnt - o), f});
@@ -2996,6 +3555,13 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
                     fw.direct.dst.push_back(P + (size_t)(R.base + (int64_t)f * R.stride) * nn);
                 }
             }
+        }
+        if (inv_all) {
+            for (int f = 0; f < nf; f++)
+                for (int o = 0; o < cnt; o += SERIES_SLICE)
+                    fw.islices.push_back(FlatSlice{kf[(size_t)f], first + o, std::min(SERIES_SLICE, cnt - o), f});
+        } else {
+            fw.inv.insert(fw.inv.end(), fw.ent.begin() + first, fw.ent.end());
         }
     }
     return true;
@@ -3011,9 +3577,13 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
     const double scale = 1.0 / (double)pl->g.K();
     Stager* st = &pl->stager;
     // series coefficients of the new keys, then every key's device pointer
-    series_form(ctx, mc, n, scale, fw.fresh, fw.fresh_key, fw.nrows, st);
+    series_form(ctx, mc, n, scale, fw.fresh, fw.fresh_key, fw.nrows, st, fw.bad);
     for (size_t k = 0; k < fw.keys.size(); k++)
-        if (!fw.keys[k].d) fw.keys[k].d = mc.series[fw.key_ids[k]].d;
+        if (!fw.keys[k].d) {
+            const ModeCache::SeriesEntry& E = mc.series[fw.key_ids[k]];
+            fw.keys[k].d = E.d;
+            fw.keys[k].e = E.e;
+        }
     if (!fw.slices.empty()) {
         DevBuf& dent = scratch_buf(ctx, SB_FENT, sizeof(FlatRegion) * fw.ent.size());
         DevBuf& dkey = scratch_buf(ctx, SB_SD, sizeof(SeriesKey) * fw.keys.size());
@@ -3024,9 +3594,21 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         TimedLaunch tl(ctx, "pseries", 0.0);
         for (size_t s0 = 0; s0 < fw.slices.size(); s0 += 65535) {
             const size_t ns = std::min<size_t>(65535, fw.slices.size() - s0);
-            hipLaunchKernelGGL(k_series_eval_flat, dim3((unsigned)((nn + 127) / 128), (unsigned)ns), dim3(128), 0, s,
-                               dsl.as<FlatSlice>() + s0, dent.as<FlatRegion>(), dkey.as<SeriesKey>(), P, (int)nn);
+            hipLaunchKernelGGL(k_series_eval_flat<false>, dim3((unsigned)((nn + 127) / 128), (unsigned)ns), dim3(128),
+                               0, s, dsl.as<FlatSlice>() + s0, dent.as<FlatRegion>(), dkey.as<SeriesKey>(), P,
+                               (int)nn);
             BF_CHECK_LAUNCH();
+        }
+        if (!fw.islices.empty()) {
+            DevBuf& disl = scratch_buf(ctx, SB_FISLICE, sizeof(FlatSlice) * fw.islices.size());
+            stage_upload(st, disl.p, fw.islices.data(), sizeof(FlatSlice) * fw.islices.size(), s);
+            for (size_t s0 = 0; s0 < fw.islices.size(); s0 += 65535) {
+                const size_t ns = std::min<size_t>(65535, fw.islices.size() - s0);
+                hipLaunchKernelGGL(k_series_eval_flat<true>, dim3((unsigned)((nn + 127) / 128), (unsigned)ns),
+                                   dim3(128), 0, s, disl.as<FlatSlice>() + s0, dent.as<FlatRegion>(),
+                                   dkey.as<SeriesKey>(), Pinv, (int)nn);
+                BF_CHECK_LAUNCH();
+            }
         }
     }
     parseval_flush(ctx, mc, n, scale, fw.direct, st);
@@ -3062,8 +3644,10 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
                        nullptr, st);
 }
 
-void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,
-                                const int32_t* set_of, const int64_t* boost_off) {
+// one run; with `bad`, flat chunks take the gaps' inverses from the inverse series and raise *bad if one
+// of them did not converge
+static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups, const int32_t* set_of,
+                          const int64_t* boost_off, int* bad) {
     bf_ctx* ctx = pl->ctx;
     hipStream_t s = ctx->stream;
     const Grid& g = pl->g;
@@ -3107,8 +3691,9 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
         for (size_t c0 = 0; c0 < confs.size();) {
             const auto h0 = clk::now();
             // chunk [c0, c1): sum of nreg * nfreq matrices within the budget (at least one geometry); the
-            // run's first chunk holds a few geometries only, so the device starts while the host prepares more
-            const size_t chunk_budget = first_chunk ? budget / 16 : budget;
+            // run's first chunk holds a few geometries only (at most 256 MiB of matrices), so the device
+            // starts while the host prepares more
+            const size_t chunk_budget = first_chunk ? std::min(budget / 16, (size_t)256 << 20) : budget;
             size_t c1 = c0, mats = 0;
             while (c1 < confs.size()) {
                 const size_t m = (size_t)setups[confs[c1]].nreg * nfreq;
@@ -3122,6 +3707,7 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
             DevBuf& dboost = scratch_buf(ctx, SB_BOOST, sizeof(zd) * n * nfreq * (c1 - c0));
             // chunks of flat, untilted geometries: frequency-independent descriptors expanded on the device
             FlatWork fw;
+            fw.bad = bad;
             if (flat_chunk_ok(pl, md, setups, confs, c0, c1) &&
                 flat_prepare(ctx, pl, set, cache, setups, confs, c0, c1, P.as<zd>(), fw)) {
                 const auto h1 = clk::now();
@@ -3219,6 +3805,27 @@ void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const 
         hg.launches += nchunks;
     }
     BF_HIP(hipStreamSynchronize(s));
+}
+
+// A run with the inverse series (option series_inverse) checks their convergence once at its end and, if
+// a key's series fell short, is redone with every gap inverse by the LU.  series_inverse = 2 forces that
+// path (tests).
+void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,
+                                const int32_t* set_of, const int64_t* boost_off) {
+    const int mode = opts().series_inverse;
+    if (mode == 0) return plan_run_once(pl, nconf, setups, set_of, boost_off, nullptr);
+    hipStream_t s = pl->ctx->stream;
+    if (!pl->dbad.p) pl->dbad.alloc(sizeof(int));
+    const int zero = 0;
+    BF_HIP(hipMemcpyAsync(pl->dbad.p, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    plan_run_once(pl, nconf, setups, set_of, boost_off, pl->dbad.as<int>());
+    int bad = 0;
+    BF_HIP(hipMemcpyAsync(&bad, pl->dbad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    BF_HIP(hipStreamSynchronize(s));
+    if (bad || mode == 2) {
+        pl->reruns++;
+        plan_run_once(pl, nconf, setups, set_of, boost_off, nullptr);
+    }
 }
 
 // copies the last run's geometry blocks to the same offsets of `boost` (the

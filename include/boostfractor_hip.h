@@ -110,8 +110,14 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * k-space tables for complex64 fields), "band" (skip the rows the aperture
  * zeroes), "chfuse", "chxf", "chwave" (cheerleader pair fusion, stored row
  * transforms, wavefront order), "graph" (dancer plans replay HIP graphs),
- * "parseval_groups", "disk_expansion" (transformer work reductions),
- * "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
+ * "parseval_groups", "disk_expansion", "parseval_series" (transformer work
+ * reductions: k-bin groups, disks from the Gram matrix, air gaps by a series
+ * in the spacing), "series_inverse" (the gaps' inverses by the inverse's
+ * series in flat sweep chunks; a run whose series fails its convergence check
+ * is redone with LU inverses, which 2 forces), "chain_tile" (register-tiled
+ * chain kernels for n = 9, 15, 25, 45), "chain_mfma" (chain on the matrix
+ * cores: 1 where it pays, 2 / 3 for every n <= 32 with four / three MFMAs
+ * per complex product), "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
  * (dancer launch shapes), "pg_wn" (2 or 4: Parseval product column blocks of
  * 128 or 256 matrix entries), "batch_bytes" (bytes of P and inv(P) per
  * chunk of a batched sweep; default 16 GiB).  Unknown names and negative

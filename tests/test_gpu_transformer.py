@@ -504,17 +504,21 @@ def test_parseval_series_matches_direct(engine_options):
             assert rel(got[q][r], want) < TOL, (q, r, rel(got[q][r], want))
 
 
-def test_transformer_batch_plan_matches_single_geometry_256():
+@pytest.mark.parametrize("flat", [False, True])
+def test_transformer_batch_plan_matches_single_geometry_256(flat):
     """The sweep benchmark's shape (256 x 256, Bessel modes n = 9 and 25, boosters
     of 12 and 24 disks with spacings 8 mm +- 0.5 mm): the resident batch plan
-    against transformer per geometry, per point within 1e-10 of the boost vector
-    (or the chain's own sensitivity, whichever is larger)."""
+    against transformer per geometry, per point within 1e-10 of the boost vector.
+    flat=True (no surface arrays, the benchmark's form) takes the plan's flat
+    chunks: gap matrices and their inverses from the spacing series, disks from
+    G and inv(G); transformer forms every matrix by the direct product and every
+    inverse by the LU."""
     coords = W.grid_coords(256)
     rng = W.SplitMix64(5200)
     geoms = []
     for nd in (12, 24):
         sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
-        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp)))
+        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp), flat=flat))
     sets = [B.SeedModes(coords, ThreeDim=True, Mmax=3, Lmax=1, diskR=DISKR),
             B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=2, diskR=DISKR)]
     set_of = [1, 0]
@@ -533,9 +537,12 @@ def test_transformer_batch_plan_matches_single_geometry_256():
 
 def test_batch_plan_flat_chunks_match_general_path(engine_options):
     """chunks of flat, untilted geometries take the frequency-expanded path (gap
-    matrices from the spacing series, disks from inv(G)); with the series off
-    the same plan takes the general path (direct products): boosts agree to
-    1e-10, every geometry and frequency, with one chunk and with many"""
+    matrices and their inverses from the spacing series, disks from inv(G));
+    with the series off the same plan takes the general path (direct products,
+    LU inverses): boosts agree to 1e-10, every geometry and frequency, with one
+    chunk and with many, and with the gaps' inverses by the LU.  A run whose
+    inverse series fails its convergence check is redone with the LU inverses
+    (forced here by series_inverse=2): bitwise the series_inverse=0 result."""
     coords = W.grid_coords(64)
     rng = W.SplitMix64(1750)
     geoms = []
@@ -548,16 +555,19 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
     emits = [B.axion_induced_modes(coords, md, diskR=DISKR) for md in sets]
     freqs = np.linspace(19.5e9, 20.5e9, 20)
     out = {}
-    for name, kw in (("flat", {}), ("flat_chunked", dict(batch_bytes=2_000_000)), ("general", dict(parseval_series=0))):
+    for name, kw in (("flat", {}), ("flat_chunked", dict(batch_bytes=2_000_000)), ("general", dict(parseval_series=0)),
+                     ("flat_lu", dict(series_inverse=0)), ("flat_rerun", dict(series_inverse=2))):
         engine_options(**kw)
         plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
         plan.run([g.copy() for g in geoms], mode_set_of=set_of)
         out[name] = plan.fetch()
         plan.close()
-        engine_options(parseval_series=1, batch_bytes=0)
+        engine_options(parseval_series=1, batch_bytes=0, series_inverse=1)
     for c in range(len(geoms)):
         assert rel(out["flat"][c], out["general"][c]) < 1e-10, (c, rel(out["flat"][c], out["general"][c]))
         assert rel(out["flat_chunked"][c], out["general"][c]) < 1e-10, c
+        assert rel(out["flat_lu"][c], out["general"][c]) < 1e-10, c
+        assert np.array_equal(out["flat_rerun"][c], out["flat_lu"][c]), c
 
 
 # one n per LDS chain kernel shape: the fixed-n kernels (9, 15, 25, 45) and the generic ones with 1, 2, 3, 4,
@@ -593,3 +603,56 @@ def test_chain_every_shape_matches_oracle(n):
     tol = max(TOL, 10 * sens)
     assert tol < 1e-7
     assert rel(boost, want_b) < tol and rel(refl, want_r) < tol, (rel(boost, want_b), rel(refl, want_r), sens)
+
+
+@pytest.mark.parametrize("n", [9, 15, 25, 45])
+def test_chain_tiled_matches_lds_chain(engine_options, n):
+    """the register-tiled chain kernels (n = 9, 15, 25, 45) against the LDS chain kernel (option
+    chain_tile=0) on the same matrices: every sum runs in the same order, so 9, 15 and 25 agree bitwise;
+    for 45 (no spare column: the LDS kernel sums the axion term as one dot product of U + V) to 1e-13"""
+    coords, sb = _flat_booster(N=16, ndisk=4)
+    modes = _lg_like_modes(coords, n, 0)
+    rng = np.random.default_rng(900 + n)
+    mats = [np.eye(n, dtype=complex)]
+    for _ in range(1, len(sb.distance)):
+        ph = np.exp(2j * np.pi * rng.uniform(size=n))
+        X = np.eye(n) + 0.2 / np.sqrt(n) * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        mats.append(ph[:, None] * X)
+    emit = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    refl_in = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    out = {}
+    for tile in (1, 0):
+        engine_options(chain_tile=tile, chain_mfma=0)
+        out[tile] = B.transformer(sb.copy(), coords, modes, f=20e9, diskR=DISKR, emit=emit, reflect=refl_in,
+                                  propagation_matrices=mats)
+    for a, b in zip(out[1], out[0]):
+        if n == 45:
+            assert rel(a, b) < 1e-13, rel(a, b)
+        else:
+            assert np.array_equal(a, b), rel(a, b)
+
+
+@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("n", [1, 7, 9, 15, 16, 17, 25, 31, 32])
+def test_chain_mfma_matches_valu_chain(engine_options, n, form):
+    """the matrix-core chain forced for every n <= 32 (operands padded to 16 or 32; complex products by
+    four real MFMAs, option chain_mfma=2, or three, 3) against the VALU chain kernels (chain_mfma=0) on
+    the same matrices, boost and reflectivity to 1e-12 (each MFMA sums its four k in an order of its own);
+    16 and 32 fill the padded size (the axion sum by its own dot product)"""
+    coords, sb = _flat_booster(N=16, ndisk=4)
+    modes = _lg_like_modes(coords, n, 0)
+    rng = np.random.default_rng(950 + n)
+    mats = [np.eye(n, dtype=complex)]
+    for _ in range(1, len(sb.distance)):
+        ph = np.exp(2j * np.pi * rng.uniform(size=n))
+        X = np.eye(n) + 0.2 / np.sqrt(n) * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        mats.append(ph[:, None] * X)
+    emit = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    refl_in = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    out = {}
+    for mf in (form, 0):
+        engine_options(chain_mfma=mf)
+        out[mf] = B.transformer(sb.copy(), coords, modes, f=20e9, diskR=DISKR, emit=emit, reflect=refl_in,
+                                propagation_matrices=mats)
+    for a, b in zip(out[form], out[0]):
+        assert rel(a, b) < 1e-12, rel(a, b)

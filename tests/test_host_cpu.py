@@ -199,3 +199,37 @@ def test_batch_plan_validation_without_device():
     assert lib.bf_transformer_batch_plan_run(None, 0, None, None, None) == L.BF_ERR_ARG
     assert lib.bf_transformer_batch_plan_fetch(None, None) == L.BF_ERR_ARG
     assert lib.bf_transformer_batch_plan_destroy(None) == L.BF_OK
+
+
+def test_batched_setup_marshalling_matches_per_geometry():
+    """Marshalled.setups (one buffer per field for a batch of geometries) gives the same structs as
+    Marshalled.setup per geometry: region counts, and every array read back through its pointer;
+    a rough geometry keeps its surface array, flat ones pass none"""
+    coords = W.grid_coords(16)
+    geoms = [W.booster(coords, ndisk=3, seed=1), W.booster(coords, ndisk=7, seed=2, flat=True),
+             W.rough_booster(coords, ndisk=2, seed=3), W.booster(coords, ndisk=1, seed=4)]
+    geoms[3].relative_tilt_x = 1e-4  # a scalar tilt, broadcast over the regions
+    m = L.Marshalled()
+    batch = m.setups(geoms, coords.ny, coords.nx)
+    for i, g in enumerate(geoms):
+        m1 = L.Marshalled()  # holds the arrays `one` points into
+        one = m1.setup(g, coords.ny, coords.nx)
+        got = batch[i]
+        assert got.nreg == one.nreg
+        nreg = one.nreg
+
+        def read(p, count, dt):
+            if not p:
+                return None
+            buf = (ctypes.c_char * (count * np.dtype(dt).itemsize)).from_address(p)
+            return np.frombuffer(bytes(buf), dtype=dt)
+
+        for field, dt, cnt in (("distance", np.float64, nreg), ("eps", np.complex128, nreg),
+                               ("tilt_x", np.float64, nreg), ("tilt_y", np.float64, nreg),
+                               ("r", np.complex128, len(np.asarray(g.r).reshape(-1)))):
+            a, b = read(getattr(got, field), cnt, dt), read(getattr(one, field), cnt, dt)
+            assert np.array_equal(a, b, equal_nan=True), (i, field)
+        assert bool(got.surfaces) == bool(one.surfaces), i
+        if one.surfaces:
+            cnt = nreg * coords.ny * coords.nx
+            assert np.array_equal(read(got.surfaces, cnt, np.complex128), read(one.surfaces, cnt, np.complex128))
