@@ -232,7 +232,7 @@ inline void stage_upload(Stager* st, void* dst, const void* src, size_t bytes, h
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
     SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0,
-    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV, SB_SINVJ, SB_SINVSEL, SB_FISLICE
+    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV, SB_SINVJ, SB_SINVSEL, SB_FISLICE, SB_FPH
 };
 }  // namespace bf
 

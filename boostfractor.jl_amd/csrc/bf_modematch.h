@@ -122,10 +122,21 @@ void disk_diag_flush(bf_ctx* ctx, ModeCache& mc, const ModesDev& md, ParsevalQue
 
 // one step of the transfer-matrix chain (src/boost_fractor_transformer.jl:326-341)
 struct ChainStep {
-    zd S;     // axion_S_factor(sqrt(eps[idx(s+1)]), sqrt(eps[idx(s)]))
-    zd a, b;  // G = [a I, b I; b I, a I]
-    int reg;  // 0-based region index idx_reg(s) - 1
-    int pad;
+    zd S;      // axion_S_factor(sqrt(eps[idx(s+1)]), sqrt(eps[idx(s)]))
+    zd a, b;   // G = [a I, b I; b I, a I]
+    int reg;   // 0-based region index idx_reg(s) - 1
+    int disk;  // 1: the region's matrix is diag(ph) G and its inverse inv(G) diag(1/ph) (ChainSrc), else P, Pinv
+};
+
+// Where the chain kernels find a region's matrix and inverse: slot m (= job.pbase + step.reg) at P + m n^2
+// and Pinv + m n^2, or, for a step marked disk, entry (i, j) as ph_m[i] G[i][j] and Ginv[i][j] / ph_m[j]
+// with ph_m = ph + m n (the flat disks' phases exp(-1im kz dz), never formed as matrices)
+struct ChainSrc {
+    const zd* P;
+    const zd* Pinv;
+    const zd* G;
+    const zd* Ginv;
+    const zd* ph;
 };
 
 // One chain per block: the steps [step0, step0 + nsteps), propagation matrices
@@ -158,7 +169,8 @@ void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainS
 // inv(P) of the matrices `sel`, then one chain per job; boost (and refl) [njobs][n]
 void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
                         const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
-                        const zd* reflect, zd* boost, zd* refl, Stager* st = nullptr);
+                        const zd* reflect, zd* boost, zd* refl, Stager* st = nullptr,
+                        const ChainSrc* disks = nullptr);
 
 }  // namespace bf
 

@@ -723,6 +723,18 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
     }
 }
 
+// Entry g = i + n j of the matrix (inv = false) or of the inverse of matrix slot m of a chain step; a
+// disk step forms it from G, Ginv and the slot's phases with the arithmetic of k_disk_flat
+__device__ __forceinline__ zd chain_entry(const ChainSrc& S, int disk, int64_t m, int n, int g, int i, int j,
+                                         bool inv) {
+    if (!disk) return inv ? S.Pinv[m * n * n + g] : S.P[m * n * n + g];
+    const zd* ph = S.ph + m * n;
+    if (!inv) return cmul(ph[i], S.G[g]);
+    const zd a = S.Ginv[g], d = ph[j];
+    const double den = d.re * d.re + d.im * d.im;
+    return cmk<double>((a.re * d.re + a.im * d.im) / den, (a.im * d.re - a.re * d.im) / den);
+}
+
 // The chain in LDS for n <= CHAIN_LDS_MAX.  Only the bottom block row [T21 T22]
 // of T feeds the axion sum and the solves, and it updates by itself:
 //     T21' = U P,  T22' = V inv(P),  U = a T21 + b T22,  V = b T21 + a T22
@@ -776,9 +788,9 @@ __host__ __device__ inline size_t chain_lds_bytes(int n) {
 
 template <int NM, int NC>
 __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ jobs,
-                                                 const ChainStep* __restrict__ steps, const zd* __restrict__ P,
-                                                 const zd* __restrict__ Pinv, int n, const zd* __restrict__ e0_all,
-                                                 const zd* __restrict__ reflect, zd* boost, zd* refl) {
+                                                 const ChainStep* __restrict__ steps, const ChainSrc src, int n,
+                                                 const zd* __restrict__ e0_all, const zd* __restrict__ reflect,
+                                                 zd* boost, zd* refl) {
     using CS = ChainShape<NM, NC>;
     extern __shared__ unsigned char smem_raw[];
     const int nn = n * n;
@@ -813,13 +825,17 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
         }
     }
     for (int i = tid; i < n; i += 256) Pl[nn + i] = Pil[nn + i] = e0_all[job.e0off + i];
-    auto region = [&](int q, int r) -> const zd* {  // entry r of [P; inv(P)] of step q (r < 2 nn)
-        const int reg = steps[job.step0 + q].reg;
-        return r < nn ? P + (size_t)(job.pbase + reg) * nn + r : Pinv + (size_t)(job.pbase + reg) * nn + (r - nn);
+    // entry r < 2 nn of [P; inv(P)] of step q
+    auto entry = [&](const ChainStep& sq, int r) -> zd {
+        const bool inv = r >= nn;
+        const int g = inv ? r - nn : r;
+        return chain_entry(src, sq.disk, job.pbase + sq.reg, n, g, g % n, g / n, inv);
     };
     auto lds_at = [&](int r) -> int { return r < nn ? r : r + nw - nn; };  // entry r of [P; inv(P)] in Pl
-    if (job.nsteps > 0)
-        for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = *region(0, r);
+    if (job.nsteps > 0) {
+        const ChainStep s0 = steps[job.step0];
+        for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = entry(s0, r);
+    }
     zd abr = cmk<double>(0.0, 0.0);  // axion_beam[row] in the thread of column group `own`
     __syncthreads();
     for (int q = 0; q < job.nsteps; q++) {
@@ -829,17 +845,10 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
         zd pn[CS::PPT];
         if constexpr (CS::PF) {
             // every element assigned on every path, so the array stays in registers
-            const zd* nP = P + (size_t)(job.pbase + sn.reg) * nn;
-            const zd* nPi = Pinv + (size_t)(job.pbase + sn.reg) * nn;
 #pragma unroll
             for (int i = 0; i < CS::PPT; i++) {
                 const int r = tid + 256 * i;
-                zd v = cmk<double>(0.0, 0.0);
-                if (more && r < nn)
-                    v = nP[r];
-                else if (more && r < 2 * nn)
-                    v = nPi[r - nn];
-                pn[i] = v;
+                pn[i] = more && r < 2 * nn ? entry(sn, r) : cmk<double>(0.0, 0.0);
             }
         }
         zd o21[NC], o22[NC];
@@ -900,7 +909,7 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
                 if (more && r < 2 * nn) Pl[lds_at(r)] = pn[i];
             }
         } else if (more) {
-            for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = *region(q + 1, r);
+            for (int r = tid; r < 2 * nn; r += 256) Pl[lds_at(r)] = entry(sn, r);
         }
         __syncthreads();
     }
@@ -952,9 +961,9 @@ struct ChainTile {
 
 template <int NF, int R, int C, int NT>
 __global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ jobs,
-                                                   const ChainStep* __restrict__ steps, const zd* __restrict__ P,
-                                                   const zd* __restrict__ Pinv, const zd* __restrict__ e0_all,
-                                                   const zd* __restrict__ reflect, zd* boost, zd* refl) {
+                                                   const ChainStep* __restrict__ steps, const ChainSrc src,
+                                                   const zd* __restrict__ e0_all, const zd* __restrict__ reflect,
+                                                   zd* boost, zd* refl) {
     using T = ChainTile<NF, R, C, NT>;
     constexpr int n = NF, nn = NF * NF, nw = NF * T::W;
     extern __shared__ unsigned char smem_raw[];
@@ -986,10 +995,14 @@ __global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ 
     }
     for (int i = tid; i < n; i += NT) Pl[nn + i] = Pil[nn + i] = e0_all[job.e0off + i];
     auto lds_at = [&](int r) -> int { return r < nn ? r : r + nw - nn; };
+    auto entry = [&](const ChainStep& sq, int r) -> zd {  // entry r < 2 nn of [P; inv(P)] of step sq
+        const bool inv = r >= nn;
+        const int g = inv ? r - nn : r;
+        return chain_entry(src, sq.disk, job.pbase + sq.reg, n, g, g % n, g / n, inv);
+    };
     if (job.nsteps > 0) {
-        const int reg = steps[job.step0].reg;
-        for (int r = tid; r < 2 * nn; r += NT)
-            Pl[lds_at(r)] = r < nn ? P[(size_t)(job.pbase + reg) * nn + r] : Pinv[(size_t)(job.pbase + reg) * nn + r - nn];
+        const ChainStep s0 = steps[job.step0];
+        for (int r = tid; r < 2 * nn; r += NT) Pl[lds_at(r)] = entry(s0, r);
     }
     zd abr[R];
 #pragma unroll
@@ -1000,19 +1013,10 @@ __global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ 
         const bool more = q + 1 < job.nsteps;
         const ChainStep sn = more ? steps[job.step0 + q + 1] : st;
         zd pn[T::PPT];
-        {
-            const zd* nP = P + (size_t)(job.pbase + sn.reg) * nn;
-            const zd* nPi = Pinv + (size_t)(job.pbase + sn.reg) * nn;
 #pragma unroll
-            for (int i = 0; i < T::PPT; i++) {
-                const int r = tid + NT * i;
-                zd v = cmk<double>(0.0, 0.0);
-                if (more && r < nn)
-                    v = nP[r];
-                else if (more && r < 2 * nn)
-                    v = nPi[r - nn];
-                pn[i] = v;
-            }
+        for (int i = 0; i < T::PPT; i++) {
+            const int r = tid + NT * i;
+            pn[i] = more && r < 2 * nn ? entry(sn, r) : cmk<double>(0.0, 0.0);
         }
         zd o21[R][C], o22[R][C];
 #pragma unroll
@@ -1109,11 +1113,11 @@ __global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ 
 }
 
 template <int NF, int R, int C, int NT>
-static void launch_chain_tile(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
-                              const zd* Pinv, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+static void launch_chain_tile(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
+                              const ChainSrc& src, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
     const size_t smem = ChainTile<NF, R, C, NT>::lds_bytes();
     allow_lds((const void*)k_chain_tile<NF, R, C, NT>, smem);
-    hipLaunchKernelGGL((k_chain_tile<NF, R, C, NT>), dim3((unsigned)nj), dim3(NT), smem, s, jobs, steps, P, Pinv, e0,
+    hipLaunchKernelGGL((k_chain_tile<NF, R, C, NT>), dim3((unsigned)nj), dim3(NT), smem, s, jobs, steps, src, e0,
                        reflect, boost, refl);
 }
 
@@ -1140,10 +1144,10 @@ struct ChainMfma {
 };
 
 template <int NP, bool GAUSS>
-__global__ void __launch_bounds__(ChainMfma<NP>::NT) k_chain_mfma(const ChainJob* __restrict__ jobs,
+__global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const ChainJob* __restrict__ jobs,
                                                                   const ChainStep* __restrict__ steps,
-                                                                  const zd* __restrict__ P, const zd* __restrict__ Pinv,
-                                                                  int n, const zd* __restrict__ e0_all,
+                                                                  const ChainSrc src, int n,
+                                                                  const zd* __restrict__ e0_all,
                                                                   const zd* __restrict__ reflect, zd* boost, zd* refl) {
     using T = ChainMfma<NP>;
     constexpr int LD = T::LD, NT = T::NT, TW = T::TW;
@@ -1183,20 +1187,22 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT) k_chain_mfma(const ChainJob
     if (spare)
         for (int k = tid; k < n; k += NT) Pt[n + LD * k] = Pit[n + LD * k] = e0_all[job.e0off + k];
     // the thread's entries of a region's matrices: global index g (column-major n x n), LDS index (transposed)
-    int gsrc[T::PPT], ldst[T::PPT];
+    int gsrc[T::PPT], gi[T::PPT], gj[T::PPT], ldst[T::PPT];
 #pragma unroll
     for (int m = 0; m < T::PPT; m++) {
         const int g = tid + NT * m;
         gsrc[m] = g < nn ? g : -1;
-        ldst[m] = g < nn ? (g / n) + LD * (g % n) : 0;
+        gi[m] = g < nn ? g % n : 0;
+        gj[m] = g < nn ? g / n : 0;
+        ldst[m] = gj[m] + LD * gi[m];
     }
     if (job.nsteps > 0) {
-        const size_t b = (size_t)(job.pbase + steps[job.step0].reg) * nn;
+        const ChainStep s0 = steps[job.step0];
 #pragma unroll
         for (int m = 0; m < T::PPT; m++)
             if (gsrc[m] >= 0) {
-                Pt[ldst[m]] = P[b + gsrc[m]];
-                Pit[ldst[m]] = Pinv[b + gsrc[m]];
+                Pt[ldst[m]] = chain_entry(src, s0.disk, job.pbase + s0.reg, n, gsrc[m], gi[m], gj[m], false);
+                Pit[ldst[m]] = chain_entry(src, s0.disk, job.pbase + s0.reg, n, gsrc[m], gi[m], gj[m], true);
             }
     }
     zd abr[4];
@@ -1209,13 +1215,13 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT) k_chain_mfma(const ChainJob
     // the loads of region q + 2 go to `nxt` (the register sets alternate)
     auto fetch = [&](int q, zd* dp, zd* dpi) {
         const bool ok = q < job.nsteps;
-        const size_t b = ok ? (size_t)(job.pbase + steps[job.step0 + q].reg) * nn : 0;
+        const ChainStep sq = steps[job.step0 + (ok ? q : 0)];
 #pragma unroll
         for (int m = 0; m < T::PPT; m++) {
             dp[m] = dpi[m] = cmk<double>(0.0, 0.0);
             if (ok && gsrc[m] >= 0) {
-                dp[m] = P[b + gsrc[m]];
-                dpi[m] = Pinv[b + gsrc[m]];
+                dp[m] = chain_entry(src, sq.disk, job.pbase + sq.reg, n, gsrc[m], gi[m], gj[m], false);
+                dpi[m] = chain_entry(src, sq.disk, job.pbase + sq.reg, n, gsrc[m], gi[m], gj[m], true);
             }
         }
     };
@@ -1340,21 +1346,21 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT) k_chain_mfma(const ChainJob
 }
 
 template <int NP, bool GAUSS>
-static void launch_chain_mfma(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
-                              const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+static void launch_chain_mfma(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
+                              const ChainSrc& src, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
     const size_t smem = ChainMfma<NP>::lds_bytes(n);
     allow_lds((const void*)k_chain_mfma<NP, GAUSS>, smem);
     hipLaunchKernelGGL((k_chain_mfma<NP, GAUSS>), dim3((unsigned)nj), dim3(ChainMfma<NP>::NT), smem, s, jobs, steps,
-                       P, Pinv, n, e0, reflect, boost, refl);
+                       src, n, e0, reflect, boost, refl);
 }
 
 // the LDS chain for n <= CHAIN_LDS_MAX: one block per job
-static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps, const zd* P,
-                             const zd* Pinv, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
+                             const ChainSrc& src, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
     const size_t smem = chain_lds_bytes(n);
     auto go = [&](const void* fn, auto kern) {
         allow_lds(fn, smem);
-        hipLaunchKernelGGL(kern, dim3((unsigned)nj), dim3(256), smem, s, jobs, steps, P, Pinv, n, e0, reflect, boost,
+        hipLaunchKernelGGL(kern, dim3((unsigned)nj), dim3(256), smem, s, jobs, steps, src, n, e0, reflect, boost,
                            refl);
     };
 #define BF_CHAIN_CASE(NM, NC)                                         \
@@ -1370,26 +1376,26 @@ static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, con
     if (n <= 32 && (mf >= 2 || (mf == 1 && mfma_pays))) {
         const bool gauss = mf != 2;
         if (n <= 16)
-            gauss ? launch_chain_mfma<16, true>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl)
-                  : launch_chain_mfma<16, false>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
+            gauss ? launch_chain_mfma<16, true>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl)
+                  : launch_chain_mfma<16, false>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
         else
-            gauss ? launch_chain_mfma<32, true>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl)
-                  : launch_chain_mfma<32, false>(s, nj, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
+            gauss ? launch_chain_mfma<32, true>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl)
+                  : launch_chain_mfma<32, false>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
         BF_CHECK_LAUNCH();
         return;
     }
     switch (opts().chain_tile ? n : 0) {  // the Bessel sets of the disk-position sweep (9, 15, 25), 45 LG modes
         case 9:
-            launch_chain_tile<9, 1, 2, 64>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
+            launch_chain_tile<9, 1, 2, 64>(s, nj, jobs, steps, src, e0, reflect, boost, refl);
             break;
         case 15:
-            launch_chain_tile<15, 2, 2, 64>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
+            launch_chain_tile<15, 2, 2, 64>(s, nj, jobs, steps, src, e0, reflect, boost, refl);
             break;
         case 25:
-            launch_chain_tile<25, 2, 3, 128>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
+            launch_chain_tile<25, 2, 3, 128>(s, nj, jobs, steps, src, e0, reflect, boost, refl);
             break;
         case 45:
-            launch_chain_tile<45, 3, 3, 256>(s, nj, jobs, steps, P, Pinv, e0, reflect, boost, refl);
+            launch_chain_tile<45, 3, 3, 256>(s, nj, jobs, steps, src, e0, reflect, boost, refl);
             break;
         default:
             if (n <= 16) {
@@ -3037,7 +3043,7 @@ static ChainStep chain_step(const zd* eps, int nreg, int q) {
     c.a = cmul(sc, cadd(n1, n0));
     c.b = cmul(sc, csub(n1, n0));
     c.reg = ir;
-    c.pad = 0;
+    c.disk = 0;
     return c;
 }
 
@@ -3047,7 +3053,7 @@ void append_chain_steps(const std::vector<zd>& eps, int nreg, std::vector<ChainS
 
 void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<int>& sel,
                         const std::vector<ChainJob>& jobs, const std::vector<ChainStep>& steps, const zd* e0,
-                        const zd* reflect, zd* boost, zd* refl, Stager* st) {
+                        const zd* reflect, zd* boost, zd* refl, Stager* st, const ChainSrc* disks) {
     hipStream_t s = ctx->stream;
     const size_t nn = (size_t)n * n;
     const int N2 = 2 * n;
@@ -3071,9 +3077,11 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         // per region: the bottom block row (16 n^3 flop) or the whole 2n x 2n product (64 n^3 flop)
         TimedLaunch tl(ctx, "chain", flops);
         if (lds) {
-            launch_chain_lds(s, nj, djobs.as<ChainJob>(), dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, boost,
-                             refl);
+            const ChainSrc src{P, Pinv, disks ? disks->G : nullptr, disks ? disks->Ginv : nullptr,
+                               disks ? disks->ph : nullptr};
+            launch_chain_lds(s, nj, djobs.as<ChainJob>(), dsteps.as<ChainStep>(), src, n, e0, reflect, boost, refl);
         } else {
+            if (disks) throw Error(BF_ERR_UNSUPPORTED, "disk steps need the LDS chain");
             hipLaunchKernelGGL(k_chain, dim3((unsigned)nj), dim3(256), 0, s, djobs.as<ChainJob>(),
                                dsteps.as<ChainStep>(), P, Pinv, n, e0, reflect, Tb.as<zd>(), cw.as<zd>(), boost,
                                refl);
@@ -3158,7 +3166,8 @@ void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps_h, int n
     hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg);
     BF_CHECK_LAUNCH();
     launch_inverse(ctx, P, Pinv, n, sel, nsel);
-    launch_chain_lds(s, (size_t)nfreq, jobs, steps, P, Pinv, n, e0, reflect, boost, refl);
+    launch_chain_lds(s, (size_t)nfreq, jobs, steps, ChainSrc{P, Pinv, nullptr, nullptr, nullptr}, n, e0, reflect,
+                     boost, refl);
     BF_HIP(hipStreamSynchronize(s));
 }
 
@@ -3392,6 +3401,20 @@ __global__ void __launch_bounds__(256) k_disk_flat(const FlatRegion* __restrict_
     });
 }
 
+// flat disks formed inside the chain (ChainSrc): block (entry, frequency) writes the phases
+// exp(-1im*kz*dz) of the entry's matrix slot m = base + f stride at ph[m n + i], as k_disk_flat forms them
+__global__ void __launch_bounds__(64) k_disk_phases(const FlatRegion* __restrict__ ent, const zd* __restrict__ k0sq,
+                                                    int nf, const zd* __restrict__ kt, int n, zd* ph) {
+    const FlatRegion R = ent[blockIdx.x];
+    const int f = blockIdx.y;
+    const zd k2 = k0sq[(size_t)R.slot * nf + f];
+    zd* o = ph + (size_t)(R.base + (int64_t)f * R.stride) * n;
+    for (int i = threadIdx.x; i < n; i += 64) {
+        const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
+        o[i] = cexp_j(rmul(R.dz, neg_i(kz)));
+    }
+}
+
 // true if the flat chunk path serves geometries confs[c0, c1)
 static bool flat_chunk_ok(const bf_transformer_batch_plan* pl, const ModesDev& md, const bf_setup* setups,
                           const std::vector<int>& confs, size_t c0, size_t c1) {
@@ -3420,6 +3443,8 @@ struct FlatWork {
     int nrows = 0;
     bool ginv = false;
     int* bad = nullptr;  // device flag of the inverse series (null: every gap's inverse by the LU)
+    int64_t nmats = 0;   // matrix slots of the chunk
+    std::vector<char> step_disk;  // per chain step: its region is a disk
 };
 
 // Host preparation of geometries confs[c0, c1) (matrices at P, chunk-relative indices).  False if the
@@ -3454,6 +3479,8 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
         }
         const int64_t step0 = (int64_t)fw.steps.size();
         append_chain_steps(eps, nreg, fw.steps);
+        for (size_t k = (size_t)step0; k < fw.steps.size(); k++)
+            fw.step_disk.push_back(eps[fw.steps[k].reg].re != 1.0);
         for (int f = 0; f < nf; f++) {
             ChainJob j;
             j.step0 = step0;
@@ -3480,8 +3507,12 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
     };
     const std::vector<zd> gk0 = k0_table(gap_eps);
     fw.dk0 = k0_table(disk_eps);
+    fw.nmats = base;
     fw.ginv = !fw.disks.empty() && ginv_usable(ctx, mc, md);
     if (!fw.ginv) fw.inv = fw.disks;
+    // with inv(G), the chain forms each disk's matrix and inverse from G, inv(G) and its phases
+    if (fw.ginv)
+        for (size_t k = 0; k < fw.steps.size(); k++) fw.steps[k].disk = fw.step_disk[k];
     // the gaps by (permittivity, bin), dz = 0 apart; their keys per frequency: formed earlier in the run,
     // formed now, or (too few gaps) the direct product
     if (mc.series_w <= 0.0 && !gk0.empty()) {
@@ -3612,17 +3643,30 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         }
     }
     parseval_flush(ctx, mc, n, scale, fw.direct, st);
+    ChainSrc disk_src{P, Pinv, mc.G.as<zd>(), fw.ginv ? mc.Ginv.as<zd>() : nullptr, nullptr};
     if (!fw.disks.empty()) {
         DevBuf& dd = scratch_buf(ctx, SB_FDISK, sizeof(FlatRegion) * fw.disks.size());
         DevBuf& dk = scratch_buf(ctx, SB_FK0, sizeof(zd) * fw.dk0.size());
         stage_upload(st, dd.p, fw.disks.data(), sizeof(FlatRegion) * fw.disks.size(), s);
         stage_upload(st, dk.p, fw.dk0.data(), sizeof(zd) * fw.dk0.size(), s);
-        for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
-            const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
-            hipLaunchKernelGGL(k_disk_flat, dim3((unsigned)nd, (unsigned)nf), dim3(256), 0, s,
-                               dd.as<FlatRegion>() + d0, dk.as<zd>(), nf, mc.G.as<zd>(),
-                               fw.ginv ? mc.Ginv.as<zd>() : nullptr, md.kt.as<zd>(), n, P, Pinv);
-            BF_CHECK_LAUNCH();
+        if (fw.ginv) {
+            // phases only: the chain forms the disks' matrices and inverses from G and inv(G)
+            DevBuf& dph = scratch_buf(ctx, SB_FPH, sizeof(zd) * (size_t)fw.nmats * n);
+            disk_src.ph = dph.as<zd>();
+            for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
+                const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
+                hipLaunchKernelGGL(k_disk_phases, dim3((unsigned)nd, (unsigned)nf), dim3(64), 0, s,
+                                   dd.as<FlatRegion>() + d0, dk.as<zd>(), nf, md.kt.as<zd>(), n, dph.as<zd>());
+                BF_CHECK_LAUNCH();
+            }
+        } else {
+            for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
+                const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
+                hipLaunchKernelGGL(k_disk_flat, dim3((unsigned)nd, (unsigned)nf), dim3(256), 0, s,
+                                   dd.as<FlatRegion>() + d0, dk.as<zd>(), nf, mc.G.as<zd>(), nullptr,
+                                   md.kt.as<zd>(), n, P, Pinv);
+                BF_CHECK_LAUNCH();
+            }
         }
     }
     if (!fw.inv.empty()) {
@@ -3641,7 +3685,7 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         }
     }
     inverse_and_chains(ctx, n, P, Pinv, std::vector<int>(), fw.jobs, fw.steps, pl->de0[set].as<zd>(), nullptr, dboost,
-                       nullptr, st);
+                       nullptr, st, disk_src.ph ? &disk_src : nullptr);
 }
 
 // one run; with `bad`, flat chunks take the gaps' inverses from the inverse series and raise *bad if one
@@ -3938,7 +3982,7 @@ void trace_back_dev(bf_ctx* ctx, const double* reflected, const double* input, c
         c.a = cmul(sc, cadd(nr, nl));
         c.b = cmul(sc, csub(nr, nl));
         c.reg = ir;
-        c.pad = 0;
+        c.disk = 0;
         steps[q] = c;
     }
     std::vector<zd> sol0(N2);
