@@ -71,12 +71,29 @@ def pmc_per_kernel(db_path, counter):
     return {k: (v[0] / v[1], v[1]) for k, v in acc.items()}
 
 
+# kernel families of the mode-matching classes: a class's traffic is the mean over every dispatch of the
+# kernels whose names start with the family prefix (the chain class runs k_chain_mfma and k_chain_tile)
+PMC_FAMILIES = ("k_chain", "k_parseval_gemm", "k_series_eval_flat", "k_series_inverse", "k_batched_inverse_wave")
+
+
 def summarize_pmc(fetch_db, write_db):
     f = pmc_per_kernel(fetch_db, "FETCH_SIZE")
     w = pmc_per_kernel(write_db, "WRITE_SIZE")
     lines = ["| kernel | dispatches | FETCH_SIZE KiB/dispatch | WRITE_SIZE KiB/dispatch |", "|---|---|---|---|"]
     for k in sorted(f, key=lambda k: -f[k][0]):
         lines.append(f"| `{k}` | {f[k][1]} | {f[k][0]:.1f} | {w.get(k, (float('nan'), 0))[0]:.1f} |")
+    lines.append("")
+    for k in f:
+        if "k_stream_copy" in k:  # the 1 GiB copy of stream_copy_probe: the byte counters' calibration
+            lines.append(f"calibration: copy_bytes={1 << 30} fetch_kib={f[k][0]:.1f} write_kib={w.get(k, (0.0, 0))[0]:.1f}")
+    for fam in PMC_FAMILIES:
+        names = [k for k in f if k.split("(")[0].split("<")[0].split("::")[-1].startswith(fam)]
+        nd = sum(f[k][1] for k in names)
+        if not nd:
+            continue
+        fk = sum(f[k][0] * f[k][1] for k in names) / nd
+        wk = sum(w.get(k, (0.0, 0))[0] * f[k][1] for k in names) / nd
+        lines.append(f"traffic: kernel={fam} freq_batch=1 fetch_kib={fk:.1f} write_kib={wk:.1f}")
     return "\n".join(lines)
 
 
@@ -222,8 +239,10 @@ MM_KERNELS = {
                 "LDS, one wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
     "sinverse": ("k_series_inverse", "k_series_inverse (coefficients of the gaps' inverse series, E_0 = inv(D_0), "
                  "E_p = -E_0 sum_j D_j E_(p-j), one block per key)"),
-    "chain": ("k_chain_lds", "k_chain_lds (bottom block row of the transfer-matrix chain in LDS, 16 n^3 flop per "
-              "region; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
+    "chain": ("k_chain", "k_chain_mfma (n = 15, 25: v_mfma_f64_16x16x4f64 on operands padded to 16 / 32, three "
+              "MFMAs per complex product) and k_chain_tile (n = 9: register-tiled FP64 VALU): bottom block row of "
+              "the transfer-matrix chain, 16 n^3 algorithmic flop per region; flat disks formed in the kernel "
+              "from G, inv(G) and their phases"),
     "pseries": ("k_series_eval", "k_series_coef / k_series_eval (spacing-series weights and Horner sums of the "
                 "gaps' matrices and inverses)"),
 }

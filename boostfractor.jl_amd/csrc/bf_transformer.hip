@@ -2501,50 +2501,53 @@ __global__ void __launch_bounds__(128) k_series_eval(const SeriesSlice* __restri
 // coefficients of t^p in P(c + t h) inv(P(c + t h)) = I give inv(P(c + t h)) = sum_p t^p E_p with
 //     E_0 = inv(D_0),    E_p = -E_0 sum_{j=1..p} D_j E_{p-j},
 // q terms like P's own series.  P is entire in the spacing and invertible on and near the real axis,
-// so the E_p fall off like the D_p (|beta|^p / p!); the block checks that the last two are below
-// SERIES_INV_TAIL of E_0 (largest |re| + |im|) and otherwise raises *bad, so that the caller takes the LU
-// for the run instead.  One block per key: E_0 (from the wave inverse) in LDS, D_j and E_{p-j}
-// staged through LDS for each (p, j); every entry sums over j, then k, in increasing order.
+// so the E_p fall off like the D_p (|beta|^p / p!); each block checks that the last two terms of its
+// columns are below SERIES_INV_TAIL of E_0's (largest |re| + |im|) and otherwise raises *bad, so that
+// the caller takes the LU for the run instead.  Column l of every E_p needs only the D_j and column l
+// of the earlier E's, so block (key, y) computes columns SINV_CW y .. SINV_CW (y + 1) - 1 on its own:
+// E_0 in LDS, D_j and the block's columns of E_{p-j} staged through LDS for each (p, j); every entry
+// sums over j, then k, in increasing order.
 constexpr double SERIES_INV_TAIL = 1e-14;
+constexpr int SINV_CW = 5;  // columns per block
 struct SeriesInvJob {
     const zd* d;  // D_0 .. D_{q-1}
     zd* e;        // E_0 (already inverted) .. E_{q-1}
     int q, pad;
 };
-constexpr int SERIES_INV_EPT = (INV_WAVE_MAX * INV_WAVE_MAX + 255) / 256;  // entries per thread
+constexpr int SINV_EPT = (INV_WAVE_MAX * SINV_CW + 127) / 128;  // entries per thread
 
-__global__ void __launch_bounds__(256) k_series_inverse(const SeriesInvJob* __restrict__ jobs, int n, int* bad) {
+__global__ void __launch_bounds__(128) k_series_inverse(const SeriesInvJob* __restrict__ jobs, int n, int* bad) {
     extern __shared__ unsigned char smem_raw[];
     const int nn = n * n;
+    const int c0 = blockIdx.y * SINV_CW, cw = min(SINV_CW, n - c0), ne = n * cw;
     zd* E0 = reinterpret_cast<zd*>(smem_raw);  // column-major n x n
-    zd* A = E0 + nn;                           // D_j, then the sum S
-    zd* Bm = A + nn;                           // E_{p-j}
-    __shared__ double red[256];
+    zd* A = E0 + nn;                           // D_j
+    zd* Bm = A + nn;                           // the block's columns of E_{p-j}, then of the sum S (n x cw)
+    __shared__ double red[128];
     const SeriesInvJob J = jobs[blockIdx.x];
     const int tid = threadIdx.x;
-    double m0 = 0.0, mt = 0.0;  // largest |re| + |im| of E_0 and of the last two terms
-    for (int e = tid; e < nn; e += 256) {
+    double m0 = 0.0, mt = 0.0;  // largest |re| + |im| in the block's columns of E_0 and of the last two terms
+    for (int e = tid; e < nn; e += 128) {
         const zd v = J.e[e];
         E0[e] = v;
-        m0 = fmax(m0, fabs(v.re) + fabs(v.im));
+        const int l = e / n;
+        if (l >= c0 && l < c0 + cw) m0 = fmax(m0, fabs(v.re) + fabs(v.im));
     }
     for (int p = 1; p < J.q; p++) {
-        zd acc[SERIES_INV_EPT];
+        zd acc[SINV_EPT];
 #pragma unroll
-        for (int r = 0; r < SERIES_INV_EPT; r++) acc[r] = cmk<double>(0.0, 0.0);
+        for (int r = 0; r < SINV_EPT; r++) acc[r] = cmk<double>(0.0, 0.0);
         for (int j = 1; j <= p; j++) {
             __syncthreads();
             const zd* Dj = J.d + (size_t)j * nn;
-            const zd* Ep = J.e + (size_t)(p - j) * nn;
-            for (int e = tid; e < nn; e += 256) {
-                A[e] = Dj[e];
-                Bm[e] = Ep[e];
-            }
+            const zd* Ep = J.e + (size_t)(p - j) * nn + (size_t)c0 * n;  // columns c0 .. c0 + cw - 1
+            for (int e = tid; e < nn; e += 128) A[e] = Dj[e];
+            for (int e = tid; e < ne; e += 128) Bm[e] = Ep[e];
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < SERIES_INV_EPT; r++) {
-                const int e = tid + 256 * r;
-                if (e < nn) {
+            for (int r = 0; r < SINV_EPT; r++) {
+                const int e = tid + 128 * r;
+                if (e < ne) {
                     const int i = e % n, l = e / n;
                     zd a = acc[r];
                     for (int k = 0; k < n; k++) a = cmac(a, A[i + n * k], Bm[k + n * l]);
@@ -2554,30 +2557,30 @@ __global__ void __launch_bounds__(256) k_series_inverse(const SeriesInvJob* __re
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < SERIES_INV_EPT; r++) {
-            const int e = tid + 256 * r;
-            if (e < nn) A[e] = acc[r];
+        for (int r = 0; r < SINV_EPT; r++) {
+            const int e = tid + 128 * r;
+            if (e < ne) Bm[e] = acc[r];
         }
         __syncthreads();
-        zd* Eo = J.e + (size_t)p * nn;
+        zd* Eo = J.e + (size_t)p * nn + (size_t)c0 * n;
 #pragma unroll
-        for (int r = 0; r < SERIES_INV_EPT; r++) {
-            const int e = tid + 256 * r;
-            if (e < nn) {
+        for (int r = 0; r < SINV_EPT; r++) {
+            const int e = tid + 128 * r;
+            if (e < ne) {
                 const int i = e % n, l = e / n;
                 zd a = cmk<double>(0.0, 0.0);
-                for (int k = 0; k < n; k++) a = cmac(a, E0[i + n * k], A[k + n * l]);
+                for (int k = 0; k < n; k++) a = cmac(a, E0[i + n * k], Bm[k + n * l]);
                 a = cmk<double>(-a.re, -a.im);
                 Eo[e] = a;
                 if (p >= J.q - 2) mt = fmax(mt, fabs(a.re) + fabs(a.im));
             }
         }
-        // E_p is read back by other threads for the next terms: the stores above are visible to the
-        // block after the barrier at the top of the next j loop
+        // the block reads its E_p back for the next terms: the stores above are visible to the block
+        // after the barrier at the top of the next j loop
     }
     red[tid] = m0;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 64; o > 0; o >>= 1) {
         if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
         __syncthreads();
     }
@@ -2585,7 +2588,7 @@ __global__ void __launch_bounds__(256) k_series_inverse(const SeriesInvJob* __re
     __syncthreads();
     red[tid] = mt;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = 64; o > 0; o >>= 1) {
         if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
         __syncthreads();
     }
@@ -2667,10 +2670,10 @@ static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const s
         hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)fresh.size()), dim3(64), smem1, s, blk, inv, n,
                            dsel.as<int>(), (const InvSel*)nullptr);
         BF_CHECK_LAUNCH();
-        const size_t smem3 = 3 * sizeof(zd) * nn;
+        const size_t smem3 = sizeof(zd) * (2 * nn + (size_t)n * SINV_CW);
         allow_lds((const void*)k_series_inverse, smem3);
-        hipLaunchKernelGGL(k_series_inverse, dim3((unsigned)fresh.size()), dim3(256), smem3, s,
-                           dj.as<SeriesInvJob>(), n, bad);
+        hipLaunchKernelGGL(k_series_inverse, dim3((unsigned)fresh.size(), (unsigned)((n + SINV_CW - 1) / SINV_CW)),
+                           dim3(128), smem3, s, dj.as<SeriesInvJob>(), n, bad);
         BF_CHECK_LAUNCH();
     }
     for (size_t k = 0; k < fresh.size(); k++)
@@ -3401,18 +3404,20 @@ __global__ void __launch_bounds__(256) k_disk_flat(const FlatRegion* __restrict_
     });
 }
 
-// flat disks formed inside the chain (ChainSrc): block (entry, frequency) writes the phases
-// exp(-1im*kz*dz) of the entry's matrix slot m = base + f stride at ph[m n + i], as k_disk_flat forms them
-__global__ void __launch_bounds__(64) k_disk_phases(const FlatRegion* __restrict__ ent, const zd* __restrict__ k0sq,
-                                                    int nf, const zd* __restrict__ kt, int n, zd* ph) {
-    const FlatRegion R = ent[blockIdx.x];
-    const int f = blockIdx.y;
+// flat disks formed inside the chain (ChainSrc): thread (entry x, frequency f, mode i) writes the phase
+// exp(-1im*kz_i*dz) of the entry's matrix slot m = base + f stride at ph[m n + i], as k_disk_flat forms it
+__global__ void __launch_bounds__(256) k_disk_phases(const FlatRegion* __restrict__ ent, size_t total,
+                                                     const zd* __restrict__ k0sq, int nf, const zd* __restrict__ kt,
+                                                     int n, zd* ph) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total) return;
+    const int i = (int)(t % n);
+    const size_t xf = t / n;
+    const int f = (int)(xf % nf);
+    const FlatRegion R = ent[xf / nf];
     const zd k2 = k0sq[(size_t)R.slot * nf + f];
-    zd* o = ph + (size_t)(R.base + (int64_t)f * R.stride) * n;
-    for (int i = threadIdx.x; i < n; i += 64) {
-        const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
-        o[i] = cexp_j(rmul(R.dz, neg_i(kz)));
-    }
+    const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
+    ph[(size_t)(R.base + (int64_t)f * R.stride) * n + i] = cexp_j(rmul(R.dz, neg_i(kz)));
 }
 
 // true if the flat chunk path serves geometries confs[c0, c1)
@@ -3653,12 +3658,10 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
             // phases only: the chain forms the disks' matrices and inverses from G and inv(G)
             DevBuf& dph = scratch_buf(ctx, SB_FPH, sizeof(zd) * (size_t)fw.nmats * n);
             disk_src.ph = dph.as<zd>();
-            for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
-                const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
-                hipLaunchKernelGGL(k_disk_phases, dim3((unsigned)nd, (unsigned)nf), dim3(64), 0, s,
-                                   dd.as<FlatRegion>() + d0, dk.as<zd>(), nf, md.kt.as<zd>(), n, dph.as<zd>());
-                BF_CHECK_LAUNCH();
-            }
+            const size_t total = fw.disks.size() * (size_t)nf * n;
+            hipLaunchKernelGGL(k_disk_phases, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                               dd.as<FlatRegion>(), total, dk.as<zd>(), nf, md.kt.as<zd>(), n, dph.as<zd>());
+            BF_CHECK_LAUNCH();
         } else {
             for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {
                 const size_t nd = std::min<size_t>(65535, fw.disks.size() - d0);
