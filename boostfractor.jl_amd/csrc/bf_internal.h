@@ -102,7 +102,8 @@ struct EngineOptions {
     int parseval_series = 1;  // air gaps of one k0 with nearby spacings by a Taylor series in the spacing
     int series_inverse = 1;   // their inverses by the inverse's series (flat sweep chunks) instead of the LU
     int chain_tile = 1;       // register-tiled chain kernels for n = 9, 15, 25, 45 (0: the LDS chain kernel)
-    int chain_mfma = 1;       // chain on the matrix cores: 1 where it pays, 2 / 3 for every n <= 32 (four / three
+    int chain_mfma = 1;       // chain on the matrix cores: 1 where it pays, 4 for every n <= 32 (products in
+                              // separate waves), 2 / 3 for every n <= 32 (both products per wave, four / three
                               // MFMAs per complex product), 0 never
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)

@@ -129,14 +129,16 @@ struct ChainStep {
 };
 
 // Where the chain kernels find a region's matrix and inverse: slot m (= job.pbase + step.reg) at P + m n^2
-// and Pinv + m n^2, or, for a step marked disk, entry (i, j) as ph_m[i] G[i][j] and Ginv[i][j] / ph_m[j]
-// with ph_m = ph + m n (the flat disks' phases exp(-1im kz dz), never formed as matrices)
+// and Pinv + m n^2, or, for a step marked disk, entry (i, j) as ph_m[i] G[i][j] and Ginv[i][j] phinv_m[j]
+// with ph_m = ph + m n, phinv_m = phinv + m n (the flat disks' phases exp(-1im kz dz) and their
+// reciprocals; the disks' matrices are never formed)
 struct ChainSrc {
     const zd* P;
     const zd* Pinv;
     const zd* G;
     const zd* Ginv;
     const zd* ph;
+    const zd* phinv;
 };
 
 // One chain per block: the steps [step0, step0 + nsteps), propagation matrices

@@ -724,15 +724,12 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
 }
 
 // Entry g = i + n j of the matrix (inv = false) or of the inverse of matrix slot m of a chain step; a
-// disk step forms it from G, Ginv and the slot's phases with the arithmetic of k_disk_flat
+// disk step forms it from G, Ginv, the slot's phases and their reciprocals
 __device__ __forceinline__ zd chain_entry(const ChainSrc& S, int disk, int64_t m, int n, int g, int i, int j,
                                          bool inv) {
     if (!disk) return inv ? S.Pinv[m * n * n + g] : S.P[m * n * n + g];
-    const zd* ph = S.ph + m * n;
-    if (!inv) return cmul(ph[i], S.G[g]);
-    const zd a = S.Ginv[g], d = ph[j];
-    const double den = d.re * d.re + d.im * d.im;
-    return cmk<double>((a.re * d.re + a.im * d.im) / den, (a.im * d.re - a.re * d.im) / den);
+    if (!inv) return cmul(S.ph[m * n + i], S.G[g]);
+    return cmul(S.Ginv[g], S.phinv[m * n + j]);
 }
 
 // The chain in LDS for n <= CHAIN_LDS_MAX.  Only the bottom block row [T21 T22]
@@ -1354,6 +1351,187 @@ static void launch_chain_mfma(hipStream_t s, size_t nj, const ChainJob* jobs, co
                        src, n, e0, reflect, boost, refl);
 }
 
+// The matrix-core chain with the two products in separate waves: wave w of 2 (NP / 16)^2 computes
+// output tile w % (NP / 16)^2 of T21' = U P (w in the first half) or T22' = V inv(P) (second half), three
+// MFMAs per complex 4-k step.  LDS holds T21 and T22 themselves; each wave forms its A operand
+// U = a T21 + b T22 or V = b T21 + a T22 as it reads it, so a region's two barriers are shared by twice
+// the waves of k_chain_mfma (four per SIMD with two jobs per CU).  The axion sum S (U + V) e0 comes as
+// column n of both products (each half adds its own term); padding, layout and prefetch as k_chain_mfma.
+template <int NP>
+struct ChainMfma2 {
+    static constexpr int TW = NP / 16;
+    static constexpr int NTH = 64 * TW * TW;  // threads per product
+    static constexpr int NT = 2 * NTH;
+    static constexpr int LD = NP + 1;
+    static constexpr int PPT = NP * NP / NTH;  // entries of the thread's matrix per region
+    static size_t lds_bytes(int n) {
+        return sizeof(zd) * ((size_t)4 * NP * LD + 4 * (size_t)n) + (sizeof(double) + sizeof(int)) * NT +
+               sizeof(int) * (size_t)n;
+    }
+};
+
+template <int NP>
+__global__ void __launch_bounds__(ChainMfma2<NP>::NT, 4) k_chain_mfma2(const ChainJob* __restrict__ jobs,
+                                                                      const ChainStep* __restrict__ steps,
+                                                                      const ChainSrc src, int n,
+                                                                      const zd* __restrict__ e0_all,
+                                                                      const zd* __restrict__ reflect, zd* boost,
+                                                                      zd* refl) {
+    using T = ChainMfma2<NP>;
+    constexpr int LD = T::LD, NT = T::NT, NTH = T::NTH, TW = T::TW;
+    constexpr int SZ = NP * LD;
+    extern __shared__ unsigned char smem_raw[];
+    zd* T21 = reinterpret_cast<zd*>(smem_raw);  // T21[i + LD k]
+    zd* T22 = T21 + SZ;
+    zd* Pt = T22 + SZ;  // Pt[j + LD k] = P[k][j]
+    zd* Pit = Pt + SZ;
+    zd* ab = Pit + SZ;  // the two products' axion terms, [2][n]
+    zd* tmp = ab + 2 * n;
+    double* sval = reinterpret_cast<double*>(tmp + 2 * n);
+    int* sidx = reinterpret_cast<int*>(sval + NT);
+    int* piv = sidx + NT;
+    const ChainJob job = jobs[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int prod = wave / (TW * TW), tile = wave % (TW * TW);  // prod 0: T21' = U P, 1: T22' = V inv(P)
+    const int tr = tile % TW, tc = tile / TW;
+    const int nn = n * n;
+    const bool spare = n < NP;
+    for (int e = tid; e < SZ; e += NT) {
+        const int i = e % LD, k = e / LD;
+        T21[e] = cmk<double>(0.0, 0.0);
+        T22[e] = cmk<double>(i == k && i < n ? 1.0 : 0.0, 0.0);
+        Pt[e] = cmk<double>(0.0, 0.0);
+        Pit[e] = cmk<double>(0.0, 0.0);
+    }
+    for (int e = tid; e < 2 * n; e += NT) ab[e] = cmk<double>(0.0, 0.0);
+    __syncthreads();
+    if (spare)
+        for (int k = tid; k < n; k += NT) Pt[n + LD * k] = Pit[n + LD * k] = e0_all[job.e0off + k];
+    // the thread's entries g = pt + NTH m of its product's matrix (P or inv(P)), stored transposed
+    const int pt = tid % NTH;
+    zd* Bl = prod ? Pit : Pt;
+    auto fetch = [&](const ChainStep& sq, bool ok, zd* dp) {
+#pragma unroll
+        for (int m = 0; m < T::PPT; m++) {
+            const int g = pt + NTH * m;
+            dp[m] = cmk<double>(0.0, 0.0);
+            if (ok && g < nn) dp[m] = chain_entry(src, sq.disk, job.pbase + sq.reg, n, g, g % n, g / n, prod != 0);
+        }
+    };
+    auto store = [&](const zd* dp) {
+#pragma unroll
+        for (int m = 0; m < T::PPT; m++) {
+            const int g = pt + NTH * m;
+            if (g < nn) Bl[g / n + LD * (g % n)] = dp[m];
+        }
+    };
+    if (job.nsteps > 0) {
+        zd d0[T::PPT];
+        fetch(steps[job.step0], true, d0);
+        store(d0);
+    }
+    zd abr[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) abr[v] = cmk<double>(0.0, 0.0);
+    const bool own = spare && tc == n / 16 && (lane & 15) == n % 16;  // lanes holding column n
+    const int ar = tr * 16 + (lane & 15), bc = tc * 16 + (lane & 15), kq = lane >> 4;
+    const int kp = (n + 3) & ~3;  // k steps cover k < n
+    __syncthreads();
+    for (int q = 0; q < job.nsteps; q++) {
+        const ChainStep st = steps[job.step0 + q];
+        const bool more = q + 1 < job.nsteps;
+        zd pn[T::PPT];  // the next region's entries, loaded while the products run
+        fetch(steps[job.step0 + (more ? q + 1 : q)], more, pn);
+        // this half's mix of T21, T22: U = a T21 + b T22 (prod 0) or V = b T21 + a T22 (prod 1)
+        const zd ma = prod ? st.b : st.a, mb = prod ? st.a : st.b;
+        d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr, cg = cr;
+#pragma unroll
+        for (int ks = 0; ks < NP; ks += 4) {
+            if (ks >= kp) break;  // k >= n are zero padding
+            const int k = ks + kq;
+            const zd t1 = T21[ar + LD * k], t2 = T22[ar + LD * k];
+            const zd x = cadd(cmul(ma, t1), cmul(mb, t2));
+            const zd bo = Bl[bc + LD * k];
+            cr = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, bo.re, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, bo.im, ci, 0, 0, 0);
+            cg = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, bo.re + bo.im, cg, 0, 0, 0);
+        }
+        d4 oref, oimf;  // re = Xr Br - Xi Bi, im = (Xr + Xi)(Br + Bi) - Xr Br - Xi Bi
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            oref[v] = cr[v] - ci[v];
+            oimf[v] = cg[v] - cr[v] - ci[v];
+        }
+        // D layout: register v of lane holds D[4 v + lane / 16][lane % 16] of the tile
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) abr[v] = cadd(abr[v], cmul(st.S, cmk<double>(oref[v], oimf[v])));
+        }
+        if (!spare && wave == 0 && lane < n) {
+            zd sacc = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++)
+                sacc = cmac(sacc, cadd(cadd(cmul(st.a, T21[lane + LD * k]), cmul(st.b, T22[lane + LD * k])),
+                                       cadd(cmul(st.b, T21[lane + LD * k]), cmul(st.a, T22[lane + LD * k]))),
+                            e0_all[job.e0off + k]);
+            abr[0] = cadd(abr[0], cmul(st.S, sacc));
+        }
+        __syncthreads();
+        zd* Tout = prod ? T22 : T21;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = tr * 16 + 4 * v + (lane >> 4), j = bc;
+            if (i < n && j < n) Tout[i + LD * j] = cmk<double>(oref[v], oimf[v]);
+        }
+        if (more) store(pn);
+        __syncthreads();
+    }
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = tr * 16 + 4 * v + (lane >> 4);
+            if (i < n) ab[prod * n + i] = abr[v];
+        }
+    }
+    if (!spare && wave == 0 && lane < n) ab[lane] = abr[0];
+    __syncthreads();
+    // T21, T22 compacted to n x n (column-major) over the P tables for the solves
+    zd* C21 = Pt;
+    zd* C22 = Pit;
+    for (int e = tid; e < nn; e += NT) {
+        const int i = e % n, j = e / n;
+        C21[e] = T21[i + LD * j];
+        C22[e] = T22[i + LD * j];
+    }
+    __syncthreads();
+    // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
+    if (reflect)
+        for (int i = tid; i < n; i += NT) {
+            zd s = cmk<double>(0.0, 0.0);
+            for (int k = 0; k < n; k++) s = cadd(s, cmul(C21[i + n * k], reflect[k]));
+            tmp[n + i] = s;
+        }
+    __syncthreads();
+    lu_factor_block(C22, n, piv, sval, sidx);
+    if (tid == 0) {
+        for (int i = 0; i < n; i++) tmp[i] = cadd(ab[i], ab[n + i]);
+        lu_solve_col(C22, n, piv, tmp);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        if (reflect) {
+            lu_solve_col(C22, n, piv, tmp + n);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+        }
+    }
+}
+
+template <int NP>
+static void launch_chain_mfma2(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
+                               const ChainSrc& src, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    const size_t smem = ChainMfma2<NP>::lds_bytes(n);
+    allow_lds((const void*)k_chain_mfma2<NP>, smem);
+    hipLaunchKernelGGL(k_chain_mfma2<NP>, dim3((unsigned)nj), dim3(ChainMfma2<NP>::NT), smem, s, jobs, steps, src, n,
+                       e0, reflect, boost, refl);
+}
+
 // the LDS chain for n <= CHAIN_LDS_MAX: one block per job
 static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
                              const ChainSrc& src, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
@@ -1369,11 +1547,19 @@ static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, con
         break
     const int nc = chain_cols(n);
     // matrix cores where the padding to 16 or 32 wastes little (measured on the sweep's Bessel sets: n = 15
-    // and 25 run faster on the MFMA kernel, n = 9 on the register-tiled one); option 2 / 3 force the
-    // four- / three-MFMA form for every n <= 32
+    // runs fastest with the products in separate waves, n = 25 with both products per wave, n = 9 on the
+    // register-tiled VALU kernel); options 4 and 2 / 3 force the MFMA forms for every n <= 32
     const int mf = opts().chain_mfma;
     const bool mfma_pays = (n > 12 && n <= 16) || (n > 24 && n <= 32);
-    if (n <= 32 && (mf >= 2 || (mf == 1 && mfma_pays))) {
+    if (n <= 32 && (mf == 4 || (mf == 1 && n > 12 && n <= 16))) {
+        if (n <= 16)
+            launch_chain_mfma2<16>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
+        else
+            launch_chain_mfma2<32>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
+        BF_CHECK_LAUNCH();
+        return;
+    }
+    if (n <= 32 && (mf == 2 || mf == 3 || (mf == 1 && mfma_pays))) {
         const bool gauss = mf != 2;
         if (n <= 16)
             gauss ? launch_chain_mfma<16, true>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl)
@@ -3081,7 +3267,7 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
         TimedLaunch tl(ctx, "chain", flops);
         if (lds) {
             const ChainSrc src{P, Pinv, disks ? disks->G : nullptr, disks ? disks->Ginv : nullptr,
-                               disks ? disks->ph : nullptr};
+                               disks ? disks->ph : nullptr, disks ? disks->phinv : nullptr};
             launch_chain_lds(s, nj, djobs.as<ChainJob>(), dsteps.as<ChainStep>(), src, n, e0, reflect, boost, refl);
         } else {
             if (disks) throw Error(BF_ERR_UNSUPPORTED, "disk steps need the LDS chain");
@@ -3169,8 +3355,8 @@ void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps_h, int n
     hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg);
     BF_CHECK_LAUNCH();
     launch_inverse(ctx, P, Pinv, n, sel, nsel);
-    launch_chain_lds(s, (size_t)nfreq, jobs, steps, ChainSrc{P, Pinv, nullptr, nullptr, nullptr}, n, e0, reflect,
-                     boost, refl);
+    launch_chain_lds(s, (size_t)nfreq, jobs, steps, ChainSrc{P, Pinv, nullptr, nullptr, nullptr, nullptr}, n, e0,
+                     reflect, boost, refl);
     BF_HIP(hipStreamSynchronize(s));
 }
 
@@ -3405,10 +3591,11 @@ __global__ void __launch_bounds__(256) k_disk_flat(const FlatRegion* __restrict_
 }
 
 // flat disks formed inside the chain (ChainSrc): thread (entry x, frequency f, mode i) writes the phase
-// exp(-1im*kz_i*dz) of the entry's matrix slot m = base + f stride at ph[m n + i], as k_disk_flat forms it
+// exp(-1im*kz_i*dz) of the entry's matrix slot m = base + f stride at ph[m n + i], as k_disk_flat forms it,
+// and its reciprocal at phinv[m n + i]
 __global__ void __launch_bounds__(256) k_disk_phases(const FlatRegion* __restrict__ ent, size_t total,
                                                      const zd* __restrict__ k0sq, int nf, const zd* __restrict__ kt,
-                                                     int n, zd* ph) {
+                                                     int n, zd* ph, zd* phinv) {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= total) return;
     const int i = (int)(t % n);
@@ -3417,7 +3604,11 @@ __global__ void __launch_bounds__(256) k_disk_phases(const FlatRegion* __restric
     const FlatRegion R = ent[xf / nf];
     const zd k2 = k0sq[(size_t)R.slot * nf + f];
     const zd kz = csqrt_j(csub(k2, cmul(kt[i], kt[i])));
-    ph[(size_t)(R.base + (int64_t)f * R.stride) * n + i] = cexp_j(rmul(R.dz, neg_i(kz)));
+    const zd d = cexp_j(rmul(R.dz, neg_i(kz)));
+    const double den = d.re * d.re + d.im * d.im;
+    const size_t o = (size_t)(R.base + (int64_t)f * R.stride) * n + i;
+    ph[o] = d;
+    phinv[o] = cmk<double>(d.re / den, -d.im / den);
 }
 
 // true if the flat chunk path serves geometries confs[c0, c1)
@@ -3648,7 +3839,7 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         }
     }
     parseval_flush(ctx, mc, n, scale, fw.direct, st);
-    ChainSrc disk_src{P, Pinv, mc.G.as<zd>(), fw.ginv ? mc.Ginv.as<zd>() : nullptr, nullptr};
+    ChainSrc disk_src{P, Pinv, mc.G.as<zd>(), fw.ginv ? mc.Ginv.as<zd>() : nullptr, nullptr, nullptr};
     if (!fw.disks.empty()) {
         DevBuf& dd = scratch_buf(ctx, SB_FDISK, sizeof(FlatRegion) * fw.disks.size());
         DevBuf& dk = scratch_buf(ctx, SB_FK0, sizeof(zd) * fw.dk0.size());
@@ -3656,11 +3847,14 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
         stage_upload(st, dk.p, fw.dk0.data(), sizeof(zd) * fw.dk0.size(), s);
         if (fw.ginv) {
             // phases only: the chain forms the disks' matrices and inverses from G and inv(G)
-            DevBuf& dph = scratch_buf(ctx, SB_FPH, sizeof(zd) * (size_t)fw.nmats * n);
-            disk_src.ph = dph.as<zd>();
+            DevBuf& dph = scratch_buf(ctx, SB_FPH, 2 * sizeof(zd) * (size_t)fw.nmats * n);
+            zd* ph = dph.as<zd>();
+            zd* phinv = ph + (size_t)fw.nmats * n;
+            disk_src.ph = ph;
+            disk_src.phinv = phinv;
             const size_t total = fw.disks.size() * (size_t)nf * n;
             hipLaunchKernelGGL(k_disk_phases, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                               dd.as<FlatRegion>(), total, dk.as<zd>(), nf, md.kt.as<zd>(), n, dph.as<zd>());
+                               dd.as<FlatRegion>(), total, dk.as<zd>(), nf, md.kt.as<zd>(), n, ph, phinv);
             BF_CHECK_LAUNCH();
         } else {
             for (size_t d0 = 0; d0 < fw.disks.size(); d0 += 65535) {

@@ -632,13 +632,14 @@ def test_chain_tiled_matches_lds_chain(engine_options, n):
             assert np.array_equal(a, b), rel(a, b)
 
 
-@pytest.mark.parametrize("form", [2, 3])
+@pytest.mark.parametrize("form", [2, 3, 4])
 @pytest.mark.parametrize("n", [1, 7, 9, 15, 16, 17, 25, 31, 32])
 def test_chain_mfma_matches_valu_chain(engine_options, n, form):
-    """the matrix-core chain forced for every n <= 32 (operands padded to 16 or 32; complex products by
-    four real MFMAs, option chain_mfma=2, or three, 3) against the VALU chain kernels (chain_mfma=0) on
-    the same matrices, boost and reflectivity to 1e-12 (each MFMA sums its four k in an order of its own);
-    16 and 32 fill the padded size (the axion sum by its own dot product)"""
+    """the matrix-core chains forced for every n <= 32 (operands padded to 16 or 32; both products in one
+    wave with four real MFMAs per complex product, option chain_mfma=2, or three, 3; the products in
+    separate waves, 4) against the VALU chain kernels (chain_mfma=0) on the same matrices, boost and
+    reflectivity to 1e-12 (each MFMA sums its four k in an order of its own); 16 and 32 fill the padded
+    size (the axion sum by its own dot product)"""
     coords, sb = _flat_booster(N=16, ndisk=4)
     modes = _lg_like_modes(coords, n, 0)
     rng = np.random.default_rng(950 + n)
