@@ -310,9 +310,11 @@ class Marshalled:
             surf = self.hold(as_c128(s))
         return Setup(len(dist), cptr(dist), cptr(r), cptr(eps), cptr(tx), cptr(ty), cptr(surf))
 
-    def setups(self, bdrys, ny, nx):
+    def setups(self, bdrys, ny, nx, nan_to_1e30=False):
         """ctypes array of Setup for many geometries: each field of all of them in one buffer, the structs
-        filled at once in a numpy array laid out like Setup (the same structs as setup() per geometry)"""
+        filled at once in a numpy array laid out like Setup (the same structs as setup() per geometry).
+        nan_to_1e30: NaN permittivities become 1e30, in the caller's setups too (transformer's mutation,
+        src/boost_fractor_transformer.jl:289), found with one scan of the joined buffer"""
         n = len(bdrys)
         arr = (Setup * max(n, 1))()
         if n == 0:
@@ -325,6 +327,14 @@ class Marshalled:
         eps = self.hold(np.concatenate([np.asarray(b.eps, dtype=np.complex128).reshape(-1) for b in bdrys]))
         if len(eps) != len(dist):
             raise DimensionMismatch("eps and distance differ in length")
+        if nan_to_1e30:
+            bad = np.flatnonzero(np.isnan(eps))
+            if len(bad):
+                eps[bad] = 1e30
+                ends = np.cumsum(lens)
+                owner = np.searchsorted(ends, bad, side="right")
+                for q, c in zip(bad.tolist(), owner.tolist()):
+                    bdrys[c].eps[q - (ends[c] - lens[c])] = 1e30
         r = self.hold(np.concatenate(rs))
         def per_region(v, l):  # a tilt array of the geometry's region count, or a value broadcast to it
             a = np.asarray(v, dtype=np.float64)

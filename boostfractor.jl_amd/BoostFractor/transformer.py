@@ -357,9 +357,7 @@ class TransformerBatchPlan:
         nconf = len(bdrys)
         mode_set_of, offs, sizes = _batch_layout(nconf, self.modesets, mode_set_of, len(self.freqs))
         m = L.Marshalled()
-        for b in bdrys:
-            _nan_to_1e30(b)
-        setups = m.setups(bdrys, self.coords.ny, self.coords.nx)
+        setups = m.setups(bdrys, self.coords.ny, self.coords.nx, nan_to_1e30=True)
         L.check(L.lib().bf_transformer_batch_plan_run(self.handle, nconf, setups, L.cptr(mode_set_of), L.cptr(offs)),
                 self.ctx.handle)
         self._layout = (offs, sizes)
@@ -404,9 +402,7 @@ def transformer_batch(bdrys, coords, modesets, f=10.0e9, mode_set_of=None, prop=
     m = L.Marshalled()
     c = m.coords(coords)
     mds, emp = _batch_sources(m, coords, modesets, emits, diskR, float(freqs[0]))
-    for b in bdrys:
-        _nan_to_1e30(b)
-    setups = m.setups(bdrys, coords.ny, coords.nx)
+    setups = m.setups(bdrys, coords.ny, coords.nx, nan_to_1e30=True)
     fr = m.hold(L.as_f64(freqs))
     nf = len(fr)
     out = np.zeros(int(sizes.sum()), dtype=np.complex128)

@@ -233,3 +233,21 @@ def test_batched_setup_marshalling_matches_per_geometry():
         if one.surfaces:
             cnt = nreg * coords.ny * coords.nx
             assert np.array_equal(read(got.surfaces, cnt, np.complex128), read(one.surfaces, cnt, np.complex128))
+
+
+def test_batched_marshalling_replaces_nan_in_place():
+    """Marshalled.setups(nan_to_1e30=True) passes 1e30 for NaN permittivities and writes 1e30 into the
+    caller's eps arrays (the reference's in-place bdry.eps[isnan] .= 1e30), leaving other entries alone"""
+    coords = W.grid_coords(16)
+    geoms = [W.booster(coords, ndisk=k, seed=10 + k) for k in (1, 3, 2)]
+    geoms[1].eps[3] = complex(np.nan, 0.0)
+    before = [g.eps.copy() for g in geoms]
+    m = L.Marshalled()
+    batch = m.setups(geoms, coords.ny, coords.nx, nan_to_1e30=True)
+    for i, g in enumerate(geoms):
+        want = before[i].copy()
+        want[np.isnan(want)] = 1e30
+        assert np.array_equal(g.eps, want), i
+        got = np.frombuffer(bytes((ctypes.c_char * (16 * batch[i].nreg)).from_address(batch[i].eps)),
+                            dtype=np.complex128)
+        assert np.array_equal(got, want), i
