@@ -142,9 +142,10 @@ struct ChainSrc {
 };
 
 // One chain per block: the steps [step0, step0 + nsteps), propagation matrices
-// P[pbase + reg], the axion source e0 + e0off and the outputs at slot `out`
-// (boost[out * n ...]).  A single geometry has one job per frequency; a batch
-// of geometries with the same mode count has one job per (geometry, frequency).
+// P[pbase + reg], the axion source e0 + e0off and the outputs at boost + out
+// (and refl + out), n entries each.  A single geometry has one job per
+// frequency; a batch of geometries with the same mode count has one job per
+// (geometry, frequency), written straight to its place in the results.
 struct ChainJob {
     int64_t step0, pbase, out, e0off;
     int nsteps, pad;

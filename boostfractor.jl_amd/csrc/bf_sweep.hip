@@ -348,7 +348,7 @@ void spacing_itp_transformer(bf_spacing_itp* it, int ncand, const double* shifts
                 j.step0 = 0;
                 j.nsteps = nreg - 1;
                 j.pbase = base;
-                j.out = (int64_t)c * nf + f;
+                j.out = ((int64_t)c * nf + f) * n;
                 j.e0off = 0;
                 j.pad = 0;
                 jobs.push_back(j);

@@ -710,7 +710,7 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
     if (threadIdx.x == 0) {
         for (int i = 0; i < n; i++) tmp[i] = ab[i];
         lu_solve_col(M, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             for (int i = 0; i < n; i++) {
                 zd s = cmk<double>(0.0, 0.0);
@@ -718,7 +718,7 @@ __global__ void k_chain(const ChainJob* __restrict__ jobs, const ChainStep* __re
                 tmp[n + i] = s;
             }
             lu_solve_col(M, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -925,10 +925,10 @@ __global__ void __launch_bounds__(256) k_chain_lds(const ChainJob* __restrict__ 
     if (tid == 0) {
         for (int i = 0; i < n; i++) tmp[i] = ab[i];
         lu_solve_col(T22, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             lu_solve_col(T22, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -1101,10 +1101,10 @@ __global__ void __launch_bounds__(NT) k_chain_tile(const ChainJob* __restrict__ 
     if (tid == 0) {
         for (int i = 0; i < n; i++) tmp[i] = ab[i];
         lu_solve_col(T22, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             lu_solve_col(T22, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -1334,10 +1334,10 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
     if (tid == 0) {
         for (int i = 0; i < n; i++) tmp[i] = ab[i];
         lu_solve_col(T22, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             lu_solve_col(T22, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -1515,10 +1515,10 @@ __global__ void __launch_bounds__(ChainMfma2<NP>::NT, 4) k_chain_mfma2(const Cha
     if (tid == 0) {
         for (int i = 0; i < n; i++) tmp[i] = cadd(ab[i], ab[n + i]);
         lu_solve_col(C22, n, piv, tmp);
-        for (int i = 0; i < n; i++) boost[(size_t)job.out * n + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
+        for (int i = 0; i < n; i++) boost[(size_t)job.out + i] = cmk<double>(-tmp[i].re, -tmp[i].im);
         if (reflect) {
             lu_solve_col(C22, n, piv, tmp + n);
-            for (int i = 0; i < n; i++) refl[(size_t)job.out * n + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
+            for (int i = 0; i < n; i++) refl[(size_t)job.out + i] = cmk<double>(-tmp[n + i].re, -tmp[n + i].im);
         }
     }
 }
@@ -3312,13 +3312,13 @@ size_t noalloc_bytes(int nreg, int n, int nfreq) {
 }
 
 // one chain per frequency over regions nreg-1 .. 1; sel lists the matrices to invert
-__global__ void k_noalloc_jobs(ChainJob* jobs, int* sel, int nfreq, int nreg) {
+__global__ void k_noalloc_jobs(ChainJob* jobs, int* sel, int nfreq, int nreg, int n) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nfreq) return;
     ChainJob j;
     j.step0 = 0;
     j.pbase = (int64_t)f * nreg;
-    j.out = f;
+    j.out = (int64_t)f * n;
     j.e0off = 0;
     j.nsteps = nreg - 1;
     j.pad = 0;
@@ -3352,7 +3352,8 @@ void transformer_chain_noalloc(bf_ctx* ctx, int nreg, const double* eps_h, int n
     }
     for (int q = 0; q < nreg - 1; q++) hsteps[q] = chain_step(eps, nreg, q);
     BF_HIP(hipMemcpyAsync(steps, hsteps, sizeof(ChainStep) * (nreg - 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg);
+    hipLaunchKernelGGL(k_noalloc_jobs, dim3((unsigned)((nfreq + 63) / 64)), dim3(64), 0, s, jobs, sel, nfreq, nreg,
+                       n);
     BF_CHECK_LAUNCH();
     launch_inverse(ctx, P, Pinv, n, sel, nsel);
     launch_chain_lds(s, (size_t)nfreq, jobs, steps, ChainSrc{P, Pinv, nullptr, nullptr, nullptr, nullptr}, n, e0,
@@ -3393,7 +3394,7 @@ void transformer_dev(bf_ctx* ctx, const bf_setup* st, const bf_coords* co, const
         j.step0 = 0;
         j.nsteps = nreg - 1;
         j.pbase = (int64_t)f * nreg;
-        j.out = f;
+        j.out = (int64_t)f * n;
         j.e0off = 0;
         j.pad = 0;
         jobs.push_back(j);
@@ -3470,7 +3471,7 @@ void transformer_plan_run(bf_transformer_plan* pl) {
     append_chain_steps(pl->eps, nreg, steps);  // the same region sequence at every frequency
     for (int f = 0; f < nf; f++) {
         for (int r = 1; r < nreg; r++) sel.push_back(f * nreg + r);
-        jobs.push_back(ChainJob{0, (int64_t)f * nreg, f, 0, nreg - 1, 0});
+        jobs.push_back(ChainJob{0, (int64_t)f * nreg, (int64_t)f * n, 0, nreg - 1, 0});
     }
     inverse_and_chains(ctx, n, pl->P.as<zd>(), pl->Pinv.as<zd>(), sel, jobs, steps, pl->de0.as<zd>(),
                        pl->dref.as<zd>(), pl->dboost.as<zd>(), pl->drefl.as<zd>());
@@ -3646,7 +3647,8 @@ struct FlatWork {
 // Host preparation of geometries confs[c0, c1) (matrices at P, chunk-relative indices).  False if the
 // mode set's tables do not allow the flat path (then nothing was queued).
 static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, ModeCache& mc, const bf_setup* setups,
-                         const std::vector<int>& confs, size_t c0, size_t c1, zd* P, FlatWork& fw) {
+                         const std::vector<int>& confs, size_t c0, size_t c1, const int64_t* boost_off, zd* P,
+                         FlatWork& fw) {
     const ModesDev& md = pl->md[set];
     const int n = md.n, nf = (int)pl->freqs.size();
     const size_t nn = (size_t)n * n;
@@ -3682,7 +3684,7 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
             j.step0 = step0;
             j.nsteps = nreg - 1;
             j.pbase = base + (int64_t)f * nreg;
-            j.out = (int64_t)((q - c0) * nf + f);
+            j.out = boost_off[confs[q]] + (int64_t)f * n;  // the plan's results, geometry-major
             j.e0off = 0;
             j.pad = 0;
             fw.jobs.push_back(j);
@@ -3945,20 +3947,15 @@ static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_set
             first_chunk = false;
             DevBuf& P = scratch_buf(ctx, SB_P, sizeof(zd) * nn * mats);
             DevBuf& Pinv = scratch_buf(ctx, SB_PINV, sizeof(zd) * nn * mats);
-            DevBuf& dboost = scratch_buf(ctx, SB_BOOST, sizeof(zd) * n * nfreq * (c1 - c0));
             // chunks of flat, untilted geometries: frequency-independent descriptors expanded on the device
             FlatWork fw;
             fw.bad = bad;
             if (flat_chunk_ok(pl, md, setups, confs, c0, c1) &&
-                flat_prepare(ctx, pl, set, cache, setups, confs, c0, c1, P.as<zd>(), fw)) {
+                flat_prepare(ctx, pl, set, cache, setups, confs, c0, c1, boost_off, P.as<zd>(), fw)) {
                 const auto h1 = clk::now();
                 pl->stager.begin_chunk();
                 const auto h2 = clk::now();
-                flat_launch(ctx, pl, set, cache, fw, P.as<zd>(), Pinv.as<zd>(), dboost.as<zd>());
-                for (size_t q = c0; q < c1; q++)
-                    BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]],
-                                          dboost.as<zd>() + (q - c0) * nfreq * n, sizeof(zd) * n * nfreq,
-                                          hipMemcpyDeviceToDevice, s));
+                flat_launch(ctx, pl, set, cache, fw, P.as<zd>(), Pinv.as<zd>(), pl->dboost.as<zd>());
                 pl->stager.end_chunk(s);
                 const auto h3 = clk::now();
                 host_ms += ms_since(h0, h1) + ms_since(h2, h3);
@@ -3995,7 +3992,7 @@ static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_set
                     j.step0 = step0;
                     j.nsteps = nreg - 1;
                     j.pbase = (int64_t)(base + (size_t)f * nreg);
-                    j.out = (int64_t)((q - c0) * nfreq + f);
+                    j.out = boost_off[confs[q]] + (int64_t)f * n;  // the plan's results, geometry-major
                     j.e0off = 0;
                     j.pad = 0;
                     jobs.push_back(j);
@@ -4021,10 +4018,7 @@ static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_set
             }
             disk_diag_flush(ctx, cache, md, pq, &pl->stager, use_ginv ? Pinv.as<zd>() : nullptr);
             inverse_and_chains(ctx, n, P.as<zd>(), Pinv.as<zd>(), sel, jobs, steps, pl->de0[set].as<zd>(), nullptr,
-                               dboost.as<zd>(), nullptr, &pl->stager);
-            for (size_t q = c0; q < c1; q++)
-                BF_HIP(hipMemcpyAsync(pl->dboost.as<zd>() + boost_off[confs[q]], dboost.as<zd>() + (q - c0) * nfreq * n,
-                                      sizeof(zd) * n * nfreq, hipMemcpyDeviceToDevice, s));
+                               pl->dboost.as<zd>(), nullptr, &pl->stager);
             pl->stager.end_chunk(s);
             const auto h3 = clk::now();
             host_ms += ms_since(h0, h1) + ms_since(h2, h3);
