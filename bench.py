@@ -239,10 +239,11 @@ MM_KERNELS = {
                 "LDS, one wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
     "sinverse": ("k_series_inverse", "k_series_inverse (coefficients of the gaps' inverse series, E_0 = inv(D_0), "
                  "E_p = -E_0 sum_j D_j E_(p-j), one block per key)"),
-    "chain": ("k_chain", "k_chain_mfma (n = 15, 25: v_mfma_f64_16x16x4f64 on operands padded to 16 / 32, three "
-              "MFMAs per complex product) and k_chain_tile (n = 9: register-tiled FP64 VALU): bottom block row of "
-              "the transfer-matrix chain, 16 n^3 algorithmic flop per region; flat disks formed in the kernel "
-              "from G, inv(G) and their phases"),
+    "chain": ("k_chain", "k_chain_mfma (n = 25: v_mfma_f64_16x16x4f64 on operands padded to 32, three MFMAs per "
+              "complex product), k_chain_mfma2 (n = 15: padded to 16, the two products in separate waves) and "
+              "k_chain_tile (n = 9: register-tiled FP64 VALU): bottom block row of the transfer-matrix chain, "
+              "16 n^3 algorithmic flop per region; flat disks formed in the kernel from G, inv(G) and their "
+              "phases"),
     "pseries": ("k_series_eval", "k_series_coef / k_series_eval (spacing-series weights and Horner sums of the "
                 "gaps' matrices and inverses)"),
 }
