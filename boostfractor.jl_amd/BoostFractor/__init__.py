@@ -12,7 +12,7 @@ from .jlrange import julia_range
 from ._lib import ArgumentError, BoundsError, HipError, InexactError, UnsupportedError, Context, context, device_count
 from .beam import gauss_profile, unnormalized_match
 from .propagators import propagator, propagator1D, propagatorMomentumSpace, propagatorNoTilts
-from .recurprop import DancerPlan, cheerleader, dance_intro, dancer
+from .recurprop import DancerPlan, cheerleader, dance_intro, dancer, dancer_batch
 from .transformer import (ErrorException, Modes, SeedModes, axion_induced_modes, calc_propagation_matrices,
                           e_field_dimensions, field2modes, index, mode, modes2field, propagation_matrix, transformer,
                           TransformerPlan, TransformerBatchPlan, transformer_batch, transformer_trace_back)
@@ -25,7 +25,7 @@ __all__ = [
     "CoordinateSystem", "SeedCoordinateSystem", "get_kspace_coords", "SetupBoundaries", "SeedSetupBoundaries",
     "wavelength", "julia_range", "DimensionMismatch", "ArgumentError", "HipError", "UnsupportedError", "Context",
     "context", "device_count", "propagator", "propagatorNoTilts", "propagatorMomentumSpace", "propagator1D",
-    "dancer", "dance_intro", "cheerleader", "DancerPlan", "Modes", "SeedModes", "mode", "index",
+    "dancer", "dancer_batch", "dance_intro", "cheerleader", "DancerPlan", "Modes", "SeedModes", "mode", "index",
     "e_field_dimensions", "axion_induced_modes", "field2modes", "modes2field", "propagation_matrix",
     "calc_propagation_matrices", "transformer", "TransformerPlan", "TransformerBatchPlan", "transformer_batch", "transformer_trace_back", "ErrorException", "InexactError",
     "gauss_profile", "unnormalized_match", "BoundsError", "calc_propagation_matrices_grid", "SpacingInterpolation",

@@ -73,6 +73,8 @@ _SIGS = {
     "bf_unnormalized_match": (ctypes.c_int, [_P, _I32, ctypes.c_int64, _P, _P, _D, _D, _P]),
     "bf_dancer": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32, _P, _P,
                                  _D, _I32, _I32, _I32, _P, _P, _P]),
+    "bf_dancer_batch": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32,
+                                       _I32, _P, _P, _D, _I32, _I32, _I32, _P, _P, _P, _P]),
     "bf_cheerleader": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32, _I32,
                                       _P, _P, _D, _I32, _P, _P]),
     "bf_dancer_plan_create": (ctypes.c_int, [_P, ctypes.POINTER(Setup), ctypes.POINTER(Coords), _D, _I32, _P, _I32,

@@ -92,6 +92,59 @@ def dancer(amin, nmax, sbdry, coords, f=10.0e9, prop=propagator, emit=None, refl
     return res
 
 
+def dancer_batch(amin, nmax, sbdrys, coords, f=10.0e9, prop=propagator, emit=None, reflect=None, diskR=0.1,
+                 returnsum=True, immediatesum=True, precision="fp64", return_power=False):
+    """dancer (src/boost_fractor_recurprop.jl:25-146) for a list of setups at the same frequencies, in one
+    call (the reference sweeps geometries point by point, test/runtests.jl:101-114); the setups may differ
+    in disk count.  emit / reflect: None or one entry per setup (each as dancer's, may be None).  Returns
+    the list of dancer's results per setup (with return_power, a list of (Eout, power) pairs)."""
+    kind = prop_kind(prop)
+    prec = precision_code(precision)
+    freqs, scalar = _freqs(f)
+    nf = len(freqs)
+    nconf = len(sbdrys)
+    m = L.Marshalled()
+    c = m.coords(coords)
+    setups = m.setups(sbdrys, coords.ny, coords.nx)
+    K = (coords.ny, coords.nx)
+    emits = [None] * nconf if emit is None else list(emit)
+    reflects = [None] * nconf if reflect is None else list(reflect)
+    if len(emits) != nconf or len(reflects) != nconf:
+        raise L.DimensionMismatch("emit and reflect need one entry per setup")
+    ep = (ctypes.c_void_p * max(nconf, 1))()
+    rp = (ctypes.c_void_p * max(nconf, 1))()
+    shapes, offs, tot = [], [], 0
+    for i, sb in enumerate(sbdrys):
+        R = len(sb.distance) - 1
+        if emits[i] is not None and reflects[i] is None:
+            em = m.hold(L.as_c128(emits[i]))
+            if em.shape != (R, 2) + K:
+                raise L.DimensionMismatch(f"emit[{i}] has shape {em.shape}, expected {(R, 2) + K}")
+            ep[i] = em.ctypes.data
+        if reflects[i] is not None:
+            rp[i] = m.hold(L.as_c128(np.broadcast_to(reflects[i], K))).ctypes.data
+        shp = (nf,) + K if returnsum else (nf, 1 if immediatesum else nmax + 1) + K
+        shapes.append(shp)
+        offs.append(tot)
+        tot += int(np.prod(shp))
+    out = np.zeros(max(tot, 1), dtype=np.complex128)
+    off = np.asarray(offs if offs else [0], dtype=np.int64)
+    power = np.zeros((max(nconf, 1), nf))
+    bounces = np.zeros((max(nconf, 1), nf), dtype=np.int32)
+    ctx = L.context()
+    L.check(L.lib().bf_dancer_batch(ctx.handle, nconf, setups, ctypes.byref(c), float(amin), int(nmax),
+                                    L.cptr(freqs), nf, kind, ep if emit is not None else None,
+                                    rp if reflect is not None else None, float(diskR), int(bool(returnsum)),
+                                    int(bool(immediatesum)), prec, L.cptr(off), L.cptr(out), L.cptr(power),
+                                    L.cptr(bounces)), ctx.handle)
+    res = []
+    for i in range(nconf):
+        e = out[offs[i]:offs[i] + int(np.prod(shapes[i]))].reshape(shapes[i])
+        e = e[0] if scalar else e
+        res.append((e, power[i][0] if scalar else power[i]) if return_power else e)
+    return res
+
+
 def cheerleader(amin, nmax, bdry, coords, f=10.0e9, prop=propagator, emit=None, reflect=None, diskR=0.1,
                 returnboth=False, precision="fp64"):
     """src/boost_fractor_recurprop.jl:226-363.  Returns the field leaving on the

@@ -367,6 +367,43 @@ int bf_dancer(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, doubl
     });
 }
 
+int bf_dancer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, const bf_coords* coords, double amin,
+                    int32_t nmax, const double* freqs, int32_t nfreq, int32_t prop_kind,
+                    const double* const* emits, const double* const* reflects, double diskR, int32_t returnsum,
+                    int32_t immediatesum, int32_t precision, const int64_t* eout_offset, double* Eout,
+                    double* power, int32_t* bounces) {
+    TRY(check_kind(ctx, prop_kind));
+    TRY(check_prec(ctx, precision));
+    TRY(check_coords(ctx, coords));
+    TRY(check_freqs(ctx, freqs, nfreq));
+    BF_REQUIRE(ctx, nconf >= 0, BF_ERR_ARG, "nconf must be >= 0");
+    BF_REQUIRE(ctx, nconf == 0 || setups, BF_ERR_ARG, "setups is NULL");
+    BF_REQUIRE(ctx, nmax >= -1, BF_ERR_ARG, "nmax must be >= -1");
+    BF_REQUIRE(ctx, Eout || power, BF_ERR_ARG, "Eout and power are both NULL");
+    BF_REQUIRE(ctx, !Eout || eout_offset, BF_ERR_ARG, "Eout needs eout_offset");
+    for (int32_t c = 0; c < nconf; c++) {
+        TRY(check_setup(ctx, &setups[c], 2, true));
+        BF_REQUIRE(ctx, !Eout || eout_offset[c] >= 0, BF_ERR_ARG, "eout_offset must be >= 0");
+    }
+    BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
+    return guarded(ctx, [&] {
+        // one plan per geometry on the context's stream: geometries may differ in region count
+        for (int32_t c = 0; c < nconf; c++) {
+            bf::DancerOptions o = ctx_options();
+            const double* rf = reflects ? reflects[c] : nullptr;
+            o.reflect = rf;  // reflect takes precedence over emit (src/boost_fractor_recurprop.jl:53-63)
+            o.emit = rf ? nullptr : (emits ? emits[c] : nullptr);
+            o.returnsum = returnsum ? 1 : 0;
+            o.immediatesum = immediatesum ? 1 : 0;
+            std::unique_ptr<bf::DancerPlanBase> p(bf::make_dancer_plan(ctx, &setups[c], coords, amin, nmax, freqs,
+                                                                       nfreq, prop_kind, diskR, precision, o));
+            p->run();
+            p->fetch(Eout ? Eout + 2 * eout_offset[c] : nullptr, power ? power + (size_t)c * nfreq : nullptr,
+                     bounces ? bounces + (size_t)c * nfreq : nullptr);
+        }
+    });
+}
+
 int bf_cheerleader(bf_ctx* ctx, const bf_setup* setup, const bf_coords* coords, double amin, int32_t nmax,
                    const double* freqs, int32_t nfreq, int32_t prop_kind, const double* emit, const double* reflect,
                    double diskR, int32_t precision, double* out_right, double* out_left) {

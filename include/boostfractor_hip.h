@@ -182,6 +182,18 @@ int bf_dancer(bf_ctx *ctx, const bf_setup *setup, const bf_coords *coords, doubl
               const double *freqs, int32_t nfreq, int32_t prop_kind, const double *emit, const double *reflect,
               double diskR, int32_t returnsum, int32_t immediatesum, int32_t precision, double *Eout,
               double *power, int32_t *bounces);
+/* dancer over many geometries at the same frequencies (the reference sweeps them
+ * point by point: test/runtests.jl:101-114, examples/Single Disk.ipynb); setups may
+ * differ in disk count.  emits / reflects: NULL or one pointer per geometry (each as
+ * bf_dancer's emit / reflect, may be NULL).  Geometry c's output, in bf_dancer's
+ * Eout layout for that geometry, starts at complex element eout_offset[c] of Eout;
+ * power and bounces (optional) are [nconf][nfreq].  Results equal bf_dancer's per
+ * geometry. */
+int bf_dancer_batch(bf_ctx *ctx, int32_t nconf, const bf_setup *setups, const bf_coords *coords, double amin,
+                    int32_t nmax, const double *freqs, int32_t nfreq, int32_t prop_kind,
+                    const double *const *emits, const double *const *reflects, double diskR, int32_t returnsum,
+                    int32_t immediatesum, int32_t precision, const int64_t *eout_offset, double *Eout,
+                    double *power, int32_t *bounces);
 
 /* cheerleader (src/boost_fractor_recurprop.jl:226-363) for nfreq frequencies.
  * emit: complex [nreg][2][ny][nx] or NULL; reflect: complex [ny][nx] or NULL.

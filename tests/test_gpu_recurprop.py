@@ -470,3 +470,33 @@ def test_cheerleader_fused_pair_rows_match_separate(prec, N):
     for key in ("10", "11"):
         for q in range(3):
             assert rel(out[key][q], out["01"][q]) < tol, (key, q, rel(out[key][q], out["01"][q]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dancer_batch_matches_per_geometry(precision):
+    """dancer_batch over boosters of 1, 3 and 2 disks (uneven region counts) at the same frequencies equals
+    dancer per geometry bit for bit (summed output, the whole history, and a reflect case), and the
+    oracle at 1e-10 on one geometry (FP64)"""
+    coords = W.grid_coords(64)
+    sbs = [W.booster(coords, ndisk=k, seed=1200 + k) for k in (1, 3, 2)]
+    freqs = [19.7e9, 20.3e9]
+    got = B.dancer_batch(0, 30, sbs, coords, f=freqs, diskR=0.15, precision=precision)
+    for sb, g in zip(sbs, got):
+        want = B.dancer(0, 30, sb, coords, f=freqs, diskR=0.15, precision=precision)
+        assert np.array_equal(g, want)
+    hist = B.dancer_batch(0, 12, sbs, coords, f=freqs, diskR=0.15, returnsum=False, immediatesum=False,
+                          precision=precision)
+    for sb, g in zip(sbs, hist):
+        want = B.dancer(0, 12, sb, coords, f=freqs, diskR=0.15, returnsum=False, immediatesum=False,
+                        precision=precision)
+        assert g.shape == want.shape and np.array_equal(g, want)
+    refl = [None, np.exp(1j * np.linspace(0, 1, 64 * 64)).reshape(64, 64), None]
+    rb = B.dancer_batch(0, 20, sbs, coords, f=freqs, diskR=0.15, reflect=refl, precision=precision)
+    for sb, r, g in zip(sbs, refl, rb):
+        want = B.dancer(0, 20, sb, coords, f=freqs, diskR=0.15, reflect=r, precision=precision)
+        assert np.array_equal(g, want)
+    if precision == "fp64":
+        oc = O.Coords(coords.X, coords.Y, coords.kX, coords.kY)
+        os_ = O.Setup(sbs[1].distance, sbs[1].eps, sbs[1].r)
+        w = O.dancer(os_, oc, 0, 30, freqs[0], O.PROP_PROPAGATOR, diskR=0.15)
+        assert np.max(np.abs(got[1][0] - w)) / np.max(np.abs(w)) < TOL

@@ -251,3 +251,22 @@ def test_batched_marshalling_replaces_nan_in_place():
         got = np.frombuffer(bytes((ctypes.c_char * (16 * batch[i].nreg)).from_address(batch[i].eps)),
                             dtype=np.complex128)
         assert np.array_equal(got, want), i
+
+
+def test_dancer_batch_validation_without_device():
+    """bf_dancer_batch checks its arguments on the host: negative count, missing setups or offsets"""
+    lib = L.lib()
+    m, c = _coords_struct()
+    sbs = [W.booster(W.grid_coords(8), ndisk=k, seed=k) for k in (1, 2)]
+    setups = m.setups(sbs, 8, 8)
+    f = np.array([20e9])
+    out = np.zeros(200, dtype=complex)
+    off = np.zeros(2, dtype=np.int64)
+    args = lambda n, st, offp: (None, n, st, ctypes.byref(c), 0.0, 10, L.cptr(f), 1, 0, None, None, 0.1, 1, 1, 0,
+                                offp, L.cptr(out), None, None)
+    assert lib.bf_dancer_batch(*args(-1, setups, L.cptr(off))) == L.BF_ERR_ARG
+    assert lib.bf_dancer_batch(*args(2, None, L.cptr(off))) == L.BF_ERR_ARG
+    assert lib.bf_dancer_batch(*args(2, setups, None)) == L.BF_ERR_ARG
+    # valid arguments but no context -> ArgumentError, raised only after the checks
+    assert lib.bf_dancer_batch(*args(2, setups, L.cptr(off))) == L.BF_ERR_ARG
+    assert b"ctx" in lib.bf_last_error(None)
