@@ -209,6 +209,7 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"series_inverse", &bf::EngineOptions::series_inverse},
         {"chain_tile", &bf::EngineOptions::chain_tile},
         {"chain_mfma", &bf::EngineOptions::chain_mfma},
+        {"chain_order", &bf::EngineOptions::chain_order},
     };
     ip = nullptr;
     for (const Int& e : ints)

@@ -105,6 +105,7 @@ struct EngineOptions {
     int chain_mfma = 1;       // chain on the matrix cores: 1 where it pays, 4 for every n <= 32 (products in
                               // separate waves), 2 / 3 for every n <= 32 (both products per wave, four / three
                               // MFMAs per complex product), 0 never
+    int chain_order = 1;      // chain blocks start longest chain first (0: in job order)
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)
 };

@@ -3254,11 +3254,24 @@ void inverse_and_chains(bf_ctx* ctx, int n, zd* P, zd* Pinv, const std::vector<i
     }
     const size_t nj = jobs.size();
     const bool lds = n <= CHAIN_LDS_MAX;
+    // Blocks start roughly in index order, so a batch of chains of unequal length (geometries with
+    // different disk counts) is laid out longest first: the last blocks to start are then the short
+    // ones, and the launch does not end on a few long chains started late.  Each job writes its own
+    // outputs, so the order changes nothing else.
+    std::vector<ChainJob> ordered;
+    const ChainJob* jl = jobs.data();
+    if (opts().chain_order && nj > 1 &&
+        std::any_of(jobs.begin(), jobs.end(), [&](const ChainJob& j) { return j.nsteps != jobs[0].nsteps; })) {
+        ordered = jobs;
+        std::stable_sort(ordered.begin(), ordered.end(),
+                         [](const ChainJob& x, const ChainJob& y) { return x.nsteps > y.nsteps; });
+        jl = ordered.data();
+    }
     DevBuf& djobs = scratch_buf(ctx, SB_JOBS, sizeof(ChainJob) * nj);
     DevBuf& dsteps = scratch_buf(ctx, SB_STEPS, sizeof(ChainStep) * steps.size());
     DevBuf& Tb = scratch_buf(ctx, SB_TB, lds ? 0 : sizeof(zd) * 2 * (size_t)N2 * N2 * nj);
     DevBuf& cw = scratch_buf(ctx, SB_CW, lds ? 0 : sizeof(zd) * (nn + 3 * (size_t)n) * nj);
-    stage_upload(st, djobs.p, jobs.data(), sizeof(ChainJob) * nj, s);
+    stage_upload(st, djobs.p, jl, sizeof(ChainJob) * nj, s);
     stage_upload(st, dsteps.p, steps.data(), sizeof(ChainStep) * steps.size(), s);
     double flops = 0.0;
     for (const ChainJob& j : jobs) flops += (lds ? 16.0 : 64.0) * (double)n * n * n * j.nsteps;

@@ -542,7 +542,9 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
     LU inverses): boosts agree to 1e-10, every geometry and frequency, with one
     chunk and with many, and with the gaps' inverses by the LU.  A run whose
     inverse series fails its convergence check is redone with the LU inverses
-    (forced here by series_inverse=2): bitwise the series_inverse=0 result."""
+    (forced here by series_inverse=2): bitwise the series_inverse=0 result.
+    The chains of the geometries (3 to 45 disks) start longest first; in job
+    order (chain_order=0) the boosts are bitwise the same."""
     coords = W.grid_coords(64)
     rng = W.SplitMix64(1750)
     geoms = []
@@ -556,18 +558,22 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
     freqs = np.linspace(19.5e9, 20.5e9, 20)
     out = {}
     for name, kw in (("flat", {}), ("flat_chunked", dict(batch_bytes=2_000_000)), ("general", dict(parseval_series=0)),
-                     ("flat_lu", dict(series_inverse=0)), ("flat_rerun", dict(series_inverse=2))):
+                     ("flat_lu", dict(series_inverse=0)), ("flat_rerun", dict(series_inverse=2)),
+                     ("flat_job_order", dict(chain_order=0)), ("general_job_order", dict(parseval_series=0,
+                                                                                          chain_order=0))):
         engine_options(**kw)
         plan = B.TransformerBatchPlan(coords, sets, freqs, diskR=DISKR, emits=emits)
         plan.run([g.copy() for g in geoms], mode_set_of=set_of)
         out[name] = plan.fetch()
         plan.close()
-        engine_options(parseval_series=1, batch_bytes=0, series_inverse=1)
+        engine_options(parseval_series=1, batch_bytes=0, series_inverse=1, chain_order=1)
     for c in range(len(geoms)):
         assert rel(out["flat"][c], out["general"][c]) < 1e-10, (c, rel(out["flat"][c], out["general"][c]))
         assert rel(out["flat_chunked"][c], out["general"][c]) < 1e-10, c
         assert rel(out["flat_lu"][c], out["general"][c]) < 1e-10, c
         assert np.array_equal(out["flat_rerun"][c], out["flat_lu"][c]), c
+        assert np.array_equal(out["flat_job_order"][c], out["flat"][c]), c
+        assert np.array_equal(out["general_job_order"][c], out["general"][c]), c
 
 
 # one n per LDS chain kernel shape: the fixed-n kernels (9, 15, 25, 45) and the generic ones with 1, 2, 3, 4,
