@@ -113,6 +113,9 @@ _SIGS = {
     "bf_transformer_batch_plan_create": (ctypes.c_int, [_P, ctypes.POINTER(Coords), _I32, ctypes.POINTER(ModesS),
                                                         ctypes.POINTER(ctypes.c_void_p), _P, _I32, _I32, _D, _I32,
                                                         ctypes.POINTER(_P)]),
+    "bf_transformer_batch_plan_create_grids": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Coords),
+                                                              ctypes.POINTER(ModesS), ctypes.POINTER(ctypes.c_void_p),
+                                                              _P, _I32, _I32, _D, _I32, ctypes.POINTER(_P)]),
     "bf_transformer_batch_plan_run": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Setup), _P, _P]),
     "bf_transformer_batch_plan_fetch": (ctypes.c_int, [_P, _P]),
     "bf_transformer_batch_plan_destroy": (ctypes.c_int, [_P]),
@@ -315,6 +318,7 @@ class Marshalled:
     def setups(self, bdrys, ny, nx, nan_to_1e30=False):
         """ctypes array of Setup for many geometries: each field of all of them in one buffer, the structs
         filled at once in a numpy array laid out like Setup (the same structs as setup() per geometry).
+        ny, nx: the grid of every geometry, or sequences with each geometry's grid (surface map shape).
         nan_to_1e30: NaN permittivities become 1e30, in the caller's setups too (transformer's mutation,
         src/boost_fractor_transformer.jl:289), found with one scan of the joined buffer"""
         n = len(bdrys)
@@ -363,9 +367,9 @@ class Marshalled:
             s = np.asarray(b.relative_surfaces)
             if s.size and any(np.asarray(s[q]).view(np.float64 if s.dtype == np.complex128 else s.dtype).any()
                               for q in range(s.shape[0])):
-                if s.shape != (int(lens[i]), ny, nx):
-                    raise DimensionMismatch(
-                        f"relative_surfaces has shape {s.shape}, expected {(int(lens[i]), ny, nx)}")
+                want = (int(lens[i]), int(ny[i] if np.ndim(ny) else ny), int(nx[i] if np.ndim(nx) else nx))
+                if s.shape != want:
+                    raise DimensionMismatch(f"relative_surfaces has shape {s.shape}, expected {want}")
                 rec["surfaces"][i] = self.hold(as_c128(s)).ctypes.data
         return arr
 

@@ -311,10 +311,16 @@ def _batch_layout(nconf, modesets, mode_set_of, nf):
 
 
 def _batch_sources(m, coords, modesets, emits, diskR, f0):
-    """marshalled mode sets and one axion source per set (default: axion_induced_modes, uniform field)"""
+    """marshalled mode sets and one axion source per set (default: axion_induced_modes, uniform field);
+    coords: the grid of every set, or a list with each set's grid"""
     nsets = len(modesets)
+    grids = coords if isinstance(coords, (list, tuple)) else [coords] * nsets
+    for g, md in zip(grids, modesets):
+        if tuple(md.mode_patterns.shape[-2:]) != (g.ny, g.nx):
+            raise L.DimensionMismatch(f"mode patterns of shape {md.mode_patterns.shape[-2:]} on a "
+                                      f"{g.ny}x{g.nx} grid")
     if emits is None:
-        emits = [axion_induced_modes(coords, md, B=None, diskR=diskR, f=f0) for md in modesets]
+        emits = [axion_induced_modes(g, md, B=None, diskR=diskR, f=f0) for g, md in zip(grids, modesets)]
     if len(emits) != nsets:
         raise L.DimensionMismatch(f"{len(emits)} emits for {nsets} mode sets")
     mds = (L.ModesS * nsets)(*[_modes_struct(m, md) for md in modesets])
@@ -335,21 +341,34 @@ class TransformerBatchPlan:
     `run(bdrys, mode_set_of)` takes the candidate setups from the host and
     computes every boost on the device; `fetch()` returns one (nfreq, n) array per
     geometry of the last run.  Like transformer, NaN permittivities of each setup
-    are replaced by 1e30 in place."""
+    are replaced by 1e30 in place.
+
+    coords: one coordinate system for every mode set, or a list with one per mode
+    set (grids of different sizes in one plan; a geometry's surface maps are
+    sampled on its mode set's grid)."""
 
     def __init__(self, coords, modesets, f, prop=propagator, diskR=0.15, emits=None, precision="fp64", ctx=None):
         self.ctx = ctx or L.context()
         self.modesets = list(modesets) if isinstance(modesets, (list, tuple)) else [modesets]
         self.freqs = np.ascontiguousarray(np.atleast_1d(np.asarray(f, dtype=np.float64)))
+        nsets = len(self.modesets)
+        per_set = isinstance(coords, (list, tuple))
+        if per_set and len(coords) != nsets:
+            raise L.DimensionMismatch(f"{len(coords)} grids for {nsets} mode sets")
         self.coords = coords
+        self._grids = list(coords) if per_set else [coords] * nsets
         m = L.Marshalled()
-        c = m.coords(coords)
         mds, emp = _batch_sources(m, coords, self.modesets, emits, diskR, float(self.freqs[0]))
         h = ctypes.c_void_p()
-        L.check(L.lib().bf_transformer_batch_plan_create(self.ctx.handle, ctypes.byref(c), len(self.modesets), mds,
-                                                         emp, L.cptr(self.freqs), len(self.freqs), prop_kind(prop),
-                                                         float(diskR), precision_code(precision), ctypes.byref(h)),
-                self.ctx.handle)
+        args = (mds, emp, L.cptr(self.freqs), len(self.freqs), prop_kind(prop), float(diskR),
+                precision_code(precision), ctypes.byref(h))
+        if per_set:
+            cs = (L.Coords * nsets)(*[m.coords(g) for g in coords])
+            rc = L.lib().bf_transformer_batch_plan_create_grids(self.ctx.handle, nsets, cs, *args)
+        else:
+            c = m.coords(coords)
+            rc = L.lib().bf_transformer_batch_plan_create(self.ctx.handle, ctypes.byref(c), nsets, *args)
+        L.check(rc, self.ctx.handle)
         self.handle = h
         self._layout = None
 
@@ -357,7 +376,9 @@ class TransformerBatchPlan:
         nconf = len(bdrys)
         mode_set_of, offs, sizes = _batch_layout(nconf, self.modesets, mode_set_of, len(self.freqs))
         m = L.Marshalled()
-        setups = m.setups(bdrys, self.coords.ny, self.coords.nx, nan_to_1e30=True)
+        ny = np.array([g.ny for g in self._grids])[mode_set_of]
+        nx = np.array([g.nx for g in self._grids])[mode_set_of]
+        setups = m.setups(bdrys, ny, nx, nan_to_1e30=True)
         L.check(L.lib().bf_transformer_batch_plan_run(self.handle, nconf, setups, L.cptr(mode_set_of), L.cptr(offs)),
                 self.ctx.handle)
         self._layout = (offs, sizes)

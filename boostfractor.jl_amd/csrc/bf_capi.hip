@@ -632,17 +632,18 @@ int bf_transformer_batch(bf_ctx* ctx, int32_t nconf, const bf_setup* setups, con
     });
 }
 
-int bf_transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* coords, int32_t nsets, const bf_modes* modesets,
-                                     const double* const* emits, const double* freqs, int32_t nfreq,
-                                     int32_t prop_kind, double diskR, int32_t precision,
-                                     bf_transformer_batch_plan** out) {
+// both plan constructors: one grid for every mode set (ncoords = 1) or one per mode set (ncoords = nsets)
+static int batch_plan_create(bf_ctx* ctx, const bf_coords* coords, int32_t ncoords, int32_t nsets,
+                             const bf_modes* modesets, const double* const* emits, const double* freqs, int32_t nfreq,
+                             int32_t prop_kind, double diskR, int32_t precision, bf_transformer_batch_plan** out) {
     BF_REQUIRE(ctx, out != nullptr, BF_ERR_ARG, "out is NULL");
     *out = nullptr;
     TRY(check_kind(ctx, prop_kind));
     TRY(check_prec(ctx, precision));
-    TRY(check_coords(ctx, coords));
-    TRY(check_freqs(ctx, freqs, nfreq));
     BF_REQUIRE(ctx, nsets >= 1, BF_ERR_ARG, "nsets must be >= 1");
+    BF_REQUIRE(ctx, coords != nullptr, BF_ERR_ARG, "coords is NULL");
+    for (int32_t k = 0; k < ncoords; k++) TRY(check_coords(ctx, &coords[k]));
+    TRY(check_freqs(ctx, freqs, nfreq));
     BF_REQUIRE(ctx, modesets && emits, BF_ERR_ARG, "modesets and emits must not be NULL");
     for (int32_t m = 0; m < nsets; m++) {
         TRY(check_modes(ctx, &modesets[m]));
@@ -650,8 +651,24 @@ int bf_transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* coords, int32
     }
     BF_REQUIRE(ctx, ctx != nullptr, BF_ERR_ARG, "ctx is NULL");
     return guarded(ctx, [&] {
-        *out = bf::transformer_batch_plan_create(ctx, coords, nsets, modesets, emits, freqs, nfreq, prop_kind, diskR);
+        *out = bf::transformer_batch_plan_create(ctx, coords, ncoords, nsets, modesets, emits, freqs, nfreq,
+                                                 prop_kind, diskR);
     });
+}
+
+int bf_transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* coords, int32_t nsets, const bf_modes* modesets,
+                                     const double* const* emits, const double* freqs, int32_t nfreq,
+                                     int32_t prop_kind, double diskR, int32_t precision,
+                                     bf_transformer_batch_plan** out) {
+    return batch_plan_create(ctx, coords, 1, nsets, modesets, emits, freqs, nfreq, prop_kind, diskR, precision, out);
+}
+
+int bf_transformer_batch_plan_create_grids(bf_ctx* ctx, int32_t nsets, const bf_coords* coords,
+                                           const bf_modes* modesets, const double* const* emits, const double* freqs,
+                                           int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                                           bf_transformer_batch_plan** out) {
+    return batch_plan_create(ctx, coords, nsets, nsets, modesets, emits, freqs, nfreq, prop_kind, diskR, precision,
+                             out);
 }
 
 int bf_transformer_batch_plan_run(bf_transformer_batch_plan* plan, int32_t nconf, const bf_setup* setups,

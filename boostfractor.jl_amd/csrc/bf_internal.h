@@ -376,7 +376,8 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                            const bf_modes* modesets, const double* const* emits, const int32_t* set_of,
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
                            double* boost);
-bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int nsets,
+// co: one grid (ncoords = 1) or one per mode set (ncoords = nsets)
+bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int ncoords, int nsets,
                                                          const bf_modes* modesets, const double* const* emits,
                                                          const double* freqs, int nfreq, int kind, double diskR);
 void transformer_batch_plan_run(bf_transformer_batch_plan* pl, int nconf, const bf_setup* setups,

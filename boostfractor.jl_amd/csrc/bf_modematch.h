@@ -200,7 +200,7 @@ struct bf_transformer_plan {
 // geometries from the host and leaves every boost vector in `dboost`.
 struct bf_transformer_batch_plan {
     bf_ctx* ctx = nullptr;
-    bf::Grid g;
+    std::vector<bf::Grid> grids;   // one grid for every mode set, or one per mode set
     std::vector<bf::ModesDev> md;  // per mode set
     std::vector<bf::DevBuf> de0;   // per mode set: axion source, complex [n]
     std::vector<bf::ModeCache> mc; // per mode set: the k-bin grouping (grid only) kept across runs
@@ -213,6 +213,8 @@ struct bf_transformer_batch_plan {
     bf::Stager stager;  // pinned staging of the chunks' descriptor uploads
     bf::DevBuf dbad;    // int: a key's inverse series did not converge in this run
     int64_t reruns = 0; // runs redone with the LU inverses for that reason
+    // the grid of mode set `set` (its patterns, and the surface maps of its geometries, are sampled on it)
+    const bf::Grid& grid(int set) const { return grids[grids.size() == 1 ? 0 : (size_t)set]; }
 };
 
 // cubic B-spline coefficients of a propagation-matrix grid over the spacing axis

@@ -3508,20 +3508,23 @@ void transformer_plan_destroy(bf_transformer_plan* pl) { delete pl; }
 // the device, and the k-bin grouping of the grid once built; every run
 // recomputes the pattern FFTs, group Grams and flat-disk Gram matrix of each
 // mode set it uses (once per set and run).
-bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int nsets,
+// With ncoords = nsets, mode set `set` lives on its own grid co[set] (grids of any size in one plan): the
+// grid enters only the set's pattern transforms, Gram tables and Parseval scale, and from there on every
+// region is an n x n matrix.
+bf_transformer_batch_plan* transformer_batch_plan_create(bf_ctx* ctx, const bf_coords* co, int ncoords, int nsets,
                                                          const bf_modes* modesets, const double* const* emits,
                                                          const double* freqs, int nfreq, int kind, double diskR) {
     hipStream_t s = ctx->stream;
     auto* pl = new bf_transformer_batch_plan;
     try {
         pl->ctx = ctx;
-        pl->g.upload(co, s);
-        const size_t K = pl->g.K();
+        pl->grids.resize((size_t)ncoords);
+        for (int k = 0; k < ncoords; k++) pl->grids[(size_t)k].upload(&co[k], s);
         pl->md.resize(nsets);
         pl->de0.resize(nsets);
         pl->mc.resize(nsets);
         for (int set = 0; set < nsets; set++) {
-            pl->md[set].upload(&modesets[set], K, s);
+            pl->md[set].upload(&modesets[set], pl->grid(set).K(), s);
             pl->de0[set].alloc(sizeof(zd) * pl->md[set].n);
             upload(pl->de0[set].p, emits[set], sizeof(zd) * pl->md[set].n, s);
         }
@@ -3704,7 +3707,7 @@ static bool flat_prepare(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mo
         }
         base += (int64_t)nreg * nf;
     }
-    const bool grouped = mode_tables(ctx, pl->g, md, mc, !fw.disks.empty(), !gaps.empty(), pl->diskR);
+    const bool grouped = mode_tables(ctx, pl->grid(set), md, mc, !fw.disks.empty(), !gaps.empty(), pl->diskR);
     if (!grouped && !gaps.empty()) return false;
     // k0^2 per permittivity and frequency
     auto k0_table = [&](const std::vector<zd>& list) {
@@ -3816,7 +3819,7 @@ static void flat_launch(bf_ctx* ctx, bf_transformer_batch_plan* pl, int set, Mod
     const ModesDev& md = pl->md[set];
     const int n = md.n, nf = (int)pl->freqs.size();
     const size_t nn = (size_t)n * n;
-    const double scale = 1.0 / (double)pl->g.K();
+    const double scale = 1.0 / (double)pl->grid(set).K();
     Stager* st = &pl->stager;
     // series coefficients of the new keys, then every key's device pointer
     series_form(ctx, mc, n, scale, fw.fresh, fw.fresh_key, fw.nrows, st, fw.bad);
@@ -3906,7 +3909,6 @@ static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_set
                           const int64_t* boost_off, int* bad) {
     bf_ctx* ctx = pl->ctx;
     hipStream_t s = ctx->stream;
-    const Grid& g = pl->g;
     const int nfreq = (int)pl->freqs.size();
     const double* freqs = pl->freqs.data();
     // the spacing-series coefficients depend on the geometries' spacings: formed anew in every run
@@ -3939,6 +3941,7 @@ static void plan_run_once(bf_transformer_batch_plan* pl, int nconf, const bf_set
             if (set_of[c] == set) confs.push_back(c);
         if (confs.empty()) continue;
         const ModesDev& md = pl->md[set];
+        const Grid& g = pl->grid(set);
         // the mode set's pattern transforms, k-bin Gram tables and flat-disk Gram depend only on the plan's
         // modes and grid: formed by the first run that needs them and kept (no wait between mode sets)
         ModeCache& cache = pl->mc[set];
@@ -4103,7 +4106,7 @@ void transformer_batch_dev(bf_ctx* ctx, int nconf, const bf_setup* setups, const
                            const double* freqs, int nfreq, int kind, double diskR, const int64_t* boost_off,
                            double* boost) {
     std::unique_ptr<bf_transformer_batch_plan> pl(
-        transformer_batch_plan_create(ctx, co, nsets, modesets, emits, freqs, nfreq, kind, diskR));
+        transformer_batch_plan_create(ctx, co, 1, nsets, modesets, emits, freqs, nfreq, kind, diskR));
     transformer_batch_plan_run(pl.get(), nconf, setups, set_of, boost_off);
     transformer_batch_plan_fetch(pl.get(), boost);
 }

@@ -308,6 +308,16 @@ int bf_transformer_batch_plan_create(bf_ctx *ctx, const bf_coords *coords, int32
                                      const double *const *emits, const double *freqs, int32_t nfreq,
                                      int32_t prop_kind, double diskR, int32_t precision,
                                      bf_transformer_batch_plan **out);
+/* bf_transformer_batch_plan_create with a grid per mode set (no reference
+ * counterpart: the reference sweeps one grid at a time): mode set m and the
+ * geometries run on it (their surface maps) are sampled on coords[m], so one
+ * plan, and one run, covers geometries on grids of different sizes.  coords:
+ * [nsets]; the rest as bf_transformer_batch_plan_create.  Each geometry's
+ * results equal those of a plan on its set's grid alone. */
+int bf_transformer_batch_plan_create_grids(bf_ctx *ctx, int32_t nsets, const bf_coords *coords,
+                                           const bf_modes *modesets, const double *const *emits, const double *freqs,
+                                           int32_t nfreq, int32_t prop_kind, double diskR, int32_t precision,
+                                           bf_transformer_batch_plan **out);
 int bf_transformer_batch_plan_run(bf_transformer_batch_plan *plan, int32_t nconf, const bf_setup *setups,
                                   const int32_t *mode_set_of, const int64_t *boost_offset);
 int bf_transformer_batch_plan_fetch(bf_transformer_batch_plan *plan, double *boost);

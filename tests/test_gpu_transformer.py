@@ -576,6 +576,49 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
         assert np.array_equal(out["general_job_order"][c], out["general"][c]), c
 
 
+def test_batch_plan_grid_per_mode_set_matches_plans_per_grid():
+    """a plan whose mode sets live on grids of different sizes (64x64 and 48x40,
+    bf_transformer_batch_plan_create_grids), run once over geometries of both
+    (uneven disk counts, flat ones and a tilted rough one sampled on its set's
+    grid), gives each geometry the boosts of a plan on its grid alone"""
+    ga = W.grid_coords(64)
+    gb = B.SeedCoordinateSystem(X=(-0.24, 0.01, 0.23), Y=(-0.2, 0.01, 0.19))
+    assert (gb.nx, gb.ny) == (48, 40)
+    sets = [B.SeedModes(ga, ThreeDim=True, Mmax=3, Lmax=1, diskR=DISKR),
+            B.SeedModes(gb, ThreeDim=True, Mmax=4, Lmax=0, diskR=DISKR)]
+    emits = [B.axion_induced_modes(g, md, diskR=DISKR) for g, md in zip((ga, gb), sets)]
+    freqs = np.linspace(19.5e9, 20.5e9, 12)
+    rng = W.SplitMix64(1760)
+    geoms, set_of = [], []
+    for nd, k in ((5, 0), (17, 1), (3, 1), (26, 0), (9, 1), (12, 0)):
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        geoms.append(W.booster(ga if k == 0 else gb, ndisk=nd, spacings=list(sp), flat=True))
+        set_of.append(k)
+    rough = W.booster(gb, ndisk=3, seed=1761)
+    nreg = len(rough.distance)
+    rough.relative_tilt_x = np.array([2e-4 * rng.normal() for _ in range(nreg)])
+    XX, YY = np.meshgrid(gb.X, gb.Y)
+    surf = np.zeros((nreg, gb.ny, gb.nx), dtype=complex)
+    for i in range(1, nreg - 1):
+        surf[i] = 3e-6 * np.cos(13 * XX + i) * np.sin(7 * YY - i)
+    rough.relative_surfaces = surf
+    geoms.append(rough)
+    set_of.append(1)
+    plan = B.TransformerBatchPlan([ga, gb], sets, freqs, diskR=DISKR, emits=emits)
+    plan.run([g.copy() for g in geoms], mode_set_of=set_of)
+    both = plan.fetch()
+    plan.close()
+    for k, g in enumerate((ga, gb)):
+        idx = [c for c in range(len(geoms)) if set_of[c] == k]
+        alone = B.TransformerBatchPlan(g, [sets[k]], freqs, diskR=DISKR, emits=[emits[k]])
+        alone.run([geoms[c].copy() for c in idx])
+        got = alone.fetch()
+        alone.close()
+        for c, a in zip(idx, got):
+            assert both[c].shape == (len(freqs), sets[k].n)
+            assert rel(both[c], a) < 1e-12, (c, rel(both[c], a))
+
+
 # one n per LDS chain kernel shape: the fixed-n kernels (9, 15, 25, 45) and the generic ones with 1, 2, 3, 4,
 # 5, 6, 7, 9 and 10 columns per thread; 16, 32, 35, 42 and 45 have no spare column of the thread grid
 # (the axion sum takes its own loop), the others form it as column n of the products

@@ -195,6 +195,19 @@ def test_batch_plan_validation_without_device():
     # NULL output handle
     rc = lib.bf_transformer_batch_plan_create(None, ctypes.byref(c), 1, None, None, L.cptr(fr), 1, 0, 0.15, 0, None)
     assert rc == L.BF_ERR_ARG
+    # a grid per mode set: no mode set, no grids, a grid of zero size
+    h = ctypes.c_void_p(1)
+    rc = lib.bf_transformer_batch_plan_create_grids(None, 0, ctypes.byref(c), None, None, L.cptr(fr), 1, 0, 0.15, 0,
+                                                    ctypes.byref(h))
+    assert rc == L.BF_ERR_ARG and b"nsets" in lib.bf_last_error(None)
+    assert not h.value
+    rc = lib.bf_transformer_batch_plan_create_grids(None, 1, None, None, None, L.cptr(fr), 1, 0, 0.15, 0,
+                                                    ctypes.byref(h))
+    assert rc == L.BF_ERR_ARG and b"coords" in lib.bf_last_error(None)
+    bad = (L.Coords * 2)(c, L.Coords(0, 4, c.X, c.Y, c.kX, c.kY))
+    rc = lib.bf_transformer_batch_plan_create_grids(None, 2, bad, None, None, L.cptr(fr), 1, 0, 0.15, 0,
+                                                    ctypes.byref(h))
+    assert rc == L.BF_ERR_DIM
     # NULL plan
     assert lib.bf_transformer_batch_plan_run(None, 0, None, None, None) == L.BF_ERR_ARG
     assert lib.bf_transformer_batch_plan_fetch(None, None) == L.BF_ERR_ARG
@@ -233,6 +246,13 @@ def test_batched_setup_marshalling_matches_per_geometry():
         if one.surfaces:
             cnt = nreg * coords.ny * coords.nx
             assert np.array_equal(read(got.surfaces, cnt, np.complex128), read(one.surfaces, cnt, np.complex128))
+    # grids per geometry (plans with a grid per mode set): each rough geometry's maps checked against its own
+    small = W.grid_coords(12)
+    mixed = [W.rough_booster(small, ndisk=2, seed=5), geoms[2]]
+    batch = L.Marshalled().setups(mixed, np.array([12, 16]), np.array([12, 16]))
+    assert batch[0].surfaces and batch[1].surfaces
+    with pytest.raises(L.DimensionMismatch):
+        L.Marshalled().setups(mixed, np.array([16, 16]), np.array([16, 16]))
 
 
 def test_batched_marshalling_replaces_nan_in_place():
