@@ -1121,40 +1121,42 @@ static void launch_chain_tile(hipStream_t s, size_t nj, const ChainJob* jobs, co
 // The chain on the matrix cores for n <= 32 (v_mfma_f64_16x16x4f64).  The operands are zero-padded to
 // NP x NP (NP = 16 or 32); wave w of the (NP / 16)^2 owns output tile (w % (NP / 16), w / (NP / 16)) of
 // both products and forms it with four MFMAs per complex 4-k step (re += Ar Br + (-Ai) Bi,
-// im += Ar Bi + Ai Br), so per k step a lane reads one complex entry each of U, V, P and inv(P) for
-// 8 x 256 multiply-adds of the wave (the VALU kernels read 2 (R + C) entries for 2 R C).  U and V are
-// column-major, P and inv(P) transposed (both with leading dimension NP + 1: every operand read is 16
-// consecutive entries per k, conflict-free); column n of P and inv(P) is e0 when n < NP (the axion sum
-// comes out of the products), else the first wave adds it by a dot product.  The padding stays zero: U
-// and V are written for rows and columns below n only, P's rows and columns beyond n never.  Each MFMA
+// im += Ar Bi + Ai Br) or, GAUSS, three (Ar Br, Ai Bi and (Ar + Ai)(Br + Bi)).  The A operands U and V
+// live in LDS, column-major with leading dimension NP + 1 (every operand read is 16 consecutive entries
+// per k, conflict-free), in two buffers: region q reads one and writes the next region's U and V into
+// the other, so one barrier per region suffices (a buffer is rewritten only after every wave has passed
+// the barrier that ends the region reading it).  The B operands, P and inv(P), never enter LDS: each
+// lane loads its own entries P[k][j] (k = 4 s + lane / 16, j = the tile's column lane % 16) from global
+// memory into registers one region ahead.  Column n of the B operands is e0 when n < NP (the axion sum
+// comes out of the products), else the first wave adds it by a dot product.  The padding stays zero:
+// U and V are written for rows and columns below n only, and B entries beyond n are zero.  Each MFMA
 // sums its four k in an order of its own, so the results agree with the VALU kernels to rounding.
 template <int NP>
 struct ChainMfma {
     static constexpr int TW = NP / 16;        // tiles per dimension
     static constexpr int NT = 64 * TW * TW;   // one wave per tile
     static constexpr int LD = NP + 1;         // LDS leading dimension (entries)
-    static constexpr int PPT = NP * NP / NT;  // matrix entries per thread (n^2 <= NP^2)
+    static constexpr int KS = NP / 4;         // MFMA k steps
     static size_t lds_bytes(int n) {
         return sizeof(zd) * ((size_t)4 * NP * LD + 3 * (size_t)n) + (sizeof(double) + sizeof(int)) * NT +
                sizeof(int) * (size_t)n;
     }
 };
 
-template <int NP, bool GAUSS>
+// KF > 0: exactly KF k steps (n in (4 KF - 4, 4 KF]), so the next region's loads are issued in the same
+// order on every path and the first use waits for its own load only; KF = 0: k steps from n at run time.
+template <int NP, bool GAUSS, int KF>
 __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const ChainJob* __restrict__ jobs,
                                                                   const ChainStep* __restrict__ steps,
                                                                   const ChainSrc src, int n,
                                                                   const zd* __restrict__ e0_all,
                                                                   const zd* __restrict__ reflect, zd* boost, zd* refl) {
     using T = ChainMfma<NP>;
-    constexpr int LD = T::LD, NT = T::NT, TW = T::TW;
+    constexpr int LD = T::LD, NT = T::NT, TW = T::TW, KS = KF > 0 ? KF : T::KS;
     constexpr int SZ = NP * LD;
     extern __shared__ unsigned char smem_raw[];
-    zd* U = reinterpret_cast<zd*>(smem_raw);  // U[i + LD k]
-    zd* V = U + SZ;
-    zd* Pt = V + SZ;  // Pt[j + LD k] = P[k][j]
-    zd* Pit = Pt + SZ;
-    zd* ab = Pit + SZ;
+    zd* buf = reinterpret_cast<zd*>(smem_raw);  // U of buffer b at buf + 2 b SZ, V at buf + (2 b + 1) SZ
+    zd* ab = buf + 4 * SZ;
     zd* tmp = ab + n;
     double* sval = reinterpret_cast<double*>(tmp + 2 * n);
     int* sidx = reinterpret_cast<int*>(sval + NT);
@@ -1164,43 +1166,19 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
     const int tr = wave % TW, tc = wave / TW;  // output tile
     const int nn = n * n;
     const bool spare = n < NP;
+    const zd* e0 = e0_all + job.e0off;
     {
-        const zd z = cmk<double>(0.0, 0.0);
-        zd du = z, dv = cmk<double>(1.0, 0.0);
+        zd du = cmk<double>(0.0, 0.0), dv = cmk<double>(1.0, 0.0);
         if (job.nsteps > 0) {
             du = steps[job.step0].b;
             dv = steps[job.step0].a;
         }
-        for (int e = tid; e < SZ; e += NT) {
-            const int i = e % LD, k = e / LD;
-            const bool d = i == k && i < n;
-            U[e] = cmk<double>(d ? du.re : 0.0, d ? du.im : 0.0);
-            V[e] = cmk<double>(d ? dv.re : 0.0, d ? dv.im : 0.0);
-            Pt[e] = z;
-            Pit[e] = z;
+        for (int e = tid; e < 4 * SZ; e += NT) {
+            const int b = e / SZ, r = e % SZ, i = r % LD, k = r / LD;
+            const bool d = b < 2 && i == k && i < n;
+            const zd dd = b == 0 ? du : dv;
+            buf[e] = cmk<double>(d ? dd.re : 0.0, d ? dd.im : 0.0);
         }
-    }
-    __syncthreads();
-    if (spare)
-        for (int k = tid; k < n; k += NT) Pt[n + LD * k] = Pit[n + LD * k] = e0_all[job.e0off + k];
-    // the thread's entries of a region's matrices: global index g (column-major n x n), LDS index (transposed)
-    int gsrc[T::PPT], gi[T::PPT], gj[T::PPT], ldst[T::PPT];
-#pragma unroll
-    for (int m = 0; m < T::PPT; m++) {
-        const int g = tid + NT * m;
-        gsrc[m] = g < nn ? g : -1;
-        gi[m] = g < nn ? g % n : 0;
-        gj[m] = g < nn ? g / n : 0;
-        ldst[m] = gj[m] + LD * gi[m];
-    }
-    if (job.nsteps > 0) {
-        const ChainStep s0 = steps[job.step0];
-#pragma unroll
-        for (int m = 0; m < T::PPT; m++)
-            if (gsrc[m] >= 0) {
-                Pt[ldst[m]] = chain_entry(src, s0.disk, job.pbase + s0.reg, n, gsrc[m], gi[m], gj[m], false);
-                Pit[ldst[m]] = chain_entry(src, s0.disk, job.pbase + s0.reg, n, gsrc[m], gi[m], gj[m], true);
-            }
     }
     zd abr[4];
 #pragma unroll
@@ -1208,33 +1186,73 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
     const bool own = spare && tc == n / 16 && (lane & 15) == n % 16;  // lanes holding column n
     const int ar = tr * 16 + (lane & 15), bc = tc * 16 + (lane & 15), kq = lane >> 4;
     const int kp = (n + 3) & ~3;  // k steps cover k < n
-    // matrices of region q arrive two regions ahead: region q + 1 sits in `cur` while region q runs, and
-    // the loads of region q + 2 go to `nxt` (the register sets alternate)
-    auto fetch = [&](int q, zd* dp, zd* dpi) {
-        const bool ok = q < job.nsteps;
-        const ChainStep sq = steps[job.step0 + (ok ? q : 0)];
+    // The lane's B entries of the coming region: P[k][bc] and inv(P)[k][bc] for k = 4 s + kq, e0[k] in
+    // column n.  Indices are clamped into the matrix instead of zeroing: rows k >= n meet the zero columns
+    // of U and V, and columns beyond n of the products are never used.  A gap's entry s is a plain load
+    // (one address per lane, e0's in column n) issued right after the current region's MFMAs of step s
+    // have read the register, and first waited for where the next region uses it; a disk's entries are
+    // formed from G, inv(G) and the phases (L2-resident) after the current region's products.
+    const int cc = bc < n ? bc : n - 1;
+    const bool col = bc == n;
+    zd bp[KS], bpi[KS];
+    auto gap_entry = [&](int64_t m, int s) {
+        const int k = 4 * s + kq, kk = k < n ? k : n - 1;
+        const int64_t g = m * nn + kk + n * cc;
+        bp[s] = *(col ? e0 + kk : src.P + g);
+        bpi[s] = *(col ? e0 + kk : src.Pinv + g);
+    };
+    // a disk's entry s in two halves: the loads (G, inv(G) into bp, bpi; the row phase, or e0's in column n,
+    // into ap), issued with the gap loads' timing, and the products, formed after the region's MFMAs
+    zd ap[KS];
+    auto disk_load = [&](int64_t m, int s) {
+        const int k = 4 * s + kq, kk = k < n ? k : n - 1;
+        ap[s] = *(col ? e0 + kk : src.ph + m * n + kk);
+        bp[s] = src.G[kk + n * cc];
+        bpi[s] = src.Ginv[kk + n * cc];
+    };
+    auto disk_form = [&](int64_t m) {
+        const zd phi = src.phinv[m * n + cc];
 #pragma unroll
-        for (int m = 0; m < T::PPT; m++) {
-            dp[m] = dpi[m] = cmk<double>(0.0, 0.0);
-            if (ok && gsrc[m] >= 0) {
-                dp[m] = chain_entry(src, sq.disk, job.pbase + sq.reg, n, gsrc[m], gi[m], gj[m], false);
-                dpi[m] = chain_entry(src, sq.disk, job.pbase + sq.reg, n, gsrc[m], gi[m], gj[m], true);
-            }
+        for (int s = 0; s < KS; s++) {
+            const zd p = cmul(ap[s], bp[s]), pi = cmul(bpi[s], phi);
+            bp[s] = cmk<double>(col ? ap[s].re : p.re, col ? ap[s].im : p.im);
+            bpi[s] = cmk<double>(col ? ap[s].re : pi.re, col ? ap[s].im : pi.im);
         }
     };
-    auto region = [&](int q, const zd* cur, const zd* curi, zd* nxt, zd* nxti) {
+    auto disk_entries = [&](int64_t m) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) disk_load(m, s);
+        disk_form(m);
+    };
+    if (job.nsteps > 0) {
+        const ChainStep s0 = steps[job.step0];
+        const int64_t m = job.pbase + s0.reg;
+        if (s0.disk) {
+            disk_entries(m);
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; s++) gap_entry(m, s);
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < job.nsteps; q++) {
         const ChainStep st = steps[job.step0 + q];
         const bool more = q + 1 < job.nsteps;
         const ChainStep sn = more ? steps[job.step0 + q + 1] : st;
-        fetch(q + 2, nxt, nxti);
+        const int64_t mn = job.pbase + sn.reg;
+        const bool gap_next = more && !sn.disk;
+        const zd* U = buf + 2 * (q & 1) * SZ;
+        const zd* V = U + SZ;
+        zd* Un = buf + 2 * ((q + 1) & 1) * SZ;
+        zd* Vn = Un + SZ;
         d4 c21r = {0.0, 0.0, 0.0, 0.0}, c21i = c21r, c22r = c21r, c22i = c21r;
         d4 g21 = c21r, g22 = c21r;  // GAUSS: sums of (Ar + Ai)(Br + Bi); c..r, c..i then hold Ar Br, Ai Bi
 #pragma unroll
-        for (int ks = 0; ks < NP; ks += 4) {
-            if (ks >= kp) break;  // k >= n are zero padding
-            const int k = ks + kq;
+        for (int s = 0; s < KS; s++) {
+            if (KF == 0 && 4 * s >= kp) break;  // k >= n are zero padding
+            const int k = 4 * s + kq;
             const zd u = U[ar + LD * k], v = V[ar + LD * k];
-            const zd p = Pt[bc + LD * k], pi = Pit[bc + LD * k];
+            const zd p = bp[s], pi = bpi[s];
             if constexpr (GAUSS) {
                 c21r = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, p.re, c21r, 0, 0, 0);
                 c21i = __builtin_amdgcn_mfma_f64_16x16x4f64(u.im, p.im, c21i, 0, 0, 0);
@@ -1252,7 +1270,10 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
                 c22r = __builtin_amdgcn_mfma_f64_16x16x4f64(-v.im, pi.im, c22r, 0, 0, 0);
                 c22i = __builtin_amdgcn_mfma_f64_16x16x4f64(v.im, pi.re, c22i, 0, 0, 0);
             }
+            if (gap_next) gap_entry(mn, s);
+            if (more && sn.disk) disk_load(mn, s);
         }
+        if (more && sn.disk) disk_form(mn);
         if constexpr (GAUSS) {  // re = Ar Br - Ai Bi, im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -1271,11 +1292,9 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
         }
         if (!spare && wave == 0 && lane < n) {
             zd sacc = cmk<double>(0.0, 0.0);
-            for (int k = 0; k < n; k++)
-                sacc = cmac(sacc, cadd(U[lane + LD * k], V[lane + LD * k]), e0_all[job.e0off + k]);
+            for (int k = 0; k < n; k++) sacc = cmac(sacc, cadd(U[lane + LD * k], V[lane + LD * k]), e0[k]);
             abr[0] = cadd(abr[0], cmul(st.S, sacc));
         }
-        __syncthreads();
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int i = tr * 16 + 4 * v + (lane >> 4), j = bc;
@@ -1286,24 +1305,11 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
                     t21 = cadd(cmul(sn.a, o21), cmul(sn.b, o22));
                     t22 = cadd(cmul(sn.b, o21), cmul(sn.a, o22));
                 }
-                U[i + LD * j] = t21;
-                V[i + LD * j] = t22;
+                Un[i + LD * j] = t21;
+                Vn[i + LD * j] = t22;
             }
         }
-#pragma unroll
-        for (int m = 0; m < T::PPT; m++)
-            if (more && gsrc[m] >= 0) {
-                Pt[ldst[m]] = cur[m];
-                Pit[ldst[m]] = curi[m];
-            }
         __syncthreads();
-    };
-    zd pa[T::PPT], pia[T::PPT], pb[T::PPT], pib[T::PPT];
-    fetch(1, pa, pia);
-    __syncthreads();
-    for (int q = 0; q < job.nsteps; q += 2) {
-        region(q, pa, pia, pb, pib);
-        if (q + 1 < job.nsteps) region(q + 1, pb, pib, pa, pia);
     }
     if (own) {
 #pragma unroll
@@ -1313,13 +1319,15 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
         }
     }
     if (!spare && wave == 0 && lane < n) ab[lane] = abr[0];
-    // T21, T22 compacted to n x n (column-major) over the P tables for the solves
-    zd* T21 = Pt;
-    zd* T22 = Pit;
+    // T21, T22 (the last region's products, in buffer nsteps & 1) compacted to n x n (column-major) over
+    // the other buffer for the solves
+    const zd* Uf = buf + 2 * (job.nsteps & 1) * SZ;
+    zd* T21 = buf + 2 * ((job.nsteps + 1) & 1) * SZ;
+    zd* T22 = T21 + SZ;
     for (int e = tid; e < nn; e += NT) {
         const int i = e % n, j = e / n;
-        T21[e] = U[i + LD * j];
-        T22[e] = V[i + LD * j];
+        T21[e] = Uf[i + LD * j];
+        T22[e] = Uf[SZ + i + LD * j];
     }
     __syncthreads();
     // boost = -T22 \ axion_beam ;  refl = -T22 \ (T21 * reflect)
@@ -1342,13 +1350,14 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
     }
 }
 
-template <int NP, bool GAUSS>
+template <int NP, bool GAUSS, int KF = 0>
 static void launch_chain_mfma(hipStream_t s, size_t nj, const ChainJob* jobs, const ChainStep* steps,
                               const ChainSrc& src, int n, const zd* e0, const zd* reflect, zd* boost, zd* refl) {
+    if (KF > 0 && (n + 3) / 4 != KF) throw Error(BF_ERR_UNSUPPORTED, "chain kernel k steps do not match n");
     const size_t smem = ChainMfma<NP>::lds_bytes(n);
-    allow_lds((const void*)k_chain_mfma<NP, GAUSS>, smem);
-    hipLaunchKernelGGL((k_chain_mfma<NP, GAUSS>), dim3((unsigned)nj), dim3(ChainMfma<NP>::NT), smem, s, jobs, steps,
-                       src, n, e0, reflect, boost, refl);
+    allow_lds((const void*)k_chain_mfma<NP, GAUSS, KF>, smem);
+    hipLaunchKernelGGL((k_chain_mfma<NP, GAUSS, KF>), dim3((unsigned)nj), dim3(ChainMfma<NP>::NT), smem, s, jobs,
+                       steps, src, n, e0, reflect, boost, refl);
 }
 
 // The matrix-core chain with the two products in separate waves: wave w of 2 (NP / 16)^2 computes
@@ -1556,6 +1565,14 @@ static void launch_chain_lds(hipStream_t s, size_t nj, const ChainJob* jobs, con
             launch_chain_mfma2<16>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
         else
             launch_chain_mfma2<32>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
+        BF_CHECK_LAUNCH();
+        return;
+    }
+    if (mf == 1 && n > 24 && n <= 32) {  // the sweep's n = 25: k steps fixed at compile time
+        if (n <= 28)
+            launch_chain_mfma<32, true, 7>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
+        else
+            launch_chain_mfma<32, true, 8>(s, nj, jobs, steps, src, n, e0, reflect, boost, refl);
         BF_CHECK_LAUNCH();
         return;
     }
