@@ -1301,9 +1301,12 @@ __global__ void __launch_bounds__(ChainMfma<NP>::NT, 2) k_chain_mfma(const Chain
             if (i < n && j < n) {
                 const zd o21 = cmk<double>(c21r[v], c21i[v]), o22 = cmk<double>(c22r[v], c22i[v]);
                 zd t21 = o21, t22 = o22;
+                // a T21 + b T22 and b T21 + a T22 with one complex product: a + b = 1 at every boundary
+                // (a = (nr + nl) / 2nr, b = (nr - nl) / 2nr), the same up to rounding
                 if (more) {
-                    t21 = cadd(cmul(sn.a, o21), cmul(sn.b, o22));
-                    t22 = cadd(cmul(sn.b, o21), cmul(sn.a, o22));
+                    const zd ad = cmul(sn.a, csub(o21, o22));
+                    t21 = cadd(o22, ad);
+                    t22 = csub(o21, ad);
                 }
                 Un[i + LD * j] = t21;
                 Vn[i + LD * j] = t22;
