@@ -210,6 +210,7 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"chain_tile", &bf::EngineOptions::chain_tile},
         {"chain_mfma", &bf::EngineOptions::chain_mfma},
         {"chain_order", &bf::EngineOptions::chain_order},
+        {"pgemm_herm", &bf::EngineOptions::pgemm_herm},
     };
     ip = nullptr;
     for (const Int& e : ints)

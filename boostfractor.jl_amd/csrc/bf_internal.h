@@ -106,6 +106,7 @@ struct EngineOptions {
                               // separate waves), 2 / 3 for every n <= 32 (both products per wave, four / three
                               // MFMAs per complex product), 0 never
     int chain_order = 1;      // chain blocks start longest chain first (0: in job order)
+    int pgemm_herm = 1;       // Parseval product over the upper triangles when the Grams are Hermitian
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)
 };
@@ -234,7 +235,7 @@ inline void stage_upload(Stager* st, void* dst, const void* src, size_t bytes, h
 enum ScratchSlot : int {
     SB_PW = 1, SB_DST, SB_SEL, SB_JOBS, SB_STEPS, SB_TB, SB_CW, SB_P, SB_PINV, SB_BOOST,
     SB_SLOTS, SB_FLAG, SB_K0, SB_DZ, SB_PGSLOT, SB_PGNK, SB_PGK0,
-    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV, SB_SINVJ, SB_SINVSEL, SB_FISLICE, SB_FPH
+    SB_SKEY, SB_SPROB, SB_SSLICE, SB_SROWDST, SB_SW, SB_SD, SB_PGPART, SB_FENT, SB_FDISK, SB_FK0, SB_FINV, SB_SINVJ, SB_SINVSEL, SB_FISLICE, SB_FPH, SB_HFLAG
 };
 }  // namespace bf
 

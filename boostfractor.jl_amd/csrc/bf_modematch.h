@@ -40,6 +40,11 @@ struct ModeCache {
     // P = sum_g w_g H_g with H_g = sum_{k in g} conj(Fhat_i(k)) FhatM_j(k).
     bool haveH = false;
     DevBuf H;  // [ngroups][n][n] (column-major)
+    // H Hermitian (masked and unmasked pattern transforms equal): its upper triangles packed, Hp[g][t] =
+    // H[g][i + n j] for tri[t] = i | j << 16 (i <= j, column by column), nt = n (n + 1) / 2 entries each
+    bool herm = false;
+    int nt = 0;
+    DevBuf Hp, tri;
     // the grouping itself depends on the grid only
     bool haveIdx = false;
     int ngroups = 0;

@@ -1744,10 +1744,11 @@ __global__ void __launch_bounds__(256) k_group_gram_lds(const zd* __restrict__ A
 // TAB = true: the weights are read from a table wtab[g][m] (M rows) instead
 // (the coefficient rows of the spacing series below); kz and the per-block
 // k0^2 lists are then unused.
-template <int WN>
+template <int WN, bool HERM = false>
 struct PgShape {
     static constexpr int WM = 2;                     // waves along problems
-    static constexpr int BM = 32 * WM, BN = 64 * WN;  // block tile
+    static constexpr int CT = HERM ? 2 : 4;          // 16-entry column tiles per wave
+    static constexpr int BM = 32 * WM, BN = 16 * CT * WN;  // block tile
     static constexpr int KC = 8;                     // groups per chunk
     static constexpr int NT = 64 * WM * WN;
     static constexpr int LDA = BM + 2, LDB = BN + 2;  // padded LDS rows (doubles)
@@ -1758,14 +1759,19 @@ struct PgShape {
 };
 constexpr int PG_BM = 64;  // rows per block (all shapes)
 
-template <int WN, bool TAB>
-__global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
+// HERM: H holds the packed upper triangles (ncol = n (n + 1) / 2 columns, tri[t] = i | j << 16): with H_g
+// Hermitian, A = sum Re(w) H and B = sum Im(w) H are Hermitian and the product is A + i B, so the four
+// real sums Wr Hr, Wr Hi, Wi Hr, Wi Hi at (i, j), i <= j, give both (i, j) = (A + i B)_ij and
+// (j, i) = conj(A_ij) + i conj(B_ij): half the entries for the same MFMAs per entry.  Else ncol = nn.
+template <int WN, bool TAB, bool HERM>
+__global__ void __launch_bounds__((PgShape<WN, HERM>::NT), 4 / WN) k_parseval_gemm(
     const zd* __restrict__ H, int nn, int NG, const double* __restrict__ gkx, const double* __restrict__ gky,
     const PwDesc* __restrict__ pd, const int* __restrict__ row_slot, const int* __restrict__ blk_nk,
     const zd* __restrict__ blk_k0sq, int M, double scale, zd* const* __restrict__ dst, int nmb,
-    const zd* __restrict__ wtab, int ncb, int cps, zd* __restrict__ part) {
-    using S = PgShape<WN>;
-    constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NT = S::NT;
+    const zd* __restrict__ wtab, int ncb, int cps, zd* __restrict__ part, int ncol, int n,
+    const int* __restrict__ tri) {
+    using S = PgShape<WN, HERM>;
+    constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NT = S::NT, CT = S::CT;
     __shared__ double swr[2][KC][S::LDA], swi[2][KC][S::LDA];
     __shared__ double shr[2][KC][S::LDB], shi[2][KC][S::LDB];
     __shared__ zd skz[TAB ? 1 : 3][BM][KC];
@@ -1788,7 +1794,7 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
     }
     // H: thread t loads entries c0 + t % BN, chunk rows t / BN + (NT / BN) j
     const int he = t % BN, hk0 = t / BN;
-    const bool hvalid = c0 + he < nn;
+    const bool hvalid = c0 + he < ncol;
     zd wreg[S::WPT], hreg[S::HPT];
     auto form_kz = [&](int c) {  // kz of chunk c into skz[c % 3]
         if constexpr (!TAB) {
@@ -1803,7 +1809,7 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
 #pragma unroll
         for (int j = 0; j < S::HPT; j++) {
             const int g = c * KC + hk0 + (NT / BN) * j;
-            hreg[j] = (hvalid && g < NG) ? H[(size_t)g * nn + c0 + he] : cmk<double>(0.0, 0.0);
+            hreg[j] = (hvalid && g < NG) ? H[(size_t)g * ncol + c0 + he] : cmk<double>(0.0, 0.0);
         }
     };
     auto form_w = [&](int c) {  // reads skz[c % 3]
@@ -1835,11 +1841,14 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
             shi[b][k][he] = hreg[j].im;
         }
     };
-    d4 are[2][4], aim[2][4];
+    constexpr int NA = HERM ? 4 : 2;  // accumulators per tile: (Re, Im) or (Wr Hr, Wr Hi, Wi Hr, Wi Hi)
+    d4 acc[2][CT][NA];
 #pragma unroll
     for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) are[rt][ct] = aim[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+            for (int a = 0; a < NA; a++) acc[rt][ct][a] = d4{0.0, 0.0, 0.0, 0.0};
     const int cbeg = ks_ * cps, cend = min((NG + KC - 1) / KC, cbeg + cps);
     form_kz(cbeg);
     form_kz(cbeg + 1);
@@ -1857,26 +1866,34 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
         for (int ks = 0; ks < KC; ks += 4) {
             // A operand 16 x 4: lane -> (row lane % 16, group ks + lane / 16); B operand 4 x 16: (group, entry)
             const int kk = ks + (lane >> 4);
-            double ar[2], ai[2], br[4], bi[4];
+            double ar[2], ai[2], br[CT], bi[CT];
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
                 ar[rt] = swr[b][kk][wm * 32 + rt * 16 + (lane & 15)];
                 ai[rt] = swi[b][kk][wm * 32 + rt * 16 + (lane & 15)];
             }
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) {
-                br[ct] = shr[b][kk][wn * 64 + ct * 16 + (lane & 15)];
-                bi[ct] = shi[b][kk][wn * 64 + ct * 16 + (lane & 15)];
+            for (int ct = 0; ct < CT; ct++) {
+                br[ct] = shr[b][kk][wn * 16 * CT + ct * 16 + (lane & 15)];
+                bi[ct] = shi[b][kk][wn * 16 * CT + ct * 16 + (lane & 15)];
             }
 #pragma unroll
             for (int rt = 0; rt < 2; rt++) {
                 const double nai = -ai[rt];
 #pragma unroll
-                for (int ct = 0; ct < 4; ct++) {
-                    are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], br[ct], are[rt][ct], 0, 0, 0);
-                    aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], bi[ct], aim[rt][ct], 0, 0, 0);
-                    are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[ct], are[rt][ct], 0, 0, 0);
-                    aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], br[ct], aim[rt][ct], 0, 0, 0);
+                for (int ct = 0; ct < CT; ct++) {
+                    d4* A = acc[rt][ct];
+                    if constexpr (HERM) {
+                        A[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], br[ct], A[0], 0, 0, 0);
+                        A[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], bi[ct], A[1], 0, 0, 0);
+                        A[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], br[ct], A[2], 0, 0, 0);
+                        A[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], bi[ct], A[3], 0, 0, 0);
+                    } else {
+                        A[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], br[ct], A[0], 0, 0, 0);
+                        A[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt], bi[ct], A[1], 0, 0, 0);
+                        A[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[ct], A[0], 0, 0, 0);
+                        A[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt], br[ct], A[1], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -1887,7 +1904,7 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
         __syncthreads();
     }
     // D layout (v_mfma_f64_16x16x4f64): register v of lane holds D[4 v + lane / 16][lane % 16]
-    const int r0 = m0 + wm * 32, e0 = c0 + wn * 64;
+    const int r0 = m0 + wm * 32, e0 = c0 + wn * 16 * CT;
 #pragma unroll
     for (int rt = 0; rt < 2; rt++) {
 #pragma unroll
@@ -1896,9 +1913,19 @@ __global__ void __launch_bounds__(PgShape<WN>::NT, 4 / WN) k_parseval_gemm(
             if (m >= M) continue;
             zd* out = part ? part + ((size_t)ks_ * M + m) * nn : dst[m];
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) {
+            for (int ct = 0; ct < CT; ct++) {
                 const int e = e0 + ct * 16 + (lane & 15);
-                if (e < nn) out[e] = cmk<double>(are[rt][ct][v], aim[rt][ct][v]);
+                if constexpr (HERM) {
+                    if (e < ncol) {
+                        const int ij = tri[e], i = ij & 0xffff, j = ij >> 16;
+                        const double rr = acc[rt][ct][0][v], ri = acc[rt][ct][1][v];
+                        const double ir = acc[rt][ct][2][v], ii = acc[rt][ct][3][v];
+                        out[i + n * j] = cmk<double>(rr - ii, ri + ir);
+                        if (i != j) out[j + n * i] = cmk<double>(rr + ii, ir - ri);
+                    }
+                } else {
+                    if (e < nn) out[e] = cmk<double>(acc[rt][ct][0][v], acc[rt][ct][1][v]);
+                }
             }
         }
     }
@@ -1993,6 +2020,25 @@ __global__ void k_combine_moments(zd* P, const zd* __restrict__ M, const CombJob
 __global__ void k_copy_z(zd* dst, const zd* __restrict__ src, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// *flag = 1 if a and b differ in any bit (n complex entries)
+__global__ void k_differs_z(const zd* __restrict__ a, const zd* __restrict__ b, size_t n, int* flag) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (__double_as_longlong(a[i].re) != __double_as_longlong(b[i].re) ||
+            __double_as_longlong(a[i].im) != __double_as_longlong(b[i].im))
+            *flag = 1;
+}
+
+// the upper triangles (i <= j) of the group Grams, packed: Hp[g][t] = H[g][i + n j] with tri[t] = i | j << 16
+__global__ void k_pack_upper(const zd* __restrict__ H, int G, int n, const int* __restrict__ tri, int nt,
+                             zd* __restrict__ Hp) {
+    const size_t total = (size_t)G * nt;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = x / nt;
+        const int ij = tri[x % nt];
+        Hp[x] = H[g * n * n + (ij & 0xffff) + (size_t)n * (ij >> 16)];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2111,6 +2157,35 @@ static bool build_parseval_groups(bf_ctx* ctx, const Grid& g, const ModesDev& md
     }
     BF_CHECK_LAUNCH();
     mc.haveH = true;
+    // With the masked and unmasked pattern transforms equal (patterns that vanish outside the aperture),
+    // H_g = sum conj(Fhat_i) Fhat_j is Hermitian, and so are sum_g Re(w_g) H_g and sum_g Im(w_g) H_g:
+    // the product needs the upper triangles only (k_parseval_gemm<.., HERM>)
+    mc.herm = false;
+    {
+        const size_t cnt = (size_t)n * md.Ktot;
+        DevBuf& dflag = scratch_buf(ctx, SB_HFLAG, sizeof(int));
+        BF_HIP(hipMemsetAsync(dflag.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_differs_z, dim3(g1(cnt, 4096)), dim3(256), 0, s, mc.Fhat.as<zd>(), mc.FhatM.as<zd>(),
+                           cnt, dflag.as<int>());
+        BF_CHECK_LAUNCH();
+        int differs = 1;
+        download(&differs, dflag.p, sizeof(int), s);
+        BF_HIP(hipStreamSynchronize(s));
+        if (!differs) {
+            std::vector<int> tri;
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i <= j; i++) tri.push_back(i | j << 16);
+            mc.nt = (int)tri.size();
+            mc.tri.alloc(sizeof(int) * tri.size());
+            upload(mc.tri.p, tri.data(), sizeof(int) * tri.size(), s);
+            mc.Hp.alloc(sizeof(zd) * (size_t)G * mc.nt);
+            hipLaunchKernelGGL(k_pack_upper, dim3(g1((size_t)G * mc.nt, 4096)), dim3(256), 0, s, mc.H.as<zd>(), G, n,
+                               mc.tri.as<int>(), mc.nt, mc.Hp.as<zd>());
+            BF_CHECK_LAUNCH();
+            BF_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+            mc.herm = true;
+        }
+    }
     return true;
 }
 
@@ -2571,9 +2646,8 @@ void disk_diag_flush(bf_ctx* ctx, ModeCache& mc, const ModesDev& md, ParsevalQue
 // to a workspace and k_pgemm_reduce adds them in split order.  The split does
 // not depend on the shape, so both shapes sum every entry in the same order.
 // Returns the split count and the chunks per split.
-static std::pair<int, int> parseval_splits(size_t nmb, size_t nn, size_t NG) {
+static std::pair<int, int> parseval_splits(size_t nmb, size_t ncb, size_t NG) {
     const int nch = (int)((NG + PgShape<2>::KC - 1) / PgShape<2>::KC);
-    const size_t ncb = (nn + PgShape<2>::BN - 1) / PgShape<2>::BN;
     const double slots = 512.0;
     int best = 1;
     double best_eff = 0.0;
@@ -2587,24 +2661,43 @@ static std::pair<int, int> parseval_splits(size_t nmb, size_t nn, size_t NG) {
     return {(nch + cps - 1) / cps, cps};
 }
 
+// the product over the packed upper triangles when the mode set's Grams are Hermitian (option pgemm_herm)
+static bool parseval_herm(const ModeCache& mc) { return mc.herm && opts().pgemm_herm; }
+// GEMM columns of the product: n (n + 1) / 2 packed entries, or all n^2
+static size_t parseval_cols(const ModeCache& mc, size_t nn) { return parseval_herm(mc) ? (size_t)mc.nt : nn; }
+
 template <bool TAB>
 static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeCache& mc, size_t nn, size_t NG,
                                  const PwDesc* pd, const int* slot, const int* nk, const zd* k0, size_t M,
                                  double scale, zd* const* dst, int nmb, const zd* wtab) {
-    const size_t BN = wn == 2 ? PgShape<2>::BN : PgShape<4>::BN;
-    const int ncb = (int)((nn + BN - 1) / BN);
-    const auto sp = parseval_splits((size_t)nmb, nn, NG);
+    const bool herm = parseval_herm(mc);
+    const size_t ncol = parseval_cols(mc, nn);
+    const size_t BN = herm ? (wn == 2 ? PgShape<2, true>::BN : PgShape<4, true>::BN)
+                           : (wn == 2 ? PgShape<2>::BN : PgShape<4>::BN);
+    const int ncb = (int)((ncol + BN - 1) / BN);
+    const auto sp = parseval_splits((size_t)nmb, (size_t)ncb, NG);
     zd* part = nullptr;
     if (sp.first > 1) part = scratch_buf(ctx, SB_PGPART, sizeof(zd) * (size_t)sp.first * M * nn).as<zd>();
     const dim3 grid((unsigned)((size_t)nmb * ncb * sp.first));
-    if (wn == 2)
-        hipLaunchKernelGGL((k_parseval_gemm<2, TAB>), grid, dim3(PgShape<2>::NT), 0, s, mc.H.as<zd>(), (int)nn,
-                           (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst,
-                           nmb, wtab, ncb, sp.second, part);
-    else
-        hipLaunchKernelGGL((k_parseval_gemm<4, TAB>), grid, dim3(PgShape<4>::NT), 0, s, mc.H.as<zd>(), (int)nn,
-                           (int)NG, mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst,
-                           nmb, wtab, ncb, sp.second, part);
+    const zd* Hs = herm ? mc.Hp.as<zd>() : mc.H.as<zd>();
+    const int n = (int)std::lround(std::sqrt((double)nn));
+    const int* tri = herm ? mc.tri.as<int>() : nullptr;
+#define BF_PGEMM(W, HM)                                                                                         \
+    hipLaunchKernelGGL((k_parseval_gemm<W, TAB, HM>), grid, dim3(PgShape<W, HM>::NT), 0, s, Hs, (int)nn, (int)NG, \
+                       mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst, nmb, wtab,  \
+                       ncb, sp.second, part, (int)ncol, n, tri)
+    if (herm) {
+        if (wn == 2)
+            BF_PGEMM(2, true);
+        else
+            BF_PGEMM(4, true);
+    } else {
+        if (wn == 2)
+            BF_PGEMM(2, false);
+        else
+            BF_PGEMM(4, false);
+    }
+#undef BF_PGEMM
     BF_CHECK_LAUNCH();
     if (part) {
         hipLaunchKernelGGL(k_pgemm_reduce, dim3((unsigned)((nn + 255) / 256), (unsigned)M), dim3(256), 0, s, part,
@@ -2616,11 +2709,14 @@ static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeC
 // Column-block width of the Parseval product for n x n matrices: the tuned one
 // if the autotuner has timed this size, else 2 (128 entries per block) unless
 // 256 fill the matrix better
-static int parseval_wn(bf_ctx* ctx, size_t nn, size_t NG) {
+static int parseval_wn(bf_ctx* ctx, const ModeCache& mc, size_t nn, size_t NG) {
     int wn = opts().pg_wn;
     const std::string key = "pgemm nn" + std::to_string(nn) + " G" + std::to_string(NG);
     if (wn != 2 && wn != 4 && ctx->tuned.count(key)) wn = (int)ctx->tuned[key][0];
-    if (wn != 2 && wn != 4) wn = ((nn + 127) / 128 * 128 <= (nn + 255) / 256 * 256) ? 2 : 4;
+    if (wn != 2 && wn != 4) {
+        const size_t c = parseval_cols(mc, nn), b2 = parseval_herm(mc) ? PgShape<2, true>::BN : PgShape<2>::BN;
+        wn = ((c + b2 - 1) / b2 * b2 <= (c + 2 * b2 - 1) / (2 * b2) * (2 * b2)) ? 2 : 4;
+    }
     return wn;
 }
 
@@ -2849,9 +2945,9 @@ static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const s
         BF_CHECK_LAUNCH();
     }
     {
-        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)nrows);
+        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)parseval_cols(mc, nn) * (double)NG * (double)nrows);
         const int nmb = (nrows + PG_BM - 1) / PG_BM;
-        launch_parseval_gemm<true>(ctx, parseval_wn(ctx, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr, nullptr,
+        launch_parseval_gemm<true>(ctx, parseval_wn(ctx, mc, nn, NG), s, mc, nn, NG, nullptr, nullptr, nullptr, nullptr,
                                    (size_t)nrows, scale, drow.as<zd*>(), nmb, dw.as<zd>());
     }
     zd* inv = nullptr;
@@ -3110,9 +3206,9 @@ void parseval_flush(bf_ctx* ctx, ModeCache& mc, int n, double scale, ParsevalQue
         ctx->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         save_tuned(ctx, key);
     }
-    wn = parseval_wn(ctx, nn, NG);
+    wn = parseval_wn(ctx, mc, nn, NG);
     {
-        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)nn * (double)NG * (double)M);
+        TimedLaunch tl(ctx, "pgemm", 8.0 * (double)parseval_cols(mc, nn) * (double)NG * (double)M);
         launch(wn);
     }
     if (!st) BF_HIP(hipStreamSynchronize(s));  // the host arrays are reused by the next flush's uploads

@@ -119,7 +119,10 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * cores: 1 where it pays, 4 for every n <= 32, 2 / 3 for every n <= 32 in
  * the earlier one-wave-per-tile form with four / three MFMAs per complex
  * product), "chain_order" (chain blocks of a batch start with the longest
- * chains; 0 keeps job order), "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
+ * chains; 0 keeps job order), "pgemm_herm" (the Parseval product over the
+ * upper triangles when the mode set's patterns vanish outside the aperture,
+ * so that its group Grams are Hermitian; 0 forms every entry), "autotune"
+ * (below).  Shapes and budgets (0 = default): "col_tx", "rows"
  * (dancer launch shapes), "pg_wn" (2 or 4: Parseval product column blocks of
  * 128 or 256 matrix entries), "batch_bytes" (bytes of P and inv(P) per
  * chunk of a batched sweep; default 16 GiB).  Unknown names and negative

@@ -576,6 +576,49 @@ def test_batch_plan_flat_chunks_match_general_path(engine_options):
         assert np.array_equal(out["general_job_order"][c], out["general"][c]), c
 
 
+def test_parseval_product_over_upper_triangles_matches_full_product(engine_options):
+    """Bessel patterns vanish outside the aperture, so the masked and unmasked
+    pattern transforms agree and every group Gram H_g is Hermitian: the
+    Parseval product then forms the upper triangles only (n (n + 1) / 2 of the
+    n^2 entries per problem, read off the executed flops) and mirrors them.
+    Boosts agree with the full product (pgemm_herm=0) to 1e-12 on the series
+    path (flat chunks) and on the direct path (parseval_series=0).  Patterns
+    that do not vanish outside the aperture keep the full product."""
+    coords = W.grid_coords(64)
+    rng = W.SplitMix64(1770)
+    geoms = []
+    for nd in (4, 11, 19):
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp), flat=True))
+    freqs = np.linspace(19.5e9, 20.5e9, 12)
+    ctx = B.context()
+
+    def run(modes, **kw):
+        engine_options(**kw)
+        plan = B.TransformerBatchPlan(coords, [modes], freqs, diskR=DISKR)
+        ctx.timing(True)
+        plan.run([g.copy() for g in geoms])
+        out = plan.fetch()
+        flops = ctx.kernel_time("pgemm")[2]
+        ctx.timing(False)
+        plan.close()
+        engine_options(pgemm_herm=1, parseval_series=1)
+        return out, flops
+
+    bessel = B.SeedModes(coords, ThreeDim=True, Mmax=3, Lmax=1, diskR=DISKR)
+    n = bessel.n
+    for kw in ({}, dict(parseval_series=0)):
+        herm, fh = run(bessel, **kw)
+        full, ff = run(bessel, pgemm_herm=0, **kw)
+        assert fh > 0 and abs(fh / ff - (n + 1) / (2 * n)) < 1e-12, (kw, fh / ff)
+        for c in range(len(geoms)):
+            assert rel(herm[c], full[c]) < 1e-12, (kw, c, rel(herm[c], full[c]))
+    lg = _lg_like_modes(coords, 3, 1)
+    _, fh = run(lg)
+    _, ff = run(lg, pgemm_herm=0)
+    assert fh == ff
+
+
 def test_batch_plan_grid_per_mode_set_matches_plans_per_grid():
     """a plan whose mode sets live on grids of different sizes (64x64 and 48x40,
     bf_transformer_batch_plan_create_grids), run once over geometries of both
