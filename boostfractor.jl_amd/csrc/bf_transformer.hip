@@ -1744,12 +1744,14 @@ __global__ void __launch_bounds__(256) k_group_gram_lds(const zd* __restrict__ A
 // TAB = true: the weights are read from a table wtab[g][m] (M rows) instead
 // (the coefficient rows of the spacing series below); kz and the per-block
 // k0^2 lists are then unused.
-template <int WN, bool HERM = false>
+template <int WN, bool HERM = false, bool TAB = false>
 struct PgShape {
     static constexpr int WM = 2;                     // waves along problems
     static constexpr int CT = HERM ? 2 : 4;          // 16-entry column tiles per wave
     static constexpr int BM = 32 * WM, BN = 16 * CT * WN;  // block tile
-    static constexpr int KC = 8;                     // groups per chunk
+    // groups per chunk: 16 for the half-width tiles of the Hermitian product on a weight table (as many
+    // MFMAs per barrier as the full product's 8; the direct product's kz staging keeps it at 8)
+    static constexpr int KC = HERM && TAB ? 16 : 8;
     static constexpr int NT = 64 * WM * WN;
     static constexpr int LDA = BM + 2, LDB = BN + 2;  // padded LDS rows (doubles)
     static constexpr int WPT = BM * KC / NT;          // weights formed per thread and chunk
@@ -1764,17 +1766,17 @@ constexpr int PG_BM = 64;  // rows per block (all shapes)
 // real sums Wr Hr, Wr Hi, Wi Hr, Wi Hi at (i, j), i <= j, give both (i, j) = (A + i B)_ij and
 // (j, i) = conj(A_ij) + i conj(B_ij): half the entries for the same MFMAs per entry.  Else ncol = nn.
 template <int WN, bool TAB, bool HERM>
-__global__ void __launch_bounds__((PgShape<WN, HERM>::NT), 4 / WN) k_parseval_gemm(
+__global__ void __launch_bounds__((PgShape<WN, HERM, TAB>::NT), 4 / WN) k_parseval_gemm(
     const zd* __restrict__ H, int nn, int NG, const double* __restrict__ gkx, const double* __restrict__ gky,
     const PwDesc* __restrict__ pd, const int* __restrict__ row_slot, const int* __restrict__ blk_nk,
     const zd* __restrict__ blk_k0sq, int M, double scale, zd* const* __restrict__ dst, int nmb,
     const zd* __restrict__ wtab, int ncb, int cps, zd* __restrict__ part, int ncol, int n,
     const int* __restrict__ tri) {
-    using S = PgShape<WN, HERM>;
+    using S = PgShape<WN, HERM, TAB>;
     constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NT = S::NT, CT = S::CT;
     __shared__ double swr[2][KC][S::LDA], swi[2][KC][S::LDA];
     __shared__ double shr[2][KC][S::LDB], shi[2][KC][S::LDB];
-    __shared__ zd skz[TAB ? 1 : 3][BM][KC];
+    __shared__ zd skz[TAB ? 1 : 3][TAB ? 1 : BM][TAB ? 1 : KC];  // kz staging (direct product only)
     // block -> (row block, column block, group split); split ks sums the chunks [ks cps, (ks + 1) cps)
     const int ks_ = blockIdx.x / (nmb * ncb), rem = blockIdx.x % (nmb * ncb);
     const int mb = rem % nmb, cb = rem / nmb;
@@ -2646,8 +2648,8 @@ void disk_diag_flush(bf_ctx* ctx, ModeCache& mc, const ModesDev& md, ParsevalQue
 // to a workspace and k_pgemm_reduce adds them in split order.  The split does
 // not depend on the shape, so both shapes sum every entry in the same order.
 // Returns the split count and the chunks per split.
-static std::pair<int, int> parseval_splits(size_t nmb, size_t ncb, size_t NG) {
-    const int nch = (int)((NG + PgShape<2>::KC - 1) / PgShape<2>::KC);
+static std::pair<int, int> parseval_splits(size_t nmb, size_t ncb, size_t NG, int kc) {
+    const int nch = (int)((NG + kc - 1) / kc);
     const double slots = 512.0;
     int best = 1;
     double best_eff = 0.0;
@@ -2674,8 +2676,9 @@ static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeC
     const size_t ncol = parseval_cols(mc, nn);
     const size_t BN = herm ? (wn == 2 ? PgShape<2, true>::BN : PgShape<4, true>::BN)
                            : (wn == 2 ? PgShape<2>::BN : PgShape<4>::BN);
+    const int kc = herm ? PgShape<2, true, TAB>::KC : PgShape<2, false, TAB>::KC;
     const int ncb = (int)((ncol + BN - 1) / BN);
-    const auto sp = parseval_splits((size_t)nmb, (size_t)ncb, NG);
+    const auto sp = parseval_splits((size_t)nmb, (size_t)ncb, NG, kc);
     zd* part = nullptr;
     if (sp.first > 1) part = scratch_buf(ctx, SB_PGPART, sizeof(zd) * (size_t)sp.first * M * nn).as<zd>();
     const dim3 grid((unsigned)((size_t)nmb * ncb * sp.first));
@@ -2683,7 +2686,8 @@ static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeC
     const int n = (int)std::lround(std::sqrt((double)nn));
     const int* tri = herm ? mc.tri.as<int>() : nullptr;
 #define BF_PGEMM(W, HM)                                                                                         \
-    hipLaunchKernelGGL((k_parseval_gemm<W, TAB, HM>), grid, dim3(PgShape<W, HM>::NT), 0, s, Hs, (int)nn, (int)NG, \
+    hipLaunchKernelGGL((k_parseval_gemm<W, TAB, HM>), grid, dim3(PgShape<W, HM, TAB>::NT), 0, s, Hs, (int)nn, \
+                       (int)NG, \
                        mc.gkx.as<double>(), mc.gky.as<double>(), pd, slot, nk, k0, (int)M, scale, dst, nmb, wtab,  \
                        ncb, sp.second, part, (int)ncol, n, tri)
     if (herm) {
