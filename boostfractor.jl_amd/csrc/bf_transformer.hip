@@ -2678,7 +2678,10 @@ static void launch_parseval_gemm(bf_ctx* ctx, int wn, hipStream_t s, const ModeC
                            : (wn == 2 ? PgShape<2>::BN : PgShape<4>::BN);
     const int kc = herm ? PgShape<2, true, TAB>::KC : PgShape<2, false, TAB>::KC;
     const int ncb = (int)((ncol + BN - 1) / BN);
-    const auto sp = parseval_splits((size_t)nmb, (size_t)ncb, NG, kc);
+    // the group split follows from the narrower width's column blocks whatever wn is, so that both widths
+    // sum every entry in the same order (bitwise the same results)
+    const size_t bn2 = herm ? PgShape<2, true>::BN : PgShape<2>::BN;
+    const auto sp = parseval_splits((size_t)nmb, (ncol + bn2 - 1) / bn2, NG, kc);
     zd* part = nullptr;
     if (sp.first > 1) part = scratch_buf(ctx, SB_PGPART, sizeof(zd) * (size_t)sp.first * M * nn).as<zd>();
     const dim3 grid((unsigned)((size_t)nmb * ncb * sp.first));
