@@ -325,12 +325,26 @@ class Marshalled:
         arr = (Setup * max(n, 1))()
         if n == 0:
             return arr
-        dists = [np.asarray(b.distance, dtype=np.float64).reshape(-1) for b in bdrys]
-        lens = np.array([len(d) for d in dists], dtype=np.int64)
-        rs = [np.asarray(b.r, dtype=np.complex128).reshape(-1) for b in bdrys]
-        rlens = np.array([len(r) for r in rs], dtype=np.int64)
+        # one pass over the geometries gathers every field (the joins and the structs are numpy-wide)
+        dists, rs, epss, txs, tys, rough = [], [], [], [], [], []
+        f64, c128 = np.float64, np.complex128
+        for i, b in enumerate(bdrys):
+            d = np.asarray(b.distance, dtype=f64).reshape(-1)
+            l = len(d)
+            dists.append(d)
+            rs.append(np.asarray(b.r, dtype=c128).reshape(-1))
+            epss.append(np.asarray(b.eps, dtype=c128).reshape(-1))
+            for v, out in ((b.relative_tilt_x, txs), (b.relative_tilt_y, tys)):
+                # a tilt array of the geometry's region count, or a value broadcast to it
+                a = np.asarray(v, dtype=f64)
+                out.append(a if a.shape == (l,) else np.broadcast_to(a, (l,)))
+            srf = b.relative_surfaces
+            if getattr(srf, "size", 1):
+                rough.append(i)
+        lens = np.fromiter((len(d) for d in dists), dtype=np.int64, count=n)
+        rlens = np.fromiter((len(x) for x in rs), dtype=np.int64, count=n)
         dist = self.hold(np.concatenate(dists))
-        eps = self.hold(np.concatenate([np.asarray(b.eps, dtype=np.complex128).reshape(-1) for b in bdrys]))
+        eps = self.hold(np.concatenate(epss))
         if len(eps) != len(dist):
             raise DimensionMismatch("eps and distance differ in length")
         if nan_to_1e30:
@@ -342,12 +356,8 @@ class Marshalled:
                 for q, c in zip(bad.tolist(), owner.tolist()):
                     bdrys[c].eps[q - (ends[c] - lens[c])] = 1e30
         r = self.hold(np.concatenate(rs))
-        def per_region(v, l):  # a tilt array of the geometry's region count, or a value broadcast to it
-            a = np.asarray(v, dtype=np.float64)
-            return a if a.shape == (l,) else np.broadcast_to(a, (l,))
-
-        tx = self.hold(np.concatenate([per_region(b.relative_tilt_x, l) for b, l in zip(bdrys, lens)]))
-        ty = self.hold(np.concatenate([per_region(b.relative_tilt_y, l) for b, l in zip(bdrys, lens)]))
+        tx = self.hold(np.concatenate(txs))
+        ty = self.hold(np.concatenate(tys))
         off = np.concatenate(([0], np.cumsum(lens)[:-1]))
         roff = np.concatenate(([0], np.cumsum(rlens)[:-1]))
         f = Setup
@@ -363,8 +373,8 @@ class Marshalled:
         rec["eps"] = eps.ctypes.data + 16 * off
         rec["tilt_x"] = tx.ctypes.data + 8 * off
         rec["tilt_y"] = ty.ctypes.data + 8 * off
-        for i, b in enumerate(bdrys):
-            s = np.asarray(b.relative_surfaces)
+        for i in rough:  # geometries with a surface stack: passed if any map is nonzero
+            s = np.asarray(bdrys[i].relative_surfaces)
             if s.size and any(np.asarray(s[q]).view(np.float64 if s.dtype == np.complex128 else s.dtype).any()
                               for q in range(s.shape[0])):
                 want = (int(lens[i]), int(ny[i] if np.ndim(ny) else ny), int(nx[i] if np.ndim(nx) else nx))
