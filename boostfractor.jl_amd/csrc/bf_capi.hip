@@ -211,6 +211,7 @@ static int64_t* option_slot(bf::EngineOptions& o, const char* name, int64_t& tmp
         {"chain_mfma", &bf::EngineOptions::chain_mfma},
         {"chain_order", &bf::EngineOptions::chain_order},
         {"pgemm_herm", &bf::EngineOptions::pgemm_herm},
+        {"sinv_mfma", &bf::EngineOptions::sinv_mfma},
     };
     ip = nullptr;
     for (const Int& e : ints)

@@ -2904,6 +2904,110 @@ __global__ void __launch_bounds__(128) k_series_inverse(const SeriesInvJob* __re
     if (tid == 0 && !(red[0] <= SERIES_INV_TAIL * e0max)) *bad = 1;
 }
 
+// k_series_inverse on the matrix cores for 16 < n <= 32 (operands padded to 32; KF = ceil(n / 4) k steps):
+// block (key, column block y) computes columns 16 y .. 16 y + 15 of every E_p, wave w rows 16 w .. 16 w + 15.
+// S_p = sum_j D_j E_{p-j} and E_p = -E_0 S_p by v_mfma_f64_16x16x4f64, three MFMAs per complex 4-k step;
+// the lanes load D_j and E_{p-j} (global memory, L2) in the MFMA layouts with indices clamped into the
+// matrix (B entries at k >= n zeroed), the next j's while the current j's products run; S_p goes through
+// LDS to the second product.  The same tail check as k_series_inverse.
+template <int KF>
+__global__ void __launch_bounds__(128) k_series_inverse_mfma(const SeriesInvJob* __restrict__ jobs, int n,
+                                                             int* bad) {
+    constexpr int LD = 33;
+    __shared__ zd S[32 * LD];  // S_p[k][c] at S[k + LD c] (rows k >= n stay zero)
+    __shared__ double red[128];
+    const SeriesInvJob J = jobs[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, kq = lane >> 4, c = lane & 15;
+    const int c0 = blockIdx.y * 16, nn = n * n;
+    const int ar = 16 * w + c, bcol = c0 + c;  // A row of the lane, B / output column
+    const int rr = ar < n ? ar : n - 1, cc = bcol < n ? bcol : n - 1;
+    for (int e = tid; e < 32 * LD; e += 128) S[e] = cmk<double>(0.0, 0.0);
+    // E_0's A operand (rows ar, k = 4 s + kq) stays in registers
+    zd e0a[KF];
+#pragma unroll
+    for (int s = 0; s < KF; s++) {
+        const int k = 4 * s + kq;
+        e0a[s] = J.e[rr + n * (k < n ? k : n - 1)];
+    }
+    double m0 = 0.0, mt = 0.0;
+    const int cw = min(16, n - c0);
+    for (int e = tid; e < n * cw; e += 128) {
+        const zd v = J.e[e % n + (size_t)n * (c0 + e / n)];
+        m0 = fmax(m0, fabs(v.re) + fabs(v.im));
+    }
+    auto gauss = [](d4& cr, d4& ci, d4& cg, const zd& a, const zd& b) {
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.im, ci, 0, 0, 0);
+        cg = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re + a.im, b.re + b.im, cg, 0, 0, 0);
+    };
+    for (int p = 1; p < J.q; p++) {
+        __syncthreads();  // E_{p-1} of both waves written; S of term p - 1 read
+        d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr, cg = cr;
+        zd an[KF], bn[KF];
+        auto load = [&](int j) {
+            const zd* Dj = J.d + (size_t)j * nn;
+            const zd* Ep = J.e + (size_t)(p - j) * nn;
+#pragma unroll
+            for (int s = 0; s < KF; s++) {
+                const int k = 4 * s + kq, kk = k < n ? k : n - 1;
+                an[s] = Dj[rr + n * kk];
+                bn[s] = Ep[kk + n * cc];
+            }
+        };
+        load(1);
+        for (int j = 1; j <= p; j++) {
+            zd a[KF], b[KF];
+#pragma unroll
+            for (int s = 0; s < KF; s++) {
+                a[s] = an[s];
+                b[s] = bn[s];
+            }
+            if (j < p) load(j + 1);
+#pragma unroll
+            for (int s = 0; s < KF; s++) {
+                const bool kin = 4 * s + kq < n;
+                const zd bz = cmk<double>(kin ? b[s].re : 0.0, kin ? b[s].im : 0.0);
+                gauss(cr, ci, cg, a[s], bz);
+            }
+        }
+        // S_p tile: register v holds S[16 w + 4 v + kq][c]
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * w + 4 * v + kq;
+            if (i < n) S[i + LD * c] = cmk<double>(cr[v] - ci[v], cg[v] - cr[v] - ci[v]);
+        }
+        __syncthreads();
+        d4 er = {0.0, 0.0, 0.0, 0.0}, ei = er, eg = er;
+#pragma unroll
+        for (int s = 0; s < KF; s++) gauss(er, ei, eg, e0a[s], S[4 * s + kq + LD * c]);
+        zd* Eo = J.e + (size_t)p * nn;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * w + 4 * v + kq;
+            if (i < n && bcol < n) {
+                const zd a = cmk<double>(-(er[v] - ei[v]), -(eg[v] - er[v] - ei[v]));
+                Eo[i + (size_t)n * bcol] = a;
+                if (p >= J.q - 2) mt = fmax(mt, fabs(a.re) + fabs(a.im));
+            }
+        }
+    }
+    red[tid] = m0;
+    __syncthreads();
+    for (int o = 64; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    const double e0max = red[0];
+    __syncthreads();
+    red[tid] = mt;
+    __syncthreads();
+    for (int o = 64; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmax(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    if (tid == 0 && !(red[0] <= SERIES_INV_TAIL * e0max)) *bad = 1;
+}
+
 using SeriesKeyId = std::array<uint64_t, 3>;  // k0^2 bits (re, im) and spacing bin; {0, 0, ~0}: dz = 0
 constexpr uint64_t SERIES_ZERO_BIN = ~(uint64_t)0;
 
@@ -2979,10 +3083,20 @@ static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const s
         hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)fresh.size()), dim3(64), smem1, s, blk, inv, n,
                            dsel.as<int>(), (const InvSel*)nullptr);
         BF_CHECK_LAUNCH();
-        const size_t smem3 = sizeof(zd) * (2 * nn + (size_t)n * SINV_CW);
-        allow_lds((const void*)k_series_inverse, smem3);
-        hipLaunchKernelGGL(k_series_inverse, dim3((unsigned)fresh.size(), (unsigned)((n + SINV_CW - 1) / SINV_CW)),
-                           dim3(128), smem3, s, dj.as<SeriesInvJob>(), n, bad);
+        if (n > 16 && n <= 32 && opts().sinv_mfma) {
+            const dim3 grid((unsigned)fresh.size(), (unsigned)((n + 15) / 16));
+            switch ((n + 3) / 4) {
+                case 5: hipLaunchKernelGGL(k_series_inverse_mfma<5>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
+                case 6: hipLaunchKernelGGL(k_series_inverse_mfma<6>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
+                case 7: hipLaunchKernelGGL(k_series_inverse_mfma<7>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
+                default: hipLaunchKernelGGL(k_series_inverse_mfma<8>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
+            }
+        } else {
+            const size_t smem3 = sizeof(zd) * (2 * nn + (size_t)n * SINV_CW);
+            allow_lds((const void*)k_series_inverse, smem3);
+            hipLaunchKernelGGL(k_series_inverse, dim3((unsigned)fresh.size(), (unsigned)((n + SINV_CW - 1) / SINV_CW)),
+                               dim3(128), smem3, s, dj.as<SeriesInvJob>(), n, bad);
+        }
         BF_CHECK_LAUNCH();
     }
     for (size_t k = 0; k < fresh.size(); k++)

@@ -619,6 +619,32 @@ def test_parseval_product_over_upper_triangles_matches_full_product(engine_optio
     assert fh == ff
 
 
+def test_inverse_series_on_matrix_cores_matches_valu(engine_options):
+    """the gaps' inverses from the inverse spacing series: the matrix-core kernel
+    (16 < n <= 32; here n = 25) against the FP64 VALU kernel (sinv_mfma=0), and
+    both against the LU inverses (series_inverse=0): boosts to 1e-10"""
+    coords = W.grid_coords(64)
+    rng = W.SplitMix64(1780)
+    geoms = []
+    for nd in (6, 23, 14):
+        sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
+        geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp), flat=True))
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=2, diskR=DISKR)
+    assert modes.n == 25
+    freqs = np.linspace(19.5e9, 20.5e9, 16)
+    out = {}
+    for name, kw in (("mfma", {}), ("valu", dict(sinv_mfma=0)), ("lu", dict(series_inverse=0))):
+        engine_options(**kw)
+        plan = B.TransformerBatchPlan(coords, [modes], freqs, diskR=DISKR)
+        plan.run([g.copy() for g in geoms])
+        out[name] = plan.fetch()
+        plan.close()
+        engine_options(sinv_mfma=1, series_inverse=1)
+    for c in range(len(geoms)):
+        assert rel(out["mfma"][c], out["valu"][c]) < 1e-10, (c, rel(out["mfma"][c], out["valu"][c]))
+        assert rel(out["mfma"][c], out["lu"][c]) < 1e-10, (c, rel(out["mfma"][c], out["lu"][c]))
+
+
 def test_batch_plan_grid_per_mode_set_matches_plans_per_grid():
     """a plan whose mode sets live on grids of different sizes (64x64 and 48x40,
     bf_transformer_batch_plan_create_grids), run once over geometries of both
