@@ -619,18 +619,19 @@ def test_parseval_product_over_upper_triangles_matches_full_product(engine_optio
     assert fh == ff
 
 
-def test_inverse_series_on_matrix_cores_matches_valu(engine_options):
+@pytest.mark.parametrize("mmax,lmax,n", [(5, 1, 15), (5, 2, 25)])
+def test_inverse_series_on_matrix_cores_matches_valu(engine_options, mmax, lmax, n):
     """the gaps' inverses from the inverse spacing series: the matrix-core kernel
-    (16 < n <= 32; here n = 25) against the FP64 VALU kernel (sinv_mfma=0), and
-    both against the LU inverses (series_inverse=0): boosts to 1e-10"""
+    (12 < n <= 32; here n = 15 and 25) against the FP64 VALU kernel (sinv_mfma=0),
+    and both against the LU inverses (series_inverse=0): boosts to 1e-10"""
     coords = W.grid_coords(64)
     rng = W.SplitMix64(1780)
     geoms = []
     for nd in (6, 23, 14):
         sp = 8e-3 + (2.0 * rng.uniforms(nd) - 1.0) * 0.5e-3
         geoms.append(W.booster(coords, ndisk=nd, spacings=list(sp), flat=True))
-    modes = B.SeedModes(coords, ThreeDim=True, Mmax=5, Lmax=2, diskR=DISKR)
-    assert modes.n == 25
+    modes = B.SeedModes(coords, ThreeDim=True, Mmax=mmax, Lmax=lmax, diskR=DISKR)
+    assert modes.n == n
     freqs = np.linspace(19.5e9, 20.5e9, 16)
     out = {}
     for name, kw in (("mfma", {}), ("valu", dict(sinv_mfma=0)), ("lu", dict(series_inverse=0))):
