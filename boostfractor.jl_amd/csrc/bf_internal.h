@@ -107,7 +107,7 @@ struct EngineOptions {
                               // MFMAs per complex product), 0 never
     int chain_order = 1;      // chain blocks start longest chain first (0: in job order)
     int pgemm_herm = 1;       // Parseval product over the upper triangles when the Grams are Hermitian
-    int sinv_mfma = 1;        // inverse series on the matrix cores for 12 < n <= 32
+    int sinv_mfma = 1;        // inverse series on the matrix cores for 8 < n <= 32
     int autotune = 0;         // 1: time candidate launch shapes on first use of a shape key (kept per context)
     int64_t batch_bytes = 0;  // matrix budget of a sweep chunk (0: BF_DEFAULT_BATCH_BYTES)
 };

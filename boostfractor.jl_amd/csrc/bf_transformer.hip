@@ -2904,7 +2904,7 @@ __global__ void __launch_bounds__(128) k_series_inverse(const SeriesInvJob* __re
     if (tid == 0 && !(red[0] <= SERIES_INV_TAIL * e0max)) *bad = 1;
 }
 
-// k_series_inverse on the matrix cores for 12 < n <= 32 (operands padded to 32; KF = ceil(n / 4) k steps):
+// k_series_inverse on the matrix cores for 8 < n <= 32 (operands padded to 32; KF = ceil(n / 4) k steps):
 // block (key, column block y) computes columns 16 y .. 16 y + 15 of every E_p, wave w rows 16 w .. 16 w + 15
 // (for n <= 16 wave 1's rows are all beyond n: its products are clamped copies and never stored).
 // S_p = sum_j D_j E_{p-j} and E_p = -E_0 S_p by v_mfma_f64_16x16x4f64, three MFMAs per complex 4-k step;
@@ -3084,9 +3084,10 @@ static void series_form(bf_ctx* ctx, ModeCache& mc, int n, double scale, const s
         hipLaunchKernelGGL(k_batched_inverse_wave, dim3((unsigned)fresh.size()), dim3(64), smem1, s, blk, inv, n,
                            dsel.as<int>(), (const InvSel*)nullptr);
         BF_CHECK_LAUNCH();
-        if (n > 12 && n <= 32 && opts().sinv_mfma) {
+        if (n > 8 && n <= 32 && opts().sinv_mfma) {
             const dim3 grid((unsigned)fresh.size(), (unsigned)((n + 15) / 16));
             switch ((n + 3) / 4) {
+                case 3: hipLaunchKernelGGL(k_series_inverse_mfma<3>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
                 case 4: hipLaunchKernelGGL(k_series_inverse_mfma<4>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
                 case 5: hipLaunchKernelGGL(k_series_inverse_mfma<5>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;
                 case 6: hipLaunchKernelGGL(k_series_inverse_mfma<6>, grid, dim3(128), 0, s, dj.as<SeriesInvJob>(), n, bad); break;

@@ -122,7 +122,7 @@ int bf_ctx_kernel_time(bf_ctx *ctx, const char *kernel, double *ms, int64_t *lau
  * chains; 0 keeps job order), "pgemm_herm" (the Parseval product over the
  * upper triangles when the mode set's patterns vanish outside the aperture,
  * so that its group Grams are Hermitian; 0 forms every entry), "sinv_mfma"
- * (the inverse series on the matrix cores for 12 < n <= 32; 0: FP64 VALU),
+ * (the inverse series on the matrix cores for 8 < n <= 32; 0: FP64 VALU),
  * "autotune" (below).  Shapes and budgets (0 = default): "col_tx", "rows"
  * (dancer launch shapes), "pg_wn" (2 or 4: Parseval product column blocks of
  * 128 or 256 matrix entries), "batch_bytes" (bytes of P and inv(P) per

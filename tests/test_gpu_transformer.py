@@ -619,10 +619,10 @@ def test_parseval_product_over_upper_triangles_matches_full_product(engine_optio
     assert fh == ff
 
 
-@pytest.mark.parametrize("mmax,lmax,n", [(5, 1, 15), (5, 2, 25)])
+@pytest.mark.parametrize("mmax,lmax,n", [(3, 1, 9), (5, 1, 15), (5, 2, 25)])
 def test_inverse_series_on_matrix_cores_matches_valu(engine_options, mmax, lmax, n):
     """the gaps' inverses from the inverse spacing series: the matrix-core kernel
-    (12 < n <= 32; here n = 15 and 25) against the FP64 VALU kernel (sinv_mfma=0),
+    (8 < n <= 32; here n = 9, 15 and 25) against the FP64 VALU kernel (sinv_mfma=0),
     and both against the LU inverses (series_inverse=0): boosts to 1e-10"""
     coords = W.grid_coords(64)
     rng = W.SplitMix64(1780)
