@@ -233,12 +233,13 @@ MM_CLASSES = ("wgram", "pgemm", "pseries", "sinverse", "inverse", "chain")
 MM_KERNELS = {
     "pgemm": ("k_parseval_gemm", "k_parseval_gemm (v_mfma_f64_16x16x4f64, weights exp(-i kz dz)/K formed in "
               "registers or read from the spacing series' table): flat air regions, P = sum over k-bin groups of "
-              "W_g H_g"),
+              "W_g H_g; over the packed upper triangles when the group Grams are Hermitian"),
     "wgram": ("k_wgram", "k_wgram (weighted Gram products A^H diag(w) B on v_mfma_f64_16x16x4f64)"),
     "inverse": ("k_batched_inverse_wave", "k_batched_inverse_wave (FP64 Gauss-Jordan with partial pivoting in place in "
                 "LDS, one wave per matrix; FP64 VALU, whose MI355X peak equals the FP64 MFMA one)"),
-    "sinverse": ("k_series_inverse", "k_series_inverse (coefficients of the gaps' inverse series, E_0 = inv(D_0), "
-                 "E_p = -E_0 sum_j D_j E_(p-j), one block per key)"),
+    "sinverse": ("k_series_inverse", "k_series_inverse_mfma / k_series_inverse (coefficients of the gaps' inverse "
+                 "series, E_0 = inv(D_0), E_p = -E_0 sum_j D_j E_(p-j): on v_mfma_f64_16x16x4f64 for 8 < n <= 32, "
+                 "else FP64 VALU; blocks of columns per key)"),
     "chain": ("k_chain", "k_chain_mfma (n = 25: v_mfma_f64_16x16x4f64 on operands padded to 32, three MFMAs per "
               "complex product), k_chain_mfma2 (n = 15: padded to 16, the two products in separate waves) and "
               "k_chain_tile (n = 9: register-tiled FP64 VALU): bottom block row of the transfer-matrix chain, "
